@@ -1,0 +1,668 @@
+/*
+ * CPU oracle for the pressure-solve path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates the reference's serial algorithm (cited per function) so that the
+ * GPU library can be checked against it.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it; the product never does.
+ *
+ * Every floating-point expression is evaluated in the order the Fortran source
+ * writes it (left to right, products rounded before sums); build with
+ * -ffp-contract=off.
+ *
+ * Pinning (which check fixes each function):
+ *   orc_mesh_geometry  -- bitwise vs the reference Fortran (oracle/_ref) on
+ *                         uniform, perturbed, graded and tet meshes; closed
+ *                         forms on the uniform box.
+ *   orc_csr_build      -- bitwise vs the reference; scipy canonical CSR.
+ *   orc_laplacian      -- bitwise vs the reference; closed form on the box.
+ *   orc_iccg/dpcg/bicgstab -- the 5x5 transcript tests/output.txt of the
+ *                         reference; bitwise iteration histories vs the
+ *                         reference on assembled mesh systems; dense solves.
+ *   orc_face_flux_central / orc_rhs_divergence -- bitwise vs a numpy
+ *                         evaluation and the divergence-theorem invariant
+ *                         (sum of su over cells equals the net boundary flux,
+ *                         zero for inner-face-only fluxes).
+ */
+#include "fcap_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+double orc_small(void) { return (double)1e-20f; }
+double orc_tol(void) { return (double)1e-13f; }
+double orc_s0(void) { return (double)1e20f; }
+double orc_tiny(void) { return (double)1e-30f; }
+
+/* 1-based accessors */
+#define I1(arr, k) ((arr)[(k)-1])
+
+/* ------------------------------------------------------------------------ */
+/* Geometry: src/mesh_geometry_and_topology.f90:106-274 (helpers) and
+ * :859-1081 (the three face sweeps).                                         */
+/* ------------------------------------------------------------------------ */
+
+static double centroid_part(double ax, double bx, double cx, double nx, double vol)
+{
+    /* 1./(2*vol) * 1./24._dp*nx * ( (ax+bx)**2 + (bx+cx)**2 + (cx+ax)**2 )
+     * (src/mesh_geometry_and_topology.f90:181) evaluated left to right. */
+    double t = 1.0 / (2.0 * vol);
+    t = t * 1.0;
+    t = t / 24.0;
+    t = t * nx;
+    double ab = ax + bx, bc = bx + cx, ca = cx + ax;
+    double s = ab * ab + bc * bc;
+    s = s + ca * ca;
+    return t * s;
+}
+
+static void intersection_point(double x1, double y1, double z1, double x2, double y2, double z2,
+                               double x3, double y3, double z3, double x4, double y4, double z4,
+                               double x5, double y5, double z5, double *xj, double *yj, double *zj)
+{
+    /* src/mesh_geometry_and_topology.f90:262-272 */
+    double num = x2 * (y3 * z4 - y4 * z3);
+    num = num - x1 * (y3 * z4 - y4 * z3);
+    num = num - x3 * (y2 * z4 - y4 * z2);
+    num = num + x1 * (y2 * z4 - y4 * z2);
+    num = num + x3 * (y1 * z4 - y4 * z1);
+    num = num - x2 * (y1 * z4 - y4 * z1);
+    num = num + x4 * (y2 * z3 - y3 * z2);
+    num = num - x1 * (y2 * z3 - y3 * z2);
+    num = num - x4 * (y1 * z3 - y3 * z1);
+    num = num + x2 * (y1 * z3 - y3 * z1);
+    num = num + x4 * (y1 * z2 - y2 * z1);
+    num = num - x3 * (y1 * z2 - y2 * z1);
+    double dz = z5 - z4, dy = y5 - y4, dx = x5 - x4;
+    double den = x2 * (y3 * dz - dy * z3);
+    den = den - x1 * (y3 * dz - dy * z3);
+    den = den - x3 * (y2 * dz - dy * z2);
+    den = den + x1 * (y2 * dz - dy * z2);
+    den = den + x3 * (y1 * dz - dy * z1);
+    den = den - x2 * (y1 * dz - dy * z1);
+    den = den + dx * (y2 * z3 - y3 * z2);
+    den = den - dx * (y1 * z3 - y3 * z1);
+    den = den + dx * (y1 * z2 - y2 * z1);
+    den = den + orc_tiny();
+    double t = -(num / den);
+    *xj = x4 + dx * t;
+    *yj = y4 + dy * t;
+    *zj = z4 + dz * t;
+}
+
+int orc_mesh_geometry(int nnodes, const double *xyz, int nfaces, int nif,
+                      const int *face_ptr, const int *face_nodes1, const int *owner1,
+                      const int *neighbour1, int ncells,
+                      double *arx, double *ary, double *arz,
+                      double *xf, double *yf, double *zf,
+                      double *vol, double *xc, double *yc, double *zc, double *facint)
+{
+    const double half = 0.5, one_third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+    (void)nnodes;
+    memset(arx, 0, sizeof(double) * nfaces);
+    memset(ary, 0, sizeof(double) * nfaces);
+    memset(arz, 0, sizeof(double) * nfaces);
+    memset(vol, 0, sizeof(double) * ncells);
+    memset(xc, 0, sizeof(double) * ncells);
+    memset(yc, 0, sizeof(double) * ncells);
+    memset(zc, 0, sizeof(double) * ncells);
+#define X(nd) xyz[3 * ((nd)-1) + 0]
+#define Y(nd) xyz[3 * ((nd)-1) + 1]
+#define Z(nd) xyz[3 * ((nd)-1) + 2]
+    /* sweep 1: face areas, volumes, face centres (:859-948) */
+    for (int f = 0; f < nfaces; ++f) {
+        const int *nd = face_nodes1 + face_ptr[f];
+        int nn = face_ptr[f + 1] - face_ptr[f];
+        int inp = owner1[f];
+        for (int i = 0; i < nn - 2; ++i) {
+            double px = X(nd[i + 1]) - X(nd[0]), py = Y(nd[i + 1]) - Y(nd[0]), pz = Z(nd[i + 1]) - Z(nd[0]);
+            double qx = X(nd[i + 2]) - X(nd[0]), qy = Y(nd[i + 2]) - Y(nd[0]), qz = Z(nd[i + 2]) - Z(nd[0]);
+            double nx = py * qz - pz * qy;
+            double ny = pz * qx - px * qz;
+            double nz = px * qy - py * qx;
+            arx[f] = arx[f] + half * nx;
+            ary[f] = ary[f] + half * ny;
+            arz[f] = arz[f] + half * nz;
+            double cx = one_third * ((X(nd[i + 2]) + X(nd[i + 1])) + X(nd[0]));
+            double cy = one_third * ((Y(nd[i + 2]) + Y(nd[i + 1])) + Y(nd[0]));
+            double cz = one_third * ((Z(nd[i + 2]) + Z(nd[i + 1])) + Z(nd[0]));
+            double vp = sixth * ((cx * nx + cy * ny) + cz * nz);
+            I1(vol, inp) = I1(vol, inp) + vp;
+            if (f < nif) {
+                int inn = neighbour1[f];
+                double vn = sixth * ((cx * (-nx) + cy * (-ny)) + cz * (-nz));
+                I1(vol, inn) = I1(vol, inn) + vn;
+            }
+        }
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int i = 0; i < nn; ++i) {
+            sx = sx + X(nd[i]);
+            sy = sy + Y(nd[i]);
+            sz = sz + Z(nd[i]);
+        }
+        xf[f] = sx / (double)nn;
+        yf[f] = sy / (double)nn;
+        zf[f] = sz / (double)nn;
+    }
+    /* sweep 2: cell centres (:974-1013) */
+    for (int f = 0; f < nfaces; ++f) {
+        const int *nd = face_nodes1 + face_ptr[f];
+        int nn = face_ptr[f + 1] - face_ptr[f];
+        int inp = owner1[f];
+        for (int i = 0; i < nn - 2; ++i) {
+            double px = X(nd[i + 1]) - X(nd[0]), py = Y(nd[i + 1]) - Y(nd[0]), pz = Z(nd[i + 1]) - Z(nd[0]);
+            double qx = X(nd[i + 2]) - X(nd[0]), qy = Y(nd[i + 2]) - Y(nd[0]), qz = Z(nd[i + 2]) - Z(nd[0]);
+            double nx = py * qz - pz * qy;
+            double ny = pz * qx - px * qz;
+            double nz = px * qy - py * qx;
+            double vo = I1(vol, inp);
+            I1(xc, inp) = I1(xc, inp) + centroid_part(X(nd[0]), X(nd[i + 1]), X(nd[i + 2]), nx, vo);
+            I1(yc, inp) = I1(yc, inp) + centroid_part(Y(nd[0]), Y(nd[i + 1]), Y(nd[i + 2]), ny, vo);
+            I1(zc, inp) = I1(zc, inp) + centroid_part(Z(nd[0]), Z(nd[i + 1]), Z(nd[i + 2]), nz, vo);
+            if (f < nif) {
+                int inn = neighbour1[f];
+                double vn = I1(vol, inn);
+                I1(xc, inn) = I1(xc, inn) + centroid_part(X(nd[0]), X(nd[i + 2]), X(nd[i + 1]), -nx, vn);
+                I1(yc, inn) = I1(yc, inn) + centroid_part(Y(nd[0]), Y(nd[i + 2]), Y(nd[i + 1]), -ny, vn);
+                I1(zc, inn) = I1(zc, inn) + centroid_part(Z(nd[0]), Z(nd[i + 2]), Z(nd[i + 1]), -nz, vn);
+            }
+        }
+    }
+    /* sweep 3: interpolation factor on inner faces (:1040-1081) */
+    for (int f = 0; f < nif; ++f) {
+        const int *nd = face_nodes1 + face_ptr[f];
+        int inp = owner1[f], inn = neighbour1[f];
+        double xpn = I1(xc, inn) - I1(xc, inp);
+        double ypn = I1(yc, inn) - I1(yc, inp);
+        double zpn = I1(zc, inn) - I1(zc, inp);
+        double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+        double xj, yj, zj;
+        intersection_point(X(nd[0]), Y(nd[0]), Z(nd[0]), X(nd[1]), Y(nd[1]), Z(nd[1]),
+                           X(nd[2]), Y(nd[2]), Z(nd[2]), I1(xc, inp), I1(yc, inp), I1(zc, inp),
+                           I1(xc, inn), I1(yc, inn), I1(zc, inn), &xj, &yj, &zj);
+        xpn = xj - I1(xc, inp);
+        ypn = yj - I1(yc, inp);
+        zpn = zj - I1(zc, inp);
+        double djn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+        facint[f] = djn / dpn;
+    }
+#undef X
+#undef Y
+#undef Z
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR structure: src/sparse_matrix.f90:42-172, src/utils.f90:76-170.
+ * The reference heap-sorts the (row,col) pairs {(i,i)} U {(o,n)} U {(n,o)};
+ * the sorted set is unique, so a per-row counting sort gives the same arrays. */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+int orc_csr_build(int n, int nif, const int *owner1, const int *neighbour1,
+                  int *ioffset, int *ja, int *diag, int *ij_slot, int *ji_slot)
+{
+    int *cnt = (int *)calloc((size_t)n + 1, sizeof(int));
+    if (!cnt) return -1;
+    for (int i = 1; i <= n; ++i) cnt[i] = 1;
+    for (int f = 0; f < nif; ++f) {
+        int o = owner1[f], nb = neighbour1[f];
+        if (o < 1 || o > n || nb < 1 || nb > n) { free(cnt); return -1; }
+        cnt[o]++;
+        cnt[nb]++;
+    }
+    ioffset[0] = 1;
+    for (int i = 1; i <= n; ++i) ioffset[i] = ioffset[i - 1] + cnt[i];
+    int *pos = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    for (int i = 1; i <= n; ++i) pos[i] = I1(ioffset, i);
+    for (int i = 1; i <= n; ++i) I1(ja, pos[i]++) = i;
+    for (int f = 0; f < nif; ++f) {
+        int o = owner1[f], nb = neighbour1[f];
+        I1(ja, pos[o]++) = nb;
+        I1(ja, pos[nb]++) = o;
+    }
+    for (int i = 1; i <= n; ++i) {
+        int s = I1(ioffset, i), e = I1(ioffset, i + 1);
+        qsort(&I1(ja, s), (size_t)(e - s), sizeof(int), cmp_int);
+        /* find_main_diag_element_positions: first (i,i) in sorted order */
+        for (int k = s; k < e; ++k)
+            if (I1(ja, k) == i) { I1(diag, i) = k; break; }
+    }
+    /* csr_to_k: first k in row i with ja(k) == j (src/utils.f90:114-125) */
+    for (int f = 0; f < nif; ++f) {
+        int o = owner1[f], nb = neighbour1[f], k;
+        ij_slot[f] = 0;
+        for (k = I1(ioffset, o); k < I1(ioffset, o + 1); ++k)
+            if (I1(ja, k) == nb) { ij_slot[f] = k; break; }
+        ji_slot[f] = 0;
+        for (k = I1(ioffset, nb); k < I1(ioffset, nb + 1); ++k)
+            if (I1(ja, k) == o) { ji_slot[f] = k; break; }
+    }
+    free(pos);
+    free(cnt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Laplacian: src/fvm_laplacian.f90:1-163, facefluxlaplacian :171-221.       */
+/* ------------------------------------------------------------------------ */
+
+static double flux_laplacian(int ijp, int ijn, double arx, double ary, double arz, double lambda,
+                             const double *mu, const double *xc, const double *yc, const double *zc)
+{
+    double fxn = lambda;
+    double fxp = 1.0 - lambda;
+    double xpn = I1(xc, ijn) - I1(xc, ijp);
+    double ypn = I1(yc, ijn) - I1(yc, ijp);
+    double zpn = I1(zc, ijn) - I1(zc, ijp);
+    double num = (arx * arx + ary * ary) + arz * arz;
+    double den = (arx * xpn + ary * ypn) + arz * zpn;
+    double smdpn = num / den;
+    double m = fxp * I1(mu, ijp) + fxn * I1(mu, ijn);
+    return m * smdpn;
+}
+
+int orc_laplacian(int n, int nif, const int *owner1, const int *neighbour1,
+                  const double *arx, const double *ary, const double *arz,
+                  const double *facint, const double *xc, const double *yc, const double *zc,
+                  const double *xf, const double *yf, const double *zf,
+                  const double *mu, const double *phi,
+                  int nwal, int iwall_faces_start, int iwall_start,
+                  int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                  const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                  double *a, double *su, double *al, double *ar)
+{
+    (void)n;
+    memset(a, 0, sizeof(double) * nnz);
+    for (int i = 0; i < nif; ++i) {
+        int ijp = owner1[i], ijn = neighbour1[i];
+        double cap = flux_laplacian(ijp, ijn, arx[i], ary[i], arz[i], facint[i], mu, xc, yc, zc);
+        double can = cap;
+        I1(a, ij_slot[i]) = can;
+        I1(a, ji_slot[i]) = cap;
+        int k = I1(diag, ijp);
+        I1(a, k) = I1(a, k) - can;
+        k = I1(diag, ijn);
+        I1(a, k) = I1(a, k) - cap;
+    }
+    for (int i = 0; i < noc; ++i) {
+        int iface = ioc_faces_start + i; /* 0-based face index of face iOCFacesStart+i+1 */
+        int ijp = ijl1[i], ijn = ijr1[i];
+        double c = flux_laplacian(ijp, ijn, arx[iface], ary[iface], arz[iface], foc[i], mu, xc, yc, zc);
+        al[i] = c;
+        ar[i] = c;
+        int k = I1(diag, ijp);
+        I1(a, k) = I1(a, k) - ar[i];
+        k = I1(diag, ijn);
+        I1(a, k) = I1(a, k) - al[i];
+    }
+    for (int i = 1; i <= nwal; ++i) {
+        int iface = iwall_faces_start + i; /* 1-based */
+        int ijp = I1(owner1, iface);
+        int ijb = iwall_start + i;         /* 1-based into phi(numTotal) */
+        int k = I1(diag, ijp);
+        double fx = I1(arx, iface), fy = I1(ary, iface), fz = I1(arz, iface);
+        double are = sqrt((fx * fx + fy * fy) + fz * fz);
+        double dx = I1(xc, ijp) - I1(xf, iface), dy = I1(yc, ijp) - I1(yf, iface), dz = I1(zc, ijp) - I1(zf, iface);
+        double dpw = sqrt((dx * dx + dy * dy) + dz * dz);
+        I1(a, k) = I1(a, k) - (I1(mu, ijp) * are) / dpw;
+        I1(su, ijp) = I1(su, ijp) + I1(a, k) * I1(phi, ijb);
+    }
+    return 0;
+}
+
+int orc_convdiff(int n, int nif, const int *owner1, const int *neighbour1,
+                 const double *arx, const double *ary, const double *arz,
+                 const double *facint, const double *xc, const double *yc, const double *zc,
+                 const double *xf, const double *yf, const double *zf,
+                 const double *vis, const double *flux, int nwal, int iwall_faces_start,
+                 const int *ioffset, const int *diag, const int *ij_slot, const int *ji_slot,
+                 int nnz, double urf, double *a)
+{
+    double *spu = (double *)calloc((size_t)n, sizeof(double));
+    if (!spu) return -1;
+    memset(a, 0, sizeof(double) * nnz);
+    for (int i = 0; i < nif; ++i) {
+        int ijp = owner1[i], ijn = neighbour1[i];
+        double fxn = facint[i], fxp = 1.0 - facint[i];
+        double xpn = I1(xc, ijn) - I1(xc, ijp);
+        double ypn = I1(yc, ijn) - I1(yc, ijp);
+        double zpn = I1(zc, ijn) - I1(zc, ijp);
+        double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+        double are = sqrt((arx[i] * arx[i] + ary[i] * ary[i]) + arz[i] * arz[i]);
+        double game = I1(vis, ijp) * fxp + I1(vis, ijn) * fxn;
+        double de = (game * are) / dpn;
+        double fl = flux[i];
+        double can = -de + (fl < 0.0 ? fl : 0.0);
+        double cap = -de - (fl > 0.0 ? fl : 0.0);
+        I1(a, ij_slot[i]) = can;
+        I1(a, ji_slot[i]) = cap;
+    }
+    for (int i = 1; i <= nwal; ++i) {
+        int iface = iwall_faces_start + i;
+        int ijp = I1(owner1, iface);
+        double fx = I1(arx, iface), fy = I1(ary, iface), fz = I1(arz, iface);
+        double are = sqrt((fx * fx + fy * fy) + fz * fz);
+        double dx = I1(xc, ijp) - I1(xf, iface), dy = I1(yc, ijp) - I1(yf, iface), dz = I1(zc, ijp) - I1(zf, iface);
+        double dpw = sqrt((dx * dx + dy * dy) + dz * dz);
+        I1(spu, ijp) = I1(spu, ijp) + (I1(vis, ijp) * are) / dpw;
+    }
+    double urfr = 1.0 / urf;
+    for (int i = 1; i <= n; ++i) {
+        double s = 0.0;
+        for (int k = I1(ioffset, i); k < I1(ioffset, i + 1); ++k) s = s + I1(a, k);
+        s = s - I1(a, I1(diag, i));
+        double dd = I1(spu, i) - s;
+        I1(a, I1(diag, i)) = dd * urfr;
+    }
+    free(spu);
+    return 0;
+}
+
+void orc_face_flux_central(int nif, const int *owner1, const int *neighbour1,
+                           const double *facint, const double *arx, const double *ary,
+                           const double *arz, const double *u, const double *v,
+                           const double *w, double *flux)
+{
+    for (int f = 0; f < nif; ++f) {
+        int p = owner1[f], q = neighbour1[f];
+        double fxn = facint[f], fxp = 1.0 - facint[f];
+        double uf = fxp * I1(u, p) + fxn * I1(u, q);
+        double vf = fxp * I1(v, p) + fxn * I1(v, q);
+        double wf = fxp * I1(w, p) + fxn * I1(w, q);
+        flux[f] = (uf * arx[f] + vf * ary[f]) + wf * arz[f];
+    }
+}
+
+void orc_rhs_divergence(int nif, const int *owner1, const int *neighbour1,
+                        const double *flux, double *su)
+{
+    for (int f = 0; f < nif; ++f) {
+        I1(su, owner1[f]) = I1(su, owner1[f]) - flux[f];
+        I1(su, neighbour1[f]) = I1(su, neighbour1[f]) + flux[f];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver kernels (src/iccg.f90, src/dpcg.f90, src/bicgstab.f90).            */
+/* ------------------------------------------------------------------------ */
+
+void orc_spmv(int n, const int *ioffset, const int *ja, const double *a, const double *x, double *y)
+{
+    /* src/iccg.f90:129-134 */
+    for (int i = 1; i <= n; ++i) {
+        double s = 0.0;
+        for (int k = I1(ioffset, i); k < I1(ioffset, i + 1); ++k) s = s + I1(a, k) * I1(x, I1(ja, k));
+        I1(y, i) = s;
+    }
+}
+
+void orc_residual(int n, const int *ioffset, const int *ja, const double *a, const double *su,
+                  const double *fi, int noc, const int *ijl1, const int *ijr1, const double *al,
+                  const double *ar, double *res)
+{
+    /* src/iccg.f90:52-62 */
+    for (int i = 1; i <= n; ++i) {
+        double r = I1(su, i);
+        for (int k = I1(ioffset, i); k < I1(ioffset, i + 1); ++k) r = r - I1(a, k) * I1(fi, I1(ja, k));
+        I1(res, i) = r;
+    }
+    for (int i = 0; i < noc; ++i) {
+        I1(res, ijl1[i]) = I1(res, ijl1[i]) - ar[i] * I1(fi, ijr1[i]);
+        I1(res, ijr1[i]) = I1(res, ijr1[i]) - al[i] * I1(fi, ijl1[i]);
+    }
+}
+
+static void oc_add(int noc, const int *ijl1, const int *ijr1, const double *al, const double *ar,
+                   const double *p, double *q)
+{
+    /* src/iccg.f90:137-140 */
+    for (int i = 0; i < noc; ++i) {
+        I1(q, ijl1[i]) = I1(q, ijl1[i]) + ar[i] * I1(p, ijr1[i]);
+        I1(q, ijr1[i]) = I1(q, ijr1[i]) + al[i] * I1(p, ijl1[i]);
+    }
+}
+
+void orc_dic_pivots(int n, const int *ioffset, const int *ja, const int *diag, const double *a, double *d)
+{
+    /* src/iccg.f90:77-83 */
+    for (int i = 1; i <= n; ++i) {
+        double di = I1(a, I1(diag, i));
+        for (int k = I1(ioffset, i); k < I1(diag, i); ++k) {
+            double ak = I1(a, k);
+            di = di - (ak * ak) * I1(d, I1(ja, k));
+        }
+        I1(d, i) = 1.0 / di;
+    }
+}
+
+int orc_dilu_pivots(int n, const int *ioffset, const int *ja, const int *diag, const double *a, double *d)
+{
+    /* src/bicgstab.f90:68-79: a_ji found by a scan of row j from its diagonal */
+    int nnz = I1(ioffset, n + 1) - 1;
+    for (int i = 1; i <= n; ++i) {
+        double di = I1(a, I1(diag, i));
+        for (int k = I1(ioffset, i); k < I1(diag, i); ++k) {
+            int j = I1(ja, k), l;
+            for (l = I1(diag, j); l <= I1(ioffset, j + 1) - 1; ++l)
+                if (I1(ja, l) == i) break;
+            if (l > nnz) return -1;   /* no transposed entry and nothing after it */
+            di = di - (I1(a, k) * I1(d, j)) * I1(a, l);
+        }
+        I1(d, i) = 1.0 / di;
+    }
+    return 0;
+}
+
+void orc_lu_apply(int n, const int *ioffset, const int *ja, const int *diag, const double *a,
+                  const double *d, double small, const double *rhs, double *z)
+{
+    /* src/iccg.f90:94-111 (identical in bicgstab.f90:117-136) */
+    for (int i = 1; i <= n; ++i) {
+        double t = I1(rhs, i);
+        for (int k = I1(ioffset, i); k < I1(diag, i); ++k) t = t - I1(a, k) * I1(z, I1(ja, k));
+        I1(z, i) = t * I1(d, i);
+    }
+    for (int i = 1; i <= n; ++i) I1(z, i) = I1(z, i) / (I1(d, i) + small);
+    for (int i = n; i >= 1; --i) {
+        double t = I1(z, i);
+        for (int k = I1(diag, i) + 1; k < I1(ioffset, i + 1); ++k) t = t - I1(a, k) * I1(z, I1(ja, k));
+        I1(z, i) = t * I1(d, i);
+    }
+}
+
+static double sum_abs(int n, const double *x)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + fabs(x[i]);
+    return s;
+}
+
+static double sum_prod(int n, const double *x, const double *y)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + x[i] * y[i];
+    return s;
+}
+
+typedef struct {
+    double small, resmax;
+    int ns, early_exit_enabled, use_oc;
+} solver_consts;
+
+static solver_consts consts_for(int variant, double sor, int nsw)
+{
+    solver_consts c;
+    if (variant == ORC_VARIANT_TESTS) {
+        /* tests/test_sparse_solvers.f90:36-37,99,262-265 */
+        c.small = (double)1e-30f;
+        c.resmax = (double)1e-13f;
+        c.ns = 100;
+        c.early_exit_enabled = 0;
+        c.use_oc = 0;
+    } else {
+        c.small = (double)1e-20f;
+        c.resmax = sor;
+        c.ns = nsw;
+        c.early_exit_enabled = 1;
+        c.use_oc = 1;
+    }
+    return c;
+}
+
+enum { PREC_DIC = 0, PREC_JACOBI = 1 };
+
+static int pcg_common(int prec, int variant, int n, const int *ioffset, const int *ja,
+                      const int *diag, const double *a, const double *su, double *fi, int noc,
+                      const int *ijl1, const int *ijr1, const double *al, const double *ar,
+                      double sor, int nsw, double *res_out, double *history, orc_stats *st)
+{
+    solver_consts c = consts_for(variant, sor, nsw);
+    if (!c.use_oc) noc = 0;
+    double *res = (double *)calloc((size_t)n, sizeof(double));
+    double *pk = (double *)calloc((size_t)n, sizeof(double));
+    double *zk = (double *)calloc((size_t)n, sizeof(double));
+    double *d = (double *)calloc((size_t)n, sizeof(double));
+    if (!res || !pk || !zk || !d) { free(res); free(pk); free(zk); free(d); return -1; }
+    orc_residual(n, ioffset, ja, a, su, fi, noc, ijl1, ijr1, al, ar, res);
+    double res0 = sum_abs(n, res);
+    memset(st, 0, sizeof(*st));
+    st->res0 = res0;
+    st->resl = res0;
+    if (c.early_exit_enabled && res0 < orc_tol()) {
+        st->early_exit = 1;
+        if (res_out) memcpy(res_out, res, sizeof(double) * n);
+        free(res); free(pk); free(zk); free(d);
+        return 2;
+    }
+    if (prec == PREC_DIC) orc_dic_pivots(n, ioffset, ja, diag, a, d);
+    double s0 = orc_s0(), resl = res0, rsm = 0.0;
+    int it = 0;
+    for (int l = 1; l <= c.ns; ++l) {
+        if (prec == PREC_DIC) {
+            orc_lu_apply(n, ioffset, ja, diag, a, d, c.small, res, zk);
+        } else {
+            for (int i = 1; i <= n; ++i) I1(zk, i) = I1(res, i) / I1(a, I1(diag, i));
+        }
+        double sk = sum_prod(n, res, zk);
+        double bet = sk / s0;
+        for (int i = 0; i < n; ++i) pk[i] = zk[i] + bet * pk[i];
+        orc_spmv(n, ioffset, ja, a, pk, zk);
+        oc_add(noc, ijl1, ijr1, al, ar, pk, zk);
+        double pkapk = sum_prod(n, pk, zk);
+        double alf = sk / pkapk;
+        for (int i = 0; i < n; ++i) fi[i] = fi[i] + alf * pk[i];
+        for (int i = 0; i < n; ++i) res[i] = res[i] - alf * zk[i];
+        resl = sum_abs(n, res);
+        s0 = sk;
+        it++;
+        if (history) history[l - 1] = resl;
+        rsm = resl / (res0 + c.small);
+        if (rsm < c.resmax) break;
+    }
+    st->resl = resl;
+    st->rsm = rsm;
+    st->iters = it;
+    if (res_out) memcpy(res_out, res, sizeof(double) * n);
+    free(res); free(pk); free(zk); free(d);
+    return (rsm < c.resmax) ? 0 : 1;
+}
+
+int orc_iccg(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+             const double *a, const double *su, double *fi, int noc, const int *ijl1,
+             const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+             double *res_out, double *history, orc_stats *st)
+{
+    (void)nnz;
+    return pcg_common(PREC_DIC, variant, n, ioffset, ja, diag, a, su, fi, noc, ijl1, ijr1, al, ar,
+                      sor, nsw, res_out, history, st);
+}
+
+int orc_dpcg(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+             const double *a, const double *su, double *fi, int noc, const int *ijl1,
+             const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+             double *res_out, double *history, orc_stats *st)
+{
+    (void)nnz;
+    return pcg_common(PREC_JACOBI, variant, n, ioffset, ja, diag, a, su, fi, noc, ijl1, ijr1, al,
+                      ar, sor, nsw, res_out, history, st);
+}
+
+int orc_bicgstab(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                 const double *a, const double *su, double *fi, int noc, const int *ijl1,
+                 const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+                 double *res_out, double *history, orc_stats *st)
+{
+    /* src/bicgstab.f90:1-230 */
+    (void)nnz;
+    solver_consts c = consts_for(variant, sor, nsw);
+    if (!c.use_oc) noc = 0;
+    size_t sz = sizeof(double) * (size_t)n;
+    double *res = (double *)calloc((size_t)n, sizeof(double));
+    double *reso = (double *)calloc((size_t)n, sizeof(double));
+    double *pk = (double *)calloc((size_t)n, sizeof(double));
+    double *uk = (double *)calloc((size_t)n, sizeof(double));
+    double *zk = (double *)calloc((size_t)n, sizeof(double));
+    double *vk = (double *)calloc((size_t)n, sizeof(double));
+    double *d = (double *)calloc((size_t)n, sizeof(double));
+    int rc = 0;
+    memset(st, 0, sizeof(*st));
+    if (!res || !reso || !pk || !uk || !zk || !vk || !d) { rc = -1; goto out; }
+    /* initial residual includes the O-C coupling (:42-52); the SpMVs below do not (:142-147) */
+    orc_residual(n, ioffset, ja, a, su, fi, noc, ijl1, ijr1, al, ar, res);
+    double res0 = sum_abs(n, res);
+    st->res0 = res0;
+    st->resl = res0;
+    if (c.early_exit_enabled && res0 < orc_tol()) {
+        st->early_exit = 1;
+        rc = 2;
+        if (res_out) memcpy(res_out, res, sz);
+        goto out;
+    }
+    if (orc_dilu_pivots(n, ioffset, ja, diag, a, d) != 0) { rc = -2; goto out; }
+    memcpy(reso, res, sz);
+    double alf = 1.0, beto = 1.0, gam = 1.0, resl = res0, rsm = 0.0;
+    int it = 0;
+    for (int l = 1; l <= c.ns; ++l) {
+        double bet = sum_prod(n, res, reso);
+        double om = (bet * gam) / ((alf * beto) + c.small);
+        beto = bet;
+        for (int i = 0; i < n; ++i) {
+            double t = alf * uk[i];
+            t = pk[i] - t;
+            t = om * t;
+            pk[i] = res[i] + t;
+        }
+        orc_lu_apply(n, ioffset, ja, diag, a, d, c.small, pk, zk);
+        orc_spmv(n, ioffset, ja, a, zk, uk);
+        double ukreso = sum_prod(n, uk, reso);
+        gam = bet / ukreso;
+        for (int i = 0; i < n; ++i) fi[i] = fi[i] + gam * zk[i];
+        for (int i = 0; i < n; ++i) res[i] = res[i] - gam * uk[i];
+        orc_lu_apply(n, ioffset, ja, diag, a, d, c.small, res, zk);
+        orc_spmv(n, ioffset, ja, a, zk, vk);
+        double vr = sum_prod(n, vk, res);
+        double vv = sum_prod(n, vk, vk);
+        alf = vr / (vv + c.small);
+        for (int i = 0; i < n; ++i) fi[i] = fi[i] + alf * zk[i];
+        for (int i = 0; i < n; ++i) res[i] = res[i] - alf * vk[i];
+        resl = sum_abs(n, res);
+        it++;
+        if (history) history[l - 1] = resl;
+        rsm = resl / (res0 + c.small);
+        if (rsm < c.resmax) break;
+    }
+    st->resl = resl;
+    st->rsm = rsm;
+    st->iters = it;
+    rc = (rsm < c.resmax) ? 0 : 1;
+    if (res_out) memcpy(res_out, res, sz);
+out:
+    free(res); free(reso); free(pk); free(uk); free(zk); free(vk); free(d);
+    return rc;
+}

@@ -1,0 +1,115 @@
+/*
+ * CPU oracle for the pressure-solve path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's serial Fortran algorithm, used by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the
+ * checker.  The product (freecappuccino_amd/libfcap.so) never links, loads or
+ * calls anything here.
+ *
+ * Index conventions follow the reference: every index array (owner, neighbour,
+ * ioffset, ja, diag, slot maps, ijl, ijr) is 1-based.  face_ptr/face_nodes are
+ * the CSR list of face vertices with 0-based pointers and 1-based node ids.
+ *
+ * Compile with -O2 -ffp-contract=off: the reference's evaluation order is
+ * reproduced operation by operation and must not be contracted into FMAs.
+ */
+#ifndef FCAP_ORACLE_H
+#define FCAP_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Constants written as default-REAL literals in the reference and widened to
+ * real(dp) (src/modules_allocatable.f90:27, src/iccg.f90:37,85). */
+double orc_small(void);      /* (double)1e-20f */
+double orc_tol(void);        /* (double)1e-13f */
+double orc_s0(void);         /* (double)1e20f  */
+double orc_tiny(void);       /* (double)1e-30f (src/mesh_geometry_and_topology.f90:58) */
+
+/* src/mesh_geometry_and_topology.f90:859-1081 (O-C / cyclic faces not handled). */
+int orc_mesh_geometry(int nnodes, const double *xyz /* nnodes*3 */, int nfaces, int nif,
+                      const int *face_ptr, const int *face_nodes1, const int *owner1,
+                      const int *neighbour1, int ncells,
+                      double *arx, double *ary, double *arz,
+                      double *xf, double *yf, double *zf,
+                      double *vol, double *xc, double *yc, double *zc, double *facint);
+
+/* src/sparse_matrix.f90:42-172 + src/utils.f90:76-170. Returns 0 or -1 on bad input. */
+int orc_csr_build(int n, int nif, const int *owner1, const int *neighbour1,
+                  int *ioffset, int *ja, int *diag, int *ij_slot, int *ji_slot);
+
+/* src/fvm_laplacian.f90:1-163 (+ facefluxlaplacian :171-221).
+ * phi has numTotal = n + nboundary entries; wall values at iwall_start+i (1-based).
+ * a is zeroed here, su is accumulated (never zeroed), as in the reference. */
+int orc_laplacian(int n, int nif, const int *owner1, const int *neighbour1,
+                  const double *arx, const double *ary, const double *arz,
+                  const double *facint, const double *xc, const double *yc, const double *zc,
+                  const double *xf, const double *yf, const double *zf,
+                  const double *mu, const double *phi,
+                  int nwal, int iwall_faces_start, int iwall_start,
+                  int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                  const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                  double *a, double *su, double *al, double *ar);
+
+/* Momentum-like convection-diffusion operator (the config-3 system):
+ * coefficients of facefluxuvw (src/faceflux_velocity.f90:97-108) with upwind
+ * convection, diagonal = (spu - sum of the row's off-diagonals) / urf
+ * (src/calcuvw.f90:417-432); spu holds the Dirichlet-wall diffusion
+ * vis*|S|/|x_P - x_f| of each wall face, accumulated in wall-face order. */
+int orc_convdiff(int n, int nif, const int *owner1, const int *neighbour1,
+                 const double *arx, const double *ary, const double *arz,
+                 const double *facint, const double *xc, const double *yc, const double *zc,
+                 const double *xf, const double *yf, const double *zf,
+                 const double *vis, const double *flux, int nwal, int iwall_faces_start,
+                 const int *ioffset, const int *diag, const int *ij_slot, const int *ji_slot,
+                 int nnz, double urf, double *a);
+
+/* Central face flux F_f = u_f . S_f with u_f = (1-facint) u_P + facint u_N
+ * (the interpolation of src/facefluxmass.f90 without Rhie-Chow), inner faces. */
+void orc_face_flux_central(int nif, const int *owner1, const int *neighbour1,
+                           const double *facint, const double *arx, const double *ary,
+                           const double *arz, const double *u, const double *v,
+                           const double *w, double *flux);
+
+/* su(o) -= F, su(n) += F in face order (src/calcp-multiple_correction_SIMPLE.f90:74-75). */
+void orc_rhs_divergence(int nif, const int *owner1, const int *neighbour1,
+                        const double *flux, double *su);
+
+/* Kernels of the solvers, each with the reference's loop order. */
+void orc_spmv(int n, const int *ioffset, const int *ja, const double *a, const double *x, double *y);
+void orc_residual(int n, const int *ioffset, const int *ja, const double *a, const double *su,
+                  const double *fi, int noc, const int *ijl1, const int *ijr1, const double *al,
+                  const double *ar, double *res);
+void orc_dic_pivots(int n, const int *ioffset, const int *ja, const int *diag, const double *a, double *d);
+int  orc_dilu_pivots(int n, const int *ioffset, const int *ja, const int *diag, const double *a, double *d);
+void orc_lu_apply(int n, const int *ioffset, const int *ja, const int *diag, const double *a,
+                  const double *d, double small, const double *rhs, double *z);
+
+/* Solver variants: the module solvers of src/ and the explicit-argument
+ * copies of tests/test_sparse_solvers.f90 differ in constants. */
+enum { ORC_VARIANT_SRC = 0, ORC_VARIANT_TESTS = 1 };
+
+typedef struct {
+    double res0, resl, rsm;
+    int iters;
+    int early_exit;      /* res0 < tol */
+} orc_stats;
+
+/* History (optional, may be NULL): resl after each iteration, capacity nsw. */
+int orc_iccg(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+             const double *a, const double *su, double *fi, int noc, const int *ijl1,
+             const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+             double *res_out, double *history, orc_stats *st);
+int orc_dpcg(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+             const double *a, const double *su, double *fi, int noc, const int *ijl1,
+             const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+             double *res_out, double *history, orc_stats *st);
+int orc_bicgstab(int variant, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                 const double *a, const double *su, double *fi, int noc, const int *ijl1,
+                 const int *ijr1, const double *al, const double *ar, double sor, int nsw,
+                 double *res_out, double *history, orc_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

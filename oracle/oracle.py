@@ -1,0 +1,282 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Wraps oracle/liboracle.so (the C restatement of the reference algorithm in
+oracle/fcap_oracle.c).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, as the checker.  The product library
+never loads it.
+
+All arrays passed in and out use the reference's 1-based index convention
+except where a docstring says otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+VARIANT_SRC = 0
+VARIANT_TESTS = 1
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"] + (["-B"] if force else []),
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(_LIB_PATH)
+        d, i, dp, ip = C.c_double, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        _lib.orc_small.restype = d
+        _lib.orc_tol.restype = d
+        _lib.orc_s0.restype = d
+        _lib.orc_tiny.restype = d
+        _lib.orc_mesh_geometry.argtypes = [i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11
+        _lib.orc_csr_build.argtypes = [i, i, ip, ip, ip, ip, ip, ip, ip]
+        _lib.orc_laplacian.argtypes = [i, i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                       i, i, i, i, i, ip, ip, dp, ip, ip, ip, i, dp, dp, dp, dp]
+        _lib.orc_convdiff.argtypes = [i, i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                      i, i, ip, ip, ip, ip, i, d, dp]
+        _lib.orc_face_flux_central.argtypes = [i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
+        _lib.orc_rhs_divergence.argtypes = [i, ip, ip, dp, dp]
+        _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
+        _lib.orc_residual.argtypes = [i, ip, ip, dp, dp, dp, i, ip, ip, dp, dp, dp]
+        _lib.orc_dic_pivots.argtypes = [i, ip, ip, ip, dp, dp]
+        _lib.orc_dilu_pivots.argtypes = [i, ip, ip, ip, dp, dp]
+        _lib.orc_lu_apply.argtypes = [i, ip, ip, ip, dp, dp, d, dp, dp]
+        for fn in (_lib.orc_iccg, _lib.orc_dpcg, _lib.orc_bicgstab):
+            fn.argtypes = [i, i, i, ip, ip, ip, dp, dp, dp, i, ip, ip, dp, dp, d, i, dp, dp,
+                           C.POINTER(Stats)]
+    return _lib
+
+
+class Stats(C.Structure):
+    _fields_ = [("res0", C.c_double), ("resl", C.c_double), ("rsm", C.c_double),
+                ("iters", C.c_int), ("early_exit", C.c_int)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def small() -> float:
+    return lib().orc_small()
+
+
+def tol() -> float:
+    return lib().orc_tol()
+
+
+@dataclass
+class Geometry:
+    arx: np.ndarray
+    ary: np.ndarray
+    arz: np.ndarray
+    xf: np.ndarray
+    yf: np.ndarray
+    zf: np.ndarray
+    vol: np.ndarray
+    xc: np.ndarray
+    yc: np.ndarray
+    zc: np.ndarray
+    facint: np.ndarray
+
+
+def mesh_geometry(mesh) -> Geometry:
+    """Geometry of a freecappuccino_amd.mesh.PolyMesh (0-based arrays in, values out)."""
+    nf, nif, n = mesh.n_faces, mesh.n_inner, mesh.n_cells
+    g = Geometry(*(np.zeros(nf) for _ in range(6)), *(np.zeros(n) for _ in range(4)), np.zeros(nif))
+    xyz = _f64(mesh.points)
+    fp = _i32(mesh.face_ptr)
+    fn = _i32(mesh.face_nodes.astype(np.int64) + 1)
+    own = _i32(mesh.owner.astype(np.int64) + 1)
+    nei = _i32(mesh.neighbour.astype(np.int64) + 1)
+    lib().orc_mesh_geometry(xyz.shape[0], _dp(xyz), nf, nif, _ip(fp), _ip(fn), _ip(own), _ip(nei), n,
+                            *(_dp(getattr(g, k)) for k in ("arx", "ary", "arz", "xf", "yf", "zf",
+                                                           "vol", "xc", "yc", "zc", "facint")))
+    return g
+
+
+@dataclass
+class CSR:
+    ioffset: np.ndarray
+    ja: np.ndarray
+    diag: np.ndarray
+    ij_slot: np.ndarray
+    ji_slot: np.ndarray
+
+
+def csr_build(n: int, owner1: np.ndarray, neighbour1: np.ndarray) -> CSR:
+    nif = neighbour1.shape[0]
+    nnz = 2 * nif + n
+    c = CSR(np.zeros(n + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(n, np.int32),
+            np.zeros(nif, np.int32), np.zeros(nif, np.int32))
+    rc = lib().orc_csr_build(n, nif, _ip(_i32(owner1)), _ip(_i32(neighbour1)), _ip(c.ioffset), _ip(c.ja),
+                             _ip(c.diag), _ip(c.ij_slot), _ip(c.ji_slot))
+    if rc != 0:
+        raise ValueError("orc_csr_build failed")
+    return c
+
+
+def laplacian(mesh, g: Geometry, csr: CSR, mu, phi, su0):
+    """Returns (a, su) for the mesh's first wall range, as fvm_laplacian.f90 assembles them."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    ws, wc = mesh.wall_range()
+    iwall_start = iwall_offset(mesh)
+    a = np.zeros(mesh.nnz)
+    su = _f64(su0).copy()
+    empty_i = np.zeros(1, np.int32)
+    empty_d = np.zeros(1)
+    lib().orc_laplacian(n, nif, _ip(own1), _ip(nei1), _dp(g.arx), _dp(g.ary), _dp(g.arz), _dp(g.facint),
+                        _dp(g.xc), _dp(g.yc), _dp(g.zc), _dp(g.xf), _dp(g.yf), _dp(g.zf),
+                        _dp(_f64(mu)), _dp(_f64(phi)), wc, ws, iwall_start, 0, 0, _ip(empty_i),
+                        _ip(empty_i), _dp(empty_d), _ip(csr.diag), _ip(csr.ij_slot), _ip(csr.ji_slot),
+                        mesh.nnz, _dp(a), _dp(su), _dp(empty_d), _dp(empty_d))
+    return a, su
+
+
+def convdiff(mesh, g: Geometry, csr: CSR, vis, flux, urf=0.7):
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    ws, wc = mesh.wall_range()
+    a = np.zeros(mesh.nnz)
+    rc = lib().orc_convdiff(n, nif, _ip(own1), _ip(nei1), _dp(g.arx), _dp(g.ary), _dp(g.arz),
+                            _dp(g.facint), _dp(g.xc), _dp(g.yc), _dp(g.zc), _dp(g.xf), _dp(g.yf),
+                            _dp(g.zf), _dp(_f64(vis)), _dp(_f64(flux)), wc, ws, _ip(csr.ioffset),
+                            _ip(csr.diag), _ip(csr.ij_slot), _ip(csr.ji_slot), mesh.nnz, float(urf),
+                            _dp(a))
+    if rc != 0:
+        raise MemoryError
+    return a
+
+
+def iwall_offset(mesh) -> int:
+    """iWallStart = numCells + ninl + nout + nsym (src/mesh_geometry_and_topology.f90:594)."""
+    cnt = {t: 0 for t in ("inlet", "outlet", "symmetry")}
+    for t, nf, _ in mesh.patches:
+        if t in cnt:
+            cnt[t] += nf
+    return mesh.n_cells + cnt["inlet"] + cnt["outlet"] + cnt["symmetry"]
+
+
+def face_flux_central(mesh, g: Geometry, u, v, w):
+    nif = mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    flux = np.zeros(nif)
+    lib().orc_face_flux_central(nif, _ip(own1), _ip(nei1), _dp(g.facint), _dp(g.arx), _dp(g.ary),
+                                _dp(g.arz), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)), _dp(flux))
+    return flux
+
+
+def rhs_divergence(mesh, flux, su0):
+    nif = mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    su = _f64(su0).copy()
+    lib().orc_rhs_divergence(nif, _ip(own1), _ip(nei1), _dp(_f64(flux)), _dp(su))
+    return su
+
+
+def spmv(csr: CSR, a, x):
+    n = csr.diag.shape[0]
+    y = np.zeros(n)
+    lib().orc_spmv(n, _ip(csr.ioffset), _ip(csr.ja), _dp(_f64(a)), _dp(_f64(x)), _dp(y))
+    return y
+
+
+def dic_pivots(csr: CSR, a):
+    n = csr.diag.shape[0]
+    d = np.zeros(n)
+    lib().orc_dic_pivots(n, _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _dp(_f64(a)), _dp(d))
+    return d
+
+
+def dilu_pivots(csr: CSR, a):
+    n = csr.diag.shape[0]
+    d = np.zeros(n)
+    if lib().orc_dilu_pivots(n, _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _dp(_f64(a)), _dp(d)) != 0:
+        raise ValueError("DILU: transposed entry missing")
+    return d
+
+
+def lu_apply(csr: CSR, a, d, rhs, small_value: Optional[float] = None):
+    n = csr.diag.shape[0]
+    z = np.zeros(n)
+    s = small() if small_value is None else small_value
+    lib().orc_lu_apply(n, _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _dp(_f64(a)), _dp(_f64(d)), s,
+                       _dp(_f64(rhs)), _dp(z))
+    return z
+
+
+@dataclass
+class SolveResult:
+    status: int
+    fi: np.ndarray
+    res: np.ndarray
+    res0: float
+    resl: float
+    rsm: float
+    iters: int
+    history: np.ndarray
+
+
+def _solve(fn, variant, csr: CSR, a, su, fi0, sor=1e-8, nsw=1000, noc_arrays=None) -> SolveResult:
+    n = csr.diag.shape[0]
+    nnz = csr.ja.shape[0]
+    fi = _f64(fi0).copy()
+    res = np.zeros(n)
+    ns = 100 if variant == VARIANT_TESTS else nsw
+    hist = np.zeros(max(ns, 1))
+    st = Stats()
+    if noc_arrays is None:
+        ijl = ijr = np.zeros(1, np.int32)
+        al = ar = np.zeros(1)
+        noc = 0
+    else:
+        ijl, ijr, al, ar = (_i32(noc_arrays[0]), _i32(noc_arrays[1]), _f64(noc_arrays[2]),
+                            _f64(noc_arrays[3]))
+        noc = ijl.shape[0]
+    rc = fn(variant, n, nnz, _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _dp(_f64(a)), _dp(_f64(su)),
+            _dp(fi), noc, _ip(ijl), _ip(ijr), _dp(al), _dp(ar), float(sor), int(nsw), _dp(res),
+            _dp(hist), C.byref(st))
+    return SolveResult(rc, fi, res, st.res0, st.resl, st.rsm, st.iters, hist[:st.iters].copy())
+
+
+def iccg(csr, a, su, fi0, sor=1e-8, nsw=1000, variant=VARIANT_SRC, noc_arrays=None):
+    return _solve(lib().orc_iccg, variant, csr, a, su, fi0, sor, nsw, noc_arrays)
+
+
+def dpcg(csr, a, su, fi0, sor=1e-8, nsw=1000, variant=VARIANT_SRC, noc_arrays=None):
+    return _solve(lib().orc_dpcg, variant, csr, a, su, fi0, sor, nsw, noc_arrays)
+
+
+def bicgstab(csr, a, su, fi0, sor=1e-8, nsw=1000, variant=VARIANT_SRC, noc_arrays=None):
+    return _solve(lib().orc_bicgstab, variant, csr, a, su, fi0, sor, nsw, noc_arrays)

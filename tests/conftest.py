@@ -1,0 +1,62 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+MESH_CASES = ["hex4_uniform", "hex8_perturbed", "hex6x5x4_graded", "tet3_permuted", "hex12_perturbed"]
+CONVDIFF_CASES = ["tet4_convdiff", "hex8_convdiff"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (run with -m gpu)")
+
+
+class Golden:
+    """One golden fixture: the mesh, the inputs and the reference's outputs."""
+
+    def __init__(self, name):
+        from freecappuccino_amd.mesh import PolyMesh
+
+        self.name = name
+        z = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+        self.z = z
+        self.mesh = PolyMesh(points=z["mesh_points"], face_ptr=z["mesh_face_ptr"],
+                             face_nodes=z["mesh_face_nodes"], owner=z["mesh_owner"],
+                             neighbour=z["mesh_neighbour"], n_cells=int(z["mesh_n_cells"][0]),
+                             patches=[tuple(p) for p in json.loads(str(z["mesh_patches"]))])
+        self.solves = json.loads(str(z["solves"]))
+
+    def inp(self, k):
+        return self.z[f"in_{k}"]
+
+    def out(self, solve, k):
+        return self.z[f"{solve}_{k}"]
+
+    @property
+    def first(self):
+        return self.solves[0]
+
+
+@pytest.fixture(scope="session")
+def golden_loader():
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            cache[name] = Golden(name)
+        return cache[name]
+
+    return get
+
+
+def load_transcript():
+    with open(os.path.join(GOLDEN, "solver5x5.json")) as f:
+        return json.load(f)

@@ -1,0 +1,203 @@
+"""Generate the golden fixtures of tests/golden/ from the reference Fortran.
+
+Runs in the build container only (it needs /root/reference and ROCm flang):
+
+    make -C oracle ref            # compiles the reference sources in place
+    python tests/golden/make_golden.py
+
+For each case it writes the mesh in polyMesh form, runs oracle/_ref/ref_driver
+(the reference's own mesh_geometry, create_CSR_matrix_from_mesh_data,
+laplacian and iccg/dpcg/bicgstab) and stores inputs and outputs as a small
+.npz.  The 5x5 solver transcript fixture is parsed from the reference's
+tests/output.txt and the matrices of tests/test_sparse_solvers.f90:605-655
+(default-REAL literals, widened to float64 as the reference does).
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from freecappuccino_amd import mesh as M  # noqa: E402
+from freecappuccino_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+REF_TESTS = "/root/reference/tests"
+
+SOLVERS = {"none": 0, "iccg": 1, "dpcg": 2, "bicgstab": 3}
+
+
+def run_reference(mesh, mu, phi, su0, fi0, solver="none", sor=1e-8, nsw=1000, fmt="native",
+                  a_override=None, su_override=None):
+    n, nf, nif = mesh.n_cells, mesh.n_faces, mesh.n_inner
+    nnz = mesh.nnz
+    with tempfile.TemporaryDirectory() as d:
+        M.write_polymesh(mesh, os.path.join(d, "polyMesh"), fmt=fmt)
+        amode = 1 if a_override is not None else 0
+        with open(os.path.join(d, "drv.txt"), "w") as f:
+            f.write(f"{SOLVERS[solver]} {sor!r} {nsw} {amode}\n")
+        with open(os.path.join(d, "drv_in.bin"), "wb") as f:
+            for arr in (mu, phi, su0, fi0):
+                f.write(np.ascontiguousarray(arr, np.float64).tobytes())
+            if amode:
+                f.write(np.ascontiguousarray(a_override, np.float64).tobytes())
+                f.write(np.ascontiguousarray(su_override, np.float64).tobytes())
+        # The reference accumulates vol/xc/yc/zc/arx/... into freshly allocated,
+        # never-zeroed arrays (src/mesh_geometry_and_topology.f90:893-895,917-920,
+        # 1000-1008).  Serving every allocation with mmap gives the zero-filled
+        # memory it relies on; a reused heap block would leave garbage in them.
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        p = subprocess.run([DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
+        raw = open(os.path.join(d, "drv_out.bin"), "rb").read()
+    off = 0
+
+    def take(dtype, count):
+        nonlocal off
+        sz = np.dtype(dtype).itemsize * count
+        arr = np.frombuffer(raw, dtype=dtype, count=count, offset=off).copy()
+        off += sz
+        return arr
+
+    hdr = take(np.int32, 4)
+    assert tuple(hdr) == (n, nf, nif, nnz), (hdr, n, nf, nif, nnz)
+    out = {}
+    for k in ("arx", "ary", "arz", "xf", "yf", "zf"):
+        out[k] = take(np.float64, nf)
+    for k in ("vol", "xc", "yc", "zc"):
+        out[k] = take(np.float64, n)
+    out["facint"] = take(np.float64, nif)
+    out["ioffset"] = take(np.int32, n + 1)
+    out["ja"] = take(np.int32, nnz)
+    out["diag"] = take(np.int32, n)
+    out["ij_slot"] = take(np.int32, nif)
+    out["ji_slot"] = take(np.int32, nif)
+    out["a"] = take(np.float64, nnz)
+    out["su"] = take(np.float64, n)
+    out["fi"] = take(np.float64, n)
+    out["res"] = take(np.float64, n)
+    out["resor"] = take(np.float64, 1)
+    assert off == len(raw)
+    # report line: '... Initial residual = x, Final residual = y, No Iterations N'
+    m = re.findall(r"No Iterations (\d+)", p.stdout)
+    out["iters"] = np.array([int(m[-1]) if m else -1], np.int32)
+    hist = re.findall(r"resl =\s*([0-9.E+-]+)", p.stdout)
+    out["resl_history_4d"] = np.array([float(h) for h in hist], np.float64)
+    rep = [ln for ln in p.stdout.splitlines() if "Solving for" in ln]
+    out["report_line"] = np.array(rep[-1] if rep else "")
+    return out
+
+
+def mesh_arrays(mesh):
+    return {
+        "points": mesh.points, "face_ptr": mesh.face_ptr, "face_nodes": mesh.face_nodes,
+        "owner": mesh.owner, "neighbour": mesh.neighbour, "n_cells": np.array([mesh.n_cells], np.int32),
+        "patches": np.array(json.dumps(mesh.patches)),
+    }
+
+
+def save_case(name, mesh, inputs, solves):
+    path = os.path.join(HERE, f"{name}.npz")
+    data = {f"mesh_{k}": v for k, v in mesh_arrays(mesh).items()}
+    data.update({f"in_{k}": v for k, v in inputs.items()})
+    for sname, out in solves.items():
+        data.update({f"{sname}_{k}": v for k, v in out.items()})
+    data["solves"] = np.array(json.dumps(list(solves.keys())))
+    np.savez_compressed(path, **data)
+    print(f"wrote {path}  ({os.path.getsize(path)} bytes)")
+
+
+def laplacian_case(name, mesh, seed, solvers=("iccg", "dpcg", "bicgstab"), fmt="native", sor=1e-8):
+    rng = np.random.default_rng(seed)
+    n = mesh.n_cells
+    ntot = n + mesh.n_boundary
+    mu = np.full(n, -1.0)
+    phi = np.zeros(ntot)
+    su0 = rng.standard_normal(n)
+    fi0 = np.zeros(n)
+    inputs = {"mu": mu, "phi": phi, "su0": su0, "fi0": fi0, "sor": np.array([sor]),
+              "nsw": np.array([1000], np.int32), "fmt": np.array(fmt)}
+    solves = {}
+    for s in solvers:
+        solves[s] = run_reference(mesh, mu, phi, su0, fi0, solver=s, sor=sor, fmt=fmt)
+    if not solvers:
+        solves["none"] = run_reference(mesh, mu, phi, su0, fi0, solver="none", fmt=fmt)
+    save_case(name, mesh, inputs, solves)
+
+
+def convdiff_case(name, mesh, seed, sor=1e-8):
+    """BiCGStab on the momentum-like system (upwind Taylor-Green convection + diffusion)."""
+    rng = np.random.default_rng(seed)
+    n = mesh.n_cells
+    ntot = n + mesh.n_boundary
+    mu = np.full(n, -1.0)
+    phi = np.zeros(ntot)
+    su0 = np.zeros(n)
+    fi0 = np.zeros(n)
+    base = run_reference(mesh, mu, phi, su0, fi0, solver="none")
+    g = O.Geometry(*(base[k] for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                       "facint")))
+    csr = O.CSR(*(base[k] for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    u, v, w = W.taylor_green(g.xc, g.yc, g.zc)
+    flux = O.face_flux_central(mesh, g, u, v, w)
+    vis = np.full(n, 1e-3)
+    a = O.convdiff(mesh, g, csr, vis, flux, urf=0.7)
+    su = rng.standard_normal(n) * g.vol
+    out = run_reference(mesh, mu, phi, su0, fi0, solver="bicgstab", sor=sor, a_override=a,
+                        su_override=su)
+    inputs = {"mu": mu, "phi": phi, "su0": su0, "fi0": fi0, "vis": vis, "u": u, "v": v, "w": w,
+              "flux": flux, "a_convdiff": a, "su_convdiff": su, "sor": np.array([sor]),
+              "nsw": np.array([1000], np.int32), "fmt": np.array("native")}
+    save_case(name, mesh, inputs, {"bicgstab": out})
+
+
+def transcript_fixture():
+    """The 5x5 systems of tests/test_sparse_solvers.f90:605-655 and the expected transcript."""
+    f32 = lambda xs: [float(np.float32(x)) for x in xs]  # noqa: E731  default-REAL literals
+    a1 = f32([6.80, -6.05, -0.45, 8.32, -9.67, -2.11, -3.30, 2.58, 2.71, -5.14,
+              5.66, 5.36, -2.70, 4.35, -7.26, 5.97, -4.44, 0.27, -7.17, 6.08,
+              8.23, 1.08, 9.04, 2.14, -6.87])
+    b1 = f32([4.02, 6.19, -8.22, -7.57, -3.03])
+    a2 = f32([3.14, 0.17, -0.90, 1.65, -0.72, 0.17, 0.79, 0.83, -0.65, 0.28,
+              -0.90, 0.83, 4.53, -3.70, 1.60, 1.65, -0.65, -3.70, 5.32, -1.37,
+              -0.72, 0.28, 1.60, -1.37, 1.98])
+    b2 = f32([-7.29, 9.25, 5.99, -1.94, -8.30])
+    text = open(os.path.join(REF_TESTS, "output.txt")).read()
+    fx = {"ioffset": [1, 6, 11, 16, 21, 26], "diag": [1, 7, 13, 19, 25],
+          "ja": [1, 2, 3, 4, 5] * 5, "a1": a1, "b1": b1, "a2": a2, "b2": b2,
+          "xa1": [-0.80, -0.70, 0.59, 1.32, 0.57], "xa2": [-6.02, 15.62, 3.02, 3.25, -8.78],
+          "transcript": text}
+    path = os.path.join(HERE, "solver5x5.json")
+    with open(path, "w") as f:
+        json.dump(fx, f, indent=1)
+    print("wrote", path)
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    transcript_fixture()
+    laplacian_case("hex4_uniform", M.hex_box(4), seed=11)
+    laplacian_case("hex8_perturbed", M.hex_box(8, perturb=0.2, seed=5), seed=12)
+    laplacian_case("hex6x5x4_graded", M.hex_box(6, 5, 4, perturb=0.2, seed=7, grade_axis=1,
+                                                  grade_ratio=1.3, grade_max_aspect=20.0), seed=13)
+    laplacian_case("tet3_permuted", M.tet_box(3, grade_axis=1, grade_ratio=1.2, grade_max_aspect=10.0,
+                                              permute_seed=4), seed=14)
+    laplacian_case("hex12_perturbed", M.hex_box(12, perturb=0.2, seed=9), seed=15)
+    convdiff_case("tet4_convdiff", M.tet_box(4, grade_axis=1, grade_ratio=1.1, grade_max_aspect=10.0,
+                                             permute_seed=8), seed=16)
+    convdiff_case("hex8_convdiff", M.hex_box(8, perturb=0.2, seed=21), seed=17)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,294 @@
+"""The CPU oracle pinned against the reference (golden fixtures), closed forms and invariants.
+
+CPU only.  Every oracle function is checked by at least one comparison that a
+plausible mistake (dropped term, wrong sign or index, transposed operand)
+would fail.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import CONVDIFF_CASES, MESH_CASES, load_transcript
+from freecappuccino_amd import mesh as M
+from oracle import oracle as O
+
+
+def _geom(g):
+    return O.mesh_geometry(g.mesh)
+
+
+def _csr(g):
+    m = g.mesh
+    return O.csr_build(m.n_cells, m.owner[:m.n_inner].astype(np.int64) + 1, m.neighbour.astype(np.int64) + 1)
+
+
+# --------------------------------------------------------------------- golden
+
+@pytest.mark.parametrize("case", MESH_CASES + CONVDIFF_CASES)
+def test_geometry_bitwise_vs_reference(golden_loader, case):
+    g = golden_loader(case)
+    geo = _geom(g)
+    for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint"):
+        np.testing.assert_array_equal(getattr(geo, k), g.out(g.first, k), err_msg=k)
+
+
+@pytest.mark.parametrize("case", MESH_CASES + CONVDIFF_CASES)
+def test_csr_bitwise_vs_reference(golden_loader, case):
+    g = golden_loader(case)
+    c = _csr(g)
+    for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot"):
+        np.testing.assert_array_equal(getattr(c, k), g.out(g.first, k), err_msg=k)
+
+
+@pytest.mark.parametrize("case", MESH_CASES)
+def test_laplacian_bitwise_vs_reference(golden_loader, case):
+    g = golden_loader(case)
+    geo, c = _geom(g), _csr(g)
+    a, su = O.laplacian(g.mesh, geo, c, g.inp("mu"), g.inp("phi"), g.inp("su0"))
+    np.testing.assert_array_equal(a, g.out(g.first, "a"))
+    np.testing.assert_array_equal(su, g.out(g.first, "su"))
+
+
+@pytest.mark.parametrize("case", MESH_CASES)
+@pytest.mark.parametrize("solver", ["iccg", "dpcg", "bicgstab"])
+def test_solvers_bitwise_vs_reference(golden_loader, case, solver):
+    g = golden_loader(case)
+    if solver not in g.solves:
+        pytest.skip("no fixture")
+    geo, c = _geom(g), _csr(g)
+    a, su = O.laplacian(g.mesh, geo, c, g.inp("mu"), g.inp("phi"), g.inp("su0"))
+    r = getattr(O, solver)(c, a, su, g.inp("fi0"), sor=float(g.inp("sor")[0]), nsw=int(g.inp("nsw")[0]))
+    assert r.iters == int(g.out(solver, "iters")[0])
+    np.testing.assert_array_equal(r.fi, g.out(solver, "fi"))
+    np.testing.assert_array_equal(r.res, g.out(solver, "res"))
+    hist4 = np.array([float(f"{h:10.3E}") for h in r.history])
+    np.testing.assert_array_equal(hist4, g.out(solver, "resl_history_4d"))
+    assert r.status == 0
+
+
+@pytest.mark.parametrize("case", CONVDIFF_CASES)
+def test_bicgstab_convdiff_bitwise_vs_reference(golden_loader, case):
+    g = golden_loader(case)
+    c = _csr(g)
+    a = g.inp("a_convdiff")
+    r = O.bicgstab(c, a, g.inp("su_convdiff"), g.inp("fi0"), sor=float(g.inp("sor")[0]),
+                   nsw=int(g.inp("nsw")[0]))
+    assert r.iters == int(g.out("bicgstab", "iters")[0])
+    np.testing.assert_array_equal(r.fi, g.out("bicgstab", "fi"))
+    np.testing.assert_array_equal(r.res, g.out("bicgstab", "res"))
+
+
+def _fmt_e(x):
+    return f"{x:10.3E}"
+
+
+def test_solver_transcript_5x5():
+    """tests/output.txt of the reference, reproduced line by line (incl. its warm-start quirks)."""
+    t = load_transcript()
+    c = O.CSR(np.array(t["ioffset"], np.int32), np.array(t["ja"], np.int32), np.array(t["diag"], np.int32),
+              np.zeros(0, np.int32), np.zeros(0, np.int32))
+    lines = []
+
+    def emit(r):
+        lines.append(" " * 20 + "res0 = " + _fmt_e(r.res0))
+        for i, h in enumerate(r.history):
+            lines.append(" " * 19 + f" iter = {i + 1:4d} resl = " + _fmt_e(h) + " rsm = "
+                         + _fmt_e(h / (r.res0 + float(np.float32(1e-30)))))
+
+    def sol(x, xa):
+        for a, b in zip(x, xa):
+            lines.append(" " * 19 + f"{a:5.2f}" + " " * 10 + f"{b:5.2f}")
+
+    x = np.zeros(5)
+    r1 = O.bicgstab(c, t["a1"], t["b1"], x, variant=O.VARIANT_TESTS)
+    emit(r1)
+    sol(r1.fi, t["xa1"])
+    r2 = O.iccg(c, t["a2"], t["b2"], r1.fi, variant=O.VARIANT_TESTS)   # x not reset in the reference
+    emit(r2)
+    sol(r2.fi, t["xa2"])
+    r3 = O.dpcg(c, t["a2"], t["b2"], r2.fi, variant=O.VARIANT_TESTS)
+    emit(r3)
+    sol(r3.fi, t["xa2"])
+    expected = [ln for ln in t["transcript"].splitlines()
+                if "res0" in ln or "iter =" in ln or ln.strip()[:1] in "-0123456789" and ln.strip()]
+    assert lines == expected
+    assert (r1.iters, r2.iters, r3.iters) == (6, 5, 5)
+
+
+# --------------------------------------------------------------- closed forms
+
+def test_uniform_box_geometry_closed_form():
+    n = 5
+    m = M.hex_box(n)
+    geo = O.mesh_geometry(m)
+    h = 1.0 / n
+    np.testing.assert_allclose(geo.vol, h ** 3, rtol=1e-12)
+    are = np.sqrt(geo.arx ** 2 + geo.ary ** 2 + geo.arz ** 2)
+    np.testing.assert_allclose(are, h * h, rtol=1e-12)
+    np.testing.assert_allclose(geo.facint, 0.5, rtol=1e-12)
+    i = np.arange(m.n_cells)
+    np.testing.assert_allclose(geo.xc, (i % n + 0.5) * h, rtol=1e-12)
+    np.testing.assert_allclose(geo.yc, ((i // n) % n + 0.5) * h, rtol=1e-12)
+    np.testing.assert_allclose(geo.zc, (i // (n * n) + 0.5) * h, rtol=1e-12)
+    # normals of inner faces point from owner to neighbour
+    nif = m.n_inner
+    d = np.stack([geo.xc[m.neighbour] - geo.xc[m.owner[:nif]], geo.yc[m.neighbour] - geo.yc[m.owner[:nif]],
+                  geo.zc[m.neighbour] - geo.zc[m.owner[:nif]]], 1)
+    s = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], 1)
+    assert (np.einsum("ij,ij->i", d, s) > 0).all()
+
+
+@pytest.mark.parametrize("maker", [lambda: M.hex_box(6, perturb=0.2, seed=3),
+                                   lambda: M.tet_box(3, grade_axis=1, grade_ratio=1.2, permute_seed=2),
+                                   lambda: M.hex_box(5, 4, 3, grade_axis=0, grade_ratio=1.2, perturb=0.1)])
+def test_geometry_invariants(maker):
+    m = maker()
+    geo = O.mesh_geometry(m)
+    assert np.isclose(geo.vol.sum(), 1.0, rtol=1e-12)
+    assert (geo.vol > 0).all()
+    # closed cells: sum of outward area vectors vanishes
+    nif = m.n_inner
+    acc = np.zeros((m.n_cells, 3))
+    S = np.stack([geo.arx, geo.ary, geo.arz], 1)
+    np.add.at(acc, m.owner, S)
+    np.add.at(acc, m.neighbour, -S[:nif])
+    assert np.abs(acc).max() < 1e-14
+    # first moment: sum vol*xc equals the domain centroid (1/2 on the unit box)
+    for c in (geo.xc, geo.yc, geo.zc):
+        assert np.isclose(np.dot(geo.vol, c), 0.5, rtol=1e-10)
+    assert ((geo.facint > 0) & (geo.facint < 1)).all()
+
+
+def test_uniform_box_laplacian_closed_form():
+    n = 4
+    m = M.hex_box(n)
+    geo = O.mesh_geometry(m)
+    c = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    a, su = O.laplacian(m, geo, c, np.full(m.n_cells, -1.0), np.zeros(m.n_cells + m.n_boundary),
+                        np.zeros(m.n_cells))
+    h = 1.0 / n
+    A = sp.csr_matrix((a, c.ja - 1, c.ioffset - 1), shape=(m.n_cells,) * 2).toarray()
+    off = A - np.diag(np.diag(A))
+    np.testing.assert_allclose(off[off != 0], -h, rtol=1e-12)
+    nb = (off != 0).sum(1)
+    i = np.arange(m.n_cells)
+    walls = (i % n == 0).astype(float)          # x=0 wall faces per cell
+    np.testing.assert_allclose(np.diag(A), nb * h + walls * 2 * h, rtol=1e-12)
+    np.testing.assert_array_equal(A, A.T)
+    assert np.all(su == 0)
+
+
+@pytest.mark.parametrize("case", MESH_CASES)
+def test_csr_canonical_scipy(golden_loader, case):
+    g = golden_loader(case)
+    c = _csr(g)
+    m = g.mesh
+    n, nif = m.n_cells, m.n_inner
+    rows = np.concatenate([np.arange(n), m.owner[:nif], m.neighbour])
+    cols = np.concatenate([np.arange(n), m.neighbour, m.owner[:nif]])
+    S = sp.coo_matrix((np.ones(rows.size), (rows, cols)), shape=(n, n)).tocsr()
+    S.sum_duplicates()
+    S.sort_indices()
+    assert S.has_canonical_format
+    np.testing.assert_array_equal(c.ioffset - 1, S.indptr)
+    np.testing.assert_array_equal(c.ja - 1, S.indices)
+    np.testing.assert_array_equal(c.ja[c.diag - 1], np.arange(1, n + 1))
+    np.testing.assert_array_equal(c.ja[c.ij_slot - 1], m.neighbour + 1)
+    np.testing.assert_array_equal(c.ja[c.ji_slot - 1], m.owner[:nif] + 1)
+
+
+# ----------------------------------------------------------- preconditioners
+
+def _dense(c, a):
+    n = c.diag.shape[0]
+    return sp.csr_matrix((a, c.ja - 1, c.ioffset - 1), shape=(n, n)).toarray()
+
+
+@pytest.mark.parametrize("kind", ["dic", "dilu"])
+def test_incomplete_factor_matches_diagonal(golden_loader, kind):
+    """M = (D+L) D^-1 (D+U) reproduces diag(A) (the defining property of DIC / DILU)."""
+    g = golden_loader("hex8_convdiff" if kind == "dilu" else "hex8_perturbed")
+    c = _csr(g)
+    a = g.inp("a_convdiff") if kind == "dilu" else g.out(g.first, "a")
+    d = O.dilu_pivots(c, a) if kind == "dilu" else O.dic_pivots(c, a)
+    A = _dense(c, a)
+    D = np.diag(1.0 / d)
+    L, U = np.tril(A, -1), np.triu(A, 1)
+    Mx = (D + L) @ np.diag(d) @ (D + U)
+    np.testing.assert_allclose(np.diag(Mx), np.diag(A), rtol=1e-10)
+    # off-diagonal pattern of M restricted to A's pattern equals A
+    mask = (A != 0) & ~np.eye(A.shape[0], dtype=bool)
+    np.testing.assert_allclose(Mx[mask], A[mask], rtol=1e-12)
+    # and the sweep applies M^-1 (small = 0 makes the rescale exact)
+    r = np.random.default_rng(0).standard_normal(A.shape[0])
+    z = O.lu_apply(c, a, d, r, small_value=0.0)
+    np.testing.assert_allclose(Mx @ z, r, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", ["iccg", "dpcg", "bicgstab"])
+def test_solvers_against_dense_solve(golden_loader, solver):
+    g = golden_loader("hex8_perturbed")
+    c = _csr(g)
+    a, su = g.out(g.first, "a"), g.out(g.first, "su")
+    r = getattr(O, solver)(c, a, su, np.zeros_like(su), sor=1e-12, nsw=2000)
+    x = np.linalg.solve(_dense(c, a), su)
+    np.testing.assert_allclose(r.fi, x, rtol=1e-8, atol=1e-10 * np.abs(x).max())
+
+
+def test_spmv_matches_scipy(golden_loader):
+    g = golden_loader("tet3_permuted")
+    c = _csr(g)
+    a = g.out(g.first, "a")
+    x = np.random.default_rng(1).standard_normal(c.diag.shape[0])
+    np.testing.assert_allclose(O.spmv(c, a, x), _dense(c, a) @ x, rtol=1e-13, atol=1e-15)
+
+
+# ----------------------------------------------------- convection-diffusion
+
+@pytest.mark.parametrize("case", CONVDIFF_CASES)
+def test_convdiff_invariants(golden_loader, case):
+    g = golden_loader(case)
+    geo, c = _geom(g), _csr(g)
+    flux, vis = g.inp("flux"), g.inp("vis")
+    a = O.convdiff(g.mesh, geo, c, vis, flux, urf=0.7)
+    np.testing.assert_array_equal(a, g.inp("a_convdiff"))
+    A = _dense(c, a)
+    m = g.mesh
+    nif = m.n_inner
+    o, nb = m.owner[:nif], m.neighbour
+    # a(ij) - a(ji) = F (upwind split of the face flux)
+    np.testing.assert_allclose(A[o, nb] - A[nb, o], flux, rtol=1e-12, atol=1e-18)
+    # symmetric part is the diffusion coefficient vis*|S|/|d| (positive, as -a)
+    de = -(A[o, nb] + A[nb, o] + np.abs(flux)) / 2
+    are = np.sqrt(geo.arx[:nif] ** 2 + geo.ary[:nif] ** 2 + geo.arz[:nif] ** 2)
+    dpn = np.sqrt((geo.xc[nb] - geo.xc[o]) ** 2 + (geo.yc[nb] - geo.yc[o]) ** 2 + (geo.zc[nb] - geo.zc[o]) ** 2)
+    np.testing.assert_allclose(de, 1e-3 * are / dpn, rtol=1e-10)
+    # diag * urf = spu - sum(offdiag): row sums vanish except on wall rows
+    rows = A.sum(1) - np.diag(A) + np.diag(A) * 0.7
+    ws, wc = m.wall_range()
+    wall_cells = np.unique(m.owner[ws:ws + wc])
+    interior = np.setdiff1d(np.arange(m.n_cells), wall_cells)
+    assert np.abs(rows[interior]).max() < 1e-15
+    assert (rows[wall_cells] > 0).all()
+
+
+def test_face_flux_and_divergence():
+    m = M.hex_box(5, perturb=0.2, seed=2)
+    geo = O.mesh_geometry(m)
+    rng = np.random.default_rng(3)
+    u, v, w = rng.uniform(-1, 1, (3, m.n_cells))
+    flux = O.face_flux_central(m, geo, u, v, w)
+    nif = m.n_inner
+    o, nb = m.owner[:nif], m.neighbour
+    fx = geo.facint
+    uf = (1 - fx) * u[o] + fx * u[nb]
+    vf = (1 - fx) * v[o] + fx * v[nb]
+    wf = (1 - fx) * w[o] + fx * w[nb]
+    np.testing.assert_array_equal(flux, (uf * geo.arx[:nif] + vf * geo.ary[:nif]) + wf * geo.arz[:nif])
+    su = O.rhs_divergence(m, flux, np.zeros(m.n_cells))
+    assert abs(su.sum()) < 1e-14
+    # a uniform velocity has zero discrete divergence on every cell without boundary faces
+    one = np.ones(m.n_cells)
+    su1 = O.rhs_divergence(m, O.face_flux_central(m, geo, one, 0 * one, 0 * one), np.zeros(m.n_cells))
+    interior = np.setdiff1d(np.arange(m.n_cells), np.unique(m.owner[nif:]))
+    assert np.abs(su1[interior]).max() < 1e-15
