@@ -1,0 +1,143 @@
+"""Benchmark of the pressure-solve hot path on one MI355X.
+
+A step is one pass of the hot path over the resident system: face-loop
+assembly of the Laplacian into the CSR matrix (fvm_laplacian) followed by the
+Krylov solve to 1e-8 FP64 relative residual ||su - A fi||_2 / ||su||_2.
+
+Default workload: 128^3 hex box with interior vertices perturbed by +-20%
+(2,097,152 cells), mu = -1, Dirichlet wall at x=0, RHS = 16 seeded Gaussian
+blobs with zero mean.  Two solvers are timed on the same system:
+  * IC(0)-PCG in FP64 (the reference's DIC-PCG, level-scheduled sweeps);
+  * the mixed-precision form: FP32 DIC-PCG inside FP64 iterative refinement.
+`value` is the time to 1e-8 of the faster of the two (ms, lower is better).
+
+Also reported: FP64 SpMV GB/s on the same matrix (the roofline object, from
+HIP events on the library's stream), and the CPU baseline: the oracle's
+sequential DIC-PCG (the reference algorithm restated in C) timed on one host
+core over a bounded number of iterations and scaled to the GPU's iteration
+count.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 128]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
+    """Algorithmic bytes of one SpMV: values + column indices + row pointers + x once + y."""
+    return nnz * (value_bytes + index_bytes) + index_bytes * (n + 1) + 2 * value_bytes * n
+
+
+def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
+    """The oracle's sequential DIC-PCG on one host core, `budget_iters` iterations, scaled."""
+    from oracle import oracle as O
+    csr = O.csr_build(mesh.n_cells, mesh.owner[:mesh.n_inner] + 1, mesh.neighbour + 1)
+    t0 = time.perf_counter()
+    r = O.iccg(csr, a, su, np.zeros(mesh.n_cells), sor=1e-30, nsw=budget_iters)
+    dt = time.perf_counter() - t0
+    per_iter = dt / max(r.iters, 1)
+    return {"value": per_iter * gpu_iters * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+            "sample": f"oracle DIC-PCG (C restatement of src/iccg.f90, gcc -O2, 1 thread) for {r.iters} "
+                      f"iterations on the same system ({dt:.1f} s incl. pivots), per-iteration time x "
+                      f"{gpu_iters} GPU iterations; host {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="cells per direction of the hex box")
+    ap.add_argument("--cpu-iters", type=int, default=40, help="iterations of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from freecappuccino_amd import fcap
+    from freecappuccino_amd import workloads as W
+
+    mesh = W.case_mesh(2, args.n)
+    ctx = fcap.Context(0)
+    s = fcap.System(mesh, ctx)
+    n, nnz = mesh.n_cells, mesh.nnz
+    xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")
+    su0 = W.blob_rhs(xc, yc, zc, vol, 16, seed=2)
+    mu = np.full(n, -1.0)
+    s.assemble_laplacian(mu, None, su0)
+    sizes = s.sizes()
+
+    def step(method):
+        s.set_solution(np.zeros(n))
+        t0 = time.perf_counter()
+        s.assemble_laplacian_dev()
+        rc, info = s.solve(method, fcap.STOP_L2, 1e-8, 20000)
+        return rc, info, (time.perf_counter() - t0) * 1e3, s.last_ms(0)
+
+    results = {}
+    for name, method in (("ic_pcg_fp64", fcap.METHOD_ICCG), ("ic_pcg_mixed", fcap.METHOD_ICCG_MIXED)):
+        for _ in range(args.warmup):
+            step(method)
+        times, infos = [], []
+        for _ in range(args.steps):
+            rc, info, wall_ms, asm_ms = step(method)
+            times.append(wall_ms)
+            infos.append((rc, info, asm_ms))
+        rc, info, asm_ms = infos[-1]
+        results[name] = {"ms": float(np.median(times)), "status": rc, "iters": info["iters"],
+                         "outer": info["outer"], "rel_l2": info["rel_l2"], "assembly_ms": asm_ms,
+                         "solve_device_ms": info["solve_ms"] + info["setup_ms"]}
+        results[name]["x"] = s.solution()
+    x64 = results["ic_pcg_fp64"].pop("x")
+    xm = results["ic_pcg_mixed"].pop("x")
+    results["ic_pcg_mixed"]["err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+
+    spmv_ms = s.spmv_bench(64, args.spmv_reps)
+    spmv32_ms = s.spmv_bench(32, args.spmv_reps)
+    b64 = spmv_bytes(n, nnz, 8, 4)
+    b32 = spmv_bytes(n, nnz, 4, 4)
+    gbs = b64 / (spmv_ms * 1e-3) / 1e9
+    best = min(results, key=lambda k: results[k]["ms"])
+    value = results[best]["ms"]
+
+    line = {
+        "metric": "pressure-solve time to 1e-8 rel. residual, learned-FCG vs IC-PCG; SpMV GB/s",
+        "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if best == "ic_pcg_fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"hex box {args.n}^3 perturbed +-20% (seeded), Laplacian mu=-1, Dirichlet x=0, "
+                               f"16 Gaussian blobs; assembly + solve to 1e-8 FP64 rel. L2 residual",
+                   "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"], "best": best},
+        "solvers": results,
+        "spmv": {"fp64_ms": spmv_ms, "fp64_GBps": gbs, "fp32_ms": spmv32_ms,
+                 "fp32_GBps": b32 / (spmv32_ms * 1e-3) / 1e9, "bytes_fp64": b64, "bytes_fp32": b32},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv<double,double,0>"},
+    }
+    if not args.no_cpu:
+        a = s.get("a")
+        su = s.get("su")
+        line["cpu_baseline"] = cpu_baseline(mesh, a, su, results["ic_pcg_fp64"]["iters"], args.cpu_iters)
+    s.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

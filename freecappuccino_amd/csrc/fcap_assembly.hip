@@ -1,0 +1,455 @@
+// Face-loop assembly of the CSR matrix on the GPU.
+//
+// The reference's face loop (src/fvm_laplacian.f90:38-65) writes each face's
+// two off-diagonal coefficients into slots no other face touches, and
+// subtracts them from the two diagonals in face order.  Here:
+//   * a face-parallel kernel computes the coefficient and writes the two
+//     off-diagonal slots (one writer per slot: no atomics, no colouring);
+//   * a row-parallel kernel rebuilds each diagonal as
+//     ((0 - a_f1) - a_f2) - ... over the row's entries taken in face order,
+//     then applies O-C cuts and wall faces in their order.  Row i's entry for
+//     face f holds exactly the value the reference subtracts from a(diag(i)) for
+//     that face, so every diagonal is bit-identical to the sequential loop.
+// For meshes in polyMesh (upper-triangular) face order, face order within a
+// row equals column order and no permutation is needed.
+
+#include <algorithm>
+
+#include "fcap_internal.h"
+
+namespace fcap {
+
+void scan_dev(fc_context *ctx, const int *in, int *out, int n);
+void sort_segments_dev(fc_context *ctx, const int *ptr, int *v, int nseg);
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int grid_for(long long n, int block = kBlock) {
+    long long g = (n + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(g, 1 << 30));
+}
+
+// facefluxlaplacian (src/fvm_laplacian.f90:171-221)
+__device__ inline double laplacian_coef(int p, int q, double sx, double sy, double sz, double lambda,
+                                        const double *__restrict__ mu, const double *__restrict__ xc,
+                                        const double *__restrict__ yc, const double *__restrict__ zc) {
+    double fxn = lambda;
+    double fxp = 1.0 - lambda;
+    double xpn = xc[q] - xc[p];
+    double ypn = yc[q] - yc[p];
+    double zpn = zc[q] - zc[p];
+    double num = (sx * sx + sy * sy) + sz * sz;
+    double den = (sx * xpn + sy * ypn) + sz * zpn;
+    double smdpn = num / den;
+    double m = fxp * mu[p] + fxn * mu[q];
+    return m * smdpn;
+}
+
+__global__ void k_laplacian_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                                  const int *__restrict__ ij, const int *__restrict__ ji,
+                                  const double *__restrict__ arx, const double *__restrict__ ary,
+                                  const double *__restrict__ arz, const double *__restrict__ facint,
+                                  const double *__restrict__ xc, const double *__restrict__ yc,
+                                  const double *__restrict__ zc, const double *__restrict__ mu,
+                                  double *__restrict__ a) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    double c = laplacian_coef(own[f], nei[f], arx[f], ary[f], arz[f], facint[f], mu, xc, yc, zc);
+    a[ij[f]] = c;   // can
+    a[ji[f]] = c;   // cap
+}
+
+__global__ void k_laplacian_oc(int noc, int oc_face0, const int *__restrict__ ijl, const int *__restrict__ ijr,
+                               const double *__restrict__ foc, const double *__restrict__ arx,
+                               const double *__restrict__ ary, const double *__restrict__ arz,
+                               const double *__restrict__ xc, const double *__restrict__ yc,
+                               const double *__restrict__ zc, const double *__restrict__ mu,
+                               double *__restrict__ al, double *__restrict__ ar) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noc) return;
+    int f = oc_face0 + i;
+    double c = laplacian_coef(ijl[i], ijr[i], arx[f], ary[f], arz[f], foc[i], mu, xc, yc, zc);
+    al[i] = c;
+    ar[i] = c;
+}
+
+// Diagonal in face order, then O-C cuts (:68-86), then wall faces (:140-154).
+__global__ void k_laplacian_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                 const int *__restrict__ kord, const int *__restrict__ co_ptr,
+                                 const int *__restrict__ co_idx, const double *__restrict__ al,
+                                 const double *__restrict__ ar, const int *__restrict__ cw_ptr,
+                                 const int *__restrict__ cw_idx, int wall_face0, int wall_val0,
+                                 const double *__restrict__ arx, const double *__restrict__ ary,
+                                 const double *__restrict__ arz, const double *__restrict__ xc,
+                                 const double *__restrict__ yc, const double *__restrict__ zc,
+                                 const double *__restrict__ xf, const double *__restrict__ yf,
+                                 const double *__restrict__ zf, const double *__restrict__ mu,
+                                 const double *__restrict__ phi, double *__restrict__ a,
+                                 double *__restrict__ su) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int dp = diagpos[i];
+    double d = 0.0;
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+        int k = kord ? kord[p] : p;
+        if (k == dp) continue;
+        d = d - a[k];
+    }
+    if (co_ptr) {
+        for (int q = co_ptr[i]; q < co_ptr[i + 1]; ++q) {
+            int code = co_idx[q];
+            int o = code >> 1;
+            d = d - ((code & 1) ? al[o] : ar[o]);
+        }
+    }
+    if (cw_ptr && cw_ptr[i] < cw_ptr[i + 1]) {
+        double s = su[i];
+        double m = mu[i];
+        for (int q = cw_ptr[i]; q < cw_ptr[i + 1]; ++q) {
+            int w = cw_idx[q];
+            int f = wall_face0 + w;
+            double fx = arx[f], fy = ary[f], fz = arz[f];
+            double are = sqrt((fx * fx + fy * fy) + fz * fz);
+            double dx = xc[i] - xf[f], dy = yc[i] - yf[f], dz = zc[i] - zf[f];
+            double dpw = sqrt((dx * dx + dy * dy) + dz * dz);
+            d = d - (m * are) / dpw;
+            s = s + d * phi[wall_val0 + w];
+        }
+        su[i] = s;
+    }
+    a[dp] = d;
+}
+
+// facefluxuvw coefficient part (src/faceflux_velocity.f90:97-108), upwind
+__global__ void k_convdiff_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                                 const int *__restrict__ ij, const int *__restrict__ ji,
+                                 const double *__restrict__ arx, const double *__restrict__ ary,
+                                 const double *__restrict__ arz, const double *__restrict__ facint,
+                                 const double *__restrict__ xc, const double *__restrict__ yc,
+                                 const double *__restrict__ zc, const double *__restrict__ vis,
+                                 const double *__restrict__ flux, double *__restrict__ a) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int p = own[f], q = nei[f];
+    double fxn = facint[f], fxp = 1.0 - facint[f];
+    double xpn = xc[q] - xc[p], ypn = yc[q] - yc[p], zpn = zc[q] - zc[p];
+    double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+    double sx = arx[f], sy = ary[f], sz = arz[f];
+    double are = sqrt((sx * sx + sy * sy) + sz * sz);
+    double game = vis[p] * fxp + vis[q] * fxn;
+    double de = (game * are) / dpn;
+    double fl = flux[f];
+    a[ij[f]] = -de + (fl < 0.0 ? fl : 0.0);
+    a[ji[f]] = -de - (fl > 0.0 ? fl : 0.0);
+}
+
+// calcuvw diagonal (src/calcuvw.f90:417-432) with spu from the wall faces
+__global__ void k_convdiff_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                const int *__restrict__ cw_ptr, const int *__restrict__ cw_idx, int wall_face0,
+                                const double *__restrict__ arx, const double *__restrict__ ary,
+                                const double *__restrict__ arz, const double *__restrict__ xc,
+                                const double *__restrict__ yc, const double *__restrict__ zc,
+                                const double *__restrict__ xf, const double *__restrict__ yf,
+                                const double *__restrict__ zf, const double *__restrict__ vis, double urfr,
+                                double *__restrict__ a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int dp = diagpos[i];
+    double spu = 0.0;
+    if (cw_ptr) {
+        for (int q = cw_ptr[i]; q < cw_ptr[i + 1]; ++q) {
+            int f = wall_face0 + cw_idx[q];
+            double fx = arx[f], fy = ary[f], fz = arz[f];
+            double are = sqrt((fx * fx + fy * fy) + fz * fz);
+            double dx = xc[i] - xf[f], dy = yc[i] - yf[f], dz = zc[i] - zf[f];
+            double dpw = sqrt((dx * dx + dy * dy) + dz * dz);
+            spu = spu + (vis[i] * are) / dpw;
+        }
+    }
+    double s = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s = s + (k == dp ? 0.0 : a[k]);
+    s = s - 0.0;
+    double dd = spu - s;
+    a[dp] = dd * urfr;
+}
+
+__global__ void k_face_flux(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                            const double *__restrict__ facint, const double *__restrict__ arx,
+                            const double *__restrict__ ary, const double *__restrict__ arz,
+                            const double *__restrict__ u, const double *__restrict__ v,
+                            const double *__restrict__ w, double *__restrict__ flux) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int p = own[f], q = nei[f];
+    double fxn = facint[f], fxp = 1.0 - facint[f];
+    double uf = fxp * u[p] + fxn * u[q];
+    double vf = fxp * v[p] + fxn * v[q];
+    double wf = fxp * w[p] + fxn * w[q];
+    flux[f] = (uf * arx[f] + vf * ary[f]) + wf * arz[f];
+}
+
+// su(o) -= F, su(n) += F accumulated per cell in face order
+__global__ void k_rhs_divergence(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                 const int *__restrict__ kord, const int *__restrict__ kface,
+                                 const int *__restrict__ ij, const double *__restrict__ flux,
+                                 double *__restrict__ su) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int dp = diagpos[i];
+    double s = su[i];
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+        int k = kord ? kord[p] : p;
+        if (k == dp) continue;
+        int f = kface[k];
+        if (ij[f] == k) s = s - flux[f];     // row i is the owner of face f
+        else s = s + flux[f];
+    }
+    su[i] = s;
+}
+
+__global__ void k_kface(int nif, const int *__restrict__ ij, const int *__restrict__ ji, int *__restrict__ kface) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    kface[ij[f]] = f;
+    kface[ji[f]] = f;
+}
+
+__global__ void k_face_order_check(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                   const int *__restrict__ kface, int *__restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int last = -1;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        if (k == diagpos[i]) continue;
+        if (kface[k] < last) { atomicOr(flag, 1); return; }
+        last = kface[k];
+    }
+}
+
+// per row: entry positions ordered by face id (diagonal last)
+__global__ void k_kord(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                       const int *__restrict__ kface, int *__restrict__ kord) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = ptr[i], e = ptr[i + 1];
+    for (int k = s; k < e; ++k) kord[k] = k;
+    auto key = [&](int k) { return k == diagpos[i] ? 0x7fffffff : kface[k]; };
+    for (int k = s + 1; k < e; ++k) {
+        int v = kord[k], kv = key(v);
+        int m = k - 1;
+        while (m >= s && key(kord[m]) > kv) { kord[m + 1] = kord[m]; --m; }
+        kord[m + 1] = v;
+    }
+}
+
+// Caller-supplied faces and slot maps are used as indices: check them first.
+__global__ void k_check_faces(int nif, int n, int nnz, const int *__restrict__ own, const int *__restrict__ nei,
+                              const int *__restrict__ ij, const int *__restrict__ ji, const int *__restrict__ ptr,
+                              const int *__restrict__ ja, int *__restrict__ err) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    int o = own[f], q = nei[f], kij = ij[f], kji = ji[f];
+    if (o < 0 || o >= n || q < 0 || q >= n) { atomicOr(err, 1); return; }
+    if (kij < ptr[o] || kij >= ptr[o + 1] || kji < ptr[q] || kji >= ptr[q + 1] || kij >= nnz || kji >= nnz) {
+        atomicOr(err, 2);
+        return;
+    }
+    if (ja[kij] != q || ja[kji] != o) atomicOr(err, 2);
+}
+
+__global__ void k_wall_count(int nwal, int wall_face0, int n, const int *__restrict__ own, int *__restrict__ cnt,
+                             int *__restrict__ err) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwal) return;
+    int c = own[wall_face0 + w];
+    if (c < 0 || c >= n) { atomicOr(err, 1); return; }
+    atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_wall_fill(int nwal, int wall_face0, const int *__restrict__ own, int *__restrict__ cursor,
+                            int *__restrict__ idx) {
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwal) return;
+    idx[atomicAdd(&cursor[own[wall_face0 + w]], 1)] = w;
+}
+
+__global__ void k_oc_count(int noc, const int *__restrict__ ijl, const int *__restrict__ ijr, int *__restrict__ cnt) {
+    int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= noc) return;
+    atomicAdd(&cnt[ijl[o]], 1);
+    atomicAdd(&cnt[ijr[o]], 1);
+}
+
+__global__ void k_oc_fill(int noc, const int *__restrict__ ijl, const int *__restrict__ ijr, int *__restrict__ cursor,
+                          int *__restrict__ idx) {
+    int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= noc) return;
+    idx[atomicAdd(&cursor[ijl[o]], 1)] = 2 * o;       // left cell: subtract ar
+    idx[atomicAdd(&cursor[ijr[o]], 1)] = 2 * o + 1;   // right cell: subtract al
+}
+
+__global__ void k_copy(const int *__restrict__ in, int *__restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+}  // namespace
+
+void build_wall_lists(fc_context *ctx, WallDev &w, const int *owner0_d, int n, int nwal, int iwall_faces_start,
+                      int iwall_start) {
+    w.nwal = nwal;
+    w.iwall_faces_start = iwall_faces_start;
+    w.iwall_start = iwall_start;
+    if (nwal <= 0) { w.cw_ptr.release(); w.cw_idx.release(); return; }
+    hipStream_t s = ctx->stream;
+    DevBuf cnt(sizeof(int) * (n + 1)), cursor(sizeof(int) * (n + 1)), err(sizeof(int));
+    FC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int) * (n + 1), s));
+    FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+    k_wall_count<<<grid_for(nwal), kBlock, 0, s>>>(nwal, iwall_faces_start, n, owner0_d, cnt.as<int>(),
+                                                   err.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    int herr = 0;
+    d2h(&herr, err.get(), sizeof(int), s);
+    FC_REQUIRE(herr == 0, FC_ERR_STRUCTURE, "wall face owner out of range");
+    w.cw_ptr.alloc(sizeof(int) * (n + 1));
+    scan_dev(ctx, cnt.as<int>(), w.cw_ptr.as<int>(), n + 1);
+    w.cw_idx.alloc(sizeof(int) * nwal);
+    k_copy<<<grid_for(n), kBlock, 0, s>>>(w.cw_ptr.as<int>(), cursor.as<int>(), n);
+    k_wall_fill<<<grid_for(nwal), kBlock, 0, s>>>(nwal, iwall_faces_start, owner0_d, cursor.as<int>(),
+                                                  w.cw_idx.as<int>());
+    sort_segments_dev(ctx, w.cw_ptr.as<int>(), w.cw_idx.as<int>(), n);
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+__global__ void k_check_range(int m, const int *__restrict__ v, int n, int *__restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && (v[i] < 0 || v[i] >= n)) atomicOr(err, 1);
+}
+
+void validate_faces(fc_context *ctx, const Structure &st, int nif, const int *own, const int *nei, const int *ij,
+                    const int *ji) {
+    if (nif <= 0) return;
+    hipStream_t s = ctx->stream;
+    DevBuf err(sizeof(int));
+    FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+    k_check_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, st.n, st.nnz, own, nei, ij, ji, st.ioffset.as<int>(),
+                                                   st.ja.as<int>(), err.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    int h = 0;
+    d2h(&h, err.get(), sizeof(int), s);
+    FC_REQUIRE(!(h & 1), FC_ERR_STRUCTURE, "owner/neighbour out of range");
+    FC_REQUIRE(!(h & 2), FC_ERR_STRUCTURE, "face slot maps do not match the CSR structure");
+}
+
+void build_oc_lists(fc_context *ctx, OcDev &o, int n) {
+    if (o.noc <= 0) { o.co_ptr.release(); o.co_idx.release(); return; }
+    hipStream_t s = ctx->stream;
+    {
+        DevBuf err(sizeof(int));
+        FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+        k_check_range<<<grid_for(o.noc), kBlock, 0, s>>>(o.noc, o.ijl.as<int>(), n, err.as<int>());
+        k_check_range<<<grid_for(o.noc), kBlock, 0, s>>>(o.noc, o.ijr.as<int>(), n, err.as<int>());
+        FC_HIP_CHECK(hipGetLastError());
+        int h = 0;
+        d2h(&h, err.get(), sizeof(int), s);
+        FC_REQUIRE(h == 0, FC_ERR_STRUCTURE, "O-C cell index out of range");
+    }
+    DevBuf cnt(sizeof(int) * (n + 1)), cursor(sizeof(int) * (n + 1));
+    FC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int) * (n + 1), s));
+    k_oc_count<<<grid_for(o.noc), kBlock, 0, s>>>(o.noc, o.ijl.as<int>(), o.ijr.as<int>(), cnt.as<int>());
+    o.co_ptr.alloc(sizeof(int) * (n + 1));
+    scan_dev(ctx, cnt.as<int>(), o.co_ptr.as<int>(), n + 1);
+    o.co_idx.alloc(sizeof(int) * 2 * o.noc);
+    k_copy<<<grid_for(n), kBlock, 0, s>>>(o.co_ptr.as<int>(), cursor.as<int>(), n);
+    k_oc_fill<<<grid_for(o.noc), kBlock, 0, s>>>(o.noc, o.ijl.as<int>(), o.ijr.as<int>(), cursor.as<int>(),
+                                                 o.co_idx.as<int>());
+    sort_segments_dev(ctx, o.co_ptr.as<int>(), o.co_idx.as<int>(), n);
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void build_kface(fc_context *ctx, const Structure &st, int nif, const int *ij_slot, const int *ji_slot,
+                 DevBuf &kface, DevBuf &kface_order, bool &needs_order) {
+    hipStream_t s = ctx->stream;
+    kface.alloc(sizeof(int) * (size_t)st.nnz);
+    FC_HIP_CHECK(hipMemsetAsync(kface.get(), 0xff, sizeof(int) * (size_t)st.nnz, s));
+    if (nif) k_kface<<<grid_for(nif), kBlock, 0, s>>>(nif, ij_slot, ji_slot, kface.as<int>());
+    DevBuf flag(sizeof(int));
+    FC_HIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    k_face_order_check<<<grid_for(st.n), kBlock, 0, s>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(),
+                                                         kface.as<int>(), flag.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    int h = 0;
+    d2h(&h, flag.get(), sizeof(int), s);
+    needs_order = h != 0;
+    if (needs_order) {
+        kface_order.alloc(sizeof(int) * (size_t)st.nnz);
+        k_kord<<<grid_for(st.n), kBlock, 0, s>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface.as<int>(),
+                                                 kface_order.as<int>());
+        FC_HIP_CHECK(hipGetLastError());
+    } else {
+        kface_order.release();
+    }
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void laplacian_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                   const int *ij_slot, const int *ji_slot, const double *arx, const double *ary,
+                   const double *arz, const double *facint, const double *xc, const double *yc,
+                   const double *zc, const double *xf, const double *yf, const double *zf, const double *mu,
+                   const double *phi, const WallDev &w, const OcDev &oc, const int *kface_order, double *a,
+                   double *su) {
+    hipStream_t s = ctx->stream;
+    if (nif)
+        k_laplacian_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, arx, ary, arz,
+                                                           facint, xc, yc, zc, mu, a);
+    if (oc.noc)
+        k_laplacian_oc<<<grid_for(oc.noc), kBlock, 0, s>>>(oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(),
+                                                           oc.ijr.as<int>(), oc.foc.as<double>(), arx, ary, arz,
+                                                           xc, yc, zc, mu, oc.al.as<double>(), oc.ar.as<double>());
+    k_laplacian_rows<<<grid_for(st.n), kBlock, 0, s>>>(
+        st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, oc.noc ? oc.co_ptr.as<int>() : nullptr,
+        oc.noc ? oc.co_idx.as<int>() : nullptr, oc.noc ? oc.al.as<double>() : nullptr,
+        oc.noc ? oc.ar.as<double>() : nullptr, w.nwal ? w.cw_ptr.as<int>() : nullptr,
+        w.nwal ? w.cw_idx.as<int>() : nullptr, w.iwall_faces_start, w.iwall_start, arx, ary, arz, xc, yc, zc, xf,
+        yf, zf, mu, phi, a, su);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void convdiff_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                  const int *ij_slot, const int *ji_slot, const double *arx, const double *ary, const double *arz,
+                  const double *facint, const double *xc, const double *yc, const double *zc, const double *xf,
+                  const double *yf, const double *zf, const double *vis, const double *flux, const WallDev &w,
+                  double urf, double *a) {
+    hipStream_t s = ctx->stream;
+    if (nif)
+        k_convdiff_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, arx, ary, arz,
+                                                          facint, xc, yc, zc, vis, flux, a);
+    double urfr = 1.0 / urf;
+    k_convdiff_rows<<<grid_for(st.n), kBlock, 0, s>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(),
+                                                      w.nwal ? w.cw_ptr.as<int>() : nullptr,
+                                                      w.nwal ? w.cw_idx.as<int>() : nullptr, w.iwall_faces_start,
+                                                      arx, ary, arz, xc, yc, zc, xf, yf, zf, vis, urfr, a);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void face_flux_dev(fc_context *ctx, int nif, const int *owner0, const int *neigh0, const double *facint,
+                   const double *arx, const double *ary, const double *arz, const double *u, const double *v,
+                   const double *w, double *flux) {
+    if (!nif) return;
+    k_face_flux<<<grid_for(nif), kBlock, 0, ctx->stream>>>(nif, owner0, neigh0, facint, arx, ary, arz, u, v, w,
+                                                           flux);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *ij_slot, const int *kface,
+                        const int *kface_order, const double *flux, double *su) {
+    k_rhs_divergence<<<grid_for(st.n), kBlock, 0, ctx->stream>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(),
+                                                                 kface_order, kface, ij_slot, flux, su);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace fcap

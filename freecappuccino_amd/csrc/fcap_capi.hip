@@ -1,0 +1,871 @@
+// C ABI of libfcap (declared in include/fcap.h).
+//
+// Every entry point converts exceptions to an fc_status and records the
+// message in the context; nothing aborts.  Host-pointer entry points upload
+// their inputs, compute on the GPU and copy the outputs back.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fcap_internal.h"
+#include "fcap_solvers.h"
+
+using namespace fcap;
+
+namespace {
+
+template <class F>
+fc_status guarded(fc_context *ctx, F &&f) {
+    if (!ctx) return FC_ERR_ARGUMENT;
+    try {
+        FC_HIP_CHECK(hipSetDevice(ctx->device));
+        return f();
+    } catch (const Error &e) {
+        ctx->last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->last_error = e.what();
+        return FC_ERR_ALLOC;
+    }
+}
+
+// 1-based host ints -> 0-based device ints
+void upload_idx(fc_context *ctx, DevBuf &dst, const int *src1, long long count) {
+    dst.alloc(sizeof(int) * (size_t)std::max<long long>(count, 1));
+    if (count <= 0) return;
+    DevBuf tmp(sizeof(int) * (size_t)count);
+    h2d(tmp.get(), src1, sizeof(int) * (size_t)count, ctx->stream);
+    sub_one_dev(ctx, tmp.as<int>(), dst.as<int>(), count);
+    FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+void upload_dbl(fc_context *ctx, DevBuf &dst, const double *src, long long count) {
+    dst.alloc(sizeof(double) * (size_t)std::max<long long>(count, 1));
+    if (count > 0 && src) h2d(dst.get(), src, sizeof(double) * (size_t)count, ctx->stream);
+}
+
+// 0-based device ints -> 1-based host ints
+void download_idx(fc_context *ctx, int *dst1, const DevBuf &src, long long count) {
+    if (count <= 0) return;
+    DevBuf tmp(sizeof(int) * (size_t)count);
+    FC_HIP_CHECK(hipMemcpyAsync(tmp.get(), src.get(), sizeof(int) * (size_t)count, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    to_one_based(ctx, tmp.as<int>(), count);
+    d2h(dst1, tmp.get(), sizeof(int) * (size_t)count, ctx->stream);
+}
+
+void download_dbl(fc_context *ctx, double *dst, const void *src, long long count) {
+    if (count > 0 && dst) d2h(dst, src, sizeof(double) * (size_t)count, ctx->stream);
+}
+
+Workspace &workspace(fc_context *ctx, int n, int nnz) {
+    if (!ctx->ws) ctx->ws.reset(new Workspace());
+    ctx->ws->ensure(n, nnz);
+    return *ctx->ws;
+}
+
+// Cached device structure of a host CSR (n, nnz, ioffset, ja, diag checksum).
+Structure &cached_structure(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag) {
+    FC_REQUIRE(n > 0 && nnz > 0 && ioffset && ja && diag, FC_ERR_ARGUMENT, "CSR arguments");
+    uint64_t key = csr_checksum(n, nnz, ioffset, ja);
+    for (int i = 0; i < n; ++i) key = (key ^ (uint64_t)(uint32_t)diag[i]) * 1099511628211ull;
+    if (!ctx->cache || ctx->cache->key != key || ctx->cache->n != n || ctx->cache->nnz != nnz) {
+        ctx->cache.reset(new Structure());
+        build_structure_from_host_csr(ctx, *ctx->cache, n, nnz, ioffset, ja, diag);
+        ctx->cache->key = key;
+    }
+    return *ctx->cache;
+}
+
+void fortran_e10_3(char *buf, size_t len, double v) {
+    // 1PE10.3: d.dddE+xx; three-digit exponents drop the 'E' (d.ddd+xxx)
+    char tmp[64];
+    std::snprintf(tmp, sizeof tmp, "%.3E", v);
+    const char *e = std::strchr(tmp, 'E');
+    if (e && std::strlen(e + 2) >= 3) {
+        std::string m(tmp, e - tmp);
+        std::snprintf(buf, len, "%10s", (m + std::string(e + 1)).c_str());
+    } else {
+        std::snprintf(buf, len, "%10s", tmp);
+    }
+}
+
+void report_line(fc_context *ctx, const char *head, const fc_solve_info &info) {
+    if (!ctx->report) return;
+    char r0[32], rl[32];
+    fortran_e10_3(r0, sizeof r0, info.res0);
+    fortran_e10_3(rl, sizeof rl, info.status == FC_EARLY_EXIT ? info.res0 : info.resl);
+    std::printf("%sSolving for %s, Initial residual = %s, Final residual = %s, No Iterations %d\n", head,
+                ctx->report_name.c_str(), r0, rl, info.iters);
+    std::fflush(stdout);
+}
+
+struct OcHost {
+    DevBuf ijl, ijr, al, ar, co_ptr, co_idx;
+};
+
+OcView make_oc(fc_context *ctx, OcHost &h, int n, int noc, const int *ijl, const int *ijr, const double *al,
+               const double *ar) {
+    OcView v;
+    if (noc <= 0) return v;
+    FC_REQUIRE(ijl && ijr && al && ar, FC_ERR_ARGUMENT, "O-C arrays");
+    OcDev od;
+    od.noc = noc;
+    upload_idx(ctx, od.ijl, ijl, noc);
+    upload_idx(ctx, od.ijr, ijr, noc);
+    build_oc_lists(ctx, od, n);
+    h.ijl = std::move(od.ijl);
+    h.ijr = std::move(od.ijr);
+    h.co_ptr = std::move(od.co_ptr);
+    h.co_idx = std::move(od.co_idx);
+    upload_dbl(ctx, h.al, al, noc);
+    upload_dbl(ctx, h.ar, ar, noc);
+    v.noc = noc;
+    v.ijl = h.ijl.as<int>();
+    v.ijr = h.ijr.as<int>();
+    v.co_ptr = h.co_ptr.as<int>();
+    v.co_idx = h.co_idx.as<int>();
+    v.al = h.al.as<double>();
+    v.ar = h.ar.as<double>();
+    return v;
+}
+
+fc_status host_solve(fc_context *ctx, int kind, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                     const double *a, const double *su, double *fi, double *res, int noc, const int *ijl,
+                     const int *ijr, const double *al, const double *ar, double sor, int nsw,
+                     fc_solve_info *info) {
+    FC_REQUIRE(a && su && fi, FC_ERR_ARGUMENT, "a/su/fi");
+    Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+    if (kind != 2) ensure_levels(ctx, st);
+    Workspace &ws = workspace(ctx, n, nnz);
+    DevBuf da, dsu;
+    upload_dbl(ctx, da, a, nnz);
+    upload_dbl(ctx, dsu, su, n);
+    h2d(ws.x.get(), fi, sizeof(double) * n, ctx->stream);
+    OcHost och;
+    SolveArgs args;
+    args.st = &st;
+    args.a = da.as<double>();
+    args.su = dsu.as<double>();
+    args.fi = ws.x.as<double>();
+    args.oc = make_oc(ctx, och, n, ctx->variant == FC_VARIANT_TESTS ? 0 : noc, ijl, ijr, al, ar);
+    args.sor = sor;
+    args.nsw = nsw;
+    args.variant = ctx->variant;
+    fc_solve_info local;
+    std::memset(&local, 0, sizeof local);
+    fc_status s;
+    if (kind == 1) s = solve_pcg64(ctx, args, true, &local);
+    else if (kind == 2) s = solve_pcg64(ctx, args, false, &local);
+    else s = solve_bicgstab64(ctx, args, &local);
+    local.rel_l2 = true_rel_l2(ctx, args);
+    d2h(fi, ws.x.get(), sizeof(double) * n, ctx->stream);
+    if (res) d2h(res, ws.r.get(), sizeof(double) * n, ctx->stream);
+    const char *head = kind == 1 ? "  PCG(IC0):  " : (kind == 2 ? "PCG(Jacobi):  " : "  BiCGStab(ILU(0)):  ");
+    if (local.status == FC_EARLY_EXIT && kind == 2) head = "  PCG(Jacobi):  ";
+    report_line(ctx, head, local);
+    if (info) *info = local;
+    return s;
+}
+
+}  // namespace
+
+// =========================================================== context
+
+extern "C" fc_status fc_init(fc_context **out, int device) {
+    if (!out) return FC_ERR_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FC_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return FC_ERR_ARGUMENT;
+    fc_context *ctx = new fc_context();
+    ctx->device = device;
+    fc_status s = guarded(ctx, [&]() {
+        FC_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->own_stream = true;
+        FC_HIP_CHECK(hipHostMalloc((void **)&ctx->host_flags, sizeof(int) * 16, hipHostMallocDefault));
+        hipDeviceProp_t prop;
+        FC_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        ctx->num_cus = prop.multiProcessorCount;
+        return FC_OK;
+    });
+    if (s != FC_OK) {
+        fc_finalize(ctx);
+        return s;
+    }
+    *out = ctx;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_finalize(fc_context *ctx) {
+    if (!ctx) return FC_ERR_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->cache.reset();
+    ctx->ws.reset();
+    ctx->timer.destroy();
+    if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_stream(fc_context *ctx, void *stream) {
+    return guarded(ctx, [&]() {
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->own_stream) FC_HIP_CHECK(hipStreamDestroy(ctx->stream));
+        if (stream) {
+            ctx->stream = (hipStream_t)stream;
+            ctx->own_stream = false;
+        } else {
+            FC_HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            ctx->own_stream = true;
+        }
+        return FC_OK;
+    });
+}
+
+extern "C" const char *fc_last_error(const fc_context *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+extern "C" fc_status fc_set_report(fc_context *ctx, int on, const char *name) {
+    if (!ctx) return FC_ERR_ARGUMENT;
+    ctx->report = on;
+    if (name) ctx->report_name = name;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_variant(fc_context *ctx, int variant) {
+    if (!ctx || (variant != FC_VARIANT_SRC && variant != FC_VARIANT_TESTS)) return FC_ERR_ARGUMENT;
+    ctx->variant = variant;
+    return FC_OK;
+}
+
+// ======================================================= mesh and CSR
+
+extern "C" fc_status fc_mesh_geometry(fc_context *ctx, int numNodes, const double *xyz, int numFaces,
+                                      int numInnerFaces, const int *face_ptr, const int *face_nodes, const int *owner,
+                                      const int *neighbour, int numCells, double *arx, double *ary, double *arz,
+                                      double *xf, double *yf, double *zf, double *vol, double *xc, double *yc,
+                                      double *zc, double *facint) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(xyz && face_ptr && face_nodes && owner && (neighbour || numInnerFaces == 0), FC_ERR_ARGUMENT,
+                   "mesh arrays");
+        MeshDev m;
+        upload_mesh(ctx, m, numNodes, xyz, numFaces, numInnerFaces, face_ptr, face_nodes, owner, neighbour, numCells);
+        compute_geometry(ctx, m);
+        download_dbl(ctx, arx, m.g.arx.get(), numFaces);
+        download_dbl(ctx, ary, m.g.ary.get(), numFaces);
+        download_dbl(ctx, arz, m.g.arz.get(), numFaces);
+        download_dbl(ctx, xf, m.g.xf.get(), numFaces);
+        download_dbl(ctx, yf, m.g.yf.get(), numFaces);
+        download_dbl(ctx, zf, m.g.zf.get(), numFaces);
+        download_dbl(ctx, vol, m.g.vol.get(), numCells);
+        download_dbl(ctx, xc, m.g.xc.get(), numCells);
+        download_dbl(ctx, yc, m.g.yc.get(), numCells);
+        download_dbl(ctx, zc, m.g.zc.get(), numCells);
+        download_dbl(ctx, facint, m.g.facint.get(), numInnerFaces);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_csr_build(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                  const int *neighbour, int *ioffset, int *ja, int *diag, int *ij_slot,
+                                  int *ji_slot) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(numCells > 0 && numInnerFaces >= 0, FC_ERR_ARGUMENT, "sizes");
+        FC_REQUIRE(ioffset && ja && diag, FC_ERR_ARGUMENT, "output arrays");
+        DevBuf own, nei, ij, ji;
+        upload_idx(ctx, own, owner, numInnerFaces);
+        upload_idx(ctx, nei, neighbour, numInnerFaces);
+        ij.alloc(sizeof(int) * std::max(numInnerFaces, 1));
+        ji.alloc(sizeof(int) * std::max(numInnerFaces, 1));
+        Structure st;
+        build_structure_from_faces_dev(ctx, st, numCells, numInnerFaces, own.as<int>(), nei.as<int>(), ij.as<int>(),
+                                       ji.as<int>());
+        download_idx(ctx, ioffset, st.ioffset, numCells + 1);
+        download_idx(ctx, ja, st.ja, st.nnz);
+        download_idx(ctx, diag, st.diag, numCells);
+        if (ij_slot) download_idx(ctx, ij_slot, ij, numInnerFaces);
+        if (ji_slot) download_idx(ctx, ji_slot, ji, numInnerFaces);
+        return FC_OK;
+    });
+}
+
+// ============================================================ assembly
+
+namespace {
+
+struct FaceSet {
+    DevBuf own, nei, ij, ji, kface, kord;
+    bool needs_order = false;
+};
+
+// Upload owner/neighbour and slot maps of the inner faces and derive kface.
+void upload_faces(fc_context *ctx, FaceSet &fs, const Structure &st, int nif, const int *owner, const int *neighbour,
+                  const int *ij_slot, const int *ji_slot) {
+    upload_idx(ctx, fs.own, owner, nif);
+    upload_idx(ctx, fs.nei, neighbour, nif);
+    upload_idx(ctx, fs.ij, ij_slot, nif);
+    upload_idx(ctx, fs.ji, ji_slot, nif);
+    validate_faces(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.ji.as<int>());
+    build_kface(ctx, st, nif, fs.ij.as<int>(), fs.ji.as<int>(), fs.kface, fs.kord, fs.needs_order);
+}
+
+}  // namespace
+
+extern "C" fc_status fc_laplacian(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                  const int *neighbour, const double *arx, const double *ary, const double *arz,
+                                  const double *facint, const double *xc, const double *yc, const double *zc,
+                                  const double *xf, const double *yf, const double *zf, const double *mu,
+                                  const double *phi, int numTotal, int nwal, int iWallFacesStart, int iWallStart,
+                                  int noc, int iOCFacesStart, const int *ijl, const int *ijr, const double *foc,
+                                  const int *ioffset, const int *ja, const int *diag, const int *ij_slot,
+                                  const int *ji_slot, int nnz, double *a, double *su, double *al, double *ar) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n, FC_ERR_ARGUMENT, "nnz must be 2*numInnerFaces+numCells");
+        FC_REQUIRE(a && su && mu && phi, FC_ERR_ARGUMENT, "a/su/mu/phi");
+        const int nfaces_needed = std::max({nif, nwal > 0 ? iWallFacesStart + nwal : 0,
+                                            noc > 0 ? iOCFacesStart + noc : 0});
+        FC_REQUIRE(numTotal >= n && (nwal <= 0 || iWallStart + nwal <= numTotal), FC_ERR_ARGUMENT, "numTotal");
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        FaceSet fs;
+        upload_faces(ctx, fs, st, nif, owner, neighbour, ij_slot, ji_slot);
+        // geometry: face arrays cover every face the loops touch
+        DevBuf dx[6], dc[3], dfi, dmu, dphi, dsu, da;
+        const double *fa[6] = {arx, ary, arz, xf, yf, zf};
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nfaces_needed);
+        const double *ca[3] = {xc, yc, zc};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dc[k], ca[k], n);
+        upload_dbl(ctx, dfi, facint, nif);
+        upload_dbl(ctx, dmu, mu, n);
+        upload_dbl(ctx, dphi, phi, numTotal);
+        upload_dbl(ctx, dsu, su, n);
+        da.alloc(sizeof(double) * nnz);
+        FC_HIP_CHECK(hipMemsetAsync(da.get(), 0, sizeof(double) * nnz, ctx->stream));   // a = 0 (fvm_laplacian.f90:33)
+        WallDev w;
+        if (nwal > 0) {
+            DevBuf owner_all;
+            upload_idx(ctx, owner_all, owner, iWallFacesStart + nwal);
+            build_wall_lists(ctx, w, owner_all.as<int>(), n, nwal, iWallFacesStart, iWallStart);
+        }
+        OcDev oc;
+        oc.noc = std::max(noc, 0);
+        oc.ioc_faces_start = iOCFacesStart;
+        if (noc > 0) {
+            FC_REQUIRE(ijl && ijr && foc && al && ar, FC_ERR_ARGUMENT, "O-C arrays");
+            upload_idx(ctx, oc.ijl, ijl, noc);
+            upload_idx(ctx, oc.ijr, ijr, noc);
+            upload_dbl(ctx, oc.foc, foc, noc);
+            oc.al.alloc(sizeof(double) * noc);
+            oc.ar.alloc(sizeof(double) * noc);
+            build_oc_lists(ctx, oc, n);
+        }
+        laplacian_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.ji.as<int>(),
+                      dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), dfi.as<double>(),
+                      dc[0].as<double>(), dc[1].as<double>(), dc[2].as<double>(), dx[3].as<double>(),
+                      dx[4].as<double>(), dx[5].as<double>(), dmu.as<double>(), dphi.as<double>(), w, oc,
+                      fs.needs_order ? fs.kord.as<int>() : nullptr, da.as<double>(), dsu.as<double>());
+        download_dbl(ctx, a, da.get(), nnz);
+        download_dbl(ctx, su, dsu.get(), n);
+        if (noc > 0) {
+            download_dbl(ctx, al, oc.al.get(), noc);
+            download_dbl(ctx, ar, oc.ar.get(), noc);
+        }
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_convdiff(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                 const int *neighbour, const double *arx, const double *ary, const double *arz,
+                                 const double *facint, const double *xc, const double *yc, const double *zc,
+                                 const double *xf, const double *yf, const double *zf, const double *vis,
+                                 const double *flux, int nwal, int iWallFacesStart, const int *ioffset,
+                                 const int *ja, const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                                 double urf, double *a) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n && urf > 0.0, FC_ERR_ARGUMENT, "sizes/urf");
+        FC_REQUIRE(a && vis && flux, FC_ERR_ARGUMENT, "a/vis/flux");
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        FaceSet fs;
+        upload_faces(ctx, fs, st, nif, owner, neighbour, ij_slot, ji_slot);
+        const int nfaces_needed = std::max(nif, nwal > 0 ? iWallFacesStart + nwal : 0);
+        DevBuf dx[6], dc[3], dfi, dvis, dflux, da;
+        const double *fa[6] = {arx, ary, arz, xf, yf, zf};
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nfaces_needed);
+        const double *ca[3] = {xc, yc, zc};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dc[k], ca[k], n);
+        upload_dbl(ctx, dfi, facint, nif);
+        upload_dbl(ctx, dvis, vis, n);
+        upload_dbl(ctx, dflux, flux, nif);
+        da.alloc(sizeof(double) * nnz);
+        FC_HIP_CHECK(hipMemsetAsync(da.get(), 0, sizeof(double) * nnz, ctx->stream));   // a = 0 (fvm_laplacian.f90:33)
+        WallDev w;
+        if (nwal > 0) {
+            DevBuf owner_all;
+            upload_idx(ctx, owner_all, owner, iWallFacesStart + nwal);
+            build_wall_lists(ctx, w, owner_all.as<int>(), n, nwal, iWallFacesStart, 0);
+        }
+        convdiff_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.ji.as<int>(),
+                     dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), dfi.as<double>(), dc[0].as<double>(),
+                     dc[1].as<double>(), dc[2].as<double>(), dx[3].as<double>(), dx[4].as<double>(),
+                     dx[5].as<double>(), dvis.as<double>(), dflux.as<double>(), w, urf, da.as<double>());
+        download_dbl(ctx, a, da.get(), nnz);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_face_flux_central(fc_context *ctx, int numInnerFaces, const int *owner, const int *neighbour,
+                                          const double *facint, const double *arx, const double *ary,
+                                          const double *arz, const double *u, const double *v, const double *w,
+                                          int numCells, double *flux) {
+    return guarded(ctx, [&]() {
+        const int nif = numInnerFaces;
+        FC_REQUIRE(nif >= 0 && numCells > 0 && flux, FC_ERR_ARGUMENT, "sizes");
+        DevBuf own, nei, dfi, dax, day, daz, du, dv, dw, dfl;
+        upload_idx(ctx, own, owner, nif);
+        upload_idx(ctx, nei, neighbour, nif);
+        upload_dbl(ctx, dfi, facint, nif);
+        upload_dbl(ctx, dax, arx, nif);
+        upload_dbl(ctx, day, ary, nif);
+        upload_dbl(ctx, daz, arz, nif);
+        upload_dbl(ctx, du, u, numCells);
+        upload_dbl(ctx, dv, v, numCells);
+        upload_dbl(ctx, dw, w, numCells);
+        dfl.alloc(sizeof(double) * std::max(nif, 1));
+        face_flux_dev(ctx, nif, own.as<int>(), nei.as<int>(), dfi.as<double>(), dax.as<double>(), day.as<double>(),
+                      daz.as<double>(), du.as<double>(), dv.as<double>(), dw.as<double>(), dfl.as<double>());
+        download_dbl(ctx, flux, dfl.get(), nif);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_rhs_divergence(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                       const int *neighbour, const double *flux, double *su) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && su && flux, FC_ERR_ARGUMENT, "sizes");
+        DevBuf own, nei, ij, ji, dfl, dsu, kface, kord;
+        upload_idx(ctx, own, owner, nif);
+        upload_idx(ctx, nei, neighbour, nif);
+        ij.alloc(sizeof(int) * std::max(nif, 1));
+        ji.alloc(sizeof(int) * std::max(nif, 1));
+        Structure st;
+        build_structure_from_faces_dev(ctx, st, n, nif, own.as<int>(), nei.as<int>(), ij.as<int>(), ji.as<int>());
+        bool needs = false;
+        build_kface(ctx, st, nif, ij.as<int>(), ji.as<int>(), kface, kord, needs);
+        upload_dbl(ctx, dfl, flux, nif);
+        upload_dbl(ctx, dsu, su, n);
+        rhs_divergence_dev(ctx, st, ij.as<int>(), kface.as<int>(), needs ? kord.as<int>() : nullptr,
+                           dfl.as<double>(), dsu.as<double>());
+        download_dbl(ctx, su, dsu.get(), n);
+        return FC_OK;
+    });
+}
+
+// ============================================================= kernels
+
+extern "C" fc_status fc_spmv(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const double *a,
+                             const double *x, double *y) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(n > 0 && nnz > 0 && ioffset && ja && a && x && y, FC_ERR_ARGUMENT, "spmv arguments");
+        FC_REQUIRE(ioffset[0] == 1 && ioffset[n] == nnz + 1, FC_ERR_STRUCTURE, "ioffset");
+        DevBuf dp, dj, da, dx, dy;
+        upload_idx(ctx, dp, ioffset, n + 1);
+        upload_idx(ctx, dj, ja, nnz);
+        upload_dbl(ctx, da, a, nnz);
+        upload_dbl(ctx, dx, x, n);
+        dy.alloc(sizeof(double) * n);
+        spmv_launch64(ctx->stream, 0, n, dp.as<int>(), dj.as<int>(), da.as<double>(), dx.as<double>(), nullptr,
+                      nullptr, dy.as<double>(), nullptr, nullptr, nullptr, nullptr);
+        FC_HIP_CHECK(hipGetLastError());
+        download_dbl(ctx, y, dy.get(), n);
+        return FC_OK;
+    });
+}
+
+static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const int *ioffset, const int *ja,
+                               const int *diag, const double *a, double *d) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(a && d, FC_ERR_ARGUMENT, "a/d");
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        ensure_levels(ctx, st);
+        if (dilu) build_tpos(ctx, st);
+        DevBuf da, dd, err(sizeof(int));
+        upload_dbl(ctx, da, a, nnz);
+        dd.alloc(sizeof(double) * n);
+        FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx->stream));
+        pivots_launch(ctx->stream, st, da.as<double>(), dd.as<double>(), dilu, err.as<int>());
+        int h = 0;
+        d2h(&h, err.get(), sizeof(int), ctx->stream);
+        download_dbl(ctx, d, dd.get(), n);
+        return h ? FC_ERR_ZERO_PIVOT : FC_OK;
+    });
+}
+
+extern "C" fc_status fc_dic_pivots(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                                   const int *diag, const double *a, double *d) {
+    return pivots_common(ctx, false, n, nnz, ioffset, ja, diag, a, d);
+}
+
+extern "C" fc_status fc_dilu_pivots(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                                    const int *diag, const double *a, double *d) {
+    return pivots_common(ctx, true, n, nnz, ioffset, ja, diag, a, d);
+}
+
+extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                                 const double *a, const double *d, const double *r, double *z) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(a && d && r && z, FC_ERR_ARGUMENT, "a/d/r/z");
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        ensure_levels(ctx, st);
+        DevBuf da, dd, dr, dz;
+        upload_dbl(ctx, da, a, nnz);
+        upload_dbl(ctx, dd, d, n);
+        upload_dbl(ctx, dr, r, n);
+        dz.alloc(sizeof(double) * n);
+        const double small = ctx->variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+        lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
+                    nullptr);
+        FC_HIP_CHECK(hipGetLastError());
+        download_dbl(ctx, z, dz.get(), n);
+        return FC_OK;
+    });
+}
+
+// ============================================================= solvers
+
+extern "C" fc_status fc_iccg(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                             const double *a, const double *su, double *fi, double *res, int noc, const int *ijl,
+                             const int *ijr, const double *al, const double *ar, double sor, int nsw,
+                             fc_solve_info *info) {
+    return guarded(ctx, [&]() {
+        return host_solve(ctx, 1, n, nnz, ioffset, ja, diag, a, su, fi, res, noc, ijl, ijr, al, ar, sor, nsw, info);
+    });
+}
+
+extern "C" fc_status fc_dpcg(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                             const double *a, const double *su, double *fi, double *res, int noc, const int *ijl,
+                             const int *ijr, const double *al, const double *ar, double sor, int nsw,
+                             fc_solve_info *info) {
+    return guarded(ctx, [&]() {
+        return host_solve(ctx, 2, n, nnz, ioffset, ja, diag, a, su, fi, res, noc, ijl, ijr, al, ar, sor, nsw, info);
+    });
+}
+
+extern "C" fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                                 const int *diag, const double *a, const double *su, double *fi, double *res,
+                                 int noc, const int *ijl, const int *ijr, const double *al, const double *ar,
+                                 double sor, int nsw, fc_solve_info *info) {
+    return guarded(ctx, [&]() {
+        return host_solve(ctx, 3, n, nnz, ioffset, ja, diag, a, su, fi, res, noc, ijl, ijr, al, ar, sor, nsw, info);
+    });
+}
+
+// ====================================================== resident system
+
+struct fc_system {
+    fc_context *ctx = nullptr;
+    MeshDev mesh;
+    Structure st;
+    DevBuf ij, ji, kface, kord;
+    bool needs_order = false;
+    WallDev walls;
+    OcDev oc;
+    int ntotal = 0;
+    DevBuf a, su, fi, mu, phi, su0, vis, flux;
+    bool su0_zero = true;
+    double last_ms[3] = {0, 0, 0};
+};
+
+extern "C" fc_status fc_sys_create(fc_context *ctx, fc_system **out, int numNodes, const double *xyz, int numFaces,
+                                   int numInnerFaces, const int *face_ptr, const int *face_nodes, const int *owner,
+                                   const int *neighbour, int numCells, int nwal, int iWallFacesStart,
+                                   int iWallStart) {
+    if (!out) return FC_ERR_ARGUMENT;
+    *out = nullptr;
+    fc_system *sys = new fc_system();
+    sys->ctx = ctx;
+    fc_status s = guarded(ctx, [&]() {
+        FC_REQUIRE(nwal >= 0 && (nwal == 0 || iWallFacesStart + nwal <= numFaces), FC_ERR_ARGUMENT, "wall range");
+        upload_mesh(ctx, sys->mesh, numNodes, xyz, numFaces, numInnerFaces, face_ptr, face_nodes, owner, neighbour,
+                    numCells);
+        compute_geometry(ctx, sys->mesh);
+        const int nif = numInnerFaces, n = numCells;
+        sys->ij.alloc(sizeof(int) * std::max(nif, 1));
+        sys->ji.alloc(sizeof(int) * std::max(nif, 1));
+        build_structure_from_faces_dev(ctx, sys->st, n, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(),
+                                       sys->ij.as<int>(), sys->ji.as<int>());
+        ensure_levels(ctx, sys->st);
+        build_kface(ctx, sys->st, nif, sys->ij.as<int>(), sys->ji.as<int>(), sys->kface, sys->kord,
+                    sys->needs_order);
+        build_wall_lists(ctx, sys->walls, sys->mesh.owner.as<int>(), n, nwal, iWallFacesStart, iWallStart);
+        sys->ntotal = n + (numFaces - nif);
+        sys->a.alloc(sizeof(double) * sys->st.nnz);
+        sys->su.alloc(sizeof(double) * n);
+        sys->fi.alloc(sizeof(double) * n);
+        FC_HIP_CHECK(hipMemsetAsync(sys->fi.get(), 0, sizeof(double) * n, ctx->stream));
+        FC_HIP_CHECK(hipMemsetAsync(sys->su.get(), 0, sizeof(double) * n, ctx->stream));
+        workspace(ctx, n, sys->st.nnz);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+    if (s != FC_OK) {
+        delete sys;
+        return s;
+    }
+    *out = sys;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_sys_destroy(fc_system *sys) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    if (sys->ctx) (void)hipStreamSynchronize(sys->ctx->stream);
+    delete sys;
+    return FC_OK;
+}
+
+static void sys_assemble_laplacian(fc_system *sys) {
+    fc_context *ctx = sys->ctx;
+    const int n = sys->st.n;
+    GeometryDev &g = sys->mesh.g;
+    if (sys->su0_zero) FC_HIP_CHECK(hipMemsetAsync(sys->su.get(), 0, sizeof(double) * n, ctx->stream));
+    else FC_HIP_CHECK(hipMemcpyAsync(sys->su.get(), sys->su0.get(), sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+    laplacian_dev(ctx, sys->st, sys->mesh.nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(),
+                  sys->ij.as<int>(), sys->ji.as<int>(), g.arx.as<double>(), g.ary.as<double>(), g.arz.as<double>(),
+                  g.facint.as<double>(), g.xc.as<double>(), g.yc.as<double>(), g.zc.as<double>(), g.xf.as<double>(),
+                  g.yf.as<double>(), g.zf.as<double>(), sys->mu.as<double>(), sys->phi.as<double>(), sys->walls,
+                  sys->oc, sys->needs_order ? sys->kord.as<int>() : nullptr, sys->a.as<double>(),
+                  sys->su.as<double>());
+}
+
+extern "C" fc_status fc_sys_assemble_laplacian(fc_system *sys, const double *mu, const double *phi,
+                                               const double *su0) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(mu && phi, FC_ERR_ARGUMENT, "mu/phi");
+        const int n = sys->st.n;
+        upload_dbl(ctx, sys->mu, mu, n);
+        upload_dbl(ctx, sys->phi, phi, sys->ntotal);
+        sys->su0_zero = su0 == nullptr;
+        if (su0) upload_dbl(ctx, sys->su0, su0, n);
+        ctx->timer.start(ctx->stream);
+        sys_assemble_laplacian(sys);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_assemble_laplacian_dev(fc_system *sys) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(sys->mu.get() && sys->phi.get(), FC_ERR_ARGUMENT, "no coefficients uploaded yet");
+        ctx->timer.start(ctx->stream);
+        sys_assemble_laplacian(sys);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis, const double *flux, double urf,
+                                              const double *su0) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(vis && flux && urf > 0.0, FC_ERR_ARGUMENT, "vis/flux/urf");
+        const int n = sys->st.n, nif = sys->mesh.nif;
+        upload_dbl(ctx, sys->vis, vis, n);
+        upload_dbl(ctx, sys->flux, flux, nif);
+        GeometryDev &g = sys->mesh.g;
+        ctx->timer.start(ctx->stream);
+        convdiff_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(), sys->ij.as<int>(),
+                     sys->ji.as<int>(), g.arx.as<double>(), g.ary.as<double>(), g.arz.as<double>(),
+                     g.facint.as<double>(), g.xc.as<double>(), g.yc.as<double>(), g.zc.as<double>(),
+                     g.xf.as<double>(), g.yf.as<double>(), g.zf.as<double>(), sys->vis.as<double>(),
+                     sys->flux.as<double>(), sys->walls, urf, sys->a.as<double>());
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        if (su0) h2d(sys->su.get(), su0, sizeof(double) * n, ctx->stream);
+        else FC_HIP_CHECK(hipMemsetAsync(sys->su.get(), 0, sizeof(double) * n, ctx->stream));
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_set_rhs(fc_system *sys, const double *su) {
+    if (!sys || !su) return FC_ERR_ARGUMENT;
+    return guarded(sys->ctx, [&]() {
+        h2d(sys->su.get(), su, sizeof(double) * sys->st.n, sys->ctx->stream);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_set_solution(fc_system *sys, const double *fi) {
+    if (!sys || !fi) return FC_ERR_ARGUMENT;
+    return guarded(sys->ctx, [&]() {
+        h2d(sys->fi.get(), fi, sizeof(double) * sys->st.n, sys->ctx->stream);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_get_solution(fc_system *sys, double *fi) {
+    if (!sys || !fi) return FC_ERR_ARGUMENT;
+    return guarded(sys->ctx, [&]() {
+        d2h(fi, sys->fi.get(), sizeof(double) * sys->st.n, sys->ctx->stream);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, double sor, int nsw,
+                                  fc_solve_info *info) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(nsw > 0 && sor > 0.0, FC_ERR_ARGUMENT, "sor/nsw");
+        SolveArgs args;
+        args.st = &sys->st;
+        args.a = sys->a.as<double>();
+        args.su = sys->su.as<double>();
+        args.fi = sys->fi.as<double>();
+        args.sor = sor;
+        args.nsw = nsw;
+        args.variant = FC_VARIANT_SRC;
+        args.stop_rule = stop_rule;
+        fc_solve_info local;
+        std::memset(&local, 0, sizeof local);
+        fc_status s;
+        switch (method) {
+        case FC_METHOD_ICCG: s = solve_pcg64(ctx, args, true, &local); break;
+        case FC_METHOD_DPCG: s = solve_pcg64(ctx, args, false, &local); break;
+        case FC_METHOD_BICGSTAB: s = solve_bicgstab64(ctx, args, &local); break;
+        case FC_METHOD_ICCG_MIXED: s = solve_mixed(ctx, args, false, &local); break;
+        case FC_METHOD_BICGSTAB_MIXED: s = solve_mixed(ctx, args, true, &local); break;
+        default: throw Error(FC_ERR_ARGUMENT, "unknown method");
+        }
+        local.rel_l2 = true_rel_l2(ctx, args);
+        sys->last_ms[1] = local.solve_ms + local.setup_ms;
+        if (info) *info = local;
+        return s;
+    });
+}
+
+extern "C" void *fc_sys_device_ptr(fc_system *sys, int which) {
+    if (!sys) return nullptr;
+    GeometryDev &g = sys->mesh.g;
+    switch (which) {
+    case FC_ARR_A: return sys->a.get();
+    case FC_ARR_SU: return sys->su.get();
+    case FC_ARR_IOFFSET: return sys->st.ioffset.get();
+    case FC_ARR_JA: return sys->st.ja.get();
+    case FC_ARR_DIAG: return sys->st.diag.get();
+    case FC_ARR_IJ_SLOT: return sys->ij.get();
+    case FC_ARR_JI_SLOT: return sys->ji.get();
+    case FC_ARR_VOL: return g.vol.get();
+    case FC_ARR_XC: return g.xc.get();
+    case FC_ARR_YC: return g.yc.get();
+    case FC_ARR_ZC: return g.zc.get();
+    case FC_ARR_FACINT: return g.facint.get();
+    case FC_ARR_ARX: return g.arx.get();
+    case FC_ARR_ARY: return g.ary.get();
+    case FC_ARR_ARZ: return g.arz.get();
+    case FC_ARR_XF: return g.xf.get();
+    case FC_ARR_YF: return g.yf.get();
+    case FC_ARR_ZF: return g.zf.get();
+    case FC_ARR_RES: return sys->ctx->ws ? sys->ctx->ws->r.get() : nullptr;
+    case FC_ARR_PIVOTS: return sys->ctx->ws ? sys->ctx->ws->d.get() : nullptr;
+    default: return nullptr;
+    }
+}
+
+extern "C" fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_t bytes) {
+    if (!sys || !host_out) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        const int n = sys->st.n, nnz = sys->st.nnz, nif = sys->mesh.nif, nf = sys->mesh.nfaces;
+        size_t expect = 0;
+        bool index = false;
+        switch (which) {
+        case FC_ARR_A: expect = sizeof(double) * nnz; break;
+        case FC_ARR_SU: case FC_ARR_VOL: case FC_ARR_XC: case FC_ARR_YC: case FC_ARR_ZC: case FC_ARR_RES:
+        case FC_ARR_PIVOTS: expect = sizeof(double) * n; break;
+        case FC_ARR_FACINT: expect = sizeof(double) * nif; break;
+        case FC_ARR_ARX: case FC_ARR_ARY: case FC_ARR_ARZ: case FC_ARR_XF: case FC_ARR_YF: case FC_ARR_ZF:
+            expect = sizeof(double) * nf; break;
+        case FC_ARR_IOFFSET: expect = sizeof(int) * (n + 1); index = true; break;
+        case FC_ARR_JA: expect = sizeof(int) * nnz; index = true; break;
+        case FC_ARR_DIAG: expect = sizeof(int) * n; index = true; break;
+        case FC_ARR_IJ_SLOT: case FC_ARR_JI_SLOT: expect = sizeof(int) * nif; index = true; break;
+        default: throw Error(FC_ERR_ARGUMENT, "unknown array");
+        }
+        FC_REQUIRE(bytes >= expect, FC_ERR_ARGUMENT, "output buffer too small");
+        void *src = fc_sys_device_ptr(sys, which);
+        FC_REQUIRE(src != nullptr, FC_ERR_ARGUMENT, "array not available");
+        if (index) {
+            DevBuf tmp(expect);
+            FC_HIP_CHECK(hipMemcpyAsync(tmp.get(), src, expect, hipMemcpyDeviceToDevice, ctx->stream));
+            to_one_based(ctx, tmp.as<int>(), (long long)(expect / sizeof(int)));
+            d2h(host_out, tmp.get(), expect, ctx->stream);
+        } else {
+            d2h(host_out, src, expect, ctx->stream);
+        }
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_sizes(fc_system *sys, int *n, int *nif, int *nnz, int *nlev_fwd, int *nlev_bwd) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    if (n) *n = sys->st.n;
+    if (nif) *nif = sys->mesh.nif;
+    if (nnz) *nnz = sys->st.nnz;
+    if (nlev_fwd) *nlev_fwd = sys->st.nlev_f;
+    if (nlev_bwd) *nlev_bwd = sys->st.nlev_b;
+    return FC_OK;
+}
+
+extern "C" double fc_sys_last_ms(fc_system *sys, int stage) {
+    if (!sys || stage < 0 || stage > 2) return -1.0;
+    return sys->last_ms[stage];
+}
+
+extern "C" double fc_sys_spmv_bench(fc_system *sys, int precision, int reps) {
+    if (!sys || reps <= 0 || (precision != 32 && precision != 64)) return -1.0;
+    fc_context *ctx = sys->ctx;
+    double out = -1.0;
+    fc_status s = guarded(ctx, [&]() {
+        Workspace &ws = workspace(ctx, sys->st.n, sys->st.nnz);
+        const Structure &st = sys->st;
+        const int n = st.n;
+        hipStream_t s = ctx->stream;
+        const double *x64 = sys->su.as<double>();
+        float *a32 = ws.a32.as<float>(), *x32 = ws.p32.as<float>(), *y32 = ws.q32.as<float>();
+        if (precision == 32) {
+            cast_d2f(s, st.nnz, sys->a.as<double>(), a32);
+            cast_d2f(s, n, x64, x32);
+        }
+        // warm-up
+        if (precision == 64)
+            spmv_launch64(s, 0, n, st.ioffset.as<int>(), st.ja.as<int>(), sys->a.as<double>(), x64, nullptr, nullptr,
+                          ws.q.as<double>(), nullptr, nullptr, nullptr, nullptr);
+        else
+            spmv_launch32(s, 0, n, st.ioffset.as<int>(), st.ja.as<int>(), a32, x32, nullptr, nullptr, y32, nullptr,
+                          nullptr, nullptr);
+        ctx->timer.start(s);
+        for (int r = 0; r < reps; ++r) {
+            if (precision == 64)
+                spmv_launch64(s, 0, n, st.ioffset.as<int>(), st.ja.as<int>(), sys->a.as<double>(), x64, nullptr,
+                              nullptr, ws.q.as<double>(), nullptr, nullptr, nullptr, nullptr);
+            else
+                spmv_launch32(s, 0, n, st.ioffset.as<int>(), st.ja.as<int>(), a32, x32, nullptr, nullptr, y32,
+                              nullptr, nullptr, nullptr);
+        }
+        FC_HIP_CHECK(hipGetLastError());
+        double ms = ctx->timer.stop_ms(s);
+        out = ms / reps;
+        sys->last_ms[2] = out;
+        return FC_OK;
+    });
+    return s == FC_OK ? out : -1.0;
+}

@@ -1,0 +1,215 @@
+// Internal declarations of libfcap: context, device buffers, error plumbing,
+// kernel launchers shared between translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fcap.h"
+
+namespace fcap {
+
+// Constants written as default-REAL literals in the reference and widened to
+// real(dp): small = 1e-20 (src/modules_allocatable.f90:27), tol = 1e-13
+// (src/iccg.f90:37), s0 = 1.e20 (src/iccg.f90:85), tiny = 1e-30
+// (src/mesh_geometry_and_topology.f90:58).
+constexpr double kSmallSrc = (double)1e-20f;
+constexpr double kSmallTests = (double)1e-30f;
+constexpr double kTol = (double)1e-13f;
+constexpr double kS0 = (double)1e20f;
+constexpr double kTiny = (double)1e-30f;
+
+struct Error : std::runtime_error {
+    fc_status code;
+    Error(fc_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define FC_HIP_CHECK(call)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::fcap::Error(FC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + \
+                                                " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define FC_REQUIRE(cond, code, msg)                                                        \
+    do {                                                                                   \
+        if (!(cond)) throw ::fcap::Error(code, msg);                                       \
+    } while (0)
+
+// Owning device allocation.
+class DevBuf {
+  public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; o.bytes_ = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p_ = o.p_; bytes_ = o.bytes_; o.p_ = nullptr; o.bytes_ = 0; }
+        return *this;
+    }
+    void alloc(size_t bytes) {
+        release();
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(&p_, bytes);
+        if (e != hipSuccess) {
+            p_ = nullptr;
+            throw Error(FC_ERR_ALLOC, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        }
+        bytes_ = bytes;
+    }
+    // Grow-only reallocation (contents not preserved).
+    void ensure(size_t bytes) { if (bytes > bytes_ || !p_) alloc(bytes); }
+    void release() { if (p_) { (void)hipFree(p_); p_ = nullptr; bytes_ = 0; } }
+    template <class T> T *as() const { return static_cast<T *>(p_); }
+    void *get() const { return p_; }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    void *p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Upload helpers (synchronous w.r.t. the stream: callers own the host arrays).
+void h2d(void *dst, const void *src, size_t bytes, hipStream_t s);
+void d2h(void *dst, const void *src, size_t bytes, hipStream_t s);
+
+// Device CSR structure with the analysis needed by the solvers.
+struct Structure {
+    int n = 0, nnz = 0;
+    uint64_t key = 0;           // checksum of (n, nnz, ioffset, ja)
+    DevBuf ioffset, ja, diag;   // 0-based on the device
+    // level schedules of the forward (strict lower) and backward (strict upper) DAGs
+    int nlev_f = 0, nlev_b = 0;
+    DevBuf rows_f, rows_b;      // rows in level order
+    std::vector<int> lvl_f, lvl_b;   // host copy of the level pointers (nlev+1)
+    int max_level_f = 0, max_level_b = 0; // largest level size
+    DevBuf tpos;                // DILU: position of (j,i) in row j for each strict-lower (i,j), -1 if none
+    bool has_tpos = false;
+    bool levels_ready = false;
+};
+
+struct Workspace;   // solver vectors, partial sums, device scalars
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    void create() { if (!a) { FC_HIP_CHECK(hipEventCreate(&a)); FC_HIP_CHECK(hipEventCreate(&b)); } }
+    void destroy() { if (a) { (void)hipEventDestroy(a); (void)hipEventDestroy(b); a = b = nullptr; } }
+    void start(hipStream_t s) { create(); FC_HIP_CHECK(hipEventRecord(a, s)); }
+    double stop_ms(hipStream_t s) {
+        FC_HIP_CHECK(hipEventRecord(b, s));
+        FC_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        FC_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        return ms;
+    }
+};
+
+}  // namespace fcap
+
+struct fc_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    int variant = FC_VARIANT_SRC;
+    int report = 0;
+    std::string report_name = "p";
+    // structure cache of the host-pointer solvers
+    std::unique_ptr<fcap::Structure> cache;
+    std::unique_ptr<fcap::Workspace> ws;
+    fcap::Timer timer;
+    int num_cus = 256;
+    // pinned host flag block read once per iteration batch
+    int *host_flags = nullptr;
+};
+
+namespace fcap {
+
+// ---------------------------------------------------------------- launchers
+// Structure analysis (fcap_mesh.hip)
+void build_structure_from_host_csr(fc_context *ctx, Structure &st, int n, int nnz,
+                                   const int *ioffset1, const int *ja1, const int *diag1);
+void build_structure_from_faces_dev(fc_context *ctx, Structure &st, int n, int nif,
+                                    const int *owner0_d, const int *neigh0_d, int *ij_slot_d,
+                                    int *ji_slot_d);
+void analyse_levels(fc_context *ctx, Structure &st);
+inline void ensure_levels(fc_context *ctx, Structure &st) {
+    if (!st.levels_ready) { analyse_levels(ctx, st); st.levels_ready = true; }
+}
+void to_one_based(fc_context *ctx, int *d, long long n);
+void sub_one_dev(fc_context *ctx, const int *in, int *out, long long n);
+void fill_int(fc_context *ctx, int *d, long long n, int v);
+void build_tpos(fc_context *ctx, Structure &st);
+uint64_t csr_checksum(int n, int nnz, const int *ioffset, const int *ja);
+
+// Geometry (fcap_mesh.hip); all device pointers, 0-based node/cell ids.
+struct GeometryDev {
+    DevBuf arx, ary, arz, xf, yf, zf, vol, xc, yc, zc, facint;
+};
+struct MeshDev {
+    int nnodes = 0, nfaces = 0, nif = 0, n = 0;
+    DevBuf xyz, face_ptr, face_nodes, owner, neighbour;   // 0-based ids
+    DevBuf cf_ptr, cf_face;   // cell -> faces in ascending face order
+    GeometryDev g;
+};
+void upload_mesh(fc_context *ctx, MeshDev &m, int nnodes, const double *xyz, int nfaces, int nif,
+                 const int *face_ptr, const int *face_nodes1, const int *owner1,
+                 const int *neighbour1, int ncells);
+void compute_geometry(fc_context *ctx, MeshDev &m);
+
+// Assembly (fcap_assembly.hip)
+struct WallDev {
+    int nwal = 0, iwall_faces_start = 0, iwall_start = 0;
+    DevBuf cw_ptr, cw_idx;   // cell -> wall indices (0..nwal-1) in wall order
+};
+void build_wall_lists(fc_context *ctx, WallDev &w, const int *owner0_d, int n, int nwal,
+                      int iwall_faces_start, int iwall_start);
+struct OcDev {
+    int noc = 0, ioc_faces_start = 0;
+    DevBuf ijl, ijr, foc, al, ar;    // 0-based cells
+    DevBuf co_ptr, co_idx;           // cell -> O-C indices, encoded (i<<1 | side)
+};
+void build_oc_lists(fc_context *ctx, OcDev &o, int n);
+// range/consistency check of caller-supplied owner/neighbour and slot maps (device, 0-based)
+void validate_faces(fc_context *ctx, const Structure &st, int nif, const int *own, const int *nei, const int *ij,
+                    const int *ji);
+void laplacian_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
+                   const int *neigh0, const int *ij_slot, const int *ji_slot,
+                   const double *arx, const double *ary, const double *arz,
+                   const double *facint, const double *xc, const double *yc, const double *zc,
+                   const double *xf, const double *yf, const double *zf, const double *mu,
+                   const double *phi, const WallDev &w, const OcDev &oc, const int *kface_order,
+                   double *a, double *su);
+void convdiff_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
+                  const int *neigh0, const int *ij_slot, const int *ji_slot,
+                  const double *arx, const double *ary, const double *arz, const double *facint,
+                  const double *xc, const double *yc, const double *zc, const double *xf,
+                  const double *yf, const double *zf, const double *vis, const double *flux,
+                  const WallDev &w, double urf, double *a);
+void face_flux_dev(fc_context *ctx, int nif, const int *owner0, const int *neigh0,
+                   const double *facint, const double *arx, const double *ary, const double *arz,
+                   const double *u, const double *v, const double *w, double *flux);
+void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *ij_slot, const int *kface,
+                        const int *kface_order, const double *flux, double *su);
+// face id of each off-diagonal CSR entry (-1 on the diagonal), and whether every
+// row lists its faces in ascending face order when read in column order
+void build_kface(fc_context *ctx, const Structure &st, int nif, const int *ij_slot,
+                 const int *ji_slot, DevBuf &kface, DevBuf &kface_order, bool &needs_order);
+
+// Linear algebra (fcap_linalg.hip)
+void spmv64(fc_context *ctx, const Structure &st, const double *a, const double *x, double *y);
+void spmv32(fc_context *ctx, const Structure &st, const float *a, const float *x, float *y);
+void pivots(fc_context *ctx, const Structure &st, const double *a, double *d, bool dilu);
+void lu_apply64(fc_context *ctx, const Structure &st, const double *a, const double *d,
+                double small, const double *r, double *z);
+
+}  // namespace fcap

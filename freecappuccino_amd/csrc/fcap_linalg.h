@@ -1,0 +1,38 @@
+// Launchers of the linear-algebra kernels (fcap_linalg.hip).
+#pragma once
+
+#include "fcap_internal.h"
+#include "fcap_kernels.h"
+
+namespace fcap {
+
+// O-C couplings of a solve (device pointers, 0-based cells), per-cell lists in cut order.
+struct OcView {
+    int noc = 0;
+    const int *ijl = nullptr, *ijr = nullptr, *co_ptr = nullptr, *co_idx = nullptr;
+    const double *al = nullptr, *ar = nullptr;
+};
+
+int red_grid(int n);
+int spmv_grid(int n);
+
+// mode 0: y = A x; 1: y = b - A x with L1 (pa) and L2^2 (pb) partials and an
+// optional FP32 copy y32; 2: y = A x with partial y.v (pa); 3: as 2 plus y.y (pb)
+void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const double *a, const double *x,
+                   const double *b, const double *v, double *y, float *y32, double *pa, double *pb,
+                   const Scalars *sc);
+void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const float *a, const float *x,
+                   const float *b, const float *v, float *y, double *pa, double *pb, const Scalars *sc);
+void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc);
+
+void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err);
+void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
+                 const double *rhs, double *w, const Scalars *sc);
+void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
+                 float *w, const Scalars *sc);
+
+void dot_launch64(hipStream_t s, int n, const double *x, const double *y, double *pa);
+void norms_launch64(hipStream_t s, int n, const double *x, double *pa, double *pb);
+void sum_parts_launch(hipStream_t s, int nparts, const double *pa, const double *pb, double *out);
+
+}  // namespace fcap

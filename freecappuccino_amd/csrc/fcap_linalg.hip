@@ -1,0 +1,312 @@
+// SpMV, reductions, level-scheduled incomplete factorisations and sweeps.
+//
+// SpMV: one wavefront per block of 64 consecutive rows.  The wave stages the
+// block's slice of a/ja (contiguous in CSR) into LDS with coalesced loads,
+// then each lane walks its own row from LDS in CSR order, gathering x from
+// global memory.  The per-row order of operations is the reference's
+// (src/iccg.f90:129-134), so the FP64 result is bit-identical.
+//
+// Triangular sweeps and pivot recurrences are level-scheduled: the rows of a
+// level of the lower (forward) or upper (backward) dependency DAG have no
+// mutual dependencies; each level is one launch over its rows, each row
+// computed in the reference's order (src/iccg.f90:77-111, src/bicgstab.f90:68-79).
+
+#include <algorithm>
+
+#include "fcap_internal.h"
+#include "fcap_kernels.h"
+#include "fcap_linalg.h"
+
+namespace fcap {
+
+namespace {
+
+inline int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(g, 1 << 30));
+}
+
+}  // namespace
+
+int red_grid(int n) { return std::min(kMaxRedGrid, grid_for(n, kRedBlock)); }
+int spmv_grid(int n) { return grid_for(n, kSpmvRows); }
+
+// ------------------------------------------------------------------- SpMV
+
+template <class T, class AT, int MODE>
+__global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict__ ptr, const int *__restrict__ ja,
+                                                    const AT *__restrict__ a, const T *__restrict__ x,
+                                                    const T *__restrict__ b, const T *__restrict__ v,
+                                                    T *__restrict__ y, float *__restrict__ y32,
+                                                    double *__restrict__ partA, double *__restrict__ partB,
+                                                    const Scalars *__restrict__ sc) {
+    __shared__ AT sa[kSpmvWaves][kSpmvCap];
+    __shared__ int sj[kSpmvWaves][kSpmvCap];
+    __shared__ double red[kSpmvWaves];
+    if (sc && sc->done) return;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const long long row0 = (long long)blockIdx.x * kSpmvRows + (long long)w * kWave;
+    const long long rowl = row0 + lane;
+    const bool valid = rowl < n;
+    const int row = (int)rowl;
+    int nrows = (int)std::min<long long>(std::max<long long>(n - row0, 0), kWave);
+    int s0 = 0, cnt = 0;
+    if (nrows > 0) {
+        s0 = ptr[row0];
+        cnt = ptr[row0 + nrows] - s0;
+    }
+    const bool staged = nrows > 0 && cnt <= kSpmvCap;
+    if (staged) {
+        for (int j = lane; j < cnt; j += kWave) {
+            sa[w][j] = a[s0 + j];
+            sj[w][j] = ja[s0 + j];
+        }
+    }
+    __syncthreads();
+    double pa = 0.0, pb = 0.0;
+    if (valid) {
+        const int rs = ptr[row], re = ptr[row + 1];
+        T acc = (MODE == 1) ? b[row] : T(0);
+        if (staged) {
+            for (int k = rs - s0; k < re - s0; ++k) {
+                T prod = (T)sa[w][k] * x[sj[w][k]];
+                acc = (MODE == 1) ? acc - prod : acc + prod;
+            }
+        } else {
+            for (int k = rs; k < re; ++k) {
+                T prod = (T)a[k] * x[ja[k]];
+                acc = (MODE == 1) ? acc - prod : acc + prod;
+            }
+        }
+        y[row] = acc;
+        if (MODE == 1) {
+            if (y32) y32[row] = (float)acc;
+            pa = fabs((double)acc);
+            pb = (double)acc * (double)acc;
+        } else if (MODE == 2) {
+            pa = (double)acc * (double)v[row];
+        } else if (MODE == 3) {
+            pa = (double)acc * (double)v[row];
+            pb = (double)acc * (double)acc;
+        }
+    }
+    if (MODE != 0) {
+        pa = wave_sum(pa);
+        if (lane == 0) red[w] = pa;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int i = 0; i < kSpmvWaves; ++i) s += red[i];
+            partA[blockIdx.x] = s;
+        }
+        if (MODE == 1 || MODE == 3) {
+            __syncthreads();
+            pb = wave_sum(pb);
+            if (lane == 0) red[w] = pb;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double s = 0.0;
+                for (int i = 0; i < kSpmvWaves; ++i) s += red[i];
+                partB[blockIdx.x] = s;
+            }
+        }
+    }
+}
+
+// O-C couplings in cut order, per cell (src/iccg.f90:59-62,137-140):
+// sign -1 for the residual form, +1 for the product form.
+template <class T>
+__global__ void k_oc_apply(int n, const int *__restrict__ co_ptr, const int *__restrict__ co_idx,
+                           const int *__restrict__ ijl, const int *__restrict__ ijr, const double *__restrict__ al,
+                           const double *__restrict__ ar, const T *__restrict__ x, T *__restrict__ y, int sign,
+                           const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = co_ptr[i], e = co_ptr[i + 1];
+    if (s == e) return;
+    T acc = y[i];
+    for (int q = s; q < e; ++q) {
+        int code = co_idx[q], o = code >> 1;
+        T prod = (code & 1) ? (T)al[o] * x[ijl[o]] : (T)ar[o] * x[ijr[o]];
+        acc = sign < 0 ? acc - prod : acc + prod;
+    }
+    y[i] = acc;
+}
+
+void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const double *a, const double *x,
+                   const double *b, const double *v, double *y, float *y32, double *pa, double *pb,
+                   const Scalars *sc) {
+    const int g = spmv_grid(n);
+    switch (mode) {
+    case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
+    case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
+    case 2: k_spmv<double, double, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
+    default: k_spmv<double, double, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
+    }
+}
+
+void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const float *a, const float *x,
+                   const float *b, const float *v, float *y, double *pa, double *pb, const Scalars *sc) {
+    const int g = spmv_grid(n);
+    switch (mode) {
+    case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
+    case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
+    case 2: k_spmv<float, float, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
+    default: k_spmv<float, float, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
+    }
+}
+
+void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc) {
+    if (!oc.noc) return;
+    k_oc_apply<double><<<grid_for(n, 256), 256, 0, s>>>(n, oc.co_ptr, oc.co_idx, oc.ijl, oc.ijr, oc.al, oc.ar, x, y,
+                                                        sign, sc);
+}
+
+// ------------------------------------------------------- pivots and sweeps
+
+__global__ void k_dic_level(const int *__restrict__ rows, int beg, int cnt, const int *__restrict__ ptr,
+                            const int *__restrict__ ja, const int *__restrict__ diag, const double *__restrict__ a,
+                            double *__restrict__ d, int *__restrict__ err) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    int i = rows[beg + t];
+    double di = a[diag[i]];
+    for (int k = ptr[i]; k < diag[i]; ++k) {
+        double ak = a[k];
+        di = di - (ak * ak) * d[ja[k]];
+    }
+    if (!(di != 0.0) || !isfinite(di)) atomicOr(err, 1);
+    d[i] = 1.0 / di;
+}
+
+__global__ void k_dilu_level(const int *__restrict__ rows, int beg, int cnt, const int *__restrict__ ptr,
+                             const int *__restrict__ ja, const int *__restrict__ diag, const int *__restrict__ tpos,
+                             const double *__restrict__ a, double *__restrict__ d, int *__restrict__ err) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    int i = rows[beg + t];
+    double di = a[diag[i]];
+    for (int k = ptr[i]; k < diag[i]; ++k) di = di - (a[k] * d[ja[k]]) * a[tpos[k]];
+    if (!(di != 0.0) || !isfinite(di)) atomicOr(err, 1);
+    d[i] = 1.0 / di;
+}
+
+// forward sweep of (D+L) w = rhs: w_i = (rhs_i - sum_{k<diag} a_k w_j) d_i
+template <class T>
+__global__ void k_fwd_level(const int *__restrict__ rows, int beg, int cnt, const int *__restrict__ ptr,
+                            const int *__restrict__ ja, const int *__restrict__ diag, const T *__restrict__ a,
+                            const T *__restrict__ d, const T *__restrict__ rhs, T *__restrict__ w,
+                            const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    int i = rows[beg + t];
+    T acc = rhs[i];
+    for (int k = ptr[i]; k < diag[i]; ++k) acc = acc - a[k] * w[ja[k]];
+    w[i] = acc * d[i];
+}
+
+// backward sweep in place: z_i = (w_i/(d_i+small) - sum_{k>diag} a_k z_j) d_i
+template <class T>
+__global__ void k_bwd_level(const int *__restrict__ rows, int beg, int cnt, const int *__restrict__ ptr,
+                            const int *__restrict__ ja, const int *__restrict__ diag, const T *__restrict__ a,
+                            const T *__restrict__ d, T small, T *__restrict__ w, const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    int i = rows[beg + t];
+    T di = d[i];
+    T acc = w[i] / (di + small);
+    for (int k = diag[i] + 1; k < ptr[i + 1]; ++k) acc = acc - a[k] * w[ja[k]];
+    w[i] = acc * di;
+}
+
+void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err) {
+    const int *rows = st.rows_f.as<int>();
+    for (int l = 0; l < st.nlev_f; ++l) {
+        int beg = st.lvl_f[l], cnt = st.lvl_f[l + 1] - beg;
+        if (dilu)
+            k_dilu_level<<<grid_for(cnt, 256), 256, 0, s>>>(rows, beg, cnt, st.ioffset.as<int>(), st.ja.as<int>(),
+                                                             st.diag.as<int>(), st.tpos.as<int>(), a, d, err);
+        else
+            k_dic_level<<<grid_for(cnt, 256), 256, 0, s>>>(rows, beg, cnt, st.ioffset.as<int>(), st.ja.as<int>(),
+                                                            st.diag.as<int>(), a, d, err);
+    }
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
+                 const Scalars *sc) {
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
+    const int *rf = st.rows_f.as<int>(), *rb = st.rows_b.as<int>();
+    for (int l = 0; l < st.nlev_f; ++l) {
+        int beg = st.lvl_f[l], cnt = st.lvl_f[l + 1] - beg;
+        k_fwd_level<T><<<grid_for(cnt, 256), 256, 0, s>>>(rf, beg, cnt, ptr, ja, dg, a, d, rhs, w, sc);
+    }
+    for (int l = 0; l < st.nlev_b; ++l) {
+        int beg = st.lvl_b[l], cnt = st.lvl_b[l + 1] - beg;
+        k_bwd_level<T><<<grid_for(cnt, 256), 256, 0, s>>>(rb, beg, cnt, ptr, ja, dg, a, d, small, w, sc);
+    }
+}
+
+void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
+                 const double *rhs, double *w, const Scalars *sc) {
+    lu_launch_t<double>(s, st, a, d, small, rhs, w, sc);
+}
+
+void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
+                 float *w, const Scalars *sc) {
+    lu_launch_t<float>(s, st, a, d, small, rhs, w, sc);
+}
+
+// ----------------------------------------------------------- reductions
+
+__global__ void k_sum_parts(int nparts, const double *__restrict__ pa, const double *__restrict__ pb,
+                            double *__restrict__ out) {
+    __shared__ double sh[kFinBlock / kWave];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kFinBlock) {
+        a += pa[i];
+        if (pb) b += pb[i];
+    }
+    a = block_sum<kFinBlock>(a, sh);
+    b = block_sum<kFinBlock>(b, sh);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+template <class TA, class TB, int OP>
+__global__ __launch_bounds__(kRedBlock) void k_reduce(long long n, const TA *__restrict__ x, const TB *__restrict__ y,
+                                                      double *__restrict__ pa, double *__restrict__ pb) {
+    __shared__ double sh[kRedBlock / kWave];
+    double a = 0.0, b = 0.0;
+    for (long long i = blockIdx.x * (long long)kRedBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kRedBlock) {
+        double xv = (double)x[i];
+        if (OP == 0) a += xv * (double)y[i];        // dot
+        else if (OP == 1) { a += fabs(xv); b += xv * xv; }   // L1 and L2^2
+    }
+    a = block_sum<kRedBlock>(a, sh);
+    if (OP == 1) b = block_sum<kRedBlock>(b, sh);
+    if (threadIdx.x == 0) {
+        pa[blockIdx.x] = a;
+        if (OP == 1) pb[blockIdx.x] = b;
+    }
+}
+
+void dot_launch64(hipStream_t s, int n, const double *x, const double *y, double *pa) {
+    k_reduce<double, double, 0><<<red_grid(n), kRedBlock, 0, s>>>(n, x, y, pa, nullptr);
+}
+
+void norms_launch64(hipStream_t s, int n, const double *x, double *pa, double *pb) {
+    k_reduce<double, double, 1><<<red_grid(n), kRedBlock, 0, s>>>(n, x, x, pa, pb);
+}
+
+void sum_parts_launch(hipStream_t s, int nparts, const double *pa, const double *pb, double *out) {
+    k_sum_parts<<<1, kFinBlock, 0, s>>>(nparts, pa, pb, out);
+}
+
+}  // namespace fcap

@@ -1,0 +1,47 @@
+// Krylov solver drivers (fcap_solvers.hip).
+#pragma once
+
+#include "fcap_internal.h"
+#include "fcap_kernels.h"
+#include "fcap_linalg.h"
+
+namespace fcap {
+
+struct Workspace {
+    // FP64 vectors
+    DevBuf r, z, p, q, x, reso, uk, vk, d, tmp;
+    // FP32 vectors and matrix copies
+    DevBuf r32, z32, p32, q32, e32, a32, d32;
+    // reduction partials, summed scalars, device solver state, error flag
+    DevBuf pa, pb, sums, sc, err;
+    // learned preconditioner scratch
+    DevBuf g0, g1, g2;
+    int n = 0, nnz = 0;
+    void ensure(int n_, int nnz_);
+};
+
+struct SolveArgs {
+    const Structure *st = nullptr;
+    const double *a = nullptr;    // device, nnz
+    const double *su = nullptr;   // device, n
+    double *fi = nullptr;         // device, n (initial guess in, solution out)
+    OcView oc;
+    double sor = 1e-8;
+    int nsw = 1000;
+    int variant = FC_VARIANT_SRC;
+    int stop_rule = FC_STOP_L1;
+    double inner_eta = 1e-3;      // relative L2 reduction of each inner FP32 solve
+    int max_outer = 40;
+};
+
+// FP64 solvers with the reference's semantics.  On return ws.r holds the
+// final (recursively updated) residual vector.
+fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve_info *info);
+fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info *info);
+// FP32 inner solves (DIC-PCG or DILU-BiCGStab) inside FP64 iterative refinement.
+fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicgstab, fc_solve_info *info);
+void cast_d2f(hipStream_t s, long long n, const double *x, float *y);
+// ||su - A fi||_2 / ||su||_2 in FP64 (O-C terms included).
+double true_rel_l2(fc_context *ctx, const SolveArgs &args);
+
+}  // namespace fcap

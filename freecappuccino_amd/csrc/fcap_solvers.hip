@@ -1,0 +1,621 @@
+// Krylov solver drivers: DIC-PCG (src/iccg.f90), Jacobi-PCG (src/dpcg.f90),
+// DILU-BiCGStab (src/bicgstab.f90) in FP64 with the reference's update order,
+// and FP32 inner solves inside FP64 iterative refinement.
+//
+// All scalars of an iteration live on the device (struct Scalars).  Each
+// reduction is written as per-block partials by the kernel that produces the
+// vector (fused), summed by a one-block "finalize" kernel that also performs
+// the scalar update of the algorithm (beta, alpha, omega, convergence test).
+// Every kernel returns at once when `done` is set, so the host enqueues
+// iterations in batches and reads one flag per batch.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+#include "fcap_solvers.h"
+
+namespace fcap {
+
+namespace {
+
+inline int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(g, 1 << 30));
+}
+
+enum FinStep : int {
+    kFinRes0 = 0,     // initial residual: res0, early exit
+    kFinSk = 1,       // sk = r.z, bet = sk/s0
+    kFinAlf = 2,      // pkapk = p.q, alf = sk/pkapk
+    kFinPcgEnd = 3,   // resl, s0 = sk, iteration count, convergence
+    kFinBet = 4,      // BiCGStab: bet, om, beto
+    kFinGam = 5,      // BiCGStab: ukreso, gam
+    kFinAlfB = 6,     // BiCGStab: alf = vr/(vv+small)
+    kFinBicgEnd = 7,  // BiCGStab: resl, iteration count, convergence
+    kFinOuter = 8     // refinement: true residual norms
+};
+
+__device__ inline void converge_test(Scalars *sc) {
+    sc->iters += 1;
+    sc->rsm = sc->resl / (sc->res0 + sc->small);
+    bool finite = isfinite(sc->resl) && isfinite(sc->rnorm2);
+    bool conv;
+    if (sc->stop_rule == 0) conv = sc->rsm < sc->resmax;
+    else if (sc->stop_rule == 1) {
+        sc->rel = sqrt(sc->rnorm2 / sc->bnorm2);
+        conv = sc->rel < sc->resmax;
+    } else conv = sc->rnorm2 <= sc->inner_tol2;
+    if (!finite) { sc->done = 1; sc->status = FC_ERR_NONFINITE; }
+    else if (conv) { sc->done = 1; sc->status = FC_OK; }
+    else if (sc->iters >= sc->nsw) { sc->done = 1; sc->status = FC_NOT_CONVERGED; }
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const double *__restrict__ pa,
+                                                  const double *__restrict__ pb, Scalars *__restrict__ sc) {
+    __shared__ double sh[kFinBlock / kWave];
+    if (sc->done) return;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kFinBlock) {
+        a += pa[i];
+        if (pb) b += pb[i];
+    }
+    a = block_sum<kFinBlock>(a, sh);
+    if (pb) b = block_sum<kFinBlock>(b, sh);
+    if (threadIdx.x != 0) return;
+    switch (step) {
+    case kFinRes0:
+        sc->res0 = a;
+        sc->resl = a;
+        sc->rnorm2 = b;
+        sc->rel = sc->bnorm2 > 0.0 ? sqrt(b / sc->bnorm2) : 0.0;
+        if (!isfinite(a) || !isfinite(b)) { sc->done = 1; sc->status = FC_ERR_NONFINITE; }
+        else if (sc->early_exit && a < (double)1e-13f) { sc->done = 1; sc->status = FC_EARLY_EXIT; }
+        else if (sc->stop_rule == 1 && sc->rel < sc->resmax) { sc->done = 1; sc->status = FC_OK; }
+        else if (sc->stop_rule == 2 && b <= sc->inner_tol2) { sc->done = 1; sc->status = FC_OK; }
+        break;
+    case kFinSk:
+        sc->sk = a;
+        sc->bet = a / sc->s0;
+        break;
+    case kFinAlf:
+        sc->pkapk = a;
+        sc->alf = sc->sk / a;
+        break;
+    case kFinPcgEnd:
+        sc->resl = a;
+        sc->rnorm2 = b;
+        sc->s0 = sc->sk;
+        converge_test(sc);
+        break;
+    case kFinBet:
+        sc->om = (a * sc->gam) / ((sc->alf * sc->beto) + sc->small);
+        sc->beto = a;
+        sc->sk = a;
+        break;
+    case kFinGam:
+        sc->ukreso = a;
+        sc->gam = sc->sk / a;
+        break;
+    case kFinAlfB:
+        sc->vr = a;
+        sc->vv = b;
+        sc->alf = a / (b + sc->small);
+        break;
+    case kFinBicgEnd:
+        sc->resl = a;
+        sc->rnorm2 = b;
+        converge_test(sc);
+        break;
+    case kFinOuter:
+        sc->rnorm2 = b;
+        sc->resl = a;
+        sc->rel = sqrt(b / sc->bnorm2);
+        break;
+    }
+}
+
+// x += alf p ; r -= alf q ; partials |r| and r^2 (src/iccg.f90:147-154)
+template <class T, class X>
+__global__ __launch_bounds__(kRedBlock) void k_pcg_xr(int n, X *__restrict__ x, T *__restrict__ r,
+                                                      const T *__restrict__ p, const T *__restrict__ q,
+                                                      const Scalars *__restrict__ sc, double *__restrict__ pa,
+                                                      double *__restrict__ pb) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    const T alf = (T)sc->alf;
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        x[i] = x[i] + (X)(alf * p[i]);
+        T rv = r[i] - alf * q[i];
+        r[i] = rv;
+        a += fabs((double)rv);
+        b += (double)rv * (double)rv;
+    }
+    a = block_sum<kRedBlock>(a, sh);
+    b = block_sum<kRedBlock>(b, sh);
+    if (threadIdx.x == 0) { pa[blockIdx.x] = a; pb[blockIdx.x] = b; }
+}
+
+// FP64 variant kept separate so x + alf*p rounds exactly as the reference
+__global__ __launch_bounds__(kRedBlock) void k_pcg_xr64(int n, double *__restrict__ x, double *__restrict__ r,
+                                                       const double *__restrict__ p, const double *__restrict__ q,
+                                                       const Scalars *__restrict__ sc, double *__restrict__ pa,
+                                                       double *__restrict__ pb) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    const double alf = sc->alf;
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        x[i] = x[i] + alf * p[i];
+        double rv = r[i] - alf * q[i];
+        r[i] = rv;
+        a += fabs(rv);
+        b += rv * rv;
+    }
+    a = block_sum<kRedBlock>(a, sh);
+    b = block_sum<kRedBlock>(b, sh);
+    if (threadIdx.x == 0) { pa[blockIdx.x] = a; pb[blockIdx.x] = b; }
+}
+
+template <class T>
+__global__ void k_p_update(int n, T *__restrict__ p, const T *__restrict__ z, const Scalars *__restrict__ sc) {
+    if (sc->done) return;
+    const T bet = (T)sc->bet;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + bet * p[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kRedBlock) void k_dot(int n, const T *__restrict__ x, const T *__restrict__ y,
+                                                   const Scalars *__restrict__ sc, double *__restrict__ pa) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc && sc->done) return;
+    double a = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock)
+        a += (double)x[i] * (double)y[i];
+    a = block_sum<kRedBlock>(a, sh);
+    if (threadIdx.x == 0) pa[blockIdx.x] = a;
+}
+
+// Jacobi: z = r / a(diag) and the partial of r.z (src/dpcg.f90:85-90)
+template <class T, class AT>
+__global__ __launch_bounds__(kRedBlock) void k_jacobi(int n, const T *__restrict__ r, const AT *__restrict__ a,
+                                                      const int *__restrict__ diag, T *__restrict__ z,
+                                                      const Scalars *__restrict__ sc, double *__restrict__ pa) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    double s = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        T zv = r[i] / (T)a[diag[i]];
+        z[i] = zv;
+        s += (double)r[i] * (double)zv;
+    }
+    s = block_sum<kRedBlock>(s, sh);
+    if (threadIdx.x == 0) pa[blockIdx.x] = s;
+}
+
+// pk = res + om*(pk - alf*uk)  (src/bicgstab.f90:111)
+template <class T>
+__global__ void k_bicg_p(int n, T *__restrict__ pk, const T *__restrict__ res, const T *__restrict__ uk,
+                         const Scalars *__restrict__ sc) {
+    if (sc->done) return;
+    const T om = (T)sc->om, alf = (T)sc->alf;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T t = alf * uk[i];
+        t = pk[i] - t;
+        t = om * t;
+        pk[i] = res[i] + t;
+    }
+}
+
+// fi += gam*zk ; res -= gam*uk  (src/bicgstab.f90:160-161)
+template <class T, class X>
+__global__ void k_bicg_xr1(int n, X *__restrict__ fi, T *__restrict__ res, const T *__restrict__ zk,
+                           const T *__restrict__ uk, const Scalars *__restrict__ sc) {
+    if (sc->done) return;
+    const T gam = (T)sc->gam;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        fi[i] = fi[i] + (X)(gam * zk[i]);
+        res[i] = res[i] - gam * uk[i];
+    }
+}
+
+template <class T>
+__global__ void k_copy(int n, const T *__restrict__ x, T *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+template <class T>
+__global__ void k_zero(int n, T *__restrict__ y) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = T(0);
+}
+
+template <class TI, class TO>
+__global__ void k_cast(long long n, const TI *__restrict__ x, TO *__restrict__ y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[i] = (TO)x[i];
+}
+
+// x(fp64) += scale * e(fp32)
+__global__ void k_refine(int n, double *__restrict__ x, const float *__restrict__ e, double scale) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] = x[i] + scale * (double)e[i];
+}
+
+// r32 = (float)(r * inv): each inner solve sees a unit-norm right-hand side, so
+// the algorithms' absolute guards (small) and FP32's range stay scale-free
+__global__ void k_scale_cast(int n, const double *__restrict__ r, float *__restrict__ r32, double inv) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        r32[i] = (float)(r[i] * inv);
+}
+
+// ----------------------------------------------------------------- helpers
+
+struct Ctx {
+    fc_context *ctx;
+    hipStream_t s;
+    Workspace &ws;
+    int n;
+    Scalars *sc;
+    double *pa, *pb;
+    int gv;   // grid of the vector kernels
+};
+
+Scalars init_scalars(const SolveArgs &args, double small) {
+    Scalars h;
+    std::memset(&h, 0, sizeof(h));
+    h.small = small;
+    h.resmax = (args.variant == FC_VARIANT_TESTS) ? (double)1e-13f : args.sor;
+    h.nsw = (args.variant == FC_VARIANT_TESTS) ? 100 : args.nsw;
+    h.s0 = kS0;
+    h.alf = 1.0;
+    h.beto = 1.0;
+    h.gam = 1.0;
+    h.early_exit = (args.variant == FC_VARIANT_TESTS) ? 0 : 1;
+    h.stop_rule = (args.variant == FC_VARIANT_TESTS) ? 0 : (args.stop_rule == FC_STOP_L2 ? 1 : 0);
+    h.status = kStatusRunning;
+    return h;
+}
+
+void upload_scalars(Ctx &c, const Scalars &h) {
+    FC_HIP_CHECK(hipMemcpyAsync(c.sc, &h, sizeof(Scalars), hipMemcpyHostToDevice, c.s));
+}
+
+Scalars read_scalars(Ctx &c) {
+    Scalars h;
+    FC_HIP_CHECK(hipMemcpyAsync(&h, c.sc, sizeof(Scalars), hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipStreamSynchronize(c.s));
+    return h;
+}
+
+int read_done(Ctx &c) {
+    int *hf = c.ctx->host_flags;
+    FC_HIP_CHECK(hipMemcpyAsync(hf, &c.sc->done, sizeof(int), hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipStreamSynchronize(c.s));
+    return hf[0];
+}
+
+void fin(Ctx &c, int step, int nparts, bool two) {
+    k_fin<<<1, kFinBlock, 0, c.s>>>(step, nparts, c.pa, two ? c.pb : nullptr, c.sc);
+}
+
+// b-norm^2 into sc->bnorm2 (host-side read; used by L2 stopping)
+double norm2_host(Ctx &c, const double *v) {
+    dot_launch64(c.s, c.n, v, v, c.pa);
+    sum_parts_launch(c.s, red_grid(c.n), c.pa, nullptr, c.ws.sums.as<double>());
+    double h[2];
+    FC_HIP_CHECK(hipMemcpyAsync(h, c.ws.sums.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipStreamSynchronize(c.s));
+    return h[0];
+}
+
+void run_batches(Ctx &c, int batch, const std::function<void()> &iter) {
+    while (true) {
+        for (int k = 0; k < batch; ++k) iter();
+        FC_HIP_CHECK(hipGetLastError());
+        if (read_done(c)) break;
+    }
+}
+
+}  // namespace
+
+void cast_d2f(hipStream_t s, long long n, const double *x, float *y) {
+    k_cast<double, float><<<std::min(grid_for(n, 256), 8192), 256, 0, s>>>(n, x, y);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void Workspace::ensure(int n_, int nnz_) {
+    const size_t D = sizeof(double) * (size_t)std::max(n_, 1), F = sizeof(float) * (size_t)std::max(n_, 1);
+    for (DevBuf *b : {&r, &z, &p, &q, &x, &reso, &uk, &vk, &d, &tmp}) b->ensure(D);
+    for (DevBuf *b : {&r32, &z32, &p32, &q32, &e32, &d32}) b->ensure(F);
+    a32.ensure(sizeof(float) * (size_t)std::max(nnz_, 1));
+    const int parts = std::max(kMaxRedGrid, spmv_grid(n_));
+    pa.ensure(sizeof(double) * parts);
+    pb.ensure(sizeof(double) * parts);
+    sums.ensure(sizeof(double) * 4);
+    sc.ensure(sizeof(Scalars));
+    err.ensure(sizeof(int) * 4);
+    n = n_;
+    nnz = nnz_;
+}
+
+static Ctx make_ctx(fc_context *ctx, int n) {
+    Workspace &ws = *ctx->ws;
+    Ctx c{ctx, ctx->stream, ws, n, ws.sc.as<Scalars>(), ws.pa.as<double>(), ws.pb.as<double>(), red_grid(n)};
+    return c;
+}
+
+static void check_pivots(Ctx &c) {
+    int h = 0;
+    FC_HIP_CHECK(hipMemcpyAsync(&h, c.ws.err.get(), sizeof(int), hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipStreamSynchronize(c.s));
+    if (h) throw Error(FC_ERR_ZERO_PIVOT, "incomplete factorisation: zero or non-finite pivot");
+}
+
+double true_rel_l2(fc_context *ctx, const SolveArgs &args) {
+    Workspace &ws = *ctx->ws;
+    const Structure &st = *args.st;
+    Ctx c = make_ctx(ctx, st.n);
+    double *r = ws.tmp.as<double>();
+    spmv_launch64(c.s, 1, st.n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r, nullptr,
+                  c.pa, c.pb, nullptr);
+    if (args.oc.noc) oc_launch64(c.s, st.n, args.oc, args.fi, r, -1, nullptr);
+    double rr = norm2_host(c, r);
+    double bb = norm2_host(c, args.su);
+    return bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+}
+
+static void fill_info(fc_solve_info *info, const Scalars &h, double setup_ms, double solve_ms) {
+    if (!info) return;
+    info->res0 = h.res0;
+    info->resl = h.resl;
+    info->rsm = h.rsm;
+    info->iters = h.iters;
+    info->status = h.status;
+    info->setup_ms = setup_ms;
+    info->solve_ms = solve_ms;
+}
+
+// Initial residual r = su - A fi (+ O-C), norms, early exit.
+static void initial_residual(Ctx &c, const SolveArgs &args, double *r) {
+    const Structure &st = *args.st;
+    if (!args.oc.noc) {
+        spmv_launch64(c.s, 1, c.n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
+                      nullptr, c.pa, c.pb, nullptr);
+        fin(c, kFinRes0, spmv_grid(c.n), true);
+    } else {
+        spmv_launch64(c.s, 1, c.n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
+                      nullptr, c.pa, c.pb, nullptr);
+        oc_launch64(c.s, c.n, args.oc, args.fi, r, -1, nullptr);
+        norms_launch64(c.s, c.n, r, c.pa, c.pb);
+        fin(c, kFinRes0, red_grid(c.n), true);
+    }
+}
+
+fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve_info *info) {
+    const Structure &st = *args.st;
+    Workspace &ws = *ctx->ws;
+    Ctx c = make_ctx(ctx, st.n);
+    const int n = st.n;
+    const double small = args.variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+    double *r = ws.r.as<double>(), *z = ws.z.as<double>(), *p = ws.p.as<double>(), *q = ws.q.as<double>();
+    double *d = ws.d.as<double>();
+    Scalars h = init_scalars(args, small);
+    if (h.stop_rule == 1) h.bnorm2 = norm2_host(c, args.su);
+    ctx->timer.start(c.s);
+    upload_scalars(c, h);
+    k_zero<double><<<c.gv, 256, 0, c.s>>>(n, p);
+    FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
+    initial_residual(c, args, r);
+    if (dic) pivots_launch(c.s, st, args.a, d, false, ws.err.as<int>());
+    double setup_ms = ctx->timer.stop_ms(c.s);
+    if (dic) check_pivots(c);
+    if (read_done(c)) {
+        Scalars hs = read_scalars(c);
+        fill_info(info, hs, setup_ms, 0.0);
+        return (fc_status)hs.status;
+    }
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
+    const int gs = spmv_grid(n);
+    auto iter = [&]() {
+        if (dic) {
+            lu_launch64(c.s, st, args.a, d, small, r, z, c.sc);
+            k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
+        } else {
+            k_jacobi<double, double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, args.a, dg, z, c.sc, c.pa);
+        }
+        fin(c, kFinSk, c.gv, false);
+        k_p_update<double><<<c.gv, 256, 0, c.s>>>(n, p, z, c.sc);
+        if (!args.oc.noc) {
+            spmv_launch64(c.s, 2, n, ptr, ja, args.a, p, nullptr, p, q, nullptr, c.pa, nullptr, c.sc);
+            fin(c, kFinAlf, gs, false);
+        } else {
+            spmv_launch64(c.s, 0, n, ptr, ja, args.a, p, nullptr, nullptr, q, nullptr, c.pa, nullptr, c.sc);
+            oc_launch64(c.s, n, args.oc, p, q, +1, c.sc);
+            k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, p, q, c.sc, c.pa);
+            fin(c, kFinAlf, c.gv, false);
+        }
+        k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, r, p, q, c.sc, c.pa, c.pb);
+        fin(c, kFinPcgEnd, c.gv, true);
+    };
+    ctx->timer.start(c.s);
+    run_batches(c, dic ? 1 : 8, iter);
+    double solve_ms = ctx->timer.stop_ms(c.s);
+    Scalars hs = read_scalars(c);
+    fill_info(info, hs, setup_ms, solve_ms);
+    return (fc_status)hs.status;
+}
+
+fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info *info) {
+    const Structure &st = *args.st;
+    Workspace &ws = *ctx->ws;
+    Ctx c = make_ctx(ctx, st.n);
+    const int n = st.n;
+    const double small = args.variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+    double *res = ws.r.as<double>(), *reso = ws.reso.as<double>(), *pk = ws.p.as<double>();
+    double *uk = ws.uk.as<double>(), *zk = ws.z.as<double>(), *vk = ws.vk.as<double>(), *d = ws.d.as<double>();
+    build_tpos(ctx, const_cast<Structure &>(st));
+    Scalars h = init_scalars(args, small);
+    if (h.stop_rule == 1) h.bnorm2 = norm2_host(c, args.su);
+    ctx->timer.start(c.s);
+    upload_scalars(c, h);
+    FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
+    initial_residual(c, args, res);
+    pivots_launch(c.s, st, args.a, d, true, ws.err.as<int>());
+    k_copy<double><<<c.gv, 256, 0, c.s>>>(n, res, reso);
+    k_zero<double><<<c.gv, 256, 0, c.s>>>(n, pk);
+    k_zero<double><<<c.gv, 256, 0, c.s>>>(n, uk);
+    double setup_ms = ctx->timer.stop_ms(c.s);
+    check_pivots(c);
+    if (read_done(c)) {
+        Scalars hs = read_scalars(c);
+        fill_info(info, hs, setup_ms, 0.0);
+        return (fc_status)hs.status;
+    }
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>();
+    const int gs = spmv_grid(n);
+    auto iter = [&]() {
+        k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, res, reso, c.sc, c.pa);
+        fin(c, kFinBet, c.gv, false);
+        k_bicg_p<double><<<c.gv, 256, 0, c.s>>>(n, pk, res, uk, c.sc);
+        lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc);
+        spmv_launch64(c.s, 2, n, ptr, ja, args.a, zk, nullptr, reso, uk, nullptr, c.pa, nullptr, c.sc);
+        fin(c, kFinGam, gs, false);
+        k_bicg_xr1<double, double><<<c.gv, 256, 0, c.s>>>(n, args.fi, res, zk, uk, c.sc);
+        lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc);
+        spmv_launch64(c.s, 3, n, ptr, ja, args.a, zk, nullptr, res, vk, nullptr, c.pa, c.pb, c.sc);
+        fin(c, kFinAlfB, gs, true);
+        k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, res, zk, vk, c.sc, c.pa, c.pb);
+        fin(c, kFinBicgEnd, c.gv, true);
+    };
+    ctx->timer.start(c.s);
+    run_batches(c, 1, iter);
+    double solve_ms = ctx->timer.stop_ms(c.s);
+    Scalars hs = read_scalars(c);
+    fill_info(info, hs, setup_ms, solve_ms);
+    return (fc_status)hs.status;
+}
+
+// FP32 inner solve of A e = r32 to ||r_inner||_2 <= eta ||r32||_2, e32 out.
+static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, double r32norm2, int cap) {
+    Workspace &ws = c.ws;
+    const int n = c.n;
+    float *a32 = ws.a32.as<float>(), *d32 = ws.d32.as<float>();
+    float *r = ws.r32.as<float>(), *z = ws.z32.as<float>(), *p = ws.p32.as<float>(), *q = ws.q32.as<float>();
+    float *e = ws.e32.as<float>();
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>();
+    const float small = (float)kSmallSrc;
+    const int gs = spmv_grid(n);
+    Scalars h;
+    std::memset(&h, 0, sizeof(h));
+    h.small = kSmallSrc;
+    h.s0 = kS0;
+    h.alf = 1.0;
+    h.beto = 1.0;
+    h.gam = 1.0;
+    h.nsw = cap;
+    h.stop_rule = 2;
+    h.inner_tol2 = eta * eta * r32norm2;
+    h.res0 = std::sqrt(r32norm2);
+    h.status = kStatusRunning;
+    upload_scalars(c, h);
+    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
+    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
+    if (!bicg) {
+        auto iter = [&]() {
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc);
+            k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
+            fin(c, kFinSk, c.gv, false);
+            k_p_update<float><<<c.gv, 256, 0, c.s>>>(n, p, z, c.sc);
+            spmv_launch32(c.s, 2, n, ptr, ja, a32, p, nullptr, p, q, c.pa, nullptr, c.sc);
+            fin(c, kFinAlf, gs, false);
+            k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, p, q, c.sc, c.pa, c.pb);
+            fin(c, kFinPcgEnd, c.gv, true);
+        };
+        run_batches(c, 1, iter);
+    } else {
+        float *reso = reinterpret_cast<float *>(ws.tmp.get());   // n floats fit in n doubles
+        float *uk = q, *vk = reinterpret_cast<float *>(ws.vk.get());
+        k_copy<float><<<c.gv, 256, 0, c.s>>>(n, r, reso);
+        k_zero<float><<<c.gv, 256, 0, c.s>>>(n, uk);
+        auto iter = [&]() {
+            k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, reso, c.sc, c.pa);
+            fin(c, kFinBet, c.gv, false);
+            k_bicg_p<float><<<c.gv, 256, 0, c.s>>>(n, p, r, uk, c.sc);
+            lu_launch32(c.s, st, a32, d32, small, p, z, c.sc);
+            spmv_launch32(c.s, 2, n, ptr, ja, a32, z, nullptr, reso, uk, c.pa, nullptr, c.sc);
+            fin(c, kFinGam, gs, false);
+            k_bicg_xr1<float, float><<<c.gv, 256, 0, c.s>>>(n, e, r, z, uk, c.sc);
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc);
+            spmv_launch32(c.s, 3, n, ptr, ja, a32, z, nullptr, r, vk, c.pa, c.pb, c.sc);
+            fin(c, kFinAlfB, gs, true);
+            k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, z, vk, c.sc, c.pa, c.pb);
+            fin(c, kFinBicgEnd, c.gv, true);
+        };
+        run_batches(c, 1, iter);
+    }
+    Scalars hs = read_scalars(c);
+    return hs.iters;
+}
+
+fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solve_info *info) {
+    const Structure &st = *args.st;
+    Workspace &ws = *ctx->ws;
+    Ctx c = make_ctx(ctx, st.n);
+    const int n = st.n;
+    double *r = ws.r.as<double>(), *d = ws.d.as<double>();
+    // set-up: FP64 pivots, FP32 copies of the matrix and pivots
+    ctx->timer.start(c.s);
+    FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
+    if (bicg) build_tpos(ctx, const_cast<Structure &>(st));
+    pivots_launch(c.s, st, args.a, d, bicg, ws.err.as<int>());
+    k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
+    k_cast<double, float><<<c.gv, 256, 0, c.s>>>(n, d, ws.d32.as<float>());
+    double setup_ms = ctx->timer.stop_ms(c.s);
+    check_pivots(c);
+    Scalars h0 = init_scalars(args, kSmallSrc);
+    h0.bnorm2 = norm2_host(c, args.su);
+    const double bnorm = std::sqrt(h0.bnorm2);
+    const bool l2 = args.stop_rule == FC_STOP_L2;
+    int total_inner = 0, outer = 0;
+    double res0 = 0.0, resl = 0.0, rel = 0.0;
+    fc_status status = FC_NOT_CONVERGED;
+    ctx->timer.start(c.s);
+    for (;;) {
+        // FP64 refinement residual r = su - A fi, its FP32 copy, L1 and L2 norms
+        h0.done = 0;
+        upload_scalars(c, h0);
+        spmv_launch64(c.s, 1, n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
+                      nullptr, c.pa, c.pb, nullptr);
+        fin(c, kFinOuter, spmv_grid(n), true);
+        Scalars ho = read_scalars(c);
+        if (outer == 0) res0 = ho.resl;
+        resl = ho.resl;
+        rel = bnorm > 0 ? std::sqrt(ho.rnorm2) / bnorm : std::sqrt(ho.rnorm2);
+        if (!std::isfinite(rel)) { status = FC_ERR_NONFINITE; break; }
+        bool conv = l2 ? rel < args.sor : (res0 > 0 ? resl / (res0 + kSmallSrc) < args.sor : true);
+        if (conv) { status = FC_OK; break; }
+        if (outer >= args.max_outer || total_inner >= args.nsw) { status = FC_NOT_CONVERGED; break; }
+        int cap = std::max(1, args.nsw - total_inner);
+        const double rn = std::sqrt(ho.rnorm2);
+        k_scale_cast<<<c.gv, 256, 0, c.s>>>(n, r, ws.r32.as<float>(), 1.0 / rn);
+        total_inner += inner_solve32(c, st, bicg, args.inner_eta, 1.0, cap);
+        k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);
+        ++outer;
+    }
+    double solve_ms = ctx->timer.stop_ms(c.s);
+    if (info) {
+        info->res0 = res0;
+        info->resl = resl;
+        info->rsm = res0 > 0 ? resl / (res0 + kSmallSrc) : 0.0;
+        info->iters = total_inner;
+        info->outer = outer;
+        info->status = status;
+        info->setup_ms = setup_ms;
+        info->solve_ms = solve_ms;
+    }
+    return status;
+}
+
+}  // namespace fcap

@@ -1,0 +1,469 @@
+"""ctypes front end of libfcap.so and the host-side mirror of the reference's solver interface.
+
+The library is the product: every computation below runs in its HIP kernels
+on the GPU.  If libfcap.so is missing or no GPU is present, the functions
+raise -- there is no CPU fallback.
+
+Mirrors of the reference entry points (explicit-argument forms, 1-based index
+arrays, as src/ and tests/test_sparse_solvers.f90 use them):
+
+    iccg(...)      src/iccg.f90:3        DIC-preconditioned CG
+    dpcg(...)      src/dpcg.f90:3        Jacobi-preconditioned CG
+    bicgstab(...)  src/bicgstab.f90:1    DILU-preconditioned BiCGStab
+    laplacian(...) src/fvm_laplacian.f90:1
+    create_csr_matrix_from_mesh_data(...) src/sparse_matrix.f90:42
+    mesh_geometry(...) src/mesh_geometry_and_topology.f90:310 (geometry part)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfcap.so")
+
+FC_OK, FC_NOT_CONVERGED, FC_EARLY_EXIT, FC_FALLBACK_USED = 0, 1, 2, 3
+STATUS_NAMES = {0: "FC_OK", 1: "FC_NOT_CONVERGED", 2: "FC_EARLY_EXIT", 3: "FC_FALLBACK_USED",
+                -1: "FC_ERR_ARGUMENT", -2: "FC_ERR_HIP", -3: "FC_ERR_NONFINITE", -4: "FC_ERR_ZERO_PIVOT",
+                -5: "FC_ERR_STRUCTURE", -6: "FC_ERR_ALLOC", -7: "FC_ERR_NO_DEVICE"}
+VARIANT_SRC, VARIANT_TESTS = 0, 1
+METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
+METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
+STOP_L1, STOP_L2 = 0, 1
+ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
+           arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
+
+
+class FcError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("res0", C.c_double), ("resl", C.c_double), ("rsm", C.c_double), ("rel_l2", C.c_double),
+                ("iters", C.c_int), ("outer", C.c_int), ("status", C.c_int), ("fallback_iters", C.c_int),
+                ("setup_ms", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libfcap.so (built in-tree).  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FcError(-7, f"{LIB_PATH} not built: run __graft_entry__.build() (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, ip, dp = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    si = C.POINTER(SolveInfo)
+    sig = {
+        "fc_init": ([C.POINTER(vp), i], i), "fc_finalize": ([vp], i), "fc_set_stream": ([vp, vp], i),
+        "fc_last_error": ([vp], C.c_char_p), "fc_set_report": ([vp, i, C.c_char_p], i),
+        "fc_set_variant": ([vp, i], i),
+        "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
+        "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
+        "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
+                                                          dp, dp, dp, dp], i),
+        "fc_convdiff": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, ip, ip, ip, ip, ip, i, d, dp], i),
+        "fc_face_flux_central": ([vp, i, ip, ip, dp, dp, dp, dp, dp, dp, dp, i, dp], i),
+        "fc_rhs_divergence": ([vp, i, i, ip, ip, dp, dp], i),
+        "fc_spmv": ([vp, i, i, ip, ip, dp, dp, dp], i),
+        "fc_dic_pivots": ([vp, i, i, ip, ip, ip, dp, dp], i),
+        "fc_dilu_pivots": ([vp, i, i, ip, ip, ip, dp, dp], i),
+        "fc_lu_apply": ([vp, i, i, ip, ip, ip, dp, dp, dp, dp], i),
+        "fc_sys_create": ([vp, C.POINTER(vp), i, dp, i, i, ip, ip, ip, ip, i, i, i, i], i),
+        "fc_sys_destroy": ([vp], i),
+        "fc_sys_assemble_laplacian": ([vp, dp, dp, dp], i),
+        "fc_sys_assemble_laplacian_dev": ([vp], i),
+        "fc_sys_assemble_convdiff": ([vp, dp, dp, d, dp], i),
+        "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
+        "fc_sys_get_solution": ([vp, dp], i),
+        "fc_sys_solve": ([vp, i, i, d, i, si], i),
+        "fc_sys_get": ([vp, i, vp, C.c_size_t], i),
+        "fc_sys_device_ptr": ([vp, i], vp),
+        "fc_sys_sizes": ([vp, ip, ip, ip, ip, ip], i),
+        "fc_sys_last_ms": ([vp, i], d),
+        "fc_sys_spmv_bench": ([vp, i, i], d),
+    }
+    for name in ("fc_iccg", "fc_dpcg", "fc_bicgstab"):
+        sig[name] = ([vp, i, i, ip, ip, ip, dp, dp, dp, dp, i, ip, ip, dp, dp, d, i, si], i)
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Context:
+    """A device context (stream, workspace, structure cache) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.L = library()
+        h = C.c_void_p()
+        rc = self.L.fc_init(C.byref(h), device)
+        if rc != 0:
+            raise FcError(rc, "fc_init failed (is a GPU visible?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fc_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, allow=(FC_OK,)):
+        if rc not in allow:
+            raise FcError(rc, self.L.fc_last_error(self.h).decode())
+        return rc
+
+    def set_variant(self, variant):
+        self.check(self.L.fc_set_variant(self.h, variant))
+
+    def set_report(self, on: bool, name: str = "p"):
+        self.check(self.L.fc_set_report(self.h, 1 if on else 0, name.encode()))
+
+    def set_stream(self, stream_ptr: int):
+        self.check(self.L.fc_set_stream(self.h, C.c_void_p(stream_ptr)))
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------- geometry
+
+@dataclass
+class Geometry:
+    arx: np.ndarray
+    ary: np.ndarray
+    arz: np.ndarray
+    xf: np.ndarray
+    yf: np.ndarray
+    zf: np.ndarray
+    vol: np.ndarray
+    xc: np.ndarray
+    yc: np.ndarray
+    zc: np.ndarray
+    facint: np.ndarray
+
+
+def mesh_geometry(mesh, ctx: Optional[Context] = None) -> Geometry:
+    """Geometry of a PolyMesh, computed on the GPU (src/mesh_geometry_and_topology.f90:859-1081)."""
+    ctx = ctx or default_context()
+    nf, nif, n = mesh.n_faces, mesh.n_inner, mesh.n_cells
+    g = Geometry(*(np.zeros(nf) for _ in range(6)), *(np.zeros(n) for _ in range(4)), np.zeros(max(nif, 1)))
+    xyz = _f64(mesh.points)
+    fn1 = _i32(mesh.face_nodes.astype(np.int64) + 1)
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1) if nif else np.zeros(1, np.int32)
+    ctx.check(ctx.L.fc_mesh_geometry(ctx.h, xyz.shape[0], _dp(xyz), nf, nif, _ip(_i32(mesh.face_ptr)), _ip(fn1),
+                                     _ip(own1), _ip(nei1), n,
+                                     *(_dp(getattr(g, k)) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol",
+                                                                    "xc", "yc", "zc", "facint"))))
+    g.facint = g.facint[:nif]
+    return g
+
+
+@dataclass
+class CSR:
+    ioffset: np.ndarray
+    ja: np.ndarray
+    diag: np.ndarray
+    ij_slot: np.ndarray
+    ji_slot: np.ndarray
+
+
+def create_csr_matrix_from_mesh_data(n_cells: int, owner1, neighbour1, ctx: Optional[Context] = None) -> CSR:
+    """CSR structure and face slot maps on the GPU (src/sparse_matrix.f90:42-172); 1-based in and out."""
+    ctx = ctx or default_context()
+    owner1, neighbour1 = _i32(owner1), _i32(neighbour1)
+    nif = neighbour1.shape[0]
+    nnz = 2 * nif + n_cells
+    c = CSR(np.zeros(n_cells + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(n_cells, np.int32),
+            np.zeros(max(nif, 1), np.int32), np.zeros(max(nif, 1), np.int32))
+    ctx.check(ctx.L.fc_csr_build(ctx.h, n_cells, nif, _ip(owner1), _ip(neighbour1), _ip(c.ioffset), _ip(c.ja),
+                                 _ip(c.diag), _ip(c.ij_slot), _ip(c.ji_slot)))
+    c.ij_slot, c.ji_slot = c.ij_slot[:nif], c.ji_slot[:nif]
+    return c
+
+
+def wall_info(mesh):
+    """(nwal, iWallFacesStart, iWallStart) as src/mesh_geometry_and_topology.f90:409-605 derives them."""
+    ws, wc = mesh.wall_range()
+    cnt = {"inlet": 0, "outlet": 0, "symmetry": 0}
+    for t, nf, _ in mesh.patches:
+        if t in cnt:
+            cnt[t] += nf
+    return wc, ws, mesh.n_cells + cnt["inlet"] + cnt["outlet"] + cnt["symmetry"]
+
+
+def laplacian(mesh, geo: Geometry, csr: CSR, mu, phi, su0, ctx: Optional[Context] = None):
+    """laplacian(mu, phi) on the GPU (src/fvm_laplacian.f90:1-163); returns (a, su)."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    nwal, wstart, iwall = wall_info(mesh)
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    a = np.zeros(mesh.nnz)
+    su = _f64(su0).copy()
+    phi = _f64(phi)
+    zi, zd = np.zeros(1, np.int32), np.zeros(1)
+    ctx.check(ctx.L.fc_laplacian(ctx.h, n, nif, _ip(own1), _ip(nei1), _dp(geo.arx), _dp(geo.ary), _dp(geo.arz),
+                                 _dp(_f64(geo.facint)), _dp(geo.xc), _dp(geo.yc), _dp(geo.zc), _dp(geo.xf),
+                                 _dp(geo.yf), _dp(geo.zf), _dp(_f64(mu)), _dp(phi), phi.shape[0], nwal, wstart,
+                                 iwall, 0, 0, _ip(zi), _ip(zi), _dp(zd), _ip(csr.ioffset), _ip(csr.ja),
+                                 _ip(csr.diag), _ip(_i32(csr.ij_slot)), _ip(_i32(csr.ji_slot)), mesh.nnz, _dp(a),
+                                 _dp(su), _dp(zd), _dp(zd)))
+    return a, su
+
+
+def convdiff(mesh, geo: Geometry, csr: CSR, vis, flux, urf=0.7, ctx: Optional[Context] = None):
+    """Momentum-like upwind convection-diffusion matrix on the GPU."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    nwal, wstart, _ = wall_info(mesh)
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    a = np.zeros(mesh.nnz)
+    ctx.check(ctx.L.fc_convdiff(ctx.h, n, nif, _ip(own1), _ip(nei1), _dp(geo.arx), _dp(geo.ary), _dp(geo.arz),
+                                _dp(_f64(geo.facint)), _dp(geo.xc), _dp(geo.yc), _dp(geo.zc), _dp(geo.xf),
+                                _dp(geo.yf), _dp(geo.zf), _dp(_f64(vis)), _dp(_f64(flux)), nwal, wstart,
+                                _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _ip(_i32(csr.ij_slot)),
+                                _ip(_i32(csr.ji_slot)), mesh.nnz, float(urf), _dp(a)))
+    return a
+
+
+def face_flux_central(mesh, geo: Geometry, u, v, w, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    nif = mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    flux = np.zeros(max(nif, 1))
+    ctx.check(ctx.L.fc_face_flux_central(ctx.h, nif, _ip(own1), _ip(nei1), _dp(_f64(geo.facint)),
+                                         _dp(_f64(geo.arx[:nif])), _dp(_f64(geo.ary[:nif])),
+                                         _dp(_f64(geo.arz[:nif])), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)),
+                                         mesh.n_cells, _dp(flux)))
+    return flux[:nif]
+
+
+def rhs_divergence(mesh, flux, su0, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    nif = mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    su = _f64(su0).copy()
+    ctx.check(ctx.L.fc_rhs_divergence(ctx.h, mesh.n_cells, nif, _ip(own1), _ip(nei1), _dp(_f64(flux)), _dp(su)))
+    return su
+
+
+# ----------------------------------------------------------------- kernels
+
+def spmv(csr: CSR, a, x, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    n = csr.diag.shape[0]
+    y = np.zeros(n)
+    ctx.check(ctx.L.fc_spmv(ctx.h, n, csr.ja.shape[0], _ip(csr.ioffset), _ip(csr.ja), _dp(_f64(a)), _dp(_f64(x)),
+                            _dp(y)))
+    return y
+
+
+def dic_pivots(csr: CSR, a, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    n = csr.diag.shape[0]
+    d = np.zeros(n)
+    ctx.check(ctx.L.fc_dic_pivots(ctx.h, n, csr.ja.shape[0], _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                  _dp(_f64(a)), _dp(d)))
+    return d
+
+
+def dilu_pivots(csr: CSR, a, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    n = csr.diag.shape[0]
+    d = np.zeros(n)
+    ctx.check(ctx.L.fc_dilu_pivots(ctx.h, n, csr.ja.shape[0], _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                   _dp(_f64(a)), _dp(d)))
+    return d
+
+
+def lu_apply(csr: CSR, a, d, r, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    n = csr.diag.shape[0]
+    z = np.zeros(n)
+    ctx.check(ctx.L.fc_lu_apply(ctx.h, n, csr.ja.shape[0], _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                _dp(_f64(a)), _dp(_f64(d)), _dp(_f64(r)), _dp(z)))
+    return z
+
+
+# ----------------------------------------------------------------- solvers
+
+@dataclass
+class SolveResult:
+    status: int
+    fi: np.ndarray
+    res: np.ndarray
+    info: dict
+
+
+def _solve(name, csr: CSR, a, su, fi0, sor, nsw, ctx, noc_arrays, variant):
+    ctx = ctx or default_context()
+    ctx.set_variant(variant)
+    n = csr.diag.shape[0]
+    fi = _f64(fi0).copy()
+    res = np.zeros(n)
+    info = SolveInfo()
+    if noc_arrays is None:
+        noc, ijl, ijr, al, ar = 0, np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1), np.zeros(1)
+    else:
+        ijl, ijr, al, ar = (_i32(noc_arrays[0]), _i32(noc_arrays[1]), _f64(noc_arrays[2]), _f64(noc_arrays[3]))
+        noc = ijl.shape[0]
+    fn = getattr(ctx.L, name)
+    rc = fn(ctx.h, n, csr.ja.shape[0], _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _dp(_f64(a)), _dp(_f64(su)),
+            _dp(fi), _dp(res), noc, _ip(ijl), _ip(ijr), _dp(al), _dp(ar), float(sor), int(nsw), C.byref(info))
+    ctx.set_variant(VARIANT_SRC)
+    ctx.check(rc, allow=(FC_OK, FC_NOT_CONVERGED, FC_EARLY_EXIT))
+    return SolveResult(rc, fi, res, info.as_dict())
+
+
+def iccg(csr, a, su, fi0, sor=1e-8, nsw=1000, ctx=None, noc_arrays=None, variant=VARIANT_SRC):
+    """iccg(fi, ifi): DIC-preconditioned CG on the GPU (src/iccg.f90)."""
+    return _solve("fc_iccg", csr, a, su, fi0, sor, nsw, ctx, noc_arrays, variant)
+
+
+def dpcg(csr, a, su, fi0, sor=1e-8, nsw=1000, ctx=None, noc_arrays=None, variant=VARIANT_SRC):
+    """dpcg(fi, ifi): Jacobi-preconditioned CG on the GPU (src/dpcg.f90)."""
+    return _solve("fc_dpcg", csr, a, su, fi0, sor, nsw, ctx, noc_arrays, variant)
+
+
+def bicgstab(csr, a, su, fi0, sor=1e-8, nsw=1000, ctx=None, noc_arrays=None, variant=VARIANT_SRC):
+    """bicgstab(fi, ifi): DILU-preconditioned BiCGStab on the GPU (src/bicgstab.f90)."""
+    return _solve("fc_bicgstab", csr, a, su, fi0, sor, nsw, ctx, noc_arrays, variant)
+
+
+# ---------------------------------------------------------- resident system
+
+class System:
+    """A mesh, its CSR structure and an assembled system resident in HBM."""
+
+    def __init__(self, mesh, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.mesh = mesh
+        L = self.ctx.L
+        nwal, wstart, iwall = wall_info(mesh)
+        xyz = _f64(mesh.points)
+        fn1 = _i32(mesh.face_nodes.astype(np.int64) + 1)
+        own1 = _i32(mesh.owner.astype(np.int64) + 1)
+        nei1 = _i32(mesh.neighbour.astype(np.int64) + 1) if mesh.n_inner else np.zeros(1, np.int32)
+        h = C.c_void_p()
+        self.ctx.check(L.fc_sys_create(self.ctx.h, C.byref(h), xyz.shape[0], _dp(xyz), mesh.n_faces, mesh.n_inner,
+                                       _ip(_i32(mesh.face_ptr)), _ip(fn1), _ip(own1), _ip(nei1), mesh.n_cells,
+                                       nwal, wstart, iwall))
+        self.h = h
+        self.n = mesh.n_cells
+        self.ntotal = mesh.n_cells + mesh.n_boundary
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.fc_sys_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble_laplacian(self, mu, phi=None, su0=None):
+        phi = np.zeros(self.ntotal) if phi is None else _f64(phi)
+        self.ctx.check(self.ctx.L.fc_sys_assemble_laplacian(self.h, _dp(_f64(mu)), _dp(phi),
+                                                            _dp(None if su0 is None else _f64(su0))))
+
+    def assemble_laplacian_dev(self):
+        self.ctx.check(self.ctx.L.fc_sys_assemble_laplacian_dev(self.h))
+
+    def assemble_convdiff(self, vis, flux, urf=0.7, su0=None):
+        self.ctx.check(self.ctx.L.fc_sys_assemble_convdiff(self.h, _dp(_f64(vis)), _dp(_f64(flux)), float(urf),
+                                                           _dp(None if su0 is None else _f64(su0))))
+
+    def set_rhs(self, su):
+        self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))
+
+    def set_solution(self, fi):
+        self.ctx.check(self.ctx.L.fc_sys_set_solution(self.h, _dp(_f64(fi))))
+
+    def solution(self):
+        fi = np.zeros(self.n)
+        self.ctx.check(self.ctx.L.fc_sys_get_solution(self.h, _dp(fi)))
+        return fi
+
+    def solve(self, method=METHOD_ICCG, stop_rule=STOP_L1, sor=1e-8, nsw=1000):
+        info = SolveInfo()
+        rc = self.ctx.L.fc_sys_solve(self.h, method, stop_rule, float(sor), int(nsw), C.byref(info))
+        self.ctx.check(rc, allow=(FC_OK, FC_NOT_CONVERGED, FC_EARLY_EXIT, FC_FALLBACK_USED))
+        return rc, info.as_dict()
+
+    def get(self, name):
+        sizes = self.sizes()
+        n, nif, nnz = sizes["n"], sizes["nif"], sizes["nnz"]
+        nf = self.mesh.n_faces
+        count = {"a": nnz, "su": n, "ioffset": n + 1, "ja": nnz, "diag": n, "ij_slot": nif, "ji_slot": nif,
+                 "vol": n, "xc": n, "yc": n, "zc": n, "facint": nif, "arx": nf, "ary": nf, "arz": nf,
+                 "xf": nf, "yf": nf, "zf": nf, "res": n, "pivots": n}[name]
+        dtype = np.int32 if name in ("ioffset", "ja", "diag", "ij_slot", "ji_slot") else np.float64
+        out = np.zeros(max(count, 1), dtype)
+        self.ctx.check(self.ctx.L.fc_sys_get(self.h, ARR[name], out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out[:count]
+
+    def device_ptr(self, name) -> int:
+        return self.ctx.L.fc_sys_device_ptr(self.h, ARR[name]) or 0
+
+    def sizes(self):
+        v = [C.c_int() for _ in range(5)]
+        self.ctx.check(self.ctx.L.fc_sys_sizes(self.h, *(C.byref(x) for x in v)))
+        return dict(zip(("n", "nif", "nnz", "nlev_fwd", "nlev_bwd"), (x.value for x in v)))
+
+    def last_ms(self, stage):
+        return self.ctx.L.fc_sys_last_ms(self.h, stage)
+
+    def spmv_bench(self, precision=64, reps=20):
+        ms = self.ctx.L.fc_sys_spmv_bench(self.h, precision, reps)
+        if ms < 0:
+            raise FcError(-2, self.ctx.L.fc_last_error(self.ctx.h).decode())
+        return ms

@@ -1,0 +1,254 @@
+/*
+ * fcap.h -- C ABI of the MI355X pressure-solve library (libfcap.so, gfx950).
+ *
+ * Drop-in replacements for the reference's hot path.  The reference passes
+ * everything through Fortran module globals; these entry points take the same
+ * arrays as explicit arguments, in the reference's layout:
+ *
+ *   - every index array is 1-based (owner, neighbour, ioffset, ja, diag, the
+ *     face slot maps, ijl, ijr, face node ids);
+ *   - CSR as in src/sparse_matrix.f90:8-22: ioffset(n+1), ja(nnz), diag(n),
+ *     a(nnz), nnz = 2*numInnerFaces + numCells;
+ *   - real(dp) = double, integer = int (32-bit).
+ *
+ * Host-pointer entry points ("fc_<name>") copy their inputs to the GPU,
+ * compute there and copy results back; the caller owns every array and nothing
+ * is allocated across the boundary.  A context caches the device copy of the
+ * CSR structure (keyed by n, nnz and a checksum of ioffset/ja) together with
+ * its level schedules, so repeated solves on one mesh upload only values.
+ *
+ * Device-resident entry points ("fc_sys_*") keep a mesh, its CSR structure,
+ * the assembled system and the solver state in HBM for callers that loop
+ * assembly + solve without host round trips.
+ *
+ * Every function returns an fc_status; none aborts.  One host thread per
+ * context.
+ */
+#ifndef FCAP_H
+#define FCAP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FC_OK = 0,                /* converged (rsm < sor)                                  */
+    FC_NOT_CONVERGED = 1,     /* nsw iterations used without reaching sor (the reference
+                                 returns silently; src/iccg.f90:89-171)                 */
+    FC_EARLY_EXIT = 2,        /* initial L1 residual below 1e-13: no iteration
+                                 (src/iccg.f90:67-71)                                   */
+    FC_FALLBACK_USED = 3,     /* flexible CG stagnated; IC(0)-PCG finished the solve    */
+    FC_ERR_ARGUMENT = -1,
+    FC_ERR_HIP = -2,
+    FC_ERR_NONFINITE = -3,    /* NaN/Inf in the matrix, RHS or an iterate               */
+    FC_ERR_ZERO_PIVOT = -4,   /* DIC/DILU pivot is zero or not finite                   */
+    FC_ERR_STRUCTURE = -5,    /* malformed owner/neighbour or CSR (duplicates, range)   */
+    FC_ERR_ALLOC = -6,
+    FC_ERR_NO_DEVICE = -7
+} fc_status;
+
+typedef struct fc_context fc_context;
+typedef struct fc_system fc_system;
+
+/* Per-solve report.  res0/resl/rsm are the reference's L1 quantities
+ * (src/iccg.f90:65,154,164); rel_l2 = ||su - A fi||_2 / ||su||_2 recomputed
+ * in FP64 from scratch after the solve. */
+typedef struct {
+    double res0, resl, rsm;
+    double rel_l2;
+    int iters;                /* inner Krylov iterations in total                        */
+    int outer;                /* refinement passes (mixed-precision methods), else 0     */
+    int status;               /* fc_status of the solve                                 */
+    int fallback_iters;       /* IC(0)-PCG iterations after a stagnation fallback        */
+    double setup_ms;          /* pivots / schedules / network set-up, device time        */
+    double solve_ms;          /* iteration loop, device time                             */
+} fc_solve_info;
+
+/* ---------------------------------------------------------------- context */
+fc_status fc_init(fc_context **ctx, int device);
+fc_status fc_finalize(fc_context *ctx);
+/* Run all work of this context on an existing hipStream_t (NULL: the context's own). */
+fc_status fc_set_stream(fc_context *ctx, void *hip_stream);
+const char *fc_last_error(const fc_context *ctx);
+/* Print the reference's one-line solver report after each host-pointer solve
+ * (src/iccg.f90:174-175): 0 off (default), 1 on. */
+fc_status fc_set_report(fc_context *ctx, int on, const char *variable_name);
+
+/* -------------------------------------------------- mesh geometry and CSR */
+
+/* Face areas, face centres, cell volumes, cell centres and the interpolation
+ * factor, as src/mesh_geometry_and_topology.f90:859-1081 computes them (fan
+ * triangulation from node 1; accumulation in face order, bit-identical).
+ * face_ptr is 0-based (numFaces+1), face_nodes 1-based.  O-C and cyclic
+ * faces are not handled. */
+fc_status fc_mesh_geometry(fc_context *ctx, int numNodes, const double *xyz /* numNodes*3 */,
+                           int numFaces, int numInnerFaces, const int *face_ptr,
+                           const int *face_nodes, const int *owner, const int *neighbour,
+                           int numCells,
+                           double *arx, double *ary, double *arz,
+                           double *xf, double *yf, double *zf,
+                           double *vol, double *xc, double *yc, double *zc, double *facint);
+
+/* create_CSR_matrix_from_mesh_data (src/sparse_matrix.f90:42-172): ioffset,
+ * ja, diag and the face slot maps icell_jcell_csr_value_index (ij_slot) and
+ * jcell_icell_csr_value_index (ji_slot).  Bit-identical to the reference. */
+fc_status fc_csr_build(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                       const int *neighbour, int *ioffset, int *ja, int *diag, int *ij_slot,
+                       int *ji_slot);
+
+/* ------------------------------------------------------------- assembly */
+
+/* laplacian(mu, phi) (src/fvm_laplacian.f90:1-163): a is overwritten, su is
+ * accumulated (the reference never zeroes it).  Wall faces are
+ * iWallFacesStart+1 .. iWallFacesStart+nwal (1-based face numbers), their
+ * boundary values phi(iWallStart+i) (phi has numTotal entries).  O-C faces
+ * (noc > 0): ijl/ijr/foc of the cuts, faces iOCFacesStart+1..; al/ar out. */
+fc_status fc_laplacian(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                       const int *neighbour, const double *arx, const double *ary,
+                       const double *arz, const double *facint, const double *xc,
+                       const double *yc, const double *zc, const double *xf, const double *yf,
+                       const double *zf, const double *mu, const double *phi, int numTotal,
+                       int nwal, int iWallFacesStart, int iWallStart, int noc, int iOCFacesStart,
+                       const int *ijl, const int *ijr, const double *foc, const int *ioffset,
+                       const int *ja, const int *diag, const int *ij_slot, const int *ji_slot,
+                       int nnz, double *a, double *su, double *al, double *ar);
+
+/* Momentum-like convection-diffusion matrix: facefluxuvw coefficients with
+ * upwind convection (src/faceflux_velocity.f90:97-108) and the calcuvw
+ * diagonal (src/calcuvw.f90:417-432): a(diag) = (spu - sum of off-diagonals)
+ * / urf, spu = vis|S|/|x_P - x_f| summed over the cell's wall faces. */
+fc_status fc_convdiff(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                      const int *neighbour, const double *arx, const double *ary,
+                      const double *arz, const double *facint, const double *xc,
+                      const double *yc, const double *zc, const double *xf, const double *yf,
+                      const double *zf, const double *vis, const double *flux, int nwal,
+                      int iWallFacesStart, const int *ioffset, const int *ja, const int *diag,
+                      const int *ij_slot, const int *ji_slot, int nnz, double urf, double *a);
+
+/* Face mass flux F_f = u_f . S_f, u_f = (1-facint) u_P + facint u_N, and the
+ * continuity source su(o) -= F, su(n) += F of
+ * src/calcp-multiple_correction_SIMPLE.f90:74-75 (accumulated in face order). */
+fc_status fc_face_flux_central(fc_context *ctx, int numInnerFaces, const int *owner,
+                               const int *neighbour, const double *facint, const double *arx,
+                               const double *ary, const double *arz, const double *u,
+                               const double *v, const double *w, int numCells, double *flux);
+fc_status fc_rhs_divergence(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                            const int *neighbour, const double *flux, double *su);
+
+/* ------------------------------------------------------------- kernels */
+
+/* y = A x in the reference's row order (src/iccg.f90:129-134), FP64. */
+fc_status fc_spmv(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                  const double *a, const double *x, double *y);
+
+/* DIC (src/iccg.f90:77-83) / DILU (src/bicgstab.f90:68-79) reciprocal pivots d. */
+fc_status fc_dic_pivots(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                        const int *diag, const double *a, double *d);
+fc_status fc_dilu_pivots(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                         const int *diag, const double *a, double *d);
+/* z = M^-1 r, M = (D+L) D^-1 (D+U): forward sweep, z/(d+small), backward
+ * sweep (src/iccg.f90:94-111), level-scheduled. */
+fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                      const int *diag, const double *a, const double *d, const double *r,
+                      double *z);
+
+/* ------------------------------------------------------------- solvers */
+
+/* Explicit-argument forms of iccg(fi,ifi), dpcg(fi,ifi), bicgstab(fi,ifi)
+ * (src/iccg.f90, src/dpcg.f90, src/bicgstab.f90).  fi holds the initial
+ * guess on entry (numCells entries are read and written).  res (may be NULL)
+ * receives the final residual vector, like the module array res.
+ * sor = relative L1 tolerance sor(ifi), nsw = iteration cap nsw(ifi).
+ * noc/ijl/ijr/al/ar: O-C couplings (noc = 0 when none).  info may be NULL. */
+fc_status fc_iccg(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                  const int *diag, const double *a, const double *su, double *fi, double *res,
+                  int noc, const int *ijl, const int *ijr, const double *al, const double *ar,
+                  double sor, int nsw, fc_solve_info *info);
+fc_status fc_dpcg(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                  const int *diag, const double *a, const double *su, double *fi, double *res,
+                  int noc, const int *ijl, const int *ijr, const double *al, const double *ar,
+                  double sor, int nsw, fc_solve_info *info);
+fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                      const int *diag, const double *a, const double *su, double *fi, double *res,
+                      int noc, const int *ijl, const int *ijr, const double *al, const double *ar,
+                      double sor, int nsw, fc_solve_info *info);
+
+/* Solver variants.  FC_VARIANT_SRC: the module solvers of src/ (small = 1e-20,
+ * early exit below 1e-13, O-C terms).  FC_VARIANT_TESTS: the explicit-argument
+ * copies of tests/test_sparse_solvers.f90 (small = 1e-30, sor = 1e-13,
+ * nsw = 100, no early exit).  Default SRC. */
+enum { FC_VARIANT_SRC = 0, FC_VARIANT_TESTS = 1 };
+fc_status fc_set_variant(fc_context *ctx, int variant);
+
+/* ------------------------------------------------ device-resident system */
+
+enum {
+    FC_METHOD_ICCG = 1,            /* DIC-PCG, FP64 (reference semantics)                 */
+    FC_METHOD_DPCG = 2,            /* Jacobi-PCG, FP64                                    */
+    FC_METHOD_BICGSTAB = 3,        /* DILU-BiCGStab, FP64                                 */
+    FC_METHOD_ICCG_MIXED = 11,     /* FP32 DIC-PCG inside FP64 iterative refinement       */
+    FC_METHOD_BICGSTAB_MIXED = 13, /* FP32 DILU-BiCGStab inside FP64 iterative refinement */
+    FC_METHOD_FCG_LEARNED = 21     /* flexible CG, BF16 message-passing preconditioner,
+                                      FP32 inner / FP64 refinement, IC(0)-PCG fallback    */
+};
+
+/* Stopping rule of fc_sys_solve: FC_STOP_L1 is the reference's
+ * ||r||_1/||r0||_1 < sor; FC_STOP_L2 is ||su - A fi||_2/||su||_2 < sor (FP64). */
+enum { FC_STOP_L1 = 0, FC_STOP_L2 = 1 };
+
+/* Create a device-resident system from a mesh (arrays as for fc_mesh_geometry;
+ * patches: wall faces iWallFacesStart+1..+nwal, wall values at iWallStart+i).
+ * Geometry, CSR structure, level schedules and slot maps are built on the GPU. */
+fc_status fc_sys_create(fc_context *ctx, fc_system **sys, int numNodes, const double *xyz,
+                        int numFaces, int numInnerFaces, const int *face_ptr,
+                        const int *face_nodes, const int *owner, const int *neighbour,
+                        int numCells, int nwal, int iWallFacesStart, int iWallStart);
+fc_status fc_sys_destroy(fc_system *sys);
+
+/* Assemble the Laplacian (mu: numCells, phi: numTotal, su0: numCells or NULL
+ * for zero) / the convection-diffusion matrix (vis, flux: host arrays; su0 as
+ * for the Laplacian) into the system, on the GPU. */
+fc_status fc_sys_assemble_laplacian(fc_system *sys, const double *mu, const double *phi,
+                                    const double *su0);
+fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis, const double *flux,
+                                   double urf, const double *su0);
+/* Upload a RHS (numCells) replacing su. */
+fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
+/* Upload / download the solution vector (numCells). */
+fc_status fc_sys_set_solution(fc_system *sys, const double *fi);
+fc_status fc_sys_get_solution(fc_system *sys, double *fi);
+
+/* Device-side assembly and solve on the resident inputs (no host copies).
+ * fc_sys_assemble_laplacian_dev uses mu/phi/su0 already on the device
+ * (uploaded by a previous fc_sys_assemble_laplacian). */
+fc_status fc_sys_assemble_laplacian_dev(fc_system *sys);
+fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, double sor, int nsw,
+                       fc_solve_info *info);
+
+/* Read back arrays of the resident system.  which: */
+enum {
+    FC_ARR_A = 1, FC_ARR_SU = 2, FC_ARR_IOFFSET = 3, FC_ARR_JA = 4, FC_ARR_DIAG = 5,
+    FC_ARR_IJ_SLOT = 6, FC_ARR_JI_SLOT = 7, FC_ARR_VOL = 8, FC_ARR_XC = 9, FC_ARR_YC = 10,
+    FC_ARR_ZC = 11, FC_ARR_FACINT = 12, FC_ARR_ARX = 13, FC_ARR_ARY = 14, FC_ARR_ARZ = 15,
+    FC_ARR_XF = 16, FC_ARR_YF = 17, FC_ARR_ZF = 18, FC_ARR_RES = 19, FC_ARR_PIVOTS = 20
+};
+fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_t bytes);
+/* Device pointer of an array of the resident system (for callers that share
+ * the device memory, e.g. a training loop); NULL if unknown. */
+void *fc_sys_device_ptr(fc_system *sys, int which);
+/* Sizes: numCells, numInnerFaces, nnz, forward and backward level counts. */
+fc_status fc_sys_sizes(fc_system *sys, int *n, int *nif, int *nnz, int *nlev_fwd, int *nlev_bwd);
+
+/* Device time of the last call of each stage, milliseconds (HIP events on the
+ * context's stream): 0 assembly, 1 solve, 2 SpMV (last fc_sys_spmv_bench). */
+double fc_sys_last_ms(fc_system *sys, int stage);
+/* Time `reps` FP64 (precision 64) or FP32 (32) SpMVs y = A x on the resident
+ * matrix; returns the average device time per SpMV in milliseconds, or < 0. */
+double fc_sys_spmv_bench(fc_system *sys, int precision, int reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCAP_H */
