@@ -60,10 +60,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=128, help="cells per direction of the hex box")
+    ap.add_argument("--case", type=int, default=2, choices=(1, 2, 5),
+                    help="1: 40^3 box; 2: 128^3 perturbed; 5: 192^3 perturbed, x-graded 1.05")
+    ap.add_argument("--n", type=int, default=None, help="cells per direction (overrides the case's size)")
     ap.add_argument("--cpu-iters", type=int, default=40, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="only assemble and run --spmv-reps FP64 SpMVs plus a 1 GiB device copy "
+                         "(the calibration kernel of the HBM counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -72,15 +77,28 @@ def main():
     from freecappuccino_amd import fcap
     from freecappuccino_amd import workloads as W
 
-    mesh = W.case_mesh(2, args.n)
+    args.n = args.n or {1: 40, 2: 128, 5: 192}[args.case]
+    mesh = W.case_mesh(args.case, args.n)
     ctx = fcap.Context(0)
     s = fcap.System(mesh, ctx)
     n, nnz = mesh.n_cells, mesh.nnz
     xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")
-    su0 = W.blob_rhs(xc, yc, zc, vol, 16, seed=2)
+    nblobs = 32 if args.case == 5 else 16
+    su0 = W.blob_rhs(xc, yc, zc, vol, nblobs, seed=2)
     mu = np.full(n, -1.0)
     s.assemble_laplacian(mu, None, su0)
     sizes = s.sizes()
+    if args.profile_spmv:
+        ms = s.spmv_bench(64, args.spmv_reps)
+        copy_bytes = 1 << 30
+        copy_ms = ctx.copy_bench(copy_bytes, 3)
+        print(json.dumps({"spmv_fp64_ms": ms, "spmv_bytes": spmv_bytes(n, nnz),
+                          "spmv_GBps": spmv_bytes(n, nnz) / ms / 1e6, "copy_bytes": copy_bytes,
+                          "copy_ms": copy_ms, "copy_GBps": 2 * copy_bytes / copy_ms / 1e6,
+                          "cells": n, "nnz": nnz}))
+        s.close()
+        ctx.close()
+        return
 
     def step(method):
         s.set_solution(np.zeros(n))
@@ -121,8 +139,10 @@ def main():
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if best == "ic_pcg_fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"hex box {args.n}^3 perturbed +-20% (seeded), Laplacian mu=-1, Dirichlet x=0, "
-                               f"16 Gaussian blobs; assembly + solve to 1e-8 FP64 rel. L2 residual",
+        "config": {"workload": f"hex box {args.n}^3 perturbed +-20% (seeded)"
+                               + (", x-graded 1.05 toward the wall" if args.case == 5 else "")
+                               + f", Laplacian mu=-1, Dirichlet x=0, {nblobs} Gaussian blobs; "
+                               "assembly + solve to 1e-8 FP64 rel. L2 residual",
                    "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"], "best": best},
         "solvers": results,
         "spmv": {"fp64_ms": spmv_ms, "fp64_GBps": gbs, "fp32_ms": spmv32_ms,

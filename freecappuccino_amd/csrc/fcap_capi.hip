@@ -6,10 +6,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "fcap_internal.h"
 #include "fcap_solvers.h"
+#include "fcap_system.h"
 
 using namespace fcap;
 
@@ -188,6 +190,8 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         hipDeviceProp_t prop;
         FC_HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
+        if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
+        if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -566,20 +570,6 @@ extern "C" fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *iof
 
 // ====================================================== resident system
 
-struct fc_system {
-    fc_context *ctx = nullptr;
-    MeshDev mesh;
-    Structure st;
-    DevBuf ij, ji, kface, kord;
-    bool needs_order = false;
-    WallDev walls;
-    OcDev oc;
-    int ntotal = 0;
-    DevBuf a, su, fi, mu, phi, su0, vis, flux;
-    bool su0_zero = true;
-    double last_ms[3] = {0, 0, 0};
-};
-
 extern "C" fc_status fc_sys_create(fc_context *ctx, fc_system **out, int numNodes, const double *xyz, int numFaces,
                                    int numInnerFaces, const int *face_ptr, const int *face_nodes, const int *owner,
                                    const int *neighbour, int numCells, int nwal, int iWallFacesStart,
@@ -656,6 +646,7 @@ extern "C" fc_status fc_sys_assemble_laplacian(fc_system *sys, const double *mu,
         ctx->timer.start(ctx->stream);
         sys_assemble_laplacian(sys);
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        sys->assembly_version += 1;
         return FC_OK;
     });
 }
@@ -668,6 +659,7 @@ extern "C" fc_status fc_sys_assemble_laplacian_dev(fc_system *sys) {
         ctx->timer.start(ctx->stream);
         sys_assemble_laplacian(sys);
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        sys->assembly_version += 1;
         return FC_OK;
     });
 }
@@ -689,6 +681,7 @@ extern "C" fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis,
                      g.xf.as<double>(), g.yf.as<double>(), g.zf.as<double>(), sys->vis.as<double>(),
                      sys->flux.as<double>(), sys->walls, urf, sys->a.as<double>());
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        sys->assembly_version += 1;
         if (su0) h2d(sys->su.get(), su0, sizeof(double) * n, ctx->stream);
         else FC_HIP_CHECK(hipMemsetAsync(sys->su.get(), 0, sizeof(double) * n, ctx->stream));
         return FC_OK;
@@ -743,6 +736,12 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_BICGSTAB: s = solve_bicgstab64(ctx, args, &local); break;
         case FC_METHOD_ICCG_MIXED: s = solve_mixed(ctx, args, false, &local); break;
         case FC_METHOD_BICGSTAB_MIXED: s = solve_mixed(ctx, args, true, &local); break;
+        case FC_METHOD_FCG_LEARNED: {
+            FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
+            PrecondHook P = gnn_hook(sys);
+            s = solve_fcg(ctx, args, P, &local);
+            break;
+        }
         default: throw Error(FC_ERR_ARGUMENT, "unknown method");
         }
         local.rel_l2 = true_rel_l2(ctx, args);
@@ -865,6 +864,42 @@ extern "C" double fc_sys_spmv_bench(fc_system *sys, int precision, int reps) {
         double ms = ctx->timer.stop_ms(s);
         out = ms / reps;
         sys->last_ms[2] = out;
+        return FC_OK;
+    });
+    return s == FC_OK ? out : -1.0;
+}
+
+// ========================================================= calibration
+
+namespace {
+// four independent 16-byte loads in flight per thread
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long long n16) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        uint4 v0 = src[i], v1 = src[i + stride], v2 = src[i + 2 * stride], v3 = src[i + 3 * stride];
+        dst[i] = v0;
+        dst[i + stride] = v1;
+        dst[i + 2 * stride] = v2;
+        dst[i + 3 * stride] = v3;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" double fc_copy_bench(fc_context *ctx, size_t bytes, int reps) {
+    if (!ctx || bytes < 16 || reps <= 0) return -1.0;
+    double out = -1.0;
+    fc_status s = guarded(ctx, [&]() {
+        const long long n16 = (long long)(bytes / 16);
+        DevBuf a(n16 * 16), b(n16 * 16);
+        FC_HIP_CHECK(hipMemsetAsync(a.get(), 1, n16 * 16, ctx->stream));
+        const int grid = std::max(1, std::min<int>((int)((n16 + 255) / 256), ctx->num_cus * 16));
+        k_copy16<<<grid, 256, 0, ctx->stream>>>(a.as<uint4>(), b.as<uint4>(), n16);
+        ctx->timer.start(ctx->stream);
+        for (int r = 0; r < reps; ++r) k_copy16<<<grid, 256, 0, ctx->stream>>>(a.as<uint4>(), b.as<uint4>(), n16);
+        FC_HIP_CHECK(hipGetLastError());
+        out = ctx->timer.stop_ms(ctx->stream) / reps;
         return FC_OK;
     });
     return s == FC_OK ? out : -1.0;

@@ -1,0 +1,40 @@
+// Learned preconditioner: message-passing network over the cell graph (fcap_gnn.hip).
+#pragma once
+
+#include "fcap_internal.h"
+#include "fcap_kernels.h"
+
+namespace fcap {
+
+constexpr int kGnnH = 32;                                  // hidden width
+constexpr int kGnnL = 4;                                   // message-passing layers
+constexpr int kGnnLayerStride = 2 * kGnnH * kGnnH + kGnnH + 3;   // Ws, Wm, b, gate(3)
+constexpr int kGnnOffEncW = 0;
+constexpr int kGnnOffEncB = kGnnH;
+constexpr int kGnnOffLayer0 = 2 * kGnnH;
+constexpr int kGnnOffDecW = kGnnOffLayer0 + kGnnL * kGnnLayerStride;
+constexpr int kGnnOffDecB = kGnnOffDecW + kGnnH;
+constexpr int kGnnParams = kGnnOffDecB + 1;
+
+// Per-system graph data the network reads: packed BF16 edge features per CSR
+// entry, FP32 diagonal, transposed-entry index, FP32 matrix copy.
+struct GnnGraph {
+    int n = 0, nnz = 0;
+    const int *ptr = nullptr, *ja = nullptr, *diagpos = nullptr;
+    DevBuf ef, dg, trans, a32;
+    bool ready = false;
+};
+
+// Build the graph data of an assembled system (device arrays of the system).
+void gnn_prepare_graph(fc_context *ctx, GnnGraph &g, const Structure &st, const int *kface, const int *owner0,
+                       const int *ij, const int *ji, int nif, const double *arx, const double *ary,
+                       const double *arz, const double *xc, const double *yc, const double *zc, const double *a);
+
+// Preconditioner hook used by flexible CG: z = P(r) on the context's stream;
+// kernels return at once when sc->done is set.
+struct PrecondHook {
+    void *obj = nullptr;
+    void (*apply)(void *obj, hipStream_t s, const float *r, float *z, const Scalars *sc) = nullptr;
+};
+
+}  // namespace fcap

@@ -91,6 +91,7 @@ struct Structure {
     int nlev_f = 0, nlev_b = 0;
     DevBuf rows_f, rows_b;      // rows in level order
     std::vector<int> lvl_f, lvl_b;   // host copy of the level pointers (nlev+1)
+    DevBuf lvl_f_d, lvl_b_d;         // device copy of the level pointers
     int max_level_f = 0, max_level_b = 0; // largest level size
     DevBuf tpos;                // DILU: position of (j,i) in row j for each strict-lower (i,j), -1 if none
     bool has_tpos = false;
@@ -128,6 +129,11 @@ struct fc_context {
     std::unique_ptr<fcap::Workspace> ws;
     fcap::Timer timer;
     int num_cus = 256;
+    // replay solver iterations as HIP graphs (FCAP_GRAPHS=0 launches kernels directly)
+    int use_graphs = 1;
+    // triangular sweeps: 0 one launch per level, 1 persistent cooperative kernel
+    // with a grid barrier per level (FCAP_SWEEP selects)
+    int sweep_mode = 0;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -18,7 +18,7 @@ namespace fcap {
 constexpr int kWave = 64;
 constexpr int kSpmvWaves = 4;                    // waves per SpMV block
 constexpr int kSpmvRows = kWave * kSpmvWaves;    // rows per SpMV block
-constexpr int kSpmvCap = 640;                    // LDS entries staged per wave
+constexpr int kSpmvCap = 512;                    // LDS entries staged per wave (8 per row on average)
 constexpr int kRedBlock = 256;                   // block size of the reduction kernels
 constexpr int kMaxRedGrid = 2048;                // grid cap of grid-stride reductions
 constexpr int kFinBlock = 1024;                  // block size of the finalize kernels
@@ -32,7 +32,8 @@ struct Scalars {
     double om, gam, beto, ukreso, vr, vv;      // BiCGStab
     double bnorm2, rnorm2, rel;                // L2 quantities (mixed precision / L2 stop)
     double inner_tol2, inner_r0;               // inner-solve stopping threshold
-    double fcg_rho, fcg_eta;                   // flexible CG
+    double fcg_pq, fcg_pr;                     // flexible CG: p.q of the previous step, p.r
+    double stag_ref;                           // flexible CG: ||r||^2 at the last stagnation check
     int iters, done, status, nsw;
     int stop_rule;     // 0: L1 rsm < resmax; 1: L2 rel < resmax; 2: rnorm2 <= inner_tol2
     int early_exit;    // 1: stop before iterating when res0 < 1e-13 (src/iccg.f90:67-71)

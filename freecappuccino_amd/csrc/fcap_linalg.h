@@ -25,11 +25,28 @@ void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja
                    const float *b, const float *v, float *y, double *pa, double *pb, const Scalars *sc);
 void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc);
 
+// Grid barrier state of the persistent sweeps.  `abort` is sticky: once a
+// barrier wait exceeds its spin limit every block leaves and the host reports
+// the solve as failed.
+struct GridBar {
+    unsigned count, gen;
+    int abort, pad;
+};
+
+// How the triangular sweeps run: one launch per level (bar == nullptr), or one
+// persistent cooperative kernel per sweep with `blocks` co-resident workgroups
+// and a grid barrier between levels.
+struct SweepCfg {
+    GridBar *bar = nullptr;
+    int blocks = 0;
+};
+SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar);
+
 void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err);
 void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
-                 const double *rhs, double *w, const Scalars *sc);
+                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg = nullptr);
 void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
-                 float *w, const Scalars *sc);
+                 float *w, const Scalars *sc, const SweepCfg *cfg = nullptr);
 
 void dot_launch64(hipStream_t s, int n, const double *x, const double *y, double *pa);
 void norms_launch64(hipStream_t s, int n, const double *x, double *pa, double *pb);

@@ -57,9 +57,27 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
     }
     const bool staged = nrows > 0 && cnt <= kSpmvCap;
     if (staged) {
-        for (int j = lane; j < cnt; j += kWave) {
-            sa[w][j] = a[s0 + j];
-            sj[w][j] = ja[s0 + j];
+        // all loads of the slice are issued before the first LDS store, so a
+        // wave keeps up to 2*kSpmvCap/64 loads in flight; a and ja are read
+        // once, so they are streamed (non-temporal) and leave the caches to x
+        constexpr int kChunks = kSpmvCap / kWave;
+        AT va[kChunks];
+        int vj[kChunks];
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int j = c * kWave + lane;
+            if (j < cnt) {
+                va[c] = __builtin_nontemporal_load(a + s0 + j);
+                vj[c] = __builtin_nontemporal_load(ja + s0 + j);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int j = c * kWave + lane;
+            if (j < cnt) {
+                sa[w][j] = va[c];
+                sj[w][j] = vj[c];
+            }
         }
     }
     __syncthreads();
@@ -68,7 +86,17 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
         const int rs = ptr[row], re = ptr[row + 1];
         T acc = (MODE == 1) ? b[row] : T(0);
         if (staged) {
-            for (int k = rs - s0; k < re - s0; ++k) {
+            // gathers issued four at a time; the sum still runs in CSR order
+            int k = rs - s0;
+            const int ke = re - s0;
+            for (; k + 4 <= ke; k += 4) {
+                const T x0 = x[sj[w][k]], x1 = x[sj[w][k + 1]], x2 = x[sj[w][k + 2]], x3 = x[sj[w][k + 3]];
+                const T p0 = (T)sa[w][k] * x0, p1 = (T)sa[w][k + 1] * x1;
+                const T p2 = (T)sa[w][k + 2] * x2, p3 = (T)sa[w][k + 3] * x3;
+                if (MODE == 1) { acc = acc - p0; acc = acc - p1; acc = acc - p2; acc = acc - p3; }
+                else { acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3; }
+            }
+            for (; k < ke; ++k) {
                 T prod = (T)sa[w][k] * x[sj[w][k]];
                 acc = (MODE == 1) ? acc - prod : acc + prod;
             }
@@ -236,9 +264,124 @@ void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *
     FC_HIP_CHECK(hipGetLastError());
 }
 
+// ----------------------------------------------- persistent level sweeps
+// One cooperative launch per sweep; the blocks walk the levels in order and
+// meet at a grid barrier between levels.  Co-residency of every block is
+// guaranteed by hipLaunchCooperativeKernel.  A barrier wait gives up after a
+// bounded number of polls and raises the sticky abort flag, so every block
+// always reaches the end of the kernel.
+
+constexpr long kSpinLimit = 4000000;
+
+__device__ inline bool grid_sync(GridBar *bar, unsigned nblocks) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long it = 0;
+            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (++it > kSpinLimit) {
+                    __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        s_ok = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fwd_persistent(const int *__restrict__ rows, const int *__restrict__ lvl,
+                                                        int nlev, const int *__restrict__ ptr,
+                                                        const int *__restrict__ ja, const int *__restrict__ diag,
+                                                        const T *__restrict__ a, const T *__restrict__ d,
+                                                        const T *__restrict__ rhs, T *w,
+                                                        const Scalars *__restrict__ sc, GridBar *bar) {
+    if (sc && sc->done) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int l = 0; l < nlev; ++l) {
+        const int beg = lvl[l], end = lvl[l + 1];
+        for (int p = beg + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += stride) {
+            int i = rows[p];
+            T acc = rhs[i];
+            for (int k = ptr[i]; k < diag[i]; ++k) acc = acc - a[k] * w[ja[k]];
+            w[i] = acc * d[i];
+        }
+        if (l + 1 < nlev && !grid_sync(bar, gridDim.x)) return;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bwd_persistent(const int *__restrict__ rows, const int *__restrict__ lvl,
+                                                        int nlev, const int *__restrict__ ptr,
+                                                        const int *__restrict__ ja, const int *__restrict__ diag,
+                                                        const T *__restrict__ a, const T *__restrict__ d, T small,
+                                                        T *w, const Scalars *__restrict__ sc, GridBar *bar) {
+    if (sc && sc->done) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int l = 0; l < nlev; ++l) {
+        const int beg = lvl[l], end = lvl[l + 1];
+        for (int p = beg + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += stride) {
+            int i = rows[p];
+            T di = d[i];
+            T acc = w[i] / (di + small);
+            for (int k = diag[i] + 1; k < ptr[i + 1]; ++k) acc = acc - a[k] * w[ja[k]];
+            w[i] = acc * di;
+        }
+        if (l + 1 < nlev && !grid_sync(bar, gridDim.x)) return;
+    }
+}
+
+SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar) {
+    SweepCfg c;
+    if (ctx->sweep_mode != 1 || !bar) return c;
+    int occ = 0;
+    FC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_fwd_persistent<double>, 256, 0));
+    int occ2 = 0;
+    FC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, (const void *)k_bwd_persistent<double>, 256, 0));
+    occ = std::min(std::min(occ, occ2), 2);
+    int want = std::max(1, (std::max(st.max_level_f, st.max_level_b) + 255) / 256);
+    c.blocks = std::max(1, std::min(occ * ctx->num_cus, want));
+    c.bar = bar;
+    return c;
+}
+
+template <class T>
+void lu_persistent(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
+                   const Scalars *sc, const SweepCfg &cfg) {
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
+    const int *rf = st.rows_f.as<int>(), *rb = st.rows_b.as<int>();
+    const int *lf = st.lvl_f_d.as<int>(), *lb = st.lvl_b_d.as<int>();
+    int nf = st.nlev_f, nb = st.nlev_b;
+    GridBar *bar = cfg.bar;
+    void *fargs[] = {(void *)&rf, (void *)&lf, (void *)&nf, (void *)&ptr, (void *)&ja, (void *)&dg,
+                     (void *)&a, (void *)&d, (void *)&rhs, (void *)&w, (void *)&sc, (void *)&bar};
+    FC_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_fwd_persistent<T>, dim3(cfg.blocks), dim3(256), fargs, 0,
+                                            s));
+    void *bargs[] = {(void *)&rb, (void *)&lb, (void *)&nb, (void *)&ptr, (void *)&ja, (void *)&dg,
+                     (void *)&a, (void *)&d, (void *)&small, (void *)&w, (void *)&sc, (void *)&bar};
+    FC_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_bwd_persistent<T>, dim3(cfg.blocks), dim3(256), bargs, 0,
+                                            s));
+}
+
 template <class T>
 void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
-                 const Scalars *sc) {
+                 const Scalars *sc, const SweepCfg *cfg) {
+    if (cfg && cfg->bar) {
+        lu_persistent<T>(s, st, a, d, small, rhs, w, sc, *cfg);
+        return;
+    }
     const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
     const int *rf = st.rows_f.as<int>(), *rb = st.rows_b.as<int>();
     for (int l = 0; l < st.nlev_f; ++l) {
@@ -252,13 +395,13 @@ void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T s
 }
 
 void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
-                 const double *rhs, double *w, const Scalars *sc) {
-    lu_launch_t<double>(s, st, a, d, small, rhs, w, sc);
+                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg) {
+    lu_launch_t<double>(s, st, a, d, small, rhs, w, sc, cfg);
 }
 
 void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
-                 float *w, const Scalars *sc) {
-    lu_launch_t<float>(s, st, a, d, small, rhs, w, sc);
+                 float *w, const Scalars *sc, const SweepCfg *cfg) {
+    lu_launch_t<float>(s, st, a, d, small, rhs, w, sc, cfg);
 }
 
 // ----------------------------------------------------------- reductions

@@ -555,6 +555,10 @@ static void check_err(fc_context *ctx, const DevBuf &err, const char *what) {
 void analyse_levels(fc_context *ctx, Structure &st) {
     levels_one(ctx, st, false, st.rows_f, st.lvl_f, st.nlev_f, st.max_level_f);
     levels_one(ctx, st, true, st.rows_b, st.lvl_b, st.nlev_b, st.max_level_b);
+    st.lvl_f_d.alloc(sizeof(int) * st.lvl_f.size());
+    h2d(st.lvl_f_d.get(), st.lvl_f.data(), sizeof(int) * st.lvl_f.size(), ctx->stream);
+    st.lvl_b_d.alloc(sizeof(int) * st.lvl_b.size());
+    h2d(st.lvl_b_d.get(), st.lvl_b.data(), sizeof(int) * st.lvl_b.size(), ctx->stream);
 }
 
 void build_tpos(fc_context *ctx, Structure &st) {

@@ -13,7 +13,7 @@ struct Workspace {
     // FP32 vectors and matrix copies
     DevBuf r32, z32, p32, q32, e32, a32, d32;
     // reduction partials, summed scalars, device solver state, error flag
-    DevBuf pa, pb, sums, sc, err;
+    DevBuf pa, pb, sums, sc, err, bar;
     // learned preconditioner scratch
     DevBuf g0, g1, g2;
     int n = 0, nnz = 0;
@@ -41,6 +41,11 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 // FP32 inner solves (DIC-PCG or DILU-BiCGStab) inside FP64 iterative refinement.
 fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicgstab, fc_solve_info *info);
 void cast_d2f(hipStream_t s, long long n, const double *x, float *y);
+// Flexible CG (FCG(1)) in FP32 with preconditioner `P` inside FP64 iterative
+// refinement; on stagnation or a non-finite iterate the solve restarts from
+// the last refined iterate with FP64 DIC-PCG (status FC_FALLBACK_USED).
+struct PrecondHook;
+fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P, fc_solve_info *info);
 // ||su - A fi||_2 / ||su||_2 in FP64 (O-C terms included).
 double true_rel_l2(fc_context *ctx, const SolveArgs &args);
 

@@ -15,6 +15,7 @@
 #include <functional>
 
 #include "fcap_solvers.h"
+#include "fcap_gnn.h"
 
 namespace fcap {
 
@@ -34,8 +35,15 @@ enum FinStep : int {
     kFinGam = 5,      // BiCGStab: ukreso, gam
     kFinAlfB = 6,     // BiCGStab: alf = vr/(vv+small)
     kFinBicgEnd = 7,  // BiCGStab: resl, iteration count, convergence
-    kFinOuter = 8     // refinement: true residual norms
+    kFinOuter = 8,    // refinement: true residual norms
+    kFinFcgBeta = 9,  // FCG(1): bet = -(z.q_prev)/(p_prev.q_prev)
+    kFinFcgPr = 10,   // FCG: p.r
+    kFinFcgAlf = 11,  // FCG: alf = p.r / p.q
+    kFinFcgEnd = 12   // FCG: residual, convergence, stagnation every 16 iterations
 };
+
+constexpr int kStagEvery = 16;          // FCG stagnation check period (iterations)
+constexpr double kStagFactor = 0.81;    // ||r||^2 must drop below 0.81x (||r|| by 10%) per period
 
 __device__ inline void converge_test(Scalars *sc) {
     sc->iters += 1;
@@ -113,7 +121,44 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
         sc->resl = a;
         sc->rel = sqrt(b / sc->bnorm2);
         break;
+    case kFinFcgBeta:
+        sc->bet = -a / sc->fcg_pq;
+        break;
+    case kFinFcgPr:
+        sc->fcg_pr = a;
+        break;
+    case kFinFcgAlf:
+        sc->pkapk = a;
+        sc->alf = sc->fcg_pr / a;
+        sc->fcg_pq = a;
+        break;
+    case kFinFcgEnd:
+        sc->resl = a;
+        sc->rnorm2 = b;
+        converge_test(sc);
+        if (!sc->done && sc->iters % kStagEvery == 0) {
+            if (sc->rnorm2 > kStagFactor * sc->stag_ref) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
+            sc->stag_ref = sc->rnorm2;
+        }
+        break;
     }
+}
+
+// p = z + bet p and the partial of p.r (flexible CG)
+__global__ __launch_bounds__(kRedBlock) void k_fcg_p(int n, float *__restrict__ p, const float *__restrict__ z,
+                                                     const float *__restrict__ r, const Scalars *__restrict__ sc,
+                                                     double *__restrict__ pa) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    const float bet = (float)sc->bet;
+    double s = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        const float pv = z[i] + bet * p[i];
+        p[i] = pv;
+        s += (double)pv * (double)r[i];
+    }
+    s = block_sum<kRedBlock>(s, sh);
+    if (threadIdx.x == 0) pa[blockIdx.x] = s;
 }
 
 // x += alf p ; r -= alf q ; partials |r| and r^2 (src/iccg.f90:147-154)
@@ -260,6 +305,7 @@ struct Ctx {
     Scalars *sc;
     double *pa, *pb;
     int gv;   // grid of the vector kernels
+    SweepCfg sw;   // how the triangular sweeps run
 };
 
 Scalars init_scalars(const SolveArgs &args, double small) {
@@ -310,9 +356,36 @@ double norm2_host(Ctx &c, const double *v) {
     return h[0];
 }
 
+// Runs `iter` until the device sets `done`.  One iteration is captured into a
+// HIP graph and replayed (the level-scheduled sweeps make an iteration hundreds
+// of small launches); the null stream cannot be captured, so it launches directly.
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() { if (exec) (void)hipGraphExecDestroy(exec); }
+};
+
 void run_batches(Ctx &c, int batch, const std::function<void()> &iter) {
+    GraphExec ge;
+    if (c.ctx->use_graphs && c.s != nullptr && !c.sw.bar) {
+        hipGraph_t g = nullptr;
+        FC_HIP_CHECK(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+        try {
+            iter();
+        } catch (...) {
+            (void)hipStreamEndCapture(c.s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        FC_HIP_CHECK(hipStreamEndCapture(c.s, &g));
+        hipError_t e = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FC_HIP_CHECK(e);
+    }
     while (true) {
-        for (int k = 0; k < batch; ++k) iter();
+        for (int k = 0; k < batch; ++k) {
+            if (ge.exec) FC_HIP_CHECK(hipGraphLaunch(ge.exec, c.s));
+            else iter();
+        }
         FC_HIP_CHECK(hipGetLastError());
         if (read_done(c)) break;
     }
@@ -336,14 +409,29 @@ void Workspace::ensure(int n_, int nnz_) {
     sums.ensure(sizeof(double) * 4);
     sc.ensure(sizeof(Scalars));
     err.ensure(sizeof(int) * 4);
+    bar.ensure(sizeof(GridBar));
     n = n_;
     nnz = nnz_;
 }
 
 static Ctx make_ctx(fc_context *ctx, int n) {
     Workspace &ws = *ctx->ws;
-    Ctx c{ctx, ctx->stream, ws, n, ws.sc.as<Scalars>(), ws.pa.as<double>(), ws.pb.as<double>(), red_grid(n)};
+    Ctx c{ctx, ctx->stream, ws, n, ws.sc.as<Scalars>(), ws.pa.as<double>(), ws.pb.as<double>(), red_grid(n), {}};
     return c;
+}
+
+static void setup_sweeps(Ctx &c, const Structure &st) {
+    if (c.ctx->sweep_mode != 1) return;
+    FC_HIP_CHECK(hipMemsetAsync(c.ws.bar.get(), 0, sizeof(GridBar), c.s));
+    c.sw = make_sweep_cfg(c.ctx, st, c.ws.bar.as<GridBar>());
+}
+
+static void check_sweeps(Ctx &c) {
+    if (!c.sw.bar) return;
+    GridBar h;
+    FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.bar, sizeof(GridBar), hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipStreamSynchronize(c.s));
+    if (h.abort) throw Error(FC_ERR_HIP, "persistent sweep: a grid barrier wait exceeded its spin limit");
 }
 
 static void check_pivots(Ctx &c) {
@@ -397,6 +485,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     const Structure &st = *args.st;
     Workspace &ws = *ctx->ws;
     Ctx c = make_ctx(ctx, st.n);
+    setup_sweeps(c, st);
     const int n = st.n;
     const double small = args.variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
     double *r = ws.r.as<double>(), *z = ws.z.as<double>(), *p = ws.p.as<double>(), *q = ws.q.as<double>();
@@ -420,7 +509,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     const int gs = spmv_grid(n);
     auto iter = [&]() {
         if (dic) {
-            lu_launch64(c.s, st, args.a, d, small, r, z, c.sc);
+            lu_launch64(c.s, st, args.a, d, small, r, z, c.sc, &c.sw);
             k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
         } else {
             k_jacobi<double, double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, args.a, dg, z, c.sc, c.pa);
@@ -442,6 +531,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     ctx->timer.start(c.s);
     run_batches(c, dic ? 1 : 8, iter);
     double solve_ms = ctx->timer.stop_ms(c.s);
+    check_sweeps(c);
     Scalars hs = read_scalars(c);
     fill_info(info, hs, setup_ms, solve_ms);
     return (fc_status)hs.status;
@@ -451,6 +541,7 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
     const Structure &st = *args.st;
     Workspace &ws = *ctx->ws;
     Ctx c = make_ctx(ctx, st.n);
+    setup_sweeps(c, st);
     const int n = st.n;
     const double small = args.variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
     double *res = ws.r.as<double>(), *reso = ws.reso.as<double>(), *pk = ws.p.as<double>();
@@ -479,11 +570,11 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
         k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, res, reso, c.sc, c.pa);
         fin(c, kFinBet, c.gv, false);
         k_bicg_p<double><<<c.gv, 256, 0, c.s>>>(n, pk, res, uk, c.sc);
-        lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc);
+        lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc, &c.sw);
         spmv_launch64(c.s, 2, n, ptr, ja, args.a, zk, nullptr, reso, uk, nullptr, c.pa, nullptr, c.sc);
         fin(c, kFinGam, gs, false);
         k_bicg_xr1<double, double><<<c.gv, 256, 0, c.s>>>(n, args.fi, res, zk, uk, c.sc);
-        lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc);
+        lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc, &c.sw);
         spmv_launch64(c.s, 3, n, ptr, ja, args.a, zk, nullptr, res, vk, nullptr, c.pa, c.pb, c.sc);
         fin(c, kFinAlfB, gs, true);
         k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, res, zk, vk, c.sc, c.pa, c.pb);
@@ -492,6 +583,7 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
     ctx->timer.start(c.s);
     run_batches(c, 1, iter);
     double solve_ms = ctx->timer.stop_ms(c.s);
+    check_sweeps(c);
     Scalars hs = read_scalars(c);
     fill_info(info, hs, setup_ms, solve_ms);
     return (fc_status)hs.status;
@@ -524,7 +616,7 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
     if (!bicg) {
         auto iter = [&]() {
-            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc);
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw);
             k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
             fin(c, kFinSk, c.gv, false);
             k_p_update<float><<<c.gv, 256, 0, c.s>>>(n, p, z, c.sc);
@@ -543,11 +635,11 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
             k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, reso, c.sc, c.pa);
             fin(c, kFinBet, c.gv, false);
             k_bicg_p<float><<<c.gv, 256, 0, c.s>>>(n, p, r, uk, c.sc);
-            lu_launch32(c.s, st, a32, d32, small, p, z, c.sc);
+            lu_launch32(c.s, st, a32, d32, small, p, z, c.sc, &c.sw);
             spmv_launch32(c.s, 2, n, ptr, ja, a32, z, nullptr, reso, uk, c.pa, nullptr, c.sc);
             fin(c, kFinGam, gs, false);
             k_bicg_xr1<float, float><<<c.gv, 256, 0, c.s>>>(n, e, r, z, uk, c.sc);
-            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc);
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw);
             spmv_launch32(c.s, 3, n, ptr, ja, a32, z, nullptr, r, vk, c.pa, c.pb, c.sc);
             fin(c, kFinAlfB, gs, true);
             k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, z, vk, c.sc, c.pa, c.pb);
@@ -563,6 +655,7 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     const Structure &st = *args.st;
     Workspace &ws = *ctx->ws;
     Ctx c = make_ctx(ctx, st.n);
+    setup_sweeps(c, st);
     const int n = st.n;
     double *r = ws.r.as<double>(), *d = ws.d.as<double>();
     // set-up: FP64 pivots, FP32 copies of the matrix and pivots
@@ -605,12 +698,125 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
         ++outer;
     }
     double solve_ms = ctx->timer.stop_ms(c.s);
+    check_sweeps(c);
     if (info) {
         info->res0 = res0;
         info->resl = resl;
         info->rsm = res0 > 0 ? resl / (res0 + kSmallSrc) : 0.0;
         info->iters = total_inner;
         info->outer = outer;
+        info->status = status;
+        info->setup_ms = setup_ms;
+        info->solve_ms = solve_ms;
+    }
+    return status;
+}
+
+// FCG(1) in FP32 on A e = r32 (unit-norm r32), preconditioner P.  Returns the
+// inner status (FC_OK, FC_NOT_CONVERGED, FC_FALLBACK_USED on stagnation,
+// FC_ERR_NONFINITE) and adds the iterations to `iters`.
+static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double eta, int cap, int &iters) {
+    Workspace &ws = c.ws;
+    const int n = c.n;
+    float *a32 = ws.a32.as<float>();
+    float *r = ws.r32.as<float>(), *z = ws.z32.as<float>(), *p = ws.p32.as<float>(), *q = ws.q32.as<float>();
+    float *e = ws.e32.as<float>();
+    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>();
+    const int gs = spmv_grid(n);
+    Scalars h;
+    std::memset(&h, 0, sizeof(h));
+    h.small = kSmallSrc;
+    h.nsw = cap;
+    h.stop_rule = 2;
+    h.inner_tol2 = eta * eta;
+    h.res0 = 1.0;
+    h.fcg_pq = 1.0;
+    h.stag_ref = 1.0;
+    h.status = kStatusRunning;
+    upload_scalars(c, h);
+    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
+    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
+    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, q);
+    auto iter = [&]() {
+        P.apply(P.obj, c.s, r, z, c.sc);
+        k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, z, q, c.sc, c.pa);
+        fin(c, kFinFcgBeta, c.gv, false);
+        k_fcg_p<<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, c.sc, c.pa);
+        fin(c, kFinFcgPr, c.gv, false);
+        spmv_launch32(c.s, 2, n, ptr, ja, a32, p, nullptr, p, q, c.pa, nullptr, c.sc);
+        fin(c, kFinFcgAlf, gs, false);
+        k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, p, q, c.sc, c.pa, c.pb);
+        fin(c, kFinFcgEnd, c.gv, true);
+    };
+    run_batches(c, 4, iter);
+    Scalars hs = read_scalars(c);
+    iters += hs.iters;
+    return hs.status;
+}
+
+fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P, fc_solve_info *info) {
+    FC_REQUIRE(P.apply != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
+    const Structure &st = *args.st;
+    Workspace &ws = *ctx->ws;
+    Ctx c = make_ctx(ctx, st.n);
+    const int n = st.n;
+    double *r = ws.r.as<double>(), *xgood = ws.tmp.as<double>();
+    ctx->timer.start(c.s);
+    k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
+    double setup_ms = ctx->timer.stop_ms(c.s);
+    Scalars h0 = init_scalars(args, kSmallSrc);
+    h0.bnorm2 = norm2_host(c, args.su);
+    const double bnorm = std::sqrt(h0.bnorm2);
+    const bool l2 = args.stop_rule == FC_STOP_L2;
+    int total_inner = 0, outer = 0, fallback_iters = 0;
+    double res0 = 0.0, resl = 0.0;
+    fc_status status = FC_NOT_CONVERGED;
+    ctx->timer.start(c.s);
+    for (;;) {
+        h0.done = 0;
+        upload_scalars(c, h0);
+        spmv_launch64(c.s, 1, n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
+                      nullptr, c.pa, c.pb, nullptr);
+        fin(c, kFinOuter, spmv_grid(n), true);
+        Scalars ho = read_scalars(c);
+        if (outer == 0) res0 = ho.resl;
+        resl = ho.resl;
+        const double rel = bnorm > 0 ? std::sqrt(ho.rnorm2) / bnorm : std::sqrt(ho.rnorm2);
+        if (!std::isfinite(rel)) { status = FC_ERR_NONFINITE; break; }
+        const bool conv = l2 ? rel < args.sor : (res0 > 0 ? resl / (res0 + kSmallSrc) < args.sor : true);
+        if (conv) { status = FC_OK; break; }
+        if (outer >= args.max_outer || total_inner >= args.nsw) { status = FC_NOT_CONVERGED; break; }
+        FC_HIP_CHECK(hipMemcpyAsync(xgood, args.fi, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+        const double rn = std::sqrt(ho.rnorm2);
+        k_scale_cast<<<c.gv, 256, 0, c.s>>>(n, r, ws.r32.as<float>(), 1.0 / rn);
+        const int cap = std::max(1, args.nsw - total_inner);
+        const int ist = inner_fcg32(c, st, P, args.inner_eta, cap, total_inner);
+        ++outer;
+        if (ist == FC_FALLBACK_USED || ist == FC_ERR_NONFINITE) {
+            // plain IC(0)-PCG (FP64) finishes the solve from the last refined iterate
+            FC_HIP_CHECK(hipMemcpyAsync(args.fi, xgood, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+            SolveArgs fb = args;
+            fb.stop_rule = FC_STOP_L2;
+            fb.nsw = std::max(1, args.nsw);
+            fc_solve_info fi;
+            std::memset(&fi, 0, sizeof fi);
+            ensure_levels(ctx, const_cast<Structure &>(st));
+            fc_status fs = solve_pcg64(ctx, fb, true, &fi);
+            fallback_iters = fi.iters;
+            resl = fi.resl;
+            status = (fs == FC_OK) ? FC_FALLBACK_USED : fs;
+            break;
+        }
+        k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);
+    }
+    double solve_ms = ctx->timer.stop_ms(c.s);
+    if (info) {
+        info->res0 = res0;
+        info->resl = resl;
+        info->rsm = res0 > 0 ? resl / (res0 + kSmallSrc) : 0.0;
+        info->iters = total_inner;
+        info->outer = outer;
+        info->fallback_iters = fallback_iters;
         info->status = status;
         info->setup_ms = setup_ms;
         info->solve_ms = solve_ms;

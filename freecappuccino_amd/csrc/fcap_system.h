@@ -1,0 +1,34 @@
+// The device-resident system (fc_system of include/fcap.h), shared by the C ABI
+// layer and the learned preconditioner.
+#pragma once
+
+#include "fcap_gnn.h"
+#include "fcap_internal.h"
+
+struct fc_gnn;
+
+struct fc_system {
+    fc_context *ctx = nullptr;
+    fcap::MeshDev mesh;
+    fcap::Structure st;
+    fcap::DevBuf ij, ji, kface, kord;
+    bool needs_order = false;
+    fcap::WallDev walls;
+    fcap::OcDev oc;
+    int ntotal = 0;
+    fcap::DevBuf a, su, fi, mu, phi, su0, vis, flux;
+    bool su0_zero = true;
+    double last_ms[3] = {0, 0, 0};
+    // bumped by every assembly; the network's graph data is rebuilt when stale
+    long assembly_version = 0;
+    fcap::GnnGraph graph;
+    long graph_version = -1;
+    fc_gnn *net = nullptr;     // preconditioner of FC_METHOD_FCG_LEARNED
+};
+
+namespace fcap {
+// Graph data of the system's current matrix (rebuilt after re-assembly).
+GnnGraph &system_graph(fc_system *sys);
+// The attached network as the preconditioner hook of flexible CG (apply == nullptr if none).
+PrecondHook gnn_hook(fc_system *sys);
+}  // namespace fcap

@@ -65,6 +65,7 @@ def library() -> C.CDLL:
         raise FcError(-7, f"{LIB_PATH} not built: run __graft_entry__.build() (no CPU fallback exists)")
     L = C.CDLL(LIB_PATH)
     vp, i, d, ip, dp = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
     si = C.POINTER(SolveInfo)
     sig = {
         "fc_init": ([C.POINTER(vp), i], i), "fc_finalize": ([vp], i), "fc_set_stream": ([vp, vp], i),
@@ -94,6 +95,20 @@ def library() -> C.CDLL:
         "fc_sys_sizes": ([vp, ip, ip, ip, ip, ip], i),
         "fc_sys_last_ms": ([vp, i], d),
         "fc_sys_spmv_bench": ([vp, i, i], d),
+        "fc_copy_bench": ([vp, C.c_size_t, i], d),
+        "fc_gnn_num_params": ([], i),
+        "fc_gnn_create": ([vp, C.POINTER(vp), fp], i),
+        "fc_gnn_destroy": ([vp], i),
+        "fc_gnn_set_params": ([vp, fp], i),
+        "fc_gnn_get_params": ([vp, fp], i),
+        "fc_gnn_apply": ([vp, vp, fp, fp], i),
+        "fc_gnn_edge_features": ([vp, vp, fp, fp, fp], i),
+        "fc_gnn_loss_grad": ([vp, vp, fp, fp, dp, fp], i),
+        "fc_gnn_add_samples": ([vp, vp, i, ip], i),
+        "fc_gnn_num_samples": ([vp], i),
+        "fc_gnn_clear_samples": ([vp], i),
+        "fc_gnn_train": ([vp, i, i, C.c_float, C.c_uint, fp], i),
+        "fc_sys_set_network": ([vp, vp], i),
     }
     for name in ("fc_iccg", "fc_dpcg", "fc_bicgstab"):
         sig[name] = ([vp, i, i, ip, ip, ip, dp, dp, dp, dp, i, ip, ip, dp, dp, d, i, si], i)
@@ -153,6 +168,13 @@ class Context:
 
     def set_report(self, on: bool, name: str = "p"):
         self.check(self.L.fc_set_report(self.h, 1 if on else 0, name.encode()))
+
+    def copy_bench(self, nbytes: int, reps: int = 10) -> float:
+        """Average ms of a plain device copy of nbytes (HBM counter calibration)."""
+        ms = self.L.fc_copy_bench(self.h, nbytes, reps)
+        if ms < 0:
+            raise FcError(-2, self.L.fc_last_error(self.h).decode())
+        return ms
 
     def set_stream(self, stream_ptr: int):
         self.check(self.L.fc_set_stream(self.h, C.c_void_p(stream_ptr)))

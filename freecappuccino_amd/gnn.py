@@ -1,0 +1,158 @@
+"""The learned preconditioner: network handle, parameter initialisation, training driver.
+
+The network (message passing over the cell graph, hidden width 32, 4 layers,
+BF16 weights/activations, MFMA node GEMMs) is evaluated and trained by the
+library's kernels (csrc/fcap_gnn.hip); this module only drives them.  It is
+the preconditioner of flexible CG (METHOD_FCG_LEARNED), with IC(0)-PCG as the
+fallback on stagnation.
+
+Training data (as for the 64^3 training case): (residual, error) pairs of
+IC(0)-PCG iterates on assembled pressure systems of perturbed hex meshes, one
+mesh seed per system, each with a zero-mean Gaussian-blob RHS.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Optional
+
+import numpy as np
+
+from . import fcap
+from . import workloads as W
+
+H = 32
+L = 4
+LAYER_STRIDE = 2 * H * H + H + 3
+OFF_ENC_W, OFF_ENC_B, OFF_LAYER0 = 0, H, 2 * H
+OFF_DEC_W = OFF_LAYER0 + L * LAYER_STRIDE
+OFF_DEC_B = OFF_DEC_W + H
+N_PARAMS = OFF_DEC_B + 1
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "pressure_gnn.npy")
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def init_params(seed: int) -> np.ndarray:
+    """Scaled-uniform initialisation; the decoder starts small so z ~ Jacobi at step 0."""
+    rng = np.random.default_rng(seed)
+    th = np.zeros(N_PARAMS, np.float32)
+    th[OFF_ENC_W:OFF_ENC_W + H] = rng.uniform(-1, 1, H)
+    th[OFF_ENC_B:OFF_ENC_B + H] = rng.uniform(0, 0.1, H)
+    sc = 0.5 / np.sqrt(H)
+    for l in range(L):
+        o = OFF_LAYER0 + l * LAYER_STRIDE
+        th[o:o + H * H] = rng.uniform(-sc, sc, H * H)
+        th[o + H * H:o + 2 * H * H] = rng.uniform(-sc, sc, H * H)
+        th[o + 2 * H * H + H:o + 2 * H * H + H + 3] = [0.0, 0.5, 0.0]
+    th[OFF_DEC_W:OFF_DEC_W + H] = rng.uniform(-0.1, 0.1, H) / np.sqrt(H)
+    return th
+
+
+class Network:
+    def __init__(self, params: Optional[np.ndarray] = None, ctx: Optional[fcap.Context] = None):
+        self.ctx = ctx or fcap.default_context()
+        L_ = self.ctx.L
+        assert L_.fc_gnn_num_params() == N_PARAMS
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        h = C.c_void_p()
+        self.ctx.check(L_.fc_gnn_create(self.ctx.h, C.byref(h), None if p is None else _fp(p)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.fc_gnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(N_PARAMS, np.float32)
+        self.ctx.check(self.ctx.L.fc_gnn_get_params(self.h, _fp(out)))
+        return out
+
+    @params.setter
+    def params(self, value):
+        v = np.ascontiguousarray(value, np.float32)
+        assert v.shape == (N_PARAMS,)
+        self.ctx.check(self.ctx.L.fc_gnn_set_params(self.h, _fp(v)))
+
+    def apply(self, system: fcap.System, r) -> np.ndarray:
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros_like(r)
+        self.ctx.check(self.ctx.L.fc_gnn_apply(self.h, system.h, _fp(r), _fp(z)))
+        return z
+
+    def edge_features(self, system: fcap.System):
+        nnz = system.sizes()["nnz"]
+        e1, e2, dg = np.zeros(nnz, np.float32), np.zeros(nnz, np.float32), np.zeros(system.n, np.float32)
+        self.ctx.check(self.ctx.L.fc_gnn_edge_features(self.h, system.h, _fp(e1), _fp(e2), _fp(dg)))
+        return e1, e2, dg
+
+    def loss_grad(self, system: fcap.System, r, e):
+        r = np.ascontiguousarray(r, np.float32)
+        e = np.ascontiguousarray(e, np.float32)
+        loss = C.c_double()
+        g = np.zeros(N_PARAMS, np.float32)
+        self.ctx.check(self.ctx.L.fc_gnn_loss_grad(self.h, system.h, _fp(r), _fp(e), C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def add_samples(self, system: fcap.System, iters: Iterable[int]):
+        it = np.ascontiguousarray(list(iters), np.int32)
+        self.ctx.check(self.ctx.L.fc_gnn_add_samples(self.h, system.h, it.shape[0],
+                                                     it.ctypes.data_as(C.POINTER(C.c_int))))
+
+    @property
+    def num_samples(self) -> int:
+        return self.ctx.L.fc_gnn_num_samples(self.h)
+
+    def clear_samples(self):
+        self.ctx.check(self.ctx.L.fc_gnn_clear_samples(self.h))
+
+    def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(steps, np.float32)
+        self.ctx.check(self.ctx.L.fc_gnn_train(self.h, steps, batch, float(lr), seed, _fp(losses)))
+        return losses
+
+    def attach(self, system: fcap.System):
+        self.ctx.check(self.ctx.L.fc_sys_set_network(system.h, self.h))
+
+    def save(self, path: str = DEFAULT_WEIGHTS):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.save(path, self.params)
+
+    @classmethod
+    def load(cls, path: str = DEFAULT_WEIGHTS, ctx: Optional[fcap.Context] = None) -> "Network":
+        return cls(np.load(path, allow_pickle=False), ctx)
+
+
+def training_system(n: int, seed: int, ctx: fcap.Context, nblobs: int = 8) -> fcap.System:
+    """A pressure system of the training case: perturbed n^3 hex mesh (seed), blob RHS (seed)."""
+    mesh = W.case_mesh(4, n, seed=seed)
+    s = fcap.System(mesh, ctx)
+    xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")
+    su0 = W.blob_rhs(xc, yc, zc, vol, nblobs, seed=10_000 + seed)
+    s.assemble_laplacian(np.full(mesh.n_cells, -1.0), None, su0)
+    return s
+
+
+def build_dataset(net: Network, n: int, n_systems: int, iters=(0, 2, 5, 10, 20, 40), seed0: int = 0,
+                  ctx: Optional[fcap.Context] = None) -> List[fcap.System]:
+    """Assemble n_systems training systems and add their IC(0)-PCG (r, e) pairs; returns the systems
+    (they must stay alive while the network trains on their samples)."""
+    ctx = ctx or net.ctx
+    systems = []
+    for k in range(n_systems):
+        s = training_system(n, seed0 + k, ctx)
+        net.add_samples(s, iters)
+        systems.append(s)
+    return systems

@@ -248,6 +248,42 @@ double fc_sys_last_ms(fc_system *sys, int stage);
  * matrix; returns the average device time per SpMV in milliseconds, or < 0. */
 double fc_sys_spmv_bench(fc_system *sys, int precision, int reps);
 
+/* --------------------------------------------- learned preconditioner */
+
+/* A message-passing network over the cell graph (hidden width 32, 4 layers,
+ * BF16 weights and activations, MFMA node GEMMs with FP32 accumulation) used
+ * as the preconditioner of flexible CG (FC_METHOD_FCG_LEARNED).  No
+ * counterpart in the reference.  Parameters are a flat FP32 vector of
+ * fc_gnn_num_params() entries (layout in oracle/gnn_oracle.py and
+ * csrc/fcap_gnn.h). */
+typedef struct fc_gnn fc_gnn;
+int fc_gnn_num_params(void);
+fc_status fc_gnn_create(fc_context *ctx, fc_gnn **net, const float *params /* NULL: zeros */);
+fc_status fc_gnn_destroy(fc_gnn *net);
+fc_status fc_gnn_set_params(fc_gnn *net, const float *params);
+fc_status fc_gnn_get_params(fc_gnn *net, float *params);
+/* z = P(r) on a system's graph (host arrays of numCells floats). */
+fc_status fc_gnn_apply(fc_gnn *net, fc_system *sys, const float *r, float *z);
+/* Edge features of the system's CSR entries (nnz each; 0 on the diagonal) and its FP32 diagonal. */
+fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2, float *diag);
+/* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
+fc_status fc_gnn_loss_grad(fc_gnn *net, fc_system *sys, const float *r, const float *e, double *loss, float *grad);
+/* Add (residual, error) pairs of IC(0)-PCG iterates on the system's current
+ * RHS, after iters[0..nk-1] iterations from zero.  The system must outlive
+ * the samples. */
+fc_status fc_gnn_add_samples(fc_gnn *net, fc_system *sys, int nk, const int *iters);
+int fc_gnn_num_samples(fc_gnn *net);
+fc_status fc_gnn_clear_samples(fc_gnn *net);
+/* Adam over `steps` mini-batches of `batch` samples; loss_out (steps floats, may be NULL). */
+fc_status fc_gnn_train(fc_gnn *net, int steps, int batch, float lr, unsigned seed, float *loss_out);
+/* Attach a network as the preconditioner of FC_METHOD_FCG_LEARNED solves of sys. */
+fc_status fc_sys_set_network(fc_system *sys, fc_gnn *net);
+
+/* No counterpart in the reference: a plain device-to-device copy kernel of
+ * `bytes` bytes, `reps` times (calibration of the HBM traffic counters and the
+ * achievable-bandwidth figure); returns the average milliseconds per copy. */
+double fc_copy_bench(fc_context *ctx, size_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif
