@@ -4,20 +4,26 @@ A step is one pass of the hot path over the resident system: face-loop
 assembly of the Laplacian into the CSR matrix (fvm_laplacian) followed by the
 Krylov solve to 1e-8 FP64 relative residual ||su - A fi||_2 / ||su||_2.
 
-Default workload: 128^3 hex box with interior vertices perturbed by +-20%
-(2,097,152 cells), mu = -1, Dirichlet wall at x=0, RHS = 16 seeded Gaussian
-blobs with zero mean.  Two solvers are timed on the same system:
-  * IC(0)-PCG in FP64 (the reference's DIC-PCG, level-scheduled sweeps);
-  * the mixed-precision form: FP32 DIC-PCG inside FP64 iterative refinement.
-`value` is the time to 1e-8 of the faster of the two (ms, lower is better).
+Default workload: 192^3 hex box (7,077,888 cells), interior vertices
+perturbed by +-20% (seeded), spacing graded along x by ratio 1.05 toward the
+wall (aspect ratio capped at 100), mu = -1, Dirichlet wall at x = 0, RHS = 32
+seeded Gaussian blobs with zero mean.  Three solvers run on the same system:
+  * ic_pcg_fp64  -- IC(0)-PCG in FP64 (the reference's DIC-PCG, level-scheduled sweeps);
+  * ic_pcg_mixed -- FP32 DIC-PCG inside FP64 iterative refinement;
+  * fcg_learned  -- flexible CG in FP32 preconditioned by the trained BF16
+                    message-passing network, inside FP64 iterative refinement,
+                    IC(0)-PCG fallback on stagnation.
+`value` is the time to 1e-8 of fcg_learned (ms per step, lower is better);
+every solver's time, iterations, FP64 residual and error against the FP64
+IC(0)-PCG solution are in `solvers`.
 
-Also reported: FP64 SpMV GB/s on the same matrix (the roofline object, from
-HIP events on the library's stream), and the CPU baseline: the oracle's
+Also reported: FP64 SpMV GB/s on the same matrix (the roofline object, timed
+with HIP events on the library's stream), and the CPU baseline: the oracle's
 sequential DIC-PCG (the reference algorithm restated in C) timed on one host
-core over a bounded number of iterations and scaled to the GPU's iteration
-count.
+core over a bounded number of iterations and scaled to the GPU's FP64
+IC(0)-PCG iteration count.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n 128]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--case 5] [--n 192]
 """
 from __future__ import annotations
 
@@ -34,6 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned")
 
 
 def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
@@ -52,17 +59,20 @@ def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
     return {"value": per_iter * gpu_iters * 1e3, "unit": "ms", "cores": 1, "kind": "port",
             "sample": f"oracle DIC-PCG (C restatement of src/iccg.f90, gcc -O2, 1 thread) for {r.iters} "
                       f"iterations on the same system ({dt:.1f} s incl. pivots), per-iteration time x "
-                      f"{gpu_iters} GPU iterations; host {platform.processor() or platform.machine()}"}
+                      f"{gpu_iters} GPU FP64 IC(0)-PCG iterations; host "
+                      f"{platform.processor() or platform.machine()}"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--case", type=int, default=2, choices=(1, 2, 5),
+    ap.add_argument("--case", type=int, default=5, choices=(1, 2, 5),
                     help="1: 40^3 box; 2: 128^3 perturbed; 5: 192^3 perturbed, x-graded 1.05")
     ap.add_argument("--n", type=int, default=None, help="cells per direction (overrides the case's size)")
+    ap.add_argument("--methods", type=str, default=",".join(METHODS))
+    ap.add_argument("--weights", type=str, default=None, help="network weights (.npy); default: the packaged ones")
     ap.add_argument("--cpu-iters", type=int, default=40, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
@@ -73,8 +83,12 @@ def main():
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    methods = [m for m in args.methods.split(",") if m]
+    for m in methods:
+        if m not in METHODS:
+            ap.error(f"unknown method {m}")
 
-    from freecappuccino_amd import fcap
+    from freecappuccino_amd import fcap, gnn
     from freecappuccino_amd import workloads as W
 
     args.n = args.n or {1: 40, 2: 128, 5: 192}[args.case]
@@ -100,6 +114,13 @@ def main():
         ctx.close()
         return
 
+    net = None
+    if "fcg_learned" in methods:
+        net = gnn.Network.load(args.weights or gnn.DEFAULT_WEIGHTS, ctx)
+        net.attach(s)
+    code = {"ic_pcg_fp64": fcap.METHOD_ICCG, "ic_pcg_mixed": fcap.METHOD_ICCG_MIXED,
+            "fcg_learned": fcap.METHOD_FCG_LEARNED}
+
     def step(method):
         s.set_solution(np.zeros(n))
         t0 = time.perf_counter()
@@ -107,50 +128,57 @@ def main():
         rc, info = s.solve(method, fcap.STOP_L2, 1e-8, 20000)
         return rc, info, (time.perf_counter() - t0) * 1e3, s.last_ms(0)
 
-    results = {}
-    for name, method in (("ic_pcg_fp64", fcap.METHOD_ICCG), ("ic_pcg_mixed", fcap.METHOD_ICCG_MIXED)):
+    results, sols = {}, {}
+    for name in methods:
         for _ in range(args.warmup):
-            step(method)
+            step(code[name])
         times, infos = [], []
         for _ in range(args.steps):
-            rc, info, wall_ms, asm_ms = step(method)
+            rc, info, wall_ms, asm_ms = step(code[name])
             times.append(wall_ms)
             infos.append((rc, info, asm_ms))
         rc, info, asm_ms = infos[-1]
-        results[name] = {"ms": float(np.median(times)), "status": rc, "iters": info["iters"],
-                         "outer": info["outer"], "rel_l2": info["rel_l2"], "assembly_ms": asm_ms,
+        results[name] = {"ms": float(np.median(times)), "status": fcap.STATUS_NAMES.get(rc, rc),
+                         "iters": info["iters"], "outer": info["outer"], "fallback_iters": info["fallback_iters"],
+                         "rel_l2": info["rel_l2"], "assembly_ms": asm_ms,
                          "solve_device_ms": info["solve_ms"] + info["setup_ms"]}
-        results[name]["x"] = s.solution()
-    x64 = results["ic_pcg_fp64"].pop("x")
-    xm = results["ic_pcg_mixed"].pop("x")
-    results["ic_pcg_mixed"]["err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+        sols[name] = s.solution()
+        print(f"{name}: {results[name]}", file=sys.stderr, flush=True)
+    if net is not None:
+        net.detach(s)
+        net.close()
+    if "ic_pcg_fp64" in sols:
+        x64 = sols["ic_pcg_fp64"]
+        for name in results:
+            results[name]["err_vs_ic_pcg_fp64"] = float(np.linalg.norm(sols[name] - x64) / np.linalg.norm(x64))
+            results[name]["speedup_vs_ic_pcg_fp64"] = results["ic_pcg_fp64"]["ms"] / results[name]["ms"]
 
     spmv_ms = s.spmv_bench(64, args.spmv_reps)
     spmv32_ms = s.spmv_bench(32, args.spmv_reps)
     b64 = spmv_bytes(n, nnz, 8, 4)
     b32 = spmv_bytes(n, nnz, 4, 4)
     gbs = b64 / (spmv_ms * 1e-3) / 1e9
-    best = min(results, key=lambda k: results[k]["ms"])
-    value = results[best]["ms"]
-
+    head = "fcg_learned" if "fcg_learned" in results else min(results, key=lambda k: results[k]["ms"])
+    value = results[head]["ms"]
+    workload = (f"hex box {args.n}^3 perturbed +-20% (seeded)"
+                + (", x-graded 1.05 toward the wall (aspect <= 100)" if args.case == 5 else "")
+                + f", Laplacian mu=-1, Dirichlet x=0, {nblobs} Gaussian blobs; "
+                "face-loop assembly + solve to 1e-8 FP64 rel. L2 residual")
     line = {
         "metric": "pressure-solve time to 1e-8 rel. residual, learned-FCG vs IC-PCG; SpMV GB/s",
         "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if best == "ic_pcg_fp64" else "f32+f64",
+        "dtype": {"ic_pcg_fp64": "f64", "ic_pcg_mixed": "f32+f64", "fcg_learned": "bf16+f32+f64"}[head],
         "data": "synthetic",
-        "config": {"workload": f"hex box {args.n}^3 perturbed +-20% (seeded)"
-                               + (", x-graded 1.05 toward the wall" if args.case == 5 else "")
-                               + f", Laplacian mu=-1, Dirichlet x=0, {nblobs} Gaussian blobs; "
-                               "assembly + solve to 1e-8 FP64 rel. L2 residual",
-                   "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"], "best": best},
+        "config": {"workload": workload, "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"],
+                   "solver": head},
         "solvers": results,
         "spmv": {"fp64_ms": spmv_ms, "fp64_GBps": gbs, "fp32_ms": spmv32_ms,
                  "fp32_GBps": b32 / (spmv32_ms * 1e-3) / 1e9, "bytes_fp64": b64, "bytes_fp32": b32},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv<double,double,0>"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and "ic_pcg_fp64" in results:
         a = s.get("a")
         su = s.get("su")
         line["cpu_baseline"] = cpu_baseline(mesh, a, su, results["ic_pcg_fp64"]["iters"], args.cpu_iters)

@@ -739,6 +739,9 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_FCG_LEARNED: {
             FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
             PrecondHook P = gnn_hook(sys);
+            // FP32 FCG keeps its recursive residual honest to ~1e-6 relative, so each
+            // inner solve takes 4 orders: two refinement steps reach 1e-8
+            args.inner_eta = 1e-4;
             s = solve_fcg(ctx, args, P, &local);
             break;
         }

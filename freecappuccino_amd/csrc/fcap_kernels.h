@@ -21,6 +21,8 @@ constexpr int kSpmvRows = kWave * kSpmvWaves;    // rows per SpMV block
 constexpr int kSpmvCap = 512;                    // LDS entries staged per wave (8 per row on average)
 constexpr int kRedBlock = 256;                   // block size of the reduction kernels
 constexpr int kMaxRedGrid = 2048;                // grid cap of grid-stride reductions
+constexpr int kFcgM = 8;                         // directions kept by truncated flexible CG
+constexpr int kXcds = 8;                         // XCDs of an MI355X (workgroups are dealt round-robin)
 constexpr int kFinBlock = 1024;                  // block size of the finalize kernels
 
 // Device-side solver state shared by the kernels of one solve.  Kernels of an
@@ -32,8 +34,9 @@ struct Scalars {
     double om, gam, beto, ukreso, vr, vv;      // BiCGStab
     double bnorm2, rnorm2, rel;                // L2 quantities (mixed precision / L2 stop)
     double inner_tol2, inner_r0;               // inner-solve stopping threshold
-    double fcg_pq, fcg_pr;                     // flexible CG: p.q of the previous step, p.r
-    double stag_ref;                           // flexible CG: ||r||^2 at the last stagnation check
+    double fcg_pr;                             // flexible CG: p.r of this step
+    double fcg_pqh[kFcgM], fcg_bet[kFcgM];     // flexible CG: p.q and beta per history slot
+    double stag_ref, stag_min;                 // flexible CG: best ||r||^2 before / within the window
     int iters, done, status, nsw;
     int stop_rule;     // 0: L1 rsm < resmax; 1: L2 rel < resmax; 2: rnorm2 <= inner_tol2
     int early_exit;    // 1: stop before iterating when res0 < 1e-13 (src/iccg.f90:67-71)

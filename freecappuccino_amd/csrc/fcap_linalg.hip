@@ -30,6 +30,8 @@ inline int grid_for(long long n, int block) {
 
 int red_grid(int n) { return std::min(kMaxRedGrid, grid_for(n, kRedBlock)); }
 int spmv_grid(int n) { return grid_for(n, kSpmvRows); }
+// launch grid of a kernel with an XCD-aware block order: `blocks` rounded up to a multiple of kXcds
+static int xcd_grid(int blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
 // ------------------------------------------------------------------- SpMV
 
@@ -44,8 +46,16 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
     __shared__ int sj[kSpmvWaves][kSpmvCap];
     __shared__ double red[kSpmvWaves];
     if (sc && sc->done) return;
+    // XCD-aware order: workgroups go to the 8 XCDs round-robin, so workgroup b
+    // takes row block (b % 8) * per + b / 8 and each XCD sweeps one contiguous
+    // eighth of the rows; the x entries that neighbouring row blocks share then
+    // meet in one L2 instead of up to eight.  The grid is a multiple of 8.
+    const int nblk = (int)(((long long)n + kSpmvRows - 1) / kSpmvRows);
+    const int per = gridDim.x / kXcds;
+    const int blk = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if (blk >= nblk) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const long long row0 = (long long)blockIdx.x * kSpmvRows + (long long)w * kWave;
+    const long long row0 = (long long)blk * kSpmvRows + (long long)w * kWave;
     const long long rowl = row0 + lane;
     const bool valid = rowl < n;
     const int row = (int)rowl;
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
         if (threadIdx.x == 0) {
             double s = 0.0;
             for (int i = 0; i < kSpmvWaves; ++i) s += red[i];
-            partA[blockIdx.x] = s;
+            partA[blk] = s;
         }
         if (MODE == 1 || MODE == 3) {
             __syncthreads();
@@ -135,7 +145,7 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
             if (threadIdx.x == 0) {
                 double s = 0.0;
                 for (int i = 0; i < kSpmvWaves; ++i) s += red[i];
-                partB[blockIdx.x] = s;
+                partB[blk] = s;
             }
         }
     }
@@ -165,7 +175,7 @@ __global__ void k_oc_apply(int n, const int *__restrict__ co_ptr, const int *__r
 void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const double *a, const double *x,
                    const double *b, const double *v, double *y, float *y32, double *pa, double *pb,
                    const Scalars *sc) {
-    const int g = spmv_grid(n);
+    const int g = xcd_grid(spmv_grid(n));
     switch (mode) {
     case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
     case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
@@ -176,7 +186,7 @@ void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja
 
 void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const float *a, const float *x,
                    const float *b, const float *v, float *y, double *pa, double *pb, const Scalars *sc) {
-    const int g = spmv_grid(n);
+    const int g = xcd_grid(spmv_grid(n));
     switch (mode) {
     case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
     case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;

@@ -14,8 +14,8 @@ struct Workspace {
     DevBuf r32, z32, p32, q32, e32, a32, d32;
     // reduction partials, summed scalars, device solver state, error flag
     DevBuf pa, pb, sums, sc, err, bar;
-    // learned preconditioner scratch
-    DevBuf g0, g1, g2;
+    // flexible CG: the last kFcgM directions p and their images A p (FP32, kFcgM x n each)
+    DevBuf ph, qh;
     int n = 0, nnz = 0;
     void ensure(int n_, int nnz_);
 };
@@ -41,9 +41,10 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 // FP32 inner solves (DIC-PCG or DILU-BiCGStab) inside FP64 iterative refinement.
 fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicgstab, fc_solve_info *info);
 void cast_d2f(hipStream_t s, long long n, const double *x, float *y);
-// Flexible CG (FCG(1)) in FP32 with preconditioner `P` inside FP64 iterative
-// refinement; on stagnation or a non-finite iterate the solve restarts from
-// the last refined iterate with FP64 DIC-PCG (status FC_FALLBACK_USED).
+// Truncated flexible CG (each direction A-orthogonalised against the previous
+// kFcgM) in FP32 with preconditioner `P` inside FP64 iterative refinement; on
+// stagnation or a non-finite iterate the solve restarts from the last refined
+// iterate with FP64 DIC-PCG (status FC_FALLBACK_USED).
 struct PrecondHook;
 fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P, fc_solve_info *info);
 // ||su - A fi||_2 / ||su||_2 in FP64 (O-C terms included).

@@ -36,14 +36,14 @@ enum FinStep : int {
     kFinAlfB = 6,     // BiCGStab: alf = vr/(vv+small)
     kFinBicgEnd = 7,  // BiCGStab: resl, iteration count, convergence
     kFinOuter = 8,    // refinement: true residual norms
-    kFinFcgBeta = 9,  // FCG(1): bet = -(z.q_prev)/(p_prev.q_prev)
+    kFinFcgBeta = 9,  // FCG: bet_s = -(z.q_s)/(p_s.q_s) for every stored direction s
     kFinFcgPr = 10,   // FCG: p.r
-    kFinFcgAlf = 11,  // FCG: alf = p.r / p.q
-    kFinFcgEnd = 12   // FCG: residual, convergence, stagnation every 16 iterations
+    kFinFcgAlf = 11,  // FCG: alf = p.r / p.q, p.q stored in this iteration's slot
+    kFinFcgEnd = 12   // FCG: residual, convergence, stagnation at the end of each window
 };
 
-constexpr int kStagEvery = 16;          // FCG stagnation check period (iterations)
-constexpr double kStagFactor = 0.81;    // ||r||^2 must drop below 0.81x (||r|| by 10%) per period
+constexpr int kStagEvery = 16;          // FCG stagnation window (iterations)
+constexpr double kStagFactor = 0.81;    // the window's best ||r||^2 must beat 0.81x the best before it
 
 __device__ inline void converge_test(Scalars *sc) {
     sc->iters += 1;
@@ -64,6 +64,17 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
                                                   const double *__restrict__ pb, Scalars *__restrict__ sc) {
     __shared__ double sh[kFinBlock / kWave];
     if (sc->done) return;
+    if (step == kFinFcgBeta) {
+        // partials of slot s at pa[s * nparts + block]
+        const int nwin = min(sc->iters, kFcgM);
+        for (int s = 0; s < nwin; ++s) {
+            double v = 0.0;
+            for (int i = threadIdx.x; i < nparts; i += kFinBlock) v += pa[(size_t)s * nparts + i];
+            v = block_sum<kFinBlock>(v, sh);
+            if (threadIdx.x == 0) sc->fcg_bet[s] = -v / sc->fcg_pqh[s];
+        }
+        return;
+    }
     double a = 0.0, b = 0.0;
     for (int i = threadIdx.x; i < nparts; i += kFinBlock) {
         a += pa[i];
@@ -121,44 +132,105 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
         sc->resl = a;
         sc->rel = sqrt(b / sc->bnorm2);
         break;
-    case kFinFcgBeta:
-        sc->bet = -a / sc->fcg_pq;
-        break;
     case kFinFcgPr:
         sc->fcg_pr = a;
         break;
     case kFinFcgAlf:
         sc->pkapk = a;
         sc->alf = sc->fcg_pr / a;
-        sc->fcg_pq = a;
+        sc->fcg_pqh[sc->iters % kFcgM] = a;
         break;
     case kFinFcgEnd:
         sc->resl = a;
         sc->rnorm2 = b;
         converge_test(sc);
+        sc->stag_min = fmin(sc->stag_min, b);
         if (!sc->done && sc->iters % kStagEvery == 0) {
-            if (sc->rnorm2 > kStagFactor * sc->stag_ref) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
-            sc->stag_ref = sc->rnorm2;
+            if (!(sc->stag_min < kStagFactor * sc->stag_ref)) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
+            sc->stag_ref = fmin(sc->stag_ref, sc->stag_min);
+            sc->stag_min = 1e300;
         }
         break;
     }
 }
 
-// p = z + bet p and the partial of p.r (flexible CG)
-__global__ __launch_bounds__(kRedBlock) void k_fcg_p(int n, float *__restrict__ p, const float *__restrict__ z,
-                                                     const float *__restrict__ r, const Scalars *__restrict__ sc,
-                                                     double *__restrict__ pa) {
+// Flexible CG with the last kFcgM directions.  Direction of iteration t lives
+// in slot t % kFcgM of ph (and A p in qh); at iteration k (= sc->iters) the
+// slots s < min(k, kFcgM) hold the directions of iterations k-1, ..., k-min(k, kFcgM).
+
+// partials of z.q_s for every stored slot s, at pa[s * gridDim.x + block]
+__global__ __launch_bounds__(kRedBlock) void k_fcg_dots(int n, const float *__restrict__ z,
+                                                        const float *__restrict__ qh, const Scalars *__restrict__ sc,
+                                                        double *__restrict__ pa) {
     __shared__ double sh[kRedBlock / kWave];
     if (sc->done) return;
-    const float bet = (float)sc->bet;
-    double s = 0.0;
+    const int nwin = min(sc->iters, kFcgM);
+    if (nwin == 0) return;
+    double acc[kFcgM];
+#pragma unroll
+    for (int s = 0; s < kFcgM; ++s) acc[s] = 0.0;
     for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-        const float pv = z[i] + bet * p[i];
-        p[i] = pv;
-        s += (double)pv * (double)r[i];
+        const double zi = (double)z[i];
+#pragma unroll
+        for (int s = 0; s < kFcgM; ++s)
+            if (s < nwin) acc[s] += zi * (double)qh[(size_t)s * n + i];
     }
-    s = block_sum<kRedBlock>(s, sh);
-    if (threadIdx.x == 0) pa[blockIdx.x] = s;
+#pragma unroll
+    for (int s = 0; s < kFcgM; ++s) {
+        if (s < nwin) {
+            const double v = block_sum<kRedBlock>(acc[s], sh);
+            if (threadIdx.x == 0) pa[(size_t)s * gridDim.x + blockIdx.x] = v;
+        }
+    }
+}
+
+// p = z + sum_s bet_s p_s and the partial of p.r
+__global__ __launch_bounds__(kRedBlock) void k_fcg_p(int n, float *__restrict__ p, const float *__restrict__ z,
+                                                     const float *__restrict__ r, const float *__restrict__ ph,
+                                                     const Scalars *__restrict__ sc, double *__restrict__ pa) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    const int nwin = min(sc->iters, kFcgM);
+    float bet[kFcgM];
+#pragma unroll
+    for (int s = 0; s < kFcgM; ++s) bet[s] = s < nwin ? (float)sc->fcg_bet[s] : 0.f;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        float pv = z[i];
+#pragma unroll
+        for (int s = 0; s < kFcgM; ++s)
+            if (s < nwin) pv = pv + bet[s] * ph[(size_t)s * n + i];
+        p[i] = pv;
+        acc += (double)pv * (double)r[i];
+    }
+    acc = block_sum<kRedBlock>(acc, sh);
+    if (threadIdx.x == 0) pa[blockIdx.x] = acc;
+}
+
+// x += alf p ; r -= alf q ; p, q stored in this iteration's slot ; partials |r| and r^2
+__global__ __launch_bounds__(kRedBlock) void k_fcg_xr(int n, float *__restrict__ x, float *__restrict__ r,
+                                                      const float *__restrict__ p, const float *__restrict__ q,
+                                                      float *__restrict__ ph, float *__restrict__ qh,
+                                                      const Scalars *__restrict__ sc, double *__restrict__ pa,
+                                                      double *__restrict__ pb) {
+    __shared__ double sh[kRedBlock / kWave];
+    if (sc->done) return;
+    const float alf = (float)sc->alf;
+    const size_t slot = (size_t)(sc->iters % kFcgM) * n;
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        const float pv = p[i], qv = q[i];
+        x[i] = x[i] + alf * pv;
+        const float rv = r[i] - alf * qv;
+        r[i] = rv;
+        ph[slot + i] = pv;
+        qh[slot + i] = qv;
+        a += fabs((double)rv);
+        b += (double)rv * (double)rv;
+    }
+    a = block_sum<kRedBlock>(a, sh);
+    b = block_sum<kRedBlock>(b, sh);
+    if (threadIdx.x == 0) { pa[blockIdx.x] = a; pb[blockIdx.x] = b; }
 }
 
 // x += alf p ; r -= alf q ; partials |r| and r^2 (src/iccg.f90:147-154)
@@ -403,7 +475,7 @@ void Workspace::ensure(int n_, int nnz_) {
     for (DevBuf *b : {&r, &z, &p, &q, &x, &reso, &uk, &vk, &d, &tmp}) b->ensure(D);
     for (DevBuf *b : {&r32, &z32, &p32, &q32, &e32, &d32}) b->ensure(F);
     a32.ensure(sizeof(float) * (size_t)std::max(nnz_, 1));
-    const int parts = std::max(kMaxRedGrid, spmv_grid(n_));
+    const int parts = std::max(kFcgM * kMaxRedGrid, spmv_grid(n_));   // kFcgM dot products at once (FCG)
     pa.ensure(sizeof(double) * parts);
     pb.ensure(sizeof(double) * parts);
     sums.ensure(sizeof(double) * 4);
@@ -712,15 +784,15 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     return status;
 }
 
-// FCG(1) in FP32 on A e = r32 (unit-norm r32), preconditioner P.  Returns the
-// inner status (FC_OK, FC_NOT_CONVERGED, FC_FALLBACK_USED on stagnation,
-// FC_ERR_NONFINITE) and adds the iterations to `iters`.
+// Truncated FCG in FP32 on A e = r32 (unit-norm r32), preconditioner P.
+// Returns the inner status (FC_OK, FC_NOT_CONVERGED, FC_FALLBACK_USED on
+// stagnation, FC_ERR_NONFINITE) and adds the iterations to `iters`.
 static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double eta, int cap, int &iters) {
     Workspace &ws = c.ws;
     const int n = c.n;
     float *a32 = ws.a32.as<float>();
     float *r = ws.r32.as<float>(), *z = ws.z32.as<float>(), *p = ws.p32.as<float>(), *q = ws.q32.as<float>();
-    float *e = ws.e32.as<float>();
+    float *e = ws.e32.as<float>(), *ph = ws.ph.as<float>(), *qh = ws.qh.as<float>();
     const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>();
     const int gs = spmv_grid(n);
     Scalars h;
@@ -730,22 +802,20 @@ static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double
     h.stop_rule = 2;
     h.inner_tol2 = eta * eta;
     h.res0 = 1.0;
-    h.fcg_pq = 1.0;
-    h.stag_ref = 1.0;
+    h.stag_ref = 1.0;      // ||r0||^2 of the unit-norm inner right-hand side
+    h.stag_min = 1e300;
     h.status = kStatusRunning;
     upload_scalars(c, h);
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
-    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
-    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, q);
     auto iter = [&]() {
         P.apply(P.obj, c.s, r, z, c.sc);
-        k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, z, q, c.sc, c.pa);
+        k_fcg_dots<<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa);
         fin(c, kFinFcgBeta, c.gv, false);
-        k_fcg_p<<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, c.sc, c.pa);
+        k_fcg_p<<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa);
         fin(c, kFinFcgPr, c.gv, false);
         spmv_launch32(c.s, 2, n, ptr, ja, a32, p, nullptr, p, q, c.pa, nullptr, c.sc);
         fin(c, kFinFcgAlf, gs, false);
-        k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, p, q, c.sc, c.pa, c.pb);
+        k_fcg_xr<<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, p, q, ph, qh, c.sc, c.pa, c.pb);
         fin(c, kFinFcgEnd, c.gv, true);
     };
     run_batches(c, 4, iter);
@@ -761,6 +831,8 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
     Ctx c = make_ctx(ctx, st.n);
     const int n = st.n;
     double *r = ws.r.as<double>(), *xgood = ws.tmp.as<double>();
+    ws.ph.ensure(sizeof(float) * (size_t)kFcgM * (size_t)n);
+    ws.qh.ensure(sizeof(float) * (size_t)kFcgM * (size_t)n);
     ctx->timer.start(c.s);
     k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
     double setup_ms = ctx->timer.stop_ms(c.s);
@@ -769,7 +841,8 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
     const double bnorm = std::sqrt(h0.bnorm2);
     const bool l2 = args.stop_rule == FC_STOP_L2;
     int total_inner = 0, outer = 0, fallback_iters = 0;
-    double res0 = 0.0, resl = 0.0;
+    double res0 = 0.0, resl = 0.0, fallback_ms = 0.0;
+    bool fell_back = false;
     fc_status status = FC_NOT_CONVERGED;
     ctx->timer.start(c.s);
     for (;;) {
@@ -793,8 +866,11 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
         const int ist = inner_fcg32(c, st, P, args.inner_eta, cap, total_inner);
         ++outer;
         if (ist == FC_FALLBACK_USED || ist == FC_ERR_NONFINITE) {
-            // plain IC(0)-PCG (FP64) finishes the solve from the last refined iterate
+            // plain IC(0)-PCG (FP64) finishes the solve from the last refined iterate;
+            // it runs its own timer, so the learned part is timed up to here
             FC_HIP_CHECK(hipMemcpyAsync(args.fi, xgood, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
+            fallback_ms = ctx->timer.stop_ms(c.s);
+            fell_back = true;
             SolveArgs fb = args;
             fb.stop_rule = FC_STOP_L2;
             fb.nsw = std::max(1, args.nsw);
@@ -803,13 +879,14 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
             ensure_levels(ctx, const_cast<Structure &>(st));
             fc_status fs = solve_pcg64(ctx, fb, true, &fi);
             fallback_iters = fi.iters;
+            fallback_ms += fi.setup_ms + fi.solve_ms;
             resl = fi.resl;
             status = (fs == FC_OK) ? FC_FALLBACK_USED : fs;
             break;
         }
         k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);
     }
-    double solve_ms = ctx->timer.stop_ms(c.s);
+    const double solve_ms = fell_back ? fallback_ms : ctx->timer.stop_ms(c.s);
     if (info) {
         info->res0 = res0;
         info->resl = resl;

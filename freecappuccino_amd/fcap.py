@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -104,9 +105,11 @@ def library() -> C.CDLL:
         "fc_gnn_apply": ([vp, vp, fp, fp], i),
         "fc_gnn_edge_features": ([vp, vp, fp, fp, fp], i),
         "fc_gnn_loss_grad": ([vp, vp, fp, fp, dp, fp], i),
-        "fc_gnn_add_samples": ([vp, vp, i, ip], i),
+        "fc_gnn_add_samples": ([vp, vp, i, ip, C.c_uint], i),
         "fc_gnn_num_samples": ([vp], i),
+        "fc_gnn_get_sample": ([vp, i, fp, fp, i], i),
         "fc_gnn_clear_samples": ([vp], i),
+        "fc_gnn_drop_system": ([vp, vp], i),
         "fc_gnn_train": ([vp, i, i, C.c_float, C.c_uint, fp], i),
         "fc_sys_set_network": ([vp, vp], i),
     }
@@ -146,9 +149,17 @@ class Context:
         if rc != 0:
             raise FcError(rc, "fc_init failed (is a GPU visible?)")
         self.h = h
+        # objects whose library handles live on this context (systems, networks):
+        # they are closed before the context, whatever order Python collects them in
+        self._children = weakref.WeakSet()
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if getattr(self, "h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             self.L.fc_finalize(self.h)
             self.h = None
 
@@ -420,10 +431,15 @@ class System:
         self.h = h
         self.n = mesh.n_cells
         self.ntotal = mesh.n_cells + mesh.n_boundary
+        self._sample_holders = weakref.WeakSet()   # networks holding training samples of this system
+        self.ctx.adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.L.fc_sys_destroy(self.h)
+            for net in list(getattr(self, "_sample_holders", ())):
+                net.drop_system(self)
+            if getattr(self.ctx, "h", None):
+                self.ctx.L.fc_sys_destroy(self.h)
             self.h = None
 
     def __del__(self):

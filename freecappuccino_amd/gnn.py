@@ -8,12 +8,19 @@ fallback on stagnation.
 
 Training data (as for the 64^3 training case): (residual, error) pairs of
 IC(0)-PCG iterates on assembled pressure systems of perturbed hex meshes, one
-mesh seed per system, each with a zero-mean Gaussian-blob RHS.
+mesh seed per system.  The pairs come from IC(0)-PCG on the homogeneous
+system A x = 0 started from seeded random vectors, recorded after 0, 1, 2, 4,
+..., 32 iterations: the errors are exact (e = -x) and span rough to smooth,
+which is what the preconditioner meets inside a Krylov method.  (Iterates
+from zero on a smooth right-hand side only show smooth errors, which a
+4-hop network cannot invert; networks trained on them help flexible CG far
+less.)
 """
 from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from typing import Iterable, List, Optional
 
 import numpy as np
@@ -28,6 +35,8 @@ OFF_ENC_W, OFF_ENC_B, OFF_LAYER0 = 0, H, 2 * H
 OFF_DEC_W = OFF_LAYER0 + L * LAYER_STRIDE
 OFF_DEC_B = OFF_DEC_W + H
 N_PARAMS = OFF_DEC_B + 1
+
+SAMPLE_ITERS = (0, 1, 2, 4, 8, 16, 32)
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "pressure_gnn.npy")
@@ -62,10 +71,16 @@ class Network:
         h = C.c_void_p()
         self.ctx.check(L_.fc_gnn_create(self.ctx.h, C.byref(h), None if p is None else _fp(p)))
         self.h = h
+        self._attached = weakref.WeakSet()
+        self.ctx.adopt(self)
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.L.fc_gnn_destroy(self.h)
+            if getattr(self.ctx, "h", None):
+                for s in list(self._attached):
+                    if getattr(s, "h", None):
+                        self.ctx.L.fc_sys_set_network(s.h, None)
+                self.ctx.L.fc_gnn_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -106,17 +121,32 @@ class Network:
         self.ctx.check(self.ctx.L.fc_gnn_loss_grad(self.h, system.h, _fp(r), _fp(e), C.byref(loss), _fp(g)))
         return loss.value, g
 
-    def add_samples(self, system: fcap.System, iters: Iterable[int]):
+    def add_samples(self, system: fcap.System, iters: Iterable[int] = SAMPLE_ITERS, seed: int = 0):
+        """(r, e) pairs of IC(0)-PCG on A x = 0 from seeded random starts, after each count in `iters`."""
         it = np.ascontiguousarray(list(iters), np.int32)
         self.ctx.check(self.ctx.L.fc_gnn_add_samples(self.h, system.h, it.shape[0],
-                                                     it.ctypes.data_as(C.POINTER(C.c_int))))
+                                                     it.ctypes.data_as(C.POINTER(C.c_int)), int(seed)))
+        # the samples point at the system's matrix: closing the system drops them first
+        system._sample_holders.add(self)
 
     @property
     def num_samples(self) -> int:
         return self.ctx.L.fc_gnn_num_samples(self.h)
 
+    def sample(self, index: int, n: int):
+        """(r, e) of training sample `index` (n = cells of its system)."""
+        r, e = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self.ctx.check(self.ctx.L.fc_gnn_get_sample(self.h, index, _fp(r), _fp(e), n))
+        return r, e
+
     def clear_samples(self):
-        self.ctx.check(self.ctx.L.fc_gnn_clear_samples(self.h))
+        if getattr(self, "h", None):
+            self.ctx.check(self.ctx.L.fc_gnn_clear_samples(self.h))
+
+    def drop_system(self, system: fcap.System):
+        """Forget the samples taken from `system` (it is about to be closed)."""
+        if getattr(self, "h", None) and getattr(system, "h", None):
+            self.ctx.check(self.ctx.L.fc_gnn_drop_system(self.h, system.h))
 
     def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         losses = np.zeros(steps, np.float32)
@@ -124,7 +154,14 @@ class Network:
         return losses
 
     def attach(self, system: fcap.System):
+        """Make this network the preconditioner of METHOD_FCG_LEARNED solves on `system`
+        (the system keeps a borrowed pointer; closing the network detaches it)."""
         self.ctx.check(self.ctx.L.fc_sys_set_network(system.h, self.h))
+        self._attached.add(system)
+
+    def detach(self, system: fcap.System):
+        self.ctx.check(self.ctx.L.fc_sys_set_network(system.h, None))
+        self._attached.discard(system)
 
     def save(self, path: str = DEFAULT_WEIGHTS):
         os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -145,14 +182,14 @@ def training_system(n: int, seed: int, ctx: fcap.Context, nblobs: int = 8) -> fc
     return s
 
 
-def build_dataset(net: Network, n: int, n_systems: int, iters=(0, 2, 5, 10, 20, 40), seed0: int = 0,
+def build_dataset(net: Network, n: int, n_systems: int, iters=SAMPLE_ITERS, seed0: int = 0,
                   ctx: Optional[fcap.Context] = None) -> List[fcap.System]:
     """Assemble n_systems training systems and add their IC(0)-PCG (r, e) pairs; returns the systems
-    (they must stay alive while the network trains on their samples)."""
+    (closing one drops its samples)."""
     ctx = ctx or net.ctx
     systems = []
     for k in range(n_systems):
         s = training_system(n, seed0 + k, ctx)
-        net.add_samples(s, iters)
+        net.add_samples(s, iters, seed=seed0 + k)
         systems.append(s)
     return systems

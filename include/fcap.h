@@ -268,12 +268,17 @@ fc_status fc_gnn_apply(fc_gnn *net, fc_system *sys, const float *r, float *z);
 fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2, float *diag);
 /* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
 fc_status fc_gnn_loss_grad(fc_gnn *net, fc_system *sys, const float *r, const float *e, double *loss, float *grad);
-/* Add (residual, error) pairs of IC(0)-PCG iterates on the system's current
- * RHS, after iters[0..nk-1] iterations from zero.  The system must outlive
- * the samples. */
-fc_status fc_gnn_add_samples(fc_gnn *net, fc_system *sys, int nk, const int *iters);
+/* Add (residual, error) pairs of IC(0)-PCG iterates on the homogeneous system
+ * A x = 0 of sys's matrix, after iters[0..nk-1] iterations from seeded
+ * standard-normal starts (e = -x, r = -A x).  The system must outlive the
+ * samples (or be passed to fc_gnn_drop_system first). */
+fc_status fc_gnn_add_samples(fc_gnn *net, fc_system *sys, int nk, const int *iters, unsigned seed);
 int fc_gnn_num_samples(fc_gnn *net);
+/* Copy sample `index` (residual and error, n floats each; n = its system's numCells). */
+fc_status fc_gnn_get_sample(fc_gnn *net, int index, float *r, float *e, int n);
 fc_status fc_gnn_clear_samples(fc_gnn *net);
+/* Drop the samples that point at `sys` (call before fc_sys_destroy of a sample source). */
+fc_status fc_gnn_drop_system(fc_gnn *net, fc_system *sys);
 /* Adam over `steps` mini-batches of `batch` samples; loss_out (steps floats, may be NULL). */
 fc_status fc_gnn_train(fc_gnn *net, int steps, int batch, float lr, unsigned seed, float *loss_out);
 /* Attach a network as the preconditioner of FC_METHOD_FCG_LEARNED solves of sys. */

@@ -60,3 +60,24 @@ def golden_loader():
 def load_transcript():
     with open(os.path.join(GOLDEN, "solver5x5.json")) as f:
         return json.load(f)
+
+
+def gnn_case(mesh, mu=-1.0):
+    """Oracle-side graph of the learned preconditioner for the Laplacian of `mesh`:
+    (geometry, CSR, matrix values, scipy matrix, oracle Graph)."""
+    import scipy.sparse as sp
+    from oracle import gnn_oracle as G
+    from oracle import oracle as O
+
+    geo = O.mesh_geometry(mesh)
+    csr = O.csr_build(mesh.n_cells, mesh.owner[:mesh.n_inner] + 1, mesh.neighbour + 1)
+    a, _ = O.laplacian(mesh, geo, csr, np.full(mesh.n_cells, mu), np.zeros(mesh.n_cells + mesh.n_boundary),
+                       np.zeros(mesh.n_cells))
+    p0, j0, d0 = csr.ioffset - 1, csr.ja - 1, csr.diag - 1
+    kface = np.full(j0.shape[0], -1, np.int64)
+    kface[csr.ij_slot - 1] = np.arange(mesh.n_inner)
+    kface[csr.ji_slot - 1] = np.arange(mesh.n_inner)
+    e1, e2, dg = G.edge_features_np(p0, j0, d0, kface, geo, a)
+    graph = G.Graph(p0, j0, d0, a[d0], e1, e2)
+    A = sp.csr_matrix((a, j0, p0), shape=(mesh.n_cells, mesh.n_cells))
+    return geo, csr, a, A, graph, (e1, e2, dg)

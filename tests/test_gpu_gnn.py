@@ -1,0 +1,221 @@
+"""Learned preconditioner on the GPU against the numpy oracle (oracle/gnn_oracle.py).
+
+Edge features bitwise; the BF16/MFMA forward pass and the hand-written backward
+pass within stated tolerances of the oracle evaluated at the same BF16
+rounding points; training lowers the loss; flexible CG with the network
+converges, falls back to IC(0)-PCG when the network output is not finite, and
+a trained network needs fewer iterations than no network (Jacobi).
+"""
+import numpy as np
+import pytest
+
+from conftest import gnn_case
+from freecappuccino_amd import fcap, gnn
+from freecappuccino_amd import mesh as M
+from freecappuccino_amd import workloads as W
+from oracle import gnn_oracle as G
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances.  The GPU evaluates in FP32 between the BF16 rounding points and
+# rounds d_out to BF16 before its MFMA products; the oracle is float64 between
+# the same points.  A value near a BF16 rounding boundary can round the other
+# way, so comparisons are in relative L2 norm, not elementwise.
+FWD_RTOL = 2e-2       # ||y_gpu - y_oracle|| / ||y_oracle||, y = z/s - u (the network's part of z)
+LOSS_RTOL = 1e-2
+GRAD_COS = 0.99       # cosine similarity of the full gradient vectors
+GRAD_RTOL = 6e-2      # ||g_gpu - g_oracle|| / ||g_oracle||
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = fcap.Context(0)
+    yield c
+    c.close()
+
+
+def _system(mesh, ctx, seed=11):
+    s = fcap.System(mesh, ctx)
+    xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")
+    su0 = W.blob_rhs(xc, yc, zc, vol, 6, seed=seed) / vol.mean()
+    s.assemble_laplacian(np.full(mesh.n_cells, -1.0), None, su0)
+    return s
+
+
+def _params(seed, dec_scale=20.0):
+    th = gnn.init_params(seed).astype(np.float64)
+    th[G.OFF_DEC_W:G.OFF_DEC_W + G.H] *= dec_scale     # make the network's share of z visible
+    return th.astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    m = M.hex_box(6, perturb=0.2, seed=2)
+    geo, csr, a, A, graph, feats = gnn_case(m)
+    s = _system(m, ctx)
+    np.testing.assert_array_equal(s.get("a"), a)
+    yield m, A, graph, feats, s
+    s.close()
+
+
+def test_edge_features_bitwise(ctx, small):
+    m, A, graph, (e1, e2, dg), s = small
+    net = gnn.Network(None, ctx)
+    g1, g2, gd = net.edge_features(s)
+    np.testing.assert_array_equal(g1.astype(np.float64), e1)
+    np.testing.assert_array_equal(g2.astype(np.float64), e2)
+    np.testing.assert_array_equal(gd.astype(np.float64), dg)
+    net.close()
+
+
+def test_zero_network_is_jacobi(ctx, small):
+    m, A, graph, feats, s = small
+    net = gnn.Network(None, ctx)
+    r = np.random.default_rng(3).standard_normal(m.n_cells).astype(np.float32)
+    z = net.apply(s, r)
+    np.testing.assert_allclose(z, r.astype(np.float64) / graph.diag, rtol=2e-6)
+    net.close()
+
+
+@pytest.mark.parametrize("pseed", [1, 7])
+def test_forward_matches_oracle(ctx, small, pseed):
+    m, A, graph, feats, s = small
+    th = _params(pseed)
+    net = gnn.Network(th, ctx)
+    r = np.random.default_rng(10 + pseed).standard_normal(m.n_cells).astype(np.float32)
+    z_gpu = net.apply(s, r).astype(np.float64)
+    z_ref, cache, _ = G.forward(th, graph, r, bf=True, keep=True)
+    sc, u = cache["s"], cache["u"]
+    y_gpu, y_ref = z_gpu / sc - u, z_ref / sc - u
+    assert np.linalg.norm(y_ref) > 0.1 * np.linalg.norm(u)        # the network part matters
+    err = np.linalg.norm(y_gpu - y_ref) / np.linalg.norm(y_ref)
+    assert err < FWD_RTOL, err
+    assert np.linalg.norm(z_gpu - z_ref) / np.linalg.norm(z_ref) < FWD_RTOL
+    net.close()
+
+
+@pytest.mark.parametrize("pseed", [3, 5])
+def test_loss_and_gradient_match_oracle(ctx, small, pseed):
+    m, A, graph, feats, s = small
+    th = _params(pseed)
+    net = gnn.Network(th, ctx)
+    rng = np.random.default_rng(20 + pseed)
+    r = rng.standard_normal(m.n_cells).astype(np.float32)
+    e = rng.standard_normal(m.n_cells).astype(np.float32)
+    loss_gpu, g_gpu = net.loss_grad(s, r, e)
+    loss_ref, g_ref = G.loss_and_grad(th, graph, A, [(r, e)], bf=True)
+    assert abs(loss_gpu - loss_ref) <= LOSS_RTOL * abs(loss_ref), (loss_gpu, loss_ref)
+    g_gpu = g_gpu.astype(np.float64)
+    cos = g_gpu @ g_ref / (np.linalg.norm(g_gpu) * np.linalg.norm(g_ref))
+    rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
+    assert cos > GRAD_COS and rel < GRAD_RTOL, (cos, rel)
+    # every parameter block receives its gradient (no block silently dropped)
+    blocks = [(G.OFF_ENC_W, G.OFF_LAYER0), (G.OFF_DEC_W, G.N_PARAMS)]
+    for l in range(G.L):
+        o = G.OFF_LAYER0 + l * G.LAYER_STRIDE
+        blocks += [(o, o + G.H * G.H), (o + G.H * G.H, o + 2 * G.H * G.H),
+                   (o + 2 * G.H * G.H, o + 2 * G.H * G.H + G.H), (o + 2 * G.H * G.H + G.H, o + G.LAYER_STRIDE)]
+    for lo, hi in blocks:
+        nb = np.linalg.norm(g_ref[lo:hi])
+        assert np.linalg.norm(g_gpu[lo:hi] - g_ref[lo:hi]) <= 0.25 * nb + 1e-6 * np.linalg.norm(g_ref), (lo, hi)
+    net.close()
+
+
+def test_samples_are_ic0_pcg_iterates_of_the_homogeneous_system(ctx, small):
+    """r = A e for every pair; e starts standard normal and gets smoother with the iteration count
+    (the Rayleigh quotient e^T A^2 e / e^T A e falls as IC(0)-PCG removes the rough components)."""
+    m, A, graph, feats, s = small
+    net = gnn.Network(None, ctx)
+    ks = (0, 1, 4, 8)
+    net.add_samples(s, ks, seed=3)
+    assert net.num_samples == len(ks)
+    rq = []
+    for i in range(len(ks)):
+        r, e = net.sample(i, m.n_cells)
+        e64 = e.astype(np.float64)
+        Ae = A @ e64
+        assert np.linalg.norm(r - Ae) <= 1e-4 * np.linalg.norm(Ae), i
+        rq.append((Ae @ Ae) / (e64 @ Ae))
+        if ks[i] == 0:
+            assert abs(e64.mean()) < 0.25 and abs(e64.std() - 1.0) < 0.2
+    assert rq[0] > rq[1] > rq[3], rq
+    net.close()
+
+
+def _train(ctx, n, n_systems, steps, seed0=0, lr=3e-3):
+    net = gnn.Network(gnn.init_params(0), ctx)
+    systems = gnn.build_dataset(net, n, n_systems, seed0=seed0, ctx=ctx)
+    losses = net.train(steps, batch=8, lr=lr, seed=1)
+    return net, systems, losses
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    net, systems, losses = _train(ctx, 16, 4, 300)
+    yield net, losses
+    net.close()
+    for s in systems:
+        s.close()
+
+
+def test_training_lowers_the_loss(trained):
+    net, losses = trained
+    assert np.all(np.isfinite(losses))
+    first, last = losses[:20].mean(), losses[-20:].mean()
+    assert last < 0.7 * first, (first, last)
+
+
+def _fcg(ctx, s, params, sor=1e-8):
+    net = gnn.Network(params, ctx)
+    net.attach(s)
+    s.set_solution(np.zeros(s.n))
+    rc, info = s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, sor, 5000)
+    net.detach(s)
+    net.close()
+    return rc, info
+
+
+def test_fcg_learned_converges_and_beats_no_network(ctx, trained):
+    net, _ = trained
+    m = M.hex_box(16, perturb=0.2, seed=99)      # not a training mesh
+    s = _system(m, ctx, seed=123)
+    rc0, info0 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
+    x0 = s.solution()
+    rc1, info1 = _fcg(ctx, s, net.params)
+    x1 = s.solution()
+    assert rc0 == fcap.FC_OK and rc1 == fcap.FC_OK, (rc0, rc1, info0, info1)
+    assert info0["rel_l2"] < 1e-8 and info1["rel_l2"] < 1e-8
+    assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-6
+    assert info1["iters"] < 0.85 * info0["iters"], (info1["iters"], info0["iters"])
+    s.close()
+
+
+def test_fcg_with_zero_network_matches_jacobi_pcg(ctx):
+    """A zero network is the fixed SPD preconditioner D^-1, and flexible CG with a fixed SPD
+    preconditioner is PCG: the iteration count matches FP64 Jacobi-PCG up to FP32 rounding and
+    the restarts of the refinement loop."""
+    m = M.hex_box(12, perturb=0.2, seed=6)
+    s = _system(m, ctx, seed=9)
+    s.set_solution(np.zeros(s.n))
+    rc_j, info_j = s.solve(fcap.METHOD_DPCG, fcap.STOP_L2, 1e-8, 5000)
+    rc_f, info_f = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
+    assert rc_j == fcap.FC_OK and rc_f == fcap.FC_OK, (info_j, info_f)
+    assert abs(info_f["iters"] - info_j["iters"]) <= 0.2 * info_j["iters"] + 10, (info_f["iters"], info_j["iters"])
+    s.close()
+
+
+def test_nonfinite_network_falls_back_to_ic0_pcg(ctx):
+    m = M.hex_box(10, perturb=0.2, seed=5)
+    s = _system(m, ctx, seed=8)
+    rc, info = _fcg(ctx, s, np.full(gnn.N_PARAMS, np.nan, np.float32))
+    assert rc == fcap.FC_FALLBACK_USED, (rc, info)
+    assert info["fallback_iters"] > 0 and info["rel_l2"] < 1e-8
+    s.close()
+
+
+def test_fcg_without_network_is_an_error(ctx):
+    m = M.hex_box(4, seed=1)
+    s = _system(m, ctx, seed=1)
+    with pytest.raises(fcap.FcError):
+        s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, 1e-8, 100)
+    s.close()
