@@ -2,8 +2,9 @@
 
     python -m freecappuccino_amd.build          # or __graft_entry__.build()
 
-Every translation unit under csrc/ is compiled with hipcc for gfx950 only and
-linked into freecappuccino_amd/libfcap.so.  Flags:
+Every .hip translation unit under csrc/ is compiled with hipcc for gfx950
+only, every .cpp (host-only code) with the host C++ compiler, and all are
+linked into freecappuccino_amd/libfcap.so.  HIP flags:
   -ffp-contract=off   the FP64 parity kernels reproduce the reference's
                       rounding sequence and must not be contracted into FMAs
                       (kernels that want FMAs call fma() explicitly);
@@ -33,23 +34,30 @@ FLAGS = [
 ]
 
 
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-Wall"]
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):        # host-only code (the polyMesh reader)
+        cmd = [CXX, *HOST_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{p.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed for {src}:\n{p.stderr}")
     return obj
 
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD_DIR, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr}")

@@ -192,6 +192,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         ctx->num_cus = prop.multiProcessorCount;
         if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
         if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
+        if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -531,8 +532,23 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         upload_dbl(ctx, dr, r, n);
         dz.alloc(sizeof(double) * n);
         const double small = ctx->variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+        // the solvers' sweeps: level-packed operands when the structure has them
+        DevBuf af, df, ab, db;
+        PackedFactor<double> pf;
+        if (ctx->packed && packed_sweeps(st)) {
+            af.alloc(sizeof(double) * (size_t)st.ell_f * n);
+            ab.alloc(sizeof(double) * (size_t)st.ell_b * n);
+            df.alloc(sizeof(double) * n);
+            db.alloc(sizeof(double) * n);
+            pack_factor64(ctx->stream, st, da.as<double>(), dd.as<double>(), af.as<double>(), df.as<double>(),
+                          ab.as<double>(), db.as<double>());
+            pf.af = af.as<double>();
+            pf.df = df.as<double>();
+            pf.ab = ab.as<double>();
+            pf.db = db.as<double>();
+        }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
-                    nullptr);
+                    nullptr, nullptr, pf.af ? &pf : nullptr);
         FC_HIP_CHECK(hipGetLastError());
         download_dbl(ctx, z, dz.get(), n);
         return FC_OK;

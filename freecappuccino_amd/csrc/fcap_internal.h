@@ -96,6 +96,13 @@ struct Structure {
     DevBuf tpos;                // DILU: position of (j,i) in row j for each strict-lower (i,j), -1 if none
     bool has_tpos = false;
     bool levels_ready = false;
+    // Level-packed sweeps: per direction, an ELL of the strict-lower (forward) or
+    // strict-upper (backward) entries of the rows in level order; slot k of
+    // packed row p sits at [k * n + p], entries in CSR order, padding 0.
+    int ell_f = 0, ell_b = 0;        // ELL widths (0: not packed)
+    DevBuf cnt_f, cnt_b;             // entries per packed row
+    DevBuf kpos_f, kpos_b;           // CSR position of each slot
+    DevBuf col_f, col_b;             // column of each slot
 };
 
 struct Workspace;   // solver vectors, partial sums, device scalars
@@ -134,6 +141,8 @@ struct fc_context {
     // triangular sweeps: 0 one launch per level, 1 persistent cooperative kernel
     // with a grid barrier per level (FCAP_SWEEP selects)
     int sweep_mode = 0;
+    // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
+    int packed = 1;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

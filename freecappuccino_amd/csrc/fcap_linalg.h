@@ -42,11 +42,29 @@ struct SweepCfg {
 };
 SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar);
 
+// The factor's values in level-packed order (Structure::kpos_f / kpos_b): strict
+// lower / upper entries as an ELL over the packed rows, and the pivots per
+// packed row.  Filled once per solve by pack_factor*; the sweeps then read
+// every operand of a row with coalesced loads and only gather w.
+template <class T>
+struct PackedFactor {
+    const T *af = nullptr, *df = nullptr, *ab = nullptr, *db = nullptr;
+};
+bool packed_sweeps(const Structure &st);   // both triangles packed
+void pack_factor64(hipStream_t s, const Structure &st, const double *a, const double *d, double *af, double *df,
+                   double *ab, double *db);
+void pack_factor32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
+                   float *ab, float *db);
+
 void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err);
+// z = M^-1 rhs with the sweeps of src/iccg.f90:94-111: level-packed kernels when
+// `pf` is given, one launch per level over the CSR rows otherwise, or the
+// persistent cooperative kernels when cfg->bar is set.
 void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
-                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg = nullptr);
+                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg = nullptr,
+                 const PackedFactor<double> *pf = nullptr);
 void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
-                 float *w, const Scalars *sc, const SweepCfg *cfg = nullptr);
+                 float *w, const Scalars *sc, const SweepCfg *cfg = nullptr, const PackedFactor<float> *pf = nullptr);
 
 void dot_launch64(hipStream_t s, int n, const double *x, const double *y, double *pa);
 void norms_launch64(hipStream_t s, int n, const double *x, double *pa, double *pb);

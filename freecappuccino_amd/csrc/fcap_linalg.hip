@@ -385,11 +385,152 @@ void lu_persistent(hipStream_t s, const Structure &st, const T *a, const T *d, T
                                             s));
 }
 
+// ------------------------------------------------- level-packed sweeps
+
+// packed row p of a level: operands at [k * n + p] (coalesced across the
+// level's threads); all of a row's slots are loaded, then all gathers of w
+// issued, then the sum runs in CSR order over the row's own entries only, so
+// the arithmetic is that of k_fwd_level / k_bwd_level.  Padding slots hold
+// column 0 and value 0 and are never added.
+template <class T, int W>
+__global__ void k_fwd_packed(int beg, int cnt, int n, const int *__restrict__ rows, const int *__restrict__ cntp,
+                             const int *__restrict__ col, const T *__restrict__ av, const T *__restrict__ dv,
+                             const T *__restrict__ rhs, T *__restrict__ w, int ell, const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int p = beg + t;
+    const int i = rows[p];
+    const int c = cntp[p];
+    T acc = rhs[i];
+    if (W > 0) {
+        int cj[W > 0 ? W : 1];
+        T ak[W > 0 ? W : 1], wk[W > 0 ? W : 1];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            cj[k] = col[(size_t)k * n + p];
+            ak[k] = av[(size_t)k * n + p];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) wk[k] = w[cj[k]];
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (k < c) acc = acc - ak[k] * wk[k];
+    } else {
+        for (int k = 0; k < c; ++k) acc = acc - av[(size_t)k * n + p] * w[col[(size_t)k * n + p]];
+    }
+    (void)ell;
+    w[i] = acc * dv[p];
+}
+
+template <class T, int W>
+__global__ void k_bwd_packed(int beg, int cnt, int n, const int *__restrict__ rows, const int *__restrict__ cntp,
+                             const int *__restrict__ col, const T *__restrict__ av, const T *__restrict__ dv, T small,
+                             T *__restrict__ w, const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const int p = beg + t;
+    const int i = rows[p];
+    const int c = cntp[p];
+    const T di = dv[p];
+    if (W > 0) {
+        int cj[W > 0 ? W : 1];
+        T ak[W > 0 ? W : 1], wk[W > 0 ? W : 1];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            cj[k] = col[(size_t)k * n + p];
+            ak[k] = av[(size_t)k * n + p];
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) wk[k] = w[cj[k]];
+        T acc = w[i] / (di + small);
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+            if (k < c) acc = acc - ak[k] * wk[k];
+        w[i] = acc * di;
+    } else {
+        T acc = w[i] / (di + small);
+        for (int k = 0; k < c; ++k) acc = acc - av[(size_t)k * n + p] * w[col[(size_t)k * n + p]];
+        w[i] = acc * di;
+    }
+}
+
+// values of the factor into packed order (one pass per solve)
+template <class T>
+__global__ void k_pack_factor(int n, int ell, const int *__restrict__ rows, const int *__restrict__ cntp,
+                              const int *__restrict__ kpos, const T *__restrict__ a, const T *__restrict__ d,
+                              T *__restrict__ av, T *__restrict__ dv) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int c = cntp[p];
+    for (int k = 0; k < ell; ++k) {
+        const size_t o = (size_t)k * n + p;
+        av[o] = k < c ? a[kpos[o]] : T(0);
+    }
+    dv[p] = d[rows[p]];
+}
+
+// Each triangle's ELL is stored at exactly the width its kernel reads (3, 4 or
+// 8 slots per row, or its widest row for the generic W = 0 kernel).
+template <class T, int W>
+void fwd_packed_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, const T *rhs, T *w,
+                  const Scalars *sc) {
+    const int *rf = st.rows_f.as<int>();
+    for (int l = 0; l < st.nlev_f; ++l) {
+        const int beg = st.lvl_f[l], cnt = st.lvl_f[l + 1] - beg;
+        k_fwd_packed<T, W><<<grid_for(cnt, 256), 256, 0, s>>>(beg, cnt, st.n, rf, st.cnt_f.as<int>(),
+                                                              st.col_f.as<int>(), pf.af, pf.df, rhs, w, st.ell_f, sc);
+    }
+}
+
+template <class T, int W>
+void bwd_packed_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, T *w, const Scalars *sc) {
+    const int *rb = st.rows_b.as<int>();
+    for (int l = 0; l < st.nlev_b; ++l) {
+        const int beg = st.lvl_b[l], cnt = st.lvl_b[l + 1] - beg;
+        k_bwd_packed<T, W><<<grid_for(cnt, 256), 256, 0, s>>>(beg, cnt, st.n, rb, st.cnt_b.as<int>(),
+                                                              st.col_b.as<int>(), pf.ab, pf.db, small, w, sc);
+    }
+}
+
+template <class T>
+void sweeps_packed(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                   const Scalars *sc) {
+    switch (st.ell_f) {
+    case 3: fwd_packed_w<T, 3>(s, st, pf, rhs, w, sc); break;
+    case 4: fwd_packed_w<T, 4>(s, st, pf, rhs, w, sc); break;
+    case 8: fwd_packed_w<T, 8>(s, st, pf, rhs, w, sc); break;
+    default: fwd_packed_w<T, 0>(s, st, pf, rhs, w, sc); break;
+    }
+    switch (st.ell_b) {
+    case 3: bwd_packed_w<T, 3>(s, st, pf, small, w, sc); break;
+    case 4: bwd_packed_w<T, 4>(s, st, pf, small, w, sc); break;
+    case 8: bwd_packed_w<T, 8>(s, st, pf, small, w, sc); break;
+    default: bwd_packed_w<T, 0>(s, st, pf, small, w, sc); break;
+    }
+}
+
+template <class T>
+void pack_factor_t(hipStream_t s, const Structure &st, const T *a, const T *d, T *af, T *df, T *ab, T *db) {
+    FC_REQUIRE(packed_sweeps(st), FC_ERR_ARGUMENT, "structure has no level-packed sweeps");
+    const int n = st.n;
+    k_pack_factor<T><<<grid_for(n, 256), 256, 0, s>>>(n, st.ell_f, st.rows_f.as<int>(), st.cnt_f.as<int>(),
+                                                      st.kpos_f.as<int>(), a, d, af, df);
+    k_pack_factor<T><<<grid_for(n, 256), 256, 0, s>>>(n, st.ell_b, st.rows_b.as<int>(), st.cnt_b.as<int>(),
+                                                      st.kpos_b.as<int>(), a, d, ab, db);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
 template <class T>
 void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
-                 const Scalars *sc, const SweepCfg *cfg) {
+                 const Scalars *sc, const SweepCfg *cfg, const PackedFactor<T> *pf) {
     if (cfg && cfg->bar) {
         lu_persistent<T>(s, st, a, d, small, rhs, w, sc, *cfg);
+        return;
+    }
+    if (pf && pf->af) {
+        sweeps_packed<T>(s, st, *pf, small, rhs, w, sc);
         return;
     }
     const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
@@ -405,13 +546,26 @@ void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T s
 }
 
 void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
-                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg) {
-    lu_launch_t<double>(s, st, a, d, small, rhs, w, sc, cfg);
+                 const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg,
+                 const PackedFactor<double> *pf) {
+    lu_launch_t<double>(s, st, a, d, small, rhs, w, sc, cfg, pf);
 }
 
 void lu_launch32(hipStream_t s, const Structure &st, const float *a, const float *d, float small, const float *rhs,
-                 float *w, const Scalars *sc, const SweepCfg *cfg) {
-    lu_launch_t<float>(s, st, a, d, small, rhs, w, sc, cfg);
+                 float *w, const Scalars *sc, const SweepCfg *cfg, const PackedFactor<float> *pf) {
+    lu_launch_t<float>(s, st, a, d, small, rhs, w, sc, cfg, pf);
+}
+
+bool packed_sweeps(const Structure &st) { return st.levels_ready && st.ell_f > 0 && st.ell_b > 0; }
+
+void pack_factor64(hipStream_t s, const Structure &st, const double *a, const double *d, double *af, double *df,
+                   double *ab, double *db) {
+    pack_factor_t<double>(s, st, a, d, af, df, ab, db);
+}
+
+void pack_factor32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
+                   float *ab, float *db) {
+    pack_factor_t<float>(s, st, a, d, af, df, ab, db);
 }
 
 // ----------------------------------------------------------- reductions

@@ -513,6 +513,32 @@ __global__ void k_facint(const double *__restrict__ xyz, const int *__restrict__
     facint[f] = djn / dpn;
 }
 
+// widest strict-lower (upper = false) or strict-upper row
+__global__ void k_tri_width(const int *__restrict__ ptr, const int *__restrict__ diag, int n, bool upper,
+                            int *__restrict__ wmax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = upper ? ptr[i + 1] - diag[i] - 1 : diag[i] - ptr[i];
+    atomicMax(wmax, c);
+}
+
+// ELL of the rows in level order: slot k of packed row p at [k * n + p]
+__global__ void k_tri_ell(const int *__restrict__ rows, const int *__restrict__ ptr, const int *__restrict__ diag,
+                          const int *__restrict__ ja, int n, int ell, bool upper, int *__restrict__ cnt,
+                          int *__restrict__ kpos, int *__restrict__ col) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = rows[p];
+    const int s = upper ? diag[i] + 1 : ptr[i];
+    const int c = upper ? ptr[i + 1] - s : diag[i] - s;
+    cnt[p] = c;
+    for (int k = 0; k < ell; ++k) {
+        const size_t o = (size_t)k * n + p;
+        kpos[o] = k < c ? s + k : 0;
+        col[o] = k < c ? ja[s + k] : 0;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host side
@@ -552,6 +578,42 @@ static void check_err(fc_context *ctx, const DevBuf &err, const char *what) {
     throw Error(FC_ERR_STRUCTURE, m);
 }
 
+// Level-packed ELL of one triangle (rows in level order); width 0 (not packed)
+// when a row is wider than kMaxEll, which would make the padding dominate.
+static void pack_triangle(fc_context *ctx, Structure &st, bool upper) {
+    constexpr int kMaxEll = 16;
+    hipStream_t s = ctx->stream;
+    const int n = st.n;
+    DevBuf wmax(sizeof(int));
+    FC_HIP_CHECK(hipMemsetAsync(wmax.get(), 0, sizeof(int), s));
+    k_tri_width<<<grid_for(n), kBlock, 0, s>>>(st.ioffset.as<int>(), st.diag.as<int>(), n, upper, wmax.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    const int widest = read_flag(ctx, wmax.as<int>());
+    // stored width = the width the sweep kernel is compiled for (3, 4, 8), or the
+    // widest row for the generic kernel; slots past a row's count are padding
+    const int ell = widest <= 3 ? 3 : widest <= 4 ? 4 : widest <= 8 ? 8 : widest;
+    int &width = upper ? st.ell_b : st.ell_f;
+    DevBuf &cnt = upper ? st.cnt_b : st.cnt_f;
+    DevBuf &kpos = upper ? st.kpos_b : st.kpos_f;
+    DevBuf &col = upper ? st.col_b : st.col_f;
+    width = 0;
+    if (widest < 1 || widest > kMaxEll) {
+        cnt.release();
+        kpos.release();
+        col.release();
+        return;
+    }
+    cnt.alloc(sizeof(int) * (size_t)n);
+    kpos.alloc(sizeof(int) * (size_t)ell * n);
+    col.alloc(sizeof(int) * (size_t)ell * n);
+    k_tri_ell<<<grid_for(n), kBlock, 0, s>>>((upper ? st.rows_b : st.rows_f).as<int>(), st.ioffset.as<int>(),
+                                             st.diag.as<int>(), st.ja.as<int>(), n, ell, upper, cnt.as<int>(),
+                                             kpos.as<int>(), col.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+    width = ell;
+}
+
 void analyse_levels(fc_context *ctx, Structure &st) {
     levels_one(ctx, st, false, st.rows_f, st.lvl_f, st.nlev_f, st.max_level_f);
     levels_one(ctx, st, true, st.rows_b, st.lvl_b, st.nlev_b, st.max_level_b);
@@ -559,6 +621,8 @@ void analyse_levels(fc_context *ctx, Structure &st) {
     h2d(st.lvl_f_d.get(), st.lvl_f.data(), sizeof(int) * st.lvl_f.size(), ctx->stream);
     st.lvl_b_d.alloc(sizeof(int) * st.lvl_b.size());
     h2d(st.lvl_b_d.get(), st.lvl_b.data(), sizeof(int) * st.lvl_b.size(), ctx->stream);
+    pack_triangle(ctx, st, false);
+    pack_triangle(ctx, st, true);
 }
 
 void build_tpos(fc_context *ctx, Structure &st) {

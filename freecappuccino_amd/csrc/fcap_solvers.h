@@ -16,6 +16,8 @@ struct Workspace {
     DevBuf pa, pb, sums, sc, err, bar;
     // flexible CG: the last kFcgM directions p and their images A p (FP32, kFcgM x n each)
     DevBuf ph, qh;
+    // the current solve's factor in level-packed order (PackedFactor), FP64 and FP32
+    DevBuf pk_af, pk_df, pk_ab, pk_db, pk_af32, pk_df32, pk_ab32, pk_db32;
     int n = 0, nnz = 0;
     void ensure(int n_, int nnz_);
 };

@@ -506,6 +506,33 @@ static void check_sweeps(Ctx &c) {
     if (h.abort) throw Error(FC_ERR_HIP, "persistent sweep: a grid barrier wait exceeded its spin limit");
 }
 
+// The factor (a, pivots d) in level-packed order for this solve's sweeps; empty
+// (sweeps over the CSR rows) when the structure is not packed, packing is
+// switched off, or the persistent sweeps run.
+template <class T>
+static PackedFactor<T> pack_for_solve(Ctx &c, const Structure &st, const T *a, const T *d) {
+    PackedFactor<T> pf;
+    if (!c.ctx->packed || c.sw.bar || !packed_sweeps(st)) return pf;
+    Workspace &ws = c.ws;
+    constexpr bool f32 = sizeof(T) == sizeof(float);
+    DevBuf &af = f32 ? ws.pk_af32 : ws.pk_af, &df = f32 ? ws.pk_df32 : ws.pk_df;
+    DevBuf &ab = f32 ? ws.pk_ab32 : ws.pk_ab, &db = f32 ? ws.pk_db32 : ws.pk_db;
+    const size_t n = (size_t)st.n;
+    af.ensure(sizeof(T) * n * (size_t)st.ell_f);
+    ab.ensure(sizeof(T) * n * (size_t)st.ell_b);
+    df.ensure(sizeof(T) * n);
+    db.ensure(sizeof(T) * n);
+    if constexpr (f32)
+        pack_factor32(c.s, st, a, d, af.as<float>(), df.as<float>(), ab.as<float>(), db.as<float>());
+    else
+        pack_factor64(c.s, st, a, d, af.as<double>(), df.as<double>(), ab.as<double>(), db.as<double>());
+    pf.af = af.as<T>();
+    pf.df = df.as<T>();
+    pf.ab = ab.as<T>();
+    pf.db = db.as<T>();
+    return pf;
+}
+
 static void check_pivots(Ctx &c) {
     int h = 0;
     FC_HIP_CHECK(hipMemcpyAsync(&h, c.ws.err.get(), sizeof(int), hipMemcpyDeviceToHost, c.s));
@@ -569,7 +596,11 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     k_zero<double><<<c.gv, 256, 0, c.s>>>(n, p);
     FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
     initial_residual(c, args, r);
-    if (dic) pivots_launch(c.s, st, args.a, d, false, ws.err.as<int>());
+    PackedFactor<double> pf;
+    if (dic) {
+        pivots_launch(c.s, st, args.a, d, false, ws.err.as<int>());
+        pf = pack_for_solve<double>(c, st, args.a, d);
+    }
     double setup_ms = ctx->timer.stop_ms(c.s);
     if (dic) check_pivots(c);
     if (read_done(c)) {
@@ -581,7 +612,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     const int gs = spmv_grid(n);
     auto iter = [&]() {
         if (dic) {
-            lu_launch64(c.s, st, args.a, d, small, r, z, c.sc, &c.sw);
+            lu_launch64(c.s, st, args.a, d, small, r, z, c.sc, &c.sw, pf.af ? &pf : nullptr);
             k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
         } else {
             k_jacobi<double, double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, args.a, dg, z, c.sc, c.pa);
@@ -626,6 +657,8 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
     FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
     initial_residual(c, args, res);
     pivots_launch(c.s, st, args.a, d, true, ws.err.as<int>());
+    const PackedFactor<double> pf = pack_for_solve<double>(c, st, args.a, d);
+    const PackedFactor<double> *ppf = pf.af ? &pf : nullptr;
     k_copy<double><<<c.gv, 256, 0, c.s>>>(n, res, reso);
     k_zero<double><<<c.gv, 256, 0, c.s>>>(n, pk);
     k_zero<double><<<c.gv, 256, 0, c.s>>>(n, uk);
@@ -642,11 +675,11 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
         k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, res, reso, c.sc, c.pa);
         fin(c, kFinBet, c.gv, false);
         k_bicg_p<double><<<c.gv, 256, 0, c.s>>>(n, pk, res, uk, c.sc);
-        lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc, &c.sw);
+        lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc, &c.sw, ppf);
         spmv_launch64(c.s, 2, n, ptr, ja, args.a, zk, nullptr, reso, uk, nullptr, c.pa, nullptr, c.sc);
         fin(c, kFinGam, gs, false);
         k_bicg_xr1<double, double><<<c.gv, 256, 0, c.s>>>(n, args.fi, res, zk, uk, c.sc);
-        lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc, &c.sw);
+        lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc, &c.sw, ppf);
         spmv_launch64(c.s, 3, n, ptr, ja, args.a, zk, nullptr, res, vk, nullptr, c.pa, c.pb, c.sc);
         fin(c, kFinAlfB, gs, true);
         k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, res, zk, vk, c.sc, c.pa, c.pb);
@@ -662,7 +695,8 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 }
 
 // FP32 inner solve of A e = r32 to ||r_inner||_2 <= eta ||r32||_2, e32 out.
-static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, double r32norm2, int cap) {
+static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, double r32norm2, int cap,
+                         const PackedFactor<float> *pf) {
     Workspace &ws = c.ws;
     const int n = c.n;
     float *a32 = ws.a32.as<float>(), *d32 = ws.d32.as<float>();
@@ -688,7 +722,7 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
     if (!bicg) {
         auto iter = [&]() {
-            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw);
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw, pf);
             k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, z, c.sc, c.pa);
             fin(c, kFinSk, c.gv, false);
             k_p_update<float><<<c.gv, 256, 0, c.s>>>(n, p, z, c.sc);
@@ -707,11 +741,11 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
             k_dot<float><<<c.gv, kRedBlock, 0, c.s>>>(n, r, reso, c.sc, c.pa);
             fin(c, kFinBet, c.gv, false);
             k_bicg_p<float><<<c.gv, 256, 0, c.s>>>(n, p, r, uk, c.sc);
-            lu_launch32(c.s, st, a32, d32, small, p, z, c.sc, &c.sw);
+            lu_launch32(c.s, st, a32, d32, small, p, z, c.sc, &c.sw, pf);
             spmv_launch32(c.s, 2, n, ptr, ja, a32, z, nullptr, reso, uk, c.pa, nullptr, c.sc);
             fin(c, kFinGam, gs, false);
             k_bicg_xr1<float, float><<<c.gv, 256, 0, c.s>>>(n, e, r, z, uk, c.sc);
-            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw);
+            lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw, pf);
             spmv_launch32(c.s, 3, n, ptr, ja, a32, z, nullptr, r, vk, c.pa, c.pb, c.sc);
             fin(c, kFinAlfB, gs, true);
             k_pcg_xr<float, float><<<c.gv, kRedBlock, 0, c.s>>>(n, e, r, z, vk, c.sc, c.pa, c.pb);
@@ -737,6 +771,7 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     pivots_launch(c.s, st, args.a, d, bicg, ws.err.as<int>());
     k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
     k_cast<double, float><<<c.gv, 256, 0, c.s>>>(n, d, ws.d32.as<float>());
+    const PackedFactor<float> pf = pack_for_solve<float>(c, st, ws.a32.as<float>(), ws.d32.as<float>());
     double setup_ms = ctx->timer.stop_ms(c.s);
     check_pivots(c);
     Scalars h0 = init_scalars(args, kSmallSrc);
@@ -765,7 +800,7 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
         int cap = std::max(1, args.nsw - total_inner);
         const double rn = std::sqrt(ho.rnorm2);
         k_scale_cast<<<c.gv, 256, 0, c.s>>>(n, r, ws.r32.as<float>(), 1.0 / rn);
-        total_inner += inner_solve32(c, st, bicg, args.inner_eta, 1.0, cap);
+        total_inner += inner_solve32(c, st, bicg, args.inner_eta, 1.0, cap, pf.af ? &pf : nullptr);
         k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);
         ++outer;
     }

@@ -284,6 +284,29 @@ fc_status fc_gnn_train(fc_gnn *net, int steps, int batch, float lr, unsigned see
 /* Attach a network as the preconditioner of FC_METHOD_FCG_LEARNED solves of sys. */
 fc_status fc_sys_set_network(fc_system *sys, fc_gnn *net);
 
+/* ---------------------------------------------------------- polyMesh reader */
+
+/* Host-side reader of the polyMesh files the reference reads in
+ * src/mesh_geometry_and_topology.f90:409-605 (points, faces, owner,
+ * neighbour, boundary), OpenFOAM ASCII (FoamFile headers, 0-based labels,
+ * nCells from the owner note as at :516-527) or the reference's native
+ * layout (:473-480), detected from the owner file.  Lists are parsed by
+ * `nthreads` threads (0: all cores, at most 16); faces may have any vertex
+ * count.  The boundary file is the reference's simplified one: '#' comment,
+ * then `bctype nFaces startFace` lines (bctypes of :417-468).  On error the
+ * message goes to err (errlen bytes). */
+typedef struct fc_polymesh fc_polymesh;
+fc_status fc_polymesh_read(const char *dir, int nthreads, fc_polymesh **mesh, char *err, int errlen);
+fc_status fc_polymesh_sizes(const fc_polymesh *mesh, int *numNodes, int *numFaces, int *numInnerFaces,
+                            int *numCells, int *numFaceNodes, int *numPatches);
+/* Caller-owned outputs: xyz (3*numNodes, x y z per node), face_ptr
+ * (numFaces+1, 0-based offsets), face_nodes (numFaceNodes), owner (numFaces),
+ * neighbour (numInnerFaces); labels 1-based, as fc_sys_create takes them. */
+fc_status fc_polymesh_arrays(const fc_polymesh *mesh, double *xyz, int *face_ptr, int *face_nodes, int *owner,
+                             int *neighbour);
+fc_status fc_polymesh_patch(const fc_polymesh *mesh, int index, char *bctype, int len, int *nFaces, int *startFace);
+void fc_polymesh_free(fc_polymesh *mesh);
+
 /* No counterpart in the reference: a plain device-to-device copy kernel of
  * `bytes` bytes, `reps` times (calibration of the HBM traffic counters and the
  * achievable-bandwidth figure); returns the average milliseconds per copy. */
