@@ -193,6 +193,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
         if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
+        if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         return FC_OK;
     });
     if (s != FC_OK) {

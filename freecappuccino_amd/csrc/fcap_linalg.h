@@ -15,6 +15,8 @@ struct OcView {
 
 int red_grid(int n);
 int spmv_grid(int n);
+// SpMV block order for the whole process: 0 natural (default), 1 XCD-aware
+void set_spmv_xcd_order(int on);
 
 // mode 0: y = A x; 1: y = b - A x with L1 (pa) and L2^2 (pb) partials and an
 // optional FP32 copy y32; 2: y = A x with partial y.v (pa); 3: as 2 plus y.y (pb)

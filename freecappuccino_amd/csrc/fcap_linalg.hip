@@ -33,6 +33,12 @@ int spmv_grid(int n) { return grid_for(n, kSpmvRows); }
 // launch grid of a kernel with an XCD-aware block order: `blocks` rounded up to a multiple of kXcds
 static int xcd_grid(int blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
+// SpMV block order, process-wide: 0 natural (workgroup b takes row block b),
+// 1 XCD-aware.  Measured at 192^3 the natural order is faster (5.10 vs
+// 4.86 TB/s), so it is the default; FCAP_XCD=1 selects the other.
+static int g_spmv_xcd = 0;
+void set_spmv_xcd_order(int on) { g_spmv_xcd = on ? 1 : 0; }
+
 // ------------------------------------------------------------------- SpMV
 
 template <class T, class AT, int MODE>
@@ -41,18 +47,17 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
                                                     const T *__restrict__ b, const T *__restrict__ v,
                                                     T *__restrict__ y, float *__restrict__ y32,
                                                     double *__restrict__ partA, double *__restrict__ partB,
-                                                    const Scalars *__restrict__ sc) {
+                                                    const Scalars *__restrict__ sc, int xcd) {
     __shared__ AT sa[kSpmvWaves][kSpmvCap];
     __shared__ int sj[kSpmvWaves][kSpmvCap];
     __shared__ double red[kSpmvWaves];
     if (sc && sc->done) return;
-    // XCD-aware order: workgroups go to the 8 XCDs round-robin, so workgroup b
-    // takes row block (b % 8) * per + b / 8 and each XCD sweeps one contiguous
-    // eighth of the rows; the x entries that neighbouring row blocks share then
-    // meet in one L2 instead of up to eight.  The grid is a multiple of 8.
+    // xcd = 1, XCD-aware order: workgroups go to the 8 XCDs round-robin, so
+    // workgroup b takes row block (b % 8) * per + b / 8 and each XCD sweeps one
+    // contiguous eighth of the rows (the grid is then a multiple of 8).
     const int nblk = (int)(((long long)n + kSpmvRows - 1) / kSpmvRows);
     const int per = gridDim.x / kXcds;
-    const int blk = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    const int blk = xcd ? (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds) : (int)blockIdx.x;
     if (blk >= nblk) return;
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const long long row0 = (long long)blk * kSpmvRows + (long long)w * kWave;
@@ -175,23 +180,25 @@ __global__ void k_oc_apply(int n, const int *__restrict__ co_ptr, const int *__r
 void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const double *a, const double *x,
                    const double *b, const double *v, double *y, float *y32, double *pa, double *pb,
                    const Scalars *sc) {
-    const int g = xcd_grid(spmv_grid(n));
+    const int xo = g_spmv_xcd;
+    const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
     switch (mode) {
-    case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
-    case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
-    case 2: k_spmv<double, double, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
-    default: k_spmv<double, double, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc); break;
+    case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
+    case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
+    case 2: k_spmv<double, double, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
+    default: k_spmv<double, double, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
     }
 }
 
 void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja, const float *a, const float *x,
                    const float *b, const float *v, float *y, double *pa, double *pb, const Scalars *sc) {
-    const int g = xcd_grid(spmv_grid(n));
+    const int xo = g_spmv_xcd;
+    const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
     switch (mode) {
-    case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
-    case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
-    case 2: k_spmv<float, float, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
-    default: k_spmv<float, float, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc); break;
+    case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
+    case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
+    case 2: k_spmv<float, float, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
+    default: k_spmv<float, float, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
     }
 }
 

@@ -26,6 +26,7 @@ from typing import Iterable, List, Optional
 import numpy as np
 
 from . import fcap
+from . import mesh as M
 from . import workloads as W
 
 H = 32
@@ -172,9 +173,21 @@ class Network:
         return cls(np.load(path, allow_pickle=False), ctx)
 
 
-def training_system(n: int, seed: int, ctx: fcap.Context, nblobs: int = 8) -> fcap.System:
-    """A pressure system of the training case: perturbed n^3 hex mesh (seed), blob RHS (seed)."""
-    mesh = W.case_mesh(4, n, seed=seed)
+def graded_slab(nx: int = 192, nyz: int = 32, seed: int = 0):
+    """The near-wall structure of the 192^3 graded mesh at 1/36 of its size: nx x nyz x nyz hex
+    cells, interior vertices perturbed +-20%, x graded by 1.05 toward the x=0 wall (aspect up to
+    100), and y, z scaled so that dy = dz = 1/nx as in the full mesh.  The network only sees
+    4-hop neighbourhoods, which such a slab reproduces."""
+    m = M.hex_box(nx, nyz, nyz, perturb=0.2, seed=seed, grade_axis=0, grade_ratio=1.05, grade_max_aspect=100.0)
+    m.points = m.points.copy()
+    m.points[:, 1:] *= nyz / nx
+    return m
+
+
+def training_system(n: int, seed: int, ctx: fcap.Context, nblobs: int = 8, kind: str = "perturbed") -> fcap.System:
+    """A pressure system for training: kind "perturbed" is the n^3 perturbed hex mesh of the
+    64^3 training case (seed); kind "graded" a graded slab (graded_slab, n cells along x)."""
+    mesh = W.case_mesh(4, n, seed=seed) if kind == "perturbed" else graded_slab(n, max(8, n // 6), seed)
     s = fcap.System(mesh, ctx)
     xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")
     su0 = W.blob_rhs(xc, yc, zc, vol, nblobs, seed=10_000 + seed)
@@ -183,13 +196,14 @@ def training_system(n: int, seed: int, ctx: fcap.Context, nblobs: int = 8) -> fc
 
 
 def build_dataset(net: Network, n: int, n_systems: int, iters=SAMPLE_ITERS, seed0: int = 0,
-                  ctx: Optional[fcap.Context] = None) -> List[fcap.System]:
-    """Assemble n_systems training systems and add their IC(0)-PCG (r, e) pairs; returns the systems
-    (closing one drops its samples)."""
+                  ctx: Optional[fcap.Context] = None, n_graded: int = 0, graded_n: int = 192) -> List[fcap.System]:
+    """Assemble n_systems perturbed n^3 systems and n_graded graded slabs (graded_n cells along
+    x) and add their IC(0)-PCG (r, e) pairs; returns the systems (closing one drops its samples)."""
     ctx = ctx or net.ctx
     systems = []
-    for k in range(n_systems):
-        s = training_system(n, seed0 + k, ctx)
+    for k in range(n_systems + n_graded):
+        graded = k >= n_systems
+        s = training_system(graded_n if graded else n, seed0 + k, ctx, kind="graded" if graded else "perturbed")
         net.add_samples(s, iters, seed=seed0 + k)
         systems.append(s)
     return systems

@@ -2,14 +2,14 @@
 
 Dataset: pressure systems (Laplacian, mu = -1, Dirichlet wall at x = 0) on
 n^3 hex meshes with interior vertices perturbed by +-20% (one mesh seed per
-system); the (residual, error) pairs are IC(0)-PCG iterates of A x = 0 from
-seeded random starts after the listed iteration counts (exact errors
-e = -x).  Training: Adam on the energy-norm loss
-(z - e)^T A (z - e) / (e^T A e), mini-batches of 8 pairs.
-Systems are generated in chunks so that only `--resident` of them hold device
-memory at once.
+system), plus graded slabs that carry the near-wall structure of the 192^3
+graded mesh (gnn.graded_slab); the (residual, error) pairs are IC(0)-PCG
+iterates of A x = 0 from seeded random starts after the listed iteration
+counts (exact errors e = -x).  Training: Adam on the energy-norm loss
+(z - e)^T A (z - e) / (e^T A e), mini-batches of 8 pairs, cosine-decayed
+learning rate.  All systems stay resident on the GPU during training.
 
-    python -m freecappuccino_amd.train_gnn --n 64 --systems 64 --steps 3000 --out weights.npy
+    python -m freecappuccino_amd.train_gnn --n 64 --systems 32 --graded 16 --steps 3000 --out weights.npy
 """
 from __future__ import annotations
 
@@ -24,9 +24,10 @@ from . import fcap, gnn
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, default=64, help="cells per direction of the training meshes")
-    ap.add_argument("--systems", type=int, default=64)
-    ap.add_argument("--resident", type=int, default=64, help="systems kept on the GPU at once")
+    ap.add_argument("--n", type=int, default=64, help="cells per direction of the perturbed training meshes")
+    ap.add_argument("--systems", type=int, default=32, help="perturbed n^3 systems")
+    ap.add_argument("--graded", type=int, default=16, help="graded slabs (gnn.graded_slab)")
+    ap.add_argument("--graded-n", type=int, default=192, help="cells along x of a graded slab")
     ap.add_argument("--iters", type=str, default=",".join(str(k) for k in gnn.SAMPLE_ITERS),
                     help="IC(0)-PCG iteration counts whose (r, e) pairs are recorded")
     ap.add_argument("--steps", type=int, default=3000)
@@ -44,39 +45,32 @@ def main(argv=None):
     net = gnn.Network(init, ctx)
     iters = [int(x) for x in args.iters.split(",")]
     t0 = time.time()
+    systems = gnn.build_dataset(net, args.n, args.systems, iters=iters, seed0=args.seed * 100000, ctx=ctx,
+                                n_graded=args.graded, graded_n=args.graded_n)
+    print(f"{args.systems} perturbed {args.n}^3 systems + {args.graded} graded slabs, {net.num_samples} pairs, "
+          f"{time.time() - t0:.1f} s", flush=True)
     history = []
-    chunks = max(1, -(-args.systems // args.resident))
-    steps_per_chunk = max(1, args.steps // chunks)
     done = 0
-    for ch in range(chunks):
-        lo = ch * args.resident
-        hi = min(args.systems, lo + args.resident)
-        net.clear_samples()
-        systems = gnn.build_dataset(net, args.n, hi - lo, iters=iters, seed0=args.seed * 100000 + lo, ctx=ctx)
-        print(f"chunk {ch + 1}/{chunks}: {hi - lo} systems, {net.num_samples} pairs, "
-              f"{time.time() - t0:.1f} s", flush=True)
-        # cosine-decayed learning rate over the whole run, in blocks of 100 steps
-        left = steps_per_chunk if ch < chunks - 1 else args.steps - done
-        while left > 0:
-            k = min(100, left)
-            frac = done / max(1, args.steps)
-            lr = args.lr_final + 0.5 * (args.lr - args.lr_final) * (1 + np.cos(np.pi * frac))
-            losses = net.train(k, batch=args.batch, lr=float(lr), seed=args.seed * 7919 + done)
-            history.extend(float(x) for x in losses)
-            done += k
-            left -= k
-            print(f"step {done}: loss {losses.mean():.4f} (lr {lr:.2e}), {time.time() - t0:.1f} s", flush=True)
-        net.clear_samples()
-        for s in systems:
-            s.close()
+    while done < args.steps:
+        # cosine-decayed learning rate, in blocks of 100 steps
+        k = min(100, args.steps - done)
+        frac = done / max(1, args.steps)
+        lr = args.lr_final + 0.5 * (args.lr - args.lr_final) * (1 + np.cos(np.pi * frac))
+        losses = net.train(k, batch=args.batch, lr=float(lr), seed=args.seed * 7919 + done)
+        history.extend(float(x) for x in losses)
+        done += k
+        print(f"step {done}: loss {losses.mean():.4f} (lr {lr:.2e}), {time.time() - t0:.1f} s", flush=True)
     net.save(args.out)
     if args.log:
         with open(args.log, "w") as f:
-            json.dump({"n": args.n, "systems": args.systems, "iters": iters, "steps": args.steps,
-                       "batch": args.batch, "lr": args.lr, "lr_final": args.lr_final,
-                       "seconds": time.time() - t0, "loss": history}, f)
+            json.dump({"n": args.n, "systems": args.systems, "graded": args.graded, "graded_n": args.graded_n,
+                       "iters": iters, "steps": args.steps, "batch": args.batch, "lr": args.lr,
+                       "lr_final": args.lr_final, "seconds": time.time() - t0, "loss": history}, f)
     print(f"saved {args.out}: first-100 mean loss {np.mean(history[:100]):.4f}, "
           f"last-100 mean loss {np.mean(history[-100:]):.4f}", flush=True)
+    net.clear_samples()
+    for s in systems:
+        s.close()
     net.close()
     ctx.close()
 
