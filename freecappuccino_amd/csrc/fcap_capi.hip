@@ -194,6 +194,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
+        if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -756,9 +757,6 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_FCG_LEARNED: {
             FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
             PrecondHook P = gnn_hook(sys);
-            // FP32 FCG keeps its recursive residual honest to ~1e-6 relative, so each
-            // inner solve takes 4 orders: two refinement steps reach 1e-8
-            args.inner_eta = 1e-4;
             s = solve_fcg(ctx, args, P, &local);
             break;
         }

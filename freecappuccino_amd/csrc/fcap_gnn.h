@@ -22,6 +22,12 @@ struct GnnGraph {
     int n = 0, nnz = 0;
     const int *ptr = nullptr, *ja = nullptr, *diagpos = nullptr;
     DevBuf ef, dg, trans, a32;
+    // The off-diagonal entries again as an ELL over the cells: slot k of cell i
+    // at [k * n + i], in CSR order; padding slots hold the cell itself and zero
+    // features and lie past ecnt[i].  Lanes of a wave own consecutive cells, so
+    // their neighbour and feature loads are coalesced.
+    int ew = 0;                      // ELL width: the largest off-diagonal count
+    DevBuf ecnt, enbr, eef, eeft;    // count; neighbour j; features of (i,j); of (j,i)
     bool ready = false;
 };
 

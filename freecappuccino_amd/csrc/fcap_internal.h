@@ -143,6 +143,8 @@ struct fc_context {
     int sweep_mode = 0;
     // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
     int packed = 1;
+    // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)
+    int gnn_unroll = 1;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -42,8 +42,9 @@ enum FinStep : int {
     kFinFcgEnd = 12   // FCG: residual, convergence, stagnation at the end of each window
 };
 
-constexpr int kStagEvery = 32;          // FCG stagnation window (iterations)
+constexpr int kStagEvery = 64;          // FCG stagnation window (iterations)
 constexpr double kStagFactor = 0.81;    // the window's best ||r||^2 must beat 0.81x the best before it
+constexpr double kStagFirst = 2.0;      // ... and the first window's 2x ||r0||^2
 
 __device__ inline void converge_test(Scalars *sc) {
     sc->iters += 1;
@@ -146,9 +147,11 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
         converge_test(sc);
         sc->stag_min = fmin(sc->stag_min, b);
         if (!sc->done && sc->iters % kStagEvery == 0) {
-            // CG's residual 2-norm is not monotone and may rise at first: the first
-            // window only has to improve on ||r0||, later ones by the factor
-            const double f = sc->iters == kStagEvery ? 1.0 : kStagFactor;
+            // CG's residual 2-norm is not monotone: on graded meshes it rises to ~3x
+            // ||r0|| in the first tens of iterations even with IC(0).  The first
+            // window may end up to 2x above ||r0||^2; later windows must beat the
+            // best before them by the factor.
+            const double f = sc->iters == kStagEvery ? kStagFirst : kStagFactor;
             if (!(sc->stag_min < f * sc->stag_ref)) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
             sc->stag_ref = fmin(sc->stag_ref, sc->stag_min);
             sc->stag_min = 1e300;
