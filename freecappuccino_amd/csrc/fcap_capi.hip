@@ -195,6 +195,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
+        if (const char *w = std::getenv("FCAP_FCG_STAG")) ctx->fcg_stag_window = std::max(0, std::atoi(w));
         return FC_OK;
     });
     if (s != FC_OK) {

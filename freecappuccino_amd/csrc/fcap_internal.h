@@ -145,6 +145,8 @@ struct fc_context {
     int packed = 1;
     // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)
     int gnn_unroll = 1;
+    // flexible CG stagnation window in iterations (FCAP_FCG_STAG; 0: no stagnation check)
+    int fcg_stag_window = 64;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

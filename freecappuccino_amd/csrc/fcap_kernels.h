@@ -37,6 +37,7 @@ struct Scalars {
     double fcg_pr;                             // flexible CG: p.r of this step
     double fcg_pqh[kFcgM], fcg_bet[kFcgM];     // flexible CG: p.q and beta per history slot
     double stag_ref, stag_min;                 // flexible CG: best ||r||^2 before / within the window
+    int stag_every;                            // flexible CG: stagnation window (0: no check)
     int iters, done, status, nsw;
     int stop_rule;     // 0: L1 rsm < resmax; 1: L2 rel < resmax; 2: rnorm2 <= inner_tol2
     int early_exit;    // 1: stop before iterating when res0 < 1e-13 (src/iccg.f90:67-71)

@@ -42,7 +42,7 @@ enum FinStep : int {
     kFinFcgEnd = 12   // FCG: residual, convergence, stagnation at the end of each window
 };
 
-constexpr int kStagEvery = 64;          // FCG stagnation window (iterations)
+// FCG stagnation: window length in fc_context::fcg_stag_window (64 by default)
 constexpr double kStagFactor = 0.81;    // the window's best ||r||^2 must beat 0.81x the best before it
 constexpr double kStagFirst = 2.0;      // ... and the first window's 2x ||r0||^2
 
@@ -146,12 +146,12 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
         sc->rnorm2 = b;
         converge_test(sc);
         sc->stag_min = fmin(sc->stag_min, b);
-        if (!sc->done && sc->iters % kStagEvery == 0) {
+        if (!sc->done && sc->stag_every > 0 && sc->iters % sc->stag_every == 0) {
             // CG's residual 2-norm is not monotone: on graded meshes it rises to ~3x
             // ||r0|| in the first tens of iterations even with IC(0).  The first
             // window may end up to 2x above ||r0||^2; later windows must beat the
             // best before them by the factor.
-            const double f = sc->iters == kStagEvery ? kStagFirst : kStagFactor;
+            const double f = sc->iters == sc->stag_every ? kStagFirst : kStagFactor;
             if (!(sc->stag_min < f * sc->stag_ref)) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
             sc->stag_ref = fmin(sc->stag_ref, sc->stag_min);
             sc->stag_min = 1e300;
@@ -845,6 +845,7 @@ static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double
     h.res0 = 1.0;
     h.stag_ref = 1.0;      // ||r0||^2 of the unit-norm inner right-hand side
     h.stag_min = 1e300;
+    h.stag_every = c.ctx->fcg_stag_window;
     h.status = kStatusRunning;
     upload_scalars(c, h);
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
