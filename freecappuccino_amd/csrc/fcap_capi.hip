@@ -758,6 +758,10 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_FCG_LEARNED: {
             FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
             PrecondHook P = gnn_hook(sys);
+            // each refinement restart discards FCG's stored directions: inner solves to
+            // 1e-4 reach 1e-8 in two steps (with 1e-3, FCG with a zero network needed
+            // 137 iterations where FP64 Jacobi-PCG needs 98)
+            args.inner_eta = 1e-4;
             s = solve_fcg(ctx, args, P, &local);
             break;
         }
