@@ -196,6 +196,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
         if (const char *w = std::getenv("FCAP_FCG_STAG")) ctx->fcg_stag_window = std::max(0, std::atoi(w));
+        if (const char *g = std::getenv("FCAP_GNN_XCD")) ctx->gnn_xcd = std::atoi(g) != 0;
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -758,10 +759,6 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_FCG_LEARNED: {
             FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
             PrecondHook P = gnn_hook(sys);
-            // each refinement restart discards FCG's stored directions: inner solves to
-            // 1e-4 reach 1e-8 in two steps (with 1e-3, FCG with a zero network needed
-            // 137 iterations where FP64 Jacobi-PCG needs 98)
-            args.inner_eta = 1e-4;
             s = solve_fcg(ctx, args, P, &local);
             break;
         }

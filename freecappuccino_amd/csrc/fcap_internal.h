@@ -147,6 +147,8 @@ struct fc_context {
     int gnn_unroll = 1;
     // flexible CG stagnation window in iterations (FCAP_FCG_STAG; 0: no stagnation check)
     int fcg_stag_window = 64;
+    // network layer kernel: XCD-contiguous cell ranges (FCAP_GNN_XCD=0: natural block order)
+    int gnn_xcd = 1;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

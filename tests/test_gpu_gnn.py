@@ -192,15 +192,15 @@ def test_fcg_learned_converges_and_beats_no_network(ctx, trained):
 
 def test_fcg_with_zero_network_matches_jacobi_pcg(ctx):
     """A zero network is the fixed SPD preconditioner D^-1, and flexible CG with a fixed SPD
-    preconditioner is PCG: the iteration count matches FP64 Jacobi-PCG up to FP32 rounding and
-    the restarts of the refinement loop."""
+    preconditioner is PCG: to 1e-3, reached within the first inner FP32 solve (no restart of the
+    refinement loop), the iteration count matches FP64 Jacobi-PCG up to FP32 rounding."""
     m = M.hex_box(12, perturb=0.2, seed=6)
     s = _system(m, ctx, seed=9)
     s.set_solution(np.zeros(s.n))
-    rc_j, info_j = s.solve(fcap.METHOD_DPCG, fcap.STOP_L2, 1e-8, 5000)
-    rc_f, info_f = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
+    rc_j, info_j = s.solve(fcap.METHOD_DPCG, fcap.STOP_L2, 1e-3, 5000)
+    rc_f, info_f = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
     assert rc_j == fcap.FC_OK and rc_f == fcap.FC_OK, (info_j, info_f)
-    assert abs(info_f["iters"] - info_j["iters"]) <= 0.2 * info_j["iters"] + 10, (info_f["iters"], info_j["iters"])
+    assert abs(info_f["iters"] - info_j["iters"]) <= 0.1 * info_j["iters"] + 5, (info_f["iters"], info_j["iters"])
     s.close()
 
 
