@@ -48,6 +48,19 @@ def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
     return nnz * (value_bytes + index_bytes) + index_bytes * (n + 1) + 2 * value_bytes * n
 
 
+def gnn_layer_bytes(n, width=6):
+    """Algorithmic bytes of one message-passing layer: the cell's own BF16 row in (64) and out
+    (64), its ELL slots (neighbour index + packed features, 8 per slot) and count (4); the
+    neighbours' rows are the other cells' own rows and are not counted again."""
+    return n * (64 + 64 + 8 * width + 4)
+
+
+def gnn_bytes(n, width=6, layers=4):
+    """One application: RMS pass (r, a_ii), input pass (r, a_ii in; u, h0 out), layers 1..L-1,
+    and the last layer (reads u, writes z instead of a row)."""
+    return n * (8 + 8 + 4 + 64) + (layers - 1) * gnn_layer_bytes(n, width) + n * (64 + 8 * width + 4 + 4 + 4)
+
+
 def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
     """The oracle's sequential DIC-PCG on one host core, `budget_iters` iterations, scaled."""
     from oracle import oracle as O
@@ -79,6 +92,8 @@ def main():
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only assemble and run --spmv-reps FP64 SpMVs plus a 1 GiB device copy "
                          "(the calibration kernel of the HBM counters), then exit")
+    ap.add_argument("--profile-gnn", type=int, default=0, metavar="REPS",
+                    help="only assemble and run REPS applications of the network, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -110,6 +125,16 @@ def main():
                           "spmv_GBps": spmv_bytes(n, nnz) / ms / 1e6, "copy_bytes": copy_bytes,
                           "copy_ms": copy_ms, "copy_GBps": 2 * copy_bytes / copy_ms / 1e6,
                           "cells": n, "nnz": nnz}))
+        s.close()
+        ctx.close()
+        return
+    if args.profile_gnn:
+        net = gnn.Network.load(args.weights or gnn.DEFAULT_WEIGHTS, ctx)
+        ms = net.bench(s, args.profile_gnn)
+        b = gnn_bytes(n)
+        print(json.dumps({"gnn_ms": ms, "gnn_bytes": b, "gnn_GBps": b / ms / 1e6,
+                          "layer_bytes": gnn_layer_bytes(n), "cells": n, "reps": args.profile_gnn}))
+        net.close()
         s.close()
         ctx.close()
         return

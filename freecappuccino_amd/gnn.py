@@ -108,6 +108,13 @@ class Network:
         self.ctx.check(self.ctx.L.fc_gnn_apply(self.h, system.h, _fp(r), _fp(z)))
         return z
 
+    def bench(self, system: fcap.System, reps: int = 20) -> float:
+        """Average ms of one application on `system` (device-resident input, HIP events)."""
+        ms = self.ctx.L.fc_gnn_bench(self.h, system.h, int(reps))
+        if ms < 0:
+            raise fcap.FcError(-2, self.ctx.L.fc_last_error(self.ctx.h).decode())
+        return ms
+
     def edge_features(self, system: fcap.System):
         nnz = system.sizes()["nnz"]
         e1, e2, dg = np.zeros(nnz, np.float32), np.zeros(nnz, np.float32), np.zeros(system.n, np.float32)

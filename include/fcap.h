@@ -264,6 +264,9 @@ fc_status fc_gnn_set_params(fc_gnn *net, const float *params);
 fc_status fc_gnn_get_params(fc_gnn *net, float *params);
 /* z = P(r) on a system's graph (host arrays of numCells floats). */
 fc_status fc_gnn_apply(fc_gnn *net, fc_system *sys, const float *r, float *z);
+/* Average milliseconds of one network application on the system (device-resident
+ * input), `reps` applications timed with events on the library's stream; -1 on error. */
+double fc_gnn_bench(fc_gnn *net, fc_system *sys, int reps);
 /* Edge features of the system's CSR entries (nnz each; 0 on the diagonal) and its FP32 diagonal. */
 fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2, float *diag);
 /* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
