@@ -197,6 +197,8 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
         if (const char *w = std::getenv("FCAP_FCG_STAG")) ctx->fcg_stag_window = std::max(0, std::atoi(w));
         if (const char *g = std::getenv("FCAP_GNN_XCD")) ctx->gnn_xcd = std::atoi(g) != 0;
+        if (const char *o = std::getenv("FCAP_GNN_ORDER")) ctx->gnn_order = std::atoi(o) != 0;
+        if (const char *pp = std::getenv("FCAP_GNN_PIPE")) ctx->gnn_pipe = std::max(-1, std::atoi(pp));
         return FC_OK;
     });
     if (s != FC_OK) {

@@ -28,6 +28,13 @@ struct GnnGraph {
     // their neighbour and feature loads are coalesced.
     int ew = 0;                      // ELL width: the largest off-diagonal count
     DevBuf ecnt, enbr, eef, eeft;    // count; neighbour j; features of (i,j); of (j,i)
+    // Processing order of the 64-cell blocks of the layer kernels: the blocks
+    // sorted by the Morton code of their centroid, so that the blocks in flight
+    // on one XCD cover a compact region and the rows their neighbours gather
+    // are still in that XCD's L2 (in index order a plane of a structured mesh
+    // separates two neighbours).
+    int nblk = 0;
+    DevBuf border;
     bool ready = false;
 };
 

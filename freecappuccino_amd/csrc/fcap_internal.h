@@ -149,6 +149,12 @@ struct fc_context {
     int fcg_stag_window = 64;
     // network layer kernel: XCD-contiguous cell ranges (FCAP_GNN_XCD=0: natural block order)
     int gnn_xcd = 1;
+    // network layer kernel: 64-cell blocks in Morton order of their centroids
+    // (FCAP_GNN_ORDER=0: block index order)
+    int gnn_order = 1;
+    // network layer kernel: persistent software-pipelined waves, this many per XCD
+    // (FCAP_GNN_PIPE; -1: as many as are resident; 0: one 64-cell block per workgroup)
+    int gnn_pipe = -1;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -43,8 +43,7 @@ enum FinStep : int {
 };
 
 // FCG stagnation: window length in fc_context::fcg_stag_window (64 by default)
-constexpr double kStagFactor = 0.81;    // the window's best ||r||^2 must beat 0.81x the best before it
-constexpr double kStagFirst = 2.0;      // ... and the first window's 2x ||r0||^2
+constexpr double kStagFactor = 0.81;    // a window's best ||r||^2 must beat 0.81x the best before it
 
 __device__ inline void converge_test(Scalars *sc) {
     sc->iters += 1;
@@ -147,13 +146,17 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
         converge_test(sc);
         sc->stag_min = fmin(sc->stag_min, b);
         if (!sc->done && sc->stag_every > 0 && sc->iters % sc->stag_every == 0) {
-            // CG's residual 2-norm is not monotone: on graded meshes it rises to ~3x
-            // ||r0|| in the first tens of iterations even with IC(0).  The first
-            // window may end up to 2x above ||r0||^2; later windows must beat the
-            // best before them by the factor.
-            const double f = sc->iters == sc->stag_every ? kStagFirst : kStagFactor;
-            if (!(sc->stag_min < f * sc->stag_ref)) { sc->done = 1; sc->status = FC_FALLBACK_USED; }
-            sc->stag_ref = fmin(sc->stag_ref, sc->stag_min);
+            // CG's residual 2-norm is not monotone: on a graded 192^3 mesh it rises to
+            // ~13x ||r0|| within a few iterations of the learned preconditioner and is
+            // back below ||r0|| only after ~100 (~3x with IC(0)).  The first window
+            // only sets the reference; every later window's best must beat the best
+            // before it by the factor.
+            const bool first = sc->iters == sc->stag_every;
+            if (!first && !(sc->stag_min < kStagFactor * sc->stag_ref)) {
+                sc->done = 1;
+                sc->status = FC_FALLBACK_USED;
+            }
+            sc->stag_ref = first ? sc->stag_min : fmin(sc->stag_ref, sc->stag_min);
             sc->stag_min = 1e300;
         }
         break;
@@ -843,7 +846,7 @@ static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double
     h.stop_rule = 2;
     h.inner_tol2 = eta * eta;
     h.res0 = 1.0;
-    h.stag_ref = 1.0;      // ||r0||^2 of the unit-norm inner right-hand side
+    h.stag_ref = 1e300;    // set by the first window
     h.stag_min = 1e300;
     h.stag_every = c.ctx->fcg_stag_window;
     h.status = kStatusRunning;

@@ -104,6 +104,7 @@ def library() -> C.CDLL:
         "fc_gnn_get_params": ([vp, fp], i),
         "fc_gnn_apply": ([vp, vp, fp, fp], i),
         "fc_gnn_bench": ([vp, vp, i], d),
+        "fc_gnn_bench_layer": ([vp, vp, i], d),
         "fc_gnn_edge_features": ([vp, vp, fp, fp, fp], i),
         "fc_gnn_loss_grad": ([vp, vp, fp, fp, dp, fp], i),
         "fc_gnn_add_samples": ([vp, vp, i, ip, C.c_uint], i),

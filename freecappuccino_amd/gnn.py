@@ -115,6 +115,13 @@ class Network:
             raise fcap.FcError(-2, self.ctx.L.fc_last_error(self.ctx.h).decode())
         return ms
 
+    def bench_layer(self, system: fcap.System, reps: int = 20) -> float:
+        """Average ms of one launch of a hidden message-passing layer on `system` (HIP events)."""
+        ms = self.ctx.L.fc_gnn_bench_layer(self.h, system.h, int(reps))
+        if ms < 0:
+            raise fcap.FcError(-2, self.ctx.L.fc_last_error(self.ctx.h).decode())
+        return ms
+
     def edge_features(self, system: fcap.System):
         nnz = system.sizes()["nnz"]
         e1, e2, dg = np.zeros(nnz, np.float32), np.zeros(nnz, np.float32), np.zeros(system.n, np.float32)

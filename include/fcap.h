@@ -267,6 +267,10 @@ fc_status fc_gnn_apply(fc_gnn *net, fc_system *sys, const float *r, float *z);
 /* Average milliseconds of one network application on the system (device-resident
  * input), `reps` applications timed with events on the library's stream; -1 on error. */
 double fc_gnn_bench(fc_gnn *net, fc_system *sys, int reps);
+/* Average milliseconds of one launch of a hidden message-passing layer (layer 2 of 4,
+ * rows in HBM) on the system, `reps` launches timed with events on the library's
+ * stream; -1 on error.  No counterpart in the reference (performance measurement). */
+double fc_gnn_bench_layer(fc_gnn *net, fc_system *sys, int reps);
 /* Edge features of the system's CSR entries (nnz each; 0 on the diagonal) and its FP32 diagonal. */
 fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2, float *diag);
 /* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
