@@ -209,6 +209,134 @@ __global__ void k_rhs_divergence(int n, const int *__restrict__ ptr, const int *
     su[i] = s;
 }
 
+// ----------------------------------------------- pressure-correction assembly
+
+// face_value_central (src/interpolation.f90:145-197); g holds 3 components per cell
+__device__ inline double face_value_central(int p, int q, double xf, double yf, double zf,
+                                            const double *__restrict__ fi, const double *__restrict__ g,
+                                            const RhieChowDev &F) {
+    const double xcp = F.xc[p], ycp = F.yc[p], zcp = F.zc[p];
+    const double xcn = F.xc[q], ycn = F.yc[q], zcn = F.zc[q];
+    const double *gp = g + 3 * (size_t)p, *gn = g + 3 * (size_t)q;
+    const double gradfidr = ((((gp[0] * (xf - xcp) + gp[1] * (yf - ycp)) + gp[2] * (zf - zcp)) + gn[0] * (xf - xcn)) +
+                             gn[1] * (yf - ycn)) +
+                            gn[2] * (zf - zcn);
+    return 0.5 * ((fi[p] + fi[q]) + gradfidr);
+}
+
+// facefluxmass (src/facefluxmass.f90:34-199): cap (= can) and the Rhie-Chow mass
+// flux of the face between cells p and q (0-based), every expression in the
+// reference's evaluation order
+__device__ inline void flux_mass(int p, int q, double xf, double yf, double zf, double arx, double ary, double arz,
+                                 double lambda, const RhieChowDev &F, double &cap, double &fluxmass) {
+    const double fxn = lambda;
+    const double fxp = 1.0 - lambda;
+    const double xcp = F.xc[p], ycp = F.yc[p], zcp = F.zc[p];
+    const double xcn = F.xc[q], ycn = F.yc[q], zcn = F.zc[q];
+    const double xpn = xcn - xcp, ypn = ycn - ycp, zpn = zcn - zcp;
+    const double are = sqrt((arx * arx + ary * ary) + arz * arz);
+    const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+    const double vn = F.vol[q], vp = F.vol[p];
+    const double Dpu = (fxn * vn) * F.apu[q] + (fxp * vp) * F.apu[p];
+    const double Dpv = (fxn * vn) * F.apv[q] + (fxp * vp) * F.apv[p];
+    const double Dpw = (fxn * vn) * F.apw[q] + (fxp * vp) * F.apw[p];
+    const double dene = F.den[p] * fxp + F.den[q] * fxn;
+    const double sfdpnr = 1.0 / (((arx * xpn) * nxx + (ary * ypn) * nyy) + (arz * zpn) * nzz);
+    const double smdpn = ((arx * arx + ary * ary) + arz * arz) * sfdpnr;
+    cap = -((dene * Dpu) * smdpn);
+    const double ui = face_value_central(p, q, xf, yf, zf, F.u, F.dudx, F);
+    const double vi = face_value_central(p, q, xf, yf, zf, F.v, F.dvdx, F);
+    const double wi = face_value_central(p, q, xf, yf, zf, F.w, F.dwdx, F);
+    const double *gpp = F.dpdx + 3 * (size_t)p, *gpn = F.dpdx + 3 * (size_t)q;
+    const double dpxi = ((Dpu * (fxn * gpn[0] + fxp * gpp[0])) * xpn) * nxx;
+    const double dpyi = ((Dpv * (fxn * gpn[1] + fxp * gpp[1])) * ypn) * nyy;
+    const double dpzi = ((Dpw * (fxn * gpn[2] + fxp * gpp[2])) * zpn) * nzz;
+    const double xpp = (xf - (xf - xcp) * nxx) - xcp, ypp = (yf - (yf - ycp) * nyy) - ycp,
+                 zpp = (zf - (zf - zcp) * nzz) - zcp;
+    const double xep = (xf - (xf - xcn) * nxx) - xcn, yep = (yf - (yf - ycn) * nyy) - ycn,
+                 zep = (zf - (zf - zcn) * nzz) - zcn;
+    // as the reference writes it: only the x term of the owner side is subtracted
+    const double dpecorr = ((((gpn[0] * xep + gpn[1] * yep) + gpn[2] * zep) - gpp[0] * xpp) + gpp[1] * ypp) +
+                           gpp[2] * zpp;
+    const double dpe = (F.p[q] - F.p[p]) + dpecorr;
+    const double dpex = ((Dpu * dpe) * sfdpnr) * arx;
+    const double dpey = ((Dpv * dpe) * sfdpnr) * ary;
+    const double dpez = ((Dpw * dpe) * sfdpnr) * arz;
+    const double ue = (ui - dpex) + dpxi;
+    const double ve = (vi - dpey) + dpyi;
+    const double we = (wi - dpez) + dpzi;
+    fluxmass = dene * ((ue * arx + ve * ary) + we * arz);
+}
+
+__global__ void k_calcp_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                              const int *__restrict__ ij, const int *__restrict__ ji, const double *__restrict__ xf,
+                              const double *__restrict__ yf, const double *__restrict__ zf,
+                              const double *__restrict__ arx, const double *__restrict__ ary,
+                              const double *__restrict__ arz, const double *__restrict__ facint, RhieChowDev F,
+                              double *__restrict__ a, double *__restrict__ flmass) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    double cap, fl;
+    flux_mass(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], F, cap, fl);
+    a[ij[f]] = cap;   // can
+    a[ji[f]] = cap;   // cap
+    flmass[f] = fl;
+}
+
+__global__ void k_calcp_oc(int noc, int oc_face0, const int *__restrict__ ijl, const int *__restrict__ ijr,
+                           const double *__restrict__ foc, const double *__restrict__ xf,
+                           const double *__restrict__ yf, const double *__restrict__ zf,
+                           const double *__restrict__ arx, const double *__restrict__ ary,
+                           const double *__restrict__ arz, RhieChowDev F, double *__restrict__ al,
+                           double *__restrict__ ar, double *__restrict__ fmoc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noc) return;
+    const int f = oc_face0 + i;
+    double cap, fl;
+    flux_mass(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], F, cap, fl);
+    al[i] = cap;
+    ar[i] = cap;
+    fmoc[i] = fl;
+}
+
+// Row i's diagonal, 0 - (the row's off-diagonals in face order) - (O-C cuts), and
+// its continuity source, 0 -/+ F in face order then the cuts' -/+ fmoc: the
+// values the reference's face loops leave in a(diag(i)) and su(i).
+__global__ void k_calcp_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                             const int *__restrict__ kord, const int *__restrict__ kface, const int *__restrict__ ij,
+                             const double *__restrict__ flmass, const int *__restrict__ co_ptr,
+                             const int *__restrict__ co_idx, const double *__restrict__ al,
+                             const double *__restrict__ ar, const double *__restrict__ fmoc, double *__restrict__ a,
+                             double *__restrict__ su) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dp = diagpos[i];
+    double d = 0.0, s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+        const int k = kord ? kord[q] : q;
+        if (k == dp) continue;
+        d = d - a[k];
+        const int f = kface[k];
+        if (ij[f] == k) s = s - flmass[f];   // row i is the owner of face f
+        else s = s + flmass[f];
+    }
+    if (co_ptr) {
+        for (int q = co_ptr[i]; q < co_ptr[i + 1]; ++q) {
+            const int code = co_idx[q];
+            const int o = code >> 1;
+            if (code & 1) {   // i is the right cell ijr of cut o
+                d = d - al[o];
+                s = s + fmoc[o];
+            } else {
+                d = d - ar[o];
+                s = s - fmoc[o];
+            }
+        }
+    }
+    a[dp] = d;
+    su[i] = s;
+}
+
 __global__ void k_kface(int nif, const int *__restrict__ ij, const int *__restrict__ ji, int *__restrict__ kface) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nif) return;
@@ -433,6 +561,26 @@ void convdiff_dev(fc_context *ctx, const Structure &st, int nif, const int *owne
                                                       w.nwal ? w.cw_ptr.as<int>() : nullptr,
                                                       w.nwal ? w.cw_idx.as<int>() : nullptr, w.iwall_faces_start,
                                                       arx, ary, arz, xc, yc, zc, xf, yf, zf, vis, urfr, a);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+               const int *ij_slot, const int *ji_slot, const int *kface, const int *kface_order, const double *xf,
+               const double *yf, const double *zf, const double *arx, const double *ary, const double *arz,
+               const double *facint, const RhieChowDev &F, const OcDev &oc, double *a, double *su, double *flmass,
+               double *fmoc) {
+    hipStream_t s = ctx->stream;
+    if (nif)
+        k_calcp_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, xf, yf, zf, arx, ary,
+                                                       arz, facint, F, a, flmass);
+    if (oc.noc)
+        k_calcp_oc<<<grid_for(oc.noc), kBlock, 0, s>>>(oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(),
+                                                       oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx, ary,
+                                                       arz, F, oc.al.as<double>(), oc.ar.as<double>(), fmoc);
+    k_calcp_rows<<<grid_for(st.n), kBlock, 0, s>>>(
+        st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface, ij_slot, flmass,
+        oc.noc ? oc.co_ptr.as<int>() : nullptr, oc.noc ? oc.co_idx.as<int>() : nullptr,
+        oc.noc ? oc.al.as<double>() : nullptr, oc.noc ? oc.ar.as<double>() : nullptr, oc.noc ? fmoc : nullptr, a, su);
     FC_HIP_CHECK(hipGetLastError());
 }
 

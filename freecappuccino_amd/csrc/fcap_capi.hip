@@ -428,6 +428,87 @@ extern "C" fc_status fc_convdiff(fc_context *ctx, int numCells, int numInnerFace
     });
 }
 
+extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                       const int *neighbour, const double *xc, const double *yc, const double *zc,
+                                       const double *vol, const double *xf, const double *yf, const double *zf,
+                                       const double *arx, const double *ary, const double *arz,
+                                       const double *facint, const double *den, const double *apu,
+                                       const double *apv, const double *apw, const double *u, const double *v,
+                                       const double *w, const double *p, const double *dUdxi, const double *dVdxi,
+                                       const double *dWdxi, const double *dPdxi, int noc, int iOCFacesStart,
+                                       const int *ijl, const int *ijr, const double *foc, const int *ioffset,
+                                       const int *ja, const int *diag, const int *ij_slot, const int *ji_slot,
+                                       int nnz, double *a, double *su, double *flmass, double *al, double *ar,
+                                       double *fmoc) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n, FC_ERR_ARGUMENT, "nnz must be 2*numInnerFaces+numCells");
+        FC_REQUIRE(a && su && (flmass || nif == 0), FC_ERR_ARGUMENT, "a/su/flmass");
+        const double *cf[12] = {xc, yc, zc, vol, den, apu, apv, apw, u, v, w, p};
+        for (const double *c : cf) FC_REQUIRE(c != nullptr, FC_ERR_ARGUMENT, "cell fields");
+        const double *gf[4] = {dUdxi, dVdxi, dWdxi, dPdxi};
+        for (const double *g : gf) FC_REQUIRE(g != nullptr, FC_ERR_ARGUMENT, "gradients");
+        const int nfaces_needed = std::max(nif, noc > 0 ? iOCFacesStart + noc : 0);
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        FaceSet fs;
+        upload_faces(ctx, fs, st, nif, owner, neighbour, ij_slot, ji_slot);
+        DevBuf dx[6], dc[12], dg[4], dfi, dsu, da, dfl, dfm;
+        const double *fa[6] = {arx, ary, arz, xf, yf, zf};
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nfaces_needed);
+        for (int k = 0; k < 12; ++k) upload_dbl(ctx, dc[k], cf[k], n);
+        for (int k = 0; k < 4; ++k) upload_dbl(ctx, dg[k], gf[k], 3 * (size_t)n);
+        upload_dbl(ctx, dfi, facint, nif);
+        dsu.alloc(sizeof(double) * n);
+        da.alloc(sizeof(double) * nnz);
+        dfl.alloc(sizeof(double) * std::max(nif, 1));
+        FC_HIP_CHECK(hipMemsetAsync(da.get(), 0, sizeof(double) * nnz, ctx->stream));   // a = 0 (calcp:34)
+        OcDev oc;
+        oc.noc = std::max(noc, 0);
+        oc.ioc_faces_start = iOCFacesStart;
+        if (noc > 0) {
+            FC_REQUIRE(ijl && ijr && foc && al && ar && fmoc, FC_ERR_ARGUMENT, "O-C arrays");
+            upload_idx(ctx, oc.ijl, ijl, noc);
+            upload_idx(ctx, oc.ijr, ijr, noc);
+            upload_dbl(ctx, oc.foc, foc, noc);
+            oc.al.alloc(sizeof(double) * noc);
+            oc.ar.alloc(sizeof(double) * noc);
+            dfm.alloc(sizeof(double) * noc);
+            build_oc_lists(ctx, oc, n);
+        }
+        RhieChowDev F;
+        F.xc = dc[0].as<double>();
+        F.yc = dc[1].as<double>();
+        F.zc = dc[2].as<double>();
+        F.vol = dc[3].as<double>();
+        F.den = dc[4].as<double>();
+        F.apu = dc[5].as<double>();
+        F.apv = dc[6].as<double>();
+        F.apw = dc[7].as<double>();
+        F.u = dc[8].as<double>();
+        F.v = dc[9].as<double>();
+        F.w = dc[10].as<double>();
+        F.p = dc[11].as<double>();
+        F.dudx = dg[0].as<double>();
+        F.dvdx = dg[1].as<double>();
+        F.dwdx = dg[2].as<double>();
+        F.dpdx = dg[3].as<double>();
+        calcp_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.ji.as<int>(),
+                  fs.kface.as<int>(), fs.needs_order ? fs.kord.as<int>() : nullptr, dx[3].as<double>(),
+                  dx[4].as<double>(), dx[5].as<double>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(),
+                  dfi.as<double>(), F, oc, da.as<double>(), dsu.as<double>(), dfl.as<double>(),
+                  noc > 0 ? dfm.as<double>() : nullptr);
+        download_dbl(ctx, a, da.get(), nnz);
+        download_dbl(ctx, su, dsu.get(), n);
+        if (nif) download_dbl(ctx, flmass, dfl.get(), nif);
+        if (noc > 0) {
+            download_dbl(ctx, al, oc.al.get(), noc);
+            download_dbl(ctx, ar, oc.ar.get(), noc);
+            download_dbl(ctx, fmoc, dfm.get(), noc);
+        }
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_face_flux_central(fc_context *ctx, int numInnerFaces, const int *owner, const int *neighbour,
                                           const double *facint, const double *arx, const double *ary,
                                           const double *arz, const double *u, const double *v, const double *w,

@@ -206,6 +206,18 @@ struct OcDev {
     DevBuf co_ptr, co_idx;           // cell -> O-C indices, encoded (i<<1 | side)
 };
 void build_oc_lists(fc_context *ctx, OcDev &o, int n);
+// Fields of the pressure-correction assembly (device, 0-based cells; gradients
+// 3 per cell, component fastest, as dUdxi(3, numCells))
+struct RhieChowDev {
+    const double *xc, *yc, *zc, *vol, *den, *apu, *apv, *apw, *u, *v, *w, *p;
+    const double *dudx, *dvdx, *dwdx, *dpdx;
+};
+// calcp's face loops (a and su overwritten, flmass of the inner faces, O-C al/ar/fmoc)
+void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+               const int *ij_slot, const int *ji_slot, const int *kface, const int *kface_order, const double *xf,
+               const double *yf, const double *zf, const double *arx, const double *ary, const double *arz,
+               const double *facint, const RhieChowDev &F, const OcDev &oc, double *a, double *su, double *flmass,
+               double *fmoc);
 // range/consistency check of caller-supplied owner/neighbour and slot maps (device, 0-based)
 void validate_faces(fc_context *ctx, const Structure &st, int nif, const int *own, const int *nei, const int *ij,
                     const int *ji);

@@ -78,6 +78,8 @@ def library() -> C.CDLL:
                                                           dp, dp, dp, dp], i),
         "fc_convdiff": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, ip, ip, ip, ip, ip, i, d, dp], i),
         "fc_face_flux_central": ([vp, i, ip, ip, dp, dp, dp, dp, dp, dp, dp, i, dp], i),
+        "fc_calcp_assemble": ([vp, i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
+                              + [dp] * 6, i),
         "fc_rhs_divergence": ([vp, i, i, ip, ip, dp, dp], i),
         "fc_spmv": ([vp, i, i, ip, ip, dp, dp, dp], i),
         "fc_dic_pivots": ([vp, i, i, ip, ip, ip, dp, dp], i),
@@ -304,6 +306,41 @@ def convdiff(mesh, geo: Geometry, csr: CSR, vis, flux, urf=0.7, ctx: Optional[Co
                                 _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _ip(_i32(csr.ij_slot)),
                                 _ip(_i32(csr.ji_slot)), mesh.nnz, float(urf), _dp(a)))
     return a
+
+
+CALCP_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p")
+CALCP_GRADS = ("dudx", "dvdx", "dwdx", "dpdx")
+
+
+def calcp_assemble(mesh, geo: Geometry, csr: CSR, fields, oc=None, ctx: Optional[Context] = None):
+    """Pressure-correction assembly on the GPU: calcp's face loops around facefluxmass
+    (src/calcp-multiple_correction_SIMPLE.f90:34-107, src/facefluxmass.f90:34-199).
+    fields: den, apu, apv, apw, u, v, w, p (n each) and dudx, dvdx, dwdx, dpdx (n x 3).
+    oc: optional O-C cuts {ijl, ijr (0-based cells), foc, faces (0-based first cut face)}.
+    Returns a dict with a, su, flmass (and al, ar, fmoc with cuts)."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    fl = [_f64(fields[k]) for k in CALCP_FIELDS]
+    gr = [_f64(np.asarray(fields[k]).reshape(n, 3)) for k in CALCP_GRADS]
+    noc = 0 if oc is None else len(oc["ijl"])
+    ijl = _i32(np.asarray(oc["ijl"]) + 1) if noc else np.zeros(1, np.int32)
+    ijr = _i32(np.asarray(oc["ijr"]) + 1) if noc else np.zeros(1, np.int32)
+    foc = _f64(oc["foc"]) if noc else np.zeros(1)
+    oc0 = int(oc["faces"]) if noc else 0
+    out = {"a": np.zeros(mesh.nnz), "su": np.zeros(n), "flmass": np.zeros(max(nif, 1)),
+           "al": np.zeros(max(noc, 1)), "ar": np.zeros(max(noc, 1)), "fmoc": np.zeros(max(noc, 1))}
+    g = [_f64(getattr(geo, k)) for k in ("xc", "yc", "zc", "vol", "xf", "yf", "zf", "arx", "ary", "arz", "facint")]
+    ctx.check(ctx.L.fc_calcp_assemble(ctx.h, n, nif, _ip(own1), _ip(nei1), *[_dp(x) for x in g],
+                                      *[_dp(x) for x in fl], *[_dp(x) for x in gr], noc, oc0, _ip(ijl), _ip(ijr),
+                                      _dp(foc), _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                      _ip(_i32(csr.ij_slot)), _ip(_i32(csr.ji_slot)), mesh.nnz,
+                                      *[_dp(out[k]) for k in ("a", "su", "flmass", "al", "ar", "fmoc")]))
+    out["flmass"] = out["flmass"][:nif]
+    for k in ("al", "ar", "fmoc"):
+        out[k] = out[k][:noc]
+    return out
 
 
 def face_flux_central(mesh, geo: Geometry, u, v, w, ctx: Optional[Context] = None):

@@ -127,6 +127,28 @@ fc_status fc_convdiff(fc_context *ctx, int numCells, int numInnerFaces, const in
                       int iWallFacesStart, const int *ioffset, const int *ja, const int *diag,
                       const int *ij_slot, const int *ji_slot, int nnz, double urf, double *a);
 
+/* Pressure-correction assembly: the face loops of calcp
+ * (src/calcp-multiple_correction_SIMPLE.f90:34-107) around facefluxmass
+ * (src/facefluxmass.f90:34-199).  a and su are overwritten (the reference
+ * zeroes both); per inner face the coefficient cap = can =
+ * -rho_f (Vol/a_P)_f |S|^2 / (S . d_PN n) goes to a(ij), a(ji) and off the two
+ * diagonals, and the Rhie-Chow mass flux flmass to su(P) -= F, su(N) += F,
+ * all in face order; then the O-C cuts (iOCFacesStart+1.., al = cap, ar = can,
+ * fmoc).  Cell fields (den, apu/apv/apw = 1/a_P, u, v, w, p) have at least
+ * numCells entries; dUdxi .. dPdxi are the reference's gradient arrays
+ * (3, numCells).  Face arrays cover every face the loops touch. */
+fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                            const int *neighbour, const double *xc, const double *yc, const double *zc,
+                            const double *vol, const double *xf, const double *yf, const double *zf,
+                            const double *arx, const double *ary, const double *arz, const double *facint,
+                            const double *den, const double *apu, const double *apv, const double *apw,
+                            const double *u, const double *v, const double *w, const double *p,
+                            const double *dUdxi, const double *dVdxi, const double *dWdxi,
+                            const double *dPdxi, int noc, int iOCFacesStart, const int *ijl,
+                            const int *ijr, const double *foc, const int *ioffset, const int *ja,
+                            const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
+                            double *su, double *flmass, double *al, double *ar, double *fmoc);
+
 /* Face mass flux F_f = u_f . S_f, u_f = (1-facint) u_P + facint u_N, and the
  * continuity source su(o) -= F, su(n) += F of
  * src/calcp-multiple_correction_SIMPLE.f90:74-75 (accumulated in face order). */

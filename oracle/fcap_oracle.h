@@ -71,6 +71,24 @@ void orc_face_flux_central(int nif, const int *owner1, const int *neighbour1,
                            const double *arz, const double *u, const double *v,
                            const double *w, double *flux);
 
+/* Pressure-correction assembly of calcp (src/calcp-multiple_correction_SIMPLE.f90:34-107):
+ * a = 0, su = 0, then per inner face facefluxmass (src/facefluxmass.f90:34-199) --
+ * cap = can and the Rhie-Chow mass flux flmass -- scattered as a(ij) = can,
+ * a(ji) = cap, a(diag(P)) -= can, a(diag(N)) -= cap, su(P) -= F, su(N) += F; then
+ * the O-C cuts (al = cap, ar = can, fmoc).  Gradients are 3 x n, component
+ * fastest (dUdxi(3, numCells)); cap_out (nif, may be NULL) receives cap. */
+int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
+                       const double *xc, const double *yc, const double *zc, const double *vol,
+                       const double *xf, const double *yf, const double *zf,
+                       const double *arx, const double *ary, const double *arz, const double *facint,
+                       const double *den, const double *apu, const double *apv, const double *apw,
+                       const double *u, const double *v, const double *w, const double *p,
+                       const double *dudx, const double *dvdx, const double *dwdx, const double *dpdx,
+                       int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                       const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                       double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
+                       double *fmoc);
+
 /* su(o) -= F, su(n) += F in face order (src/calcp-multiple_correction_SIMPLE.f90:74-75). */
 void orc_rhs_divergence(int nif, const int *owner1, const int *neighbour1,
                         const double *flux, double *su);

@@ -51,6 +51,8 @@ def lib():
                                       i, i, ip, ip, ip, ip, i, d, dp]
         _lib.orc_face_flux_central.argtypes = [i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
         _lib.orc_rhs_divergence.argtypes = [i, ip, ip, dp, dp]
+        _lib.orc_calcp_assemble.argtypes = ([i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, i]
+                                            + [dp] * 7)
         _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
         _lib.orc_residual.argtypes = [i, ip, ip, dp, dp, dp, i, ip, ip, dp, dp, dp]
         _lib.orc_dic_pivots.argtypes = [i, ip, ip, ip, dp, dp]
@@ -194,6 +196,41 @@ def face_flux_central(mesh, g: Geometry, u, v, w):
     lib().orc_face_flux_central(nif, _ip(own1), _ip(nei1), _dp(g.facint), _dp(g.arx), _dp(g.ary),
                                 _dp(g.arz), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)), _dp(flux))
     return flux
+
+
+RC_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p")
+RC_GRADS = ("dudx", "dvdx", "dwdx", "dpdx")
+
+
+def calcp_assemble(mesh, g: Geometry, csr: CSR, fields, oc=None):
+    """Pressure-correction matrix, continuity source and Rhie-Chow mass fluxes, as the face
+    loops of calcp (src/calcp-multiple_correction_SIMPLE.f90:34-107) around facefluxmass
+    (src/facefluxmass.f90:34-199) assemble them.  fields: den, apu, apv, apw, u, v, w, p
+    (n each) and dudx, dvdx, dwdx, dpdx (n x 3).  oc: optional dict of O-C cuts (ijl, ijr
+    0-based cells, foc, faces: 0-based index of the first cut face).  Returns a dict with a,
+    su, flmass, cap (and al, ar, fmoc with cuts)."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    fl = [_f64(fields[k]) for k in RC_FIELDS]
+    gr = [_f64(np.asarray(fields[k]).reshape(n, 3)) for k in RC_GRADS]
+    noc = 0 if oc is None else len(oc["ijl"])
+    ijl = _i32(np.asarray(oc["ijl"]) + 1) if noc else np.zeros(1, np.int32)
+    ijr = _i32(np.asarray(oc["ijr"]) + 1) if noc else np.zeros(1, np.int32)
+    foc = _f64(oc["foc"]) if noc else np.zeros(1)
+    oc0 = int(oc["faces"]) if noc else 0
+    out = {"a": np.zeros(mesh.nnz), "su": np.zeros(n), "flmass": np.zeros(max(nif, 1)),
+           "cap": np.zeros(max(nif, 1)), "al": np.zeros(max(noc, 1)), "ar": np.zeros(max(noc, 1)),
+           "fmoc": np.zeros(max(noc, 1))}
+    lib().orc_calcp_assemble(n, nif, _ip(own1), _ip(nei1), _dp(g.xc), _dp(g.yc), _dp(g.zc), _dp(g.vol),
+                             _dp(g.xf), _dp(g.yf), _dp(g.zf), _dp(g.arx), _dp(g.ary), _dp(g.arz),
+                             _dp(g.facint), *[_dp(x) for x in fl], *[_dp(x) for x in gr], noc, oc0,
+                             _ip(ijl), _ip(ijr), _dp(foc), _ip(csr.diag), _ip(csr.ij_slot), _ip(csr.ji_slot),
+                             mesh.nnz, *[_dp(out[k]) for k in ("a", "su", "flmass", "cap", "al", "ar", "fmoc")])
+    out["flmass"], out["cap"] = out["flmass"][:nif], out["cap"][:nif]
+    for k in ("al", "ar", "fmoc"):
+        out[k] = out[k][:noc]
+    return out
 
 
 def rhs_divergence(mesh, flux, su0):

@@ -13,6 +13,23 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 MESH_CASES = ["hex4_uniform", "hex8_perturbed", "hex6x5x4_graded", "tet3_permuted", "hex12_perturbed"]
 CONVDIFF_CASES = ["tet4_convdiff", "hex8_convdiff"]
+CALCP_CASES = ["hex6_calcp", "tet3_calcp"]
+CALCP_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p", "dudx", "dvdx", "dwdx", "dpdx")
+
+
+def calcp_uniform_case(n=4, rho=1.3, vel=(0.7, -0.4, 0.25), grad=(0.3, -1.1, 0.6)):
+    """A box of n^3 unit cubes with constant density and velocity and the linear pressure
+    p = grad . x with its exact gradient: there the Rhie-Chow terms of facefluxmass cancel
+    (dpxi = Dpu g_x |d| n_x and dpex = Dpu g_x for |d| = 1), so the mass flux is rho u . S
+    and cap = -rho (Vol/a_P)_f |S| / |d|."""
+    from freecappuccino_amd import mesh as M
+
+    m = M.hex_box(n)
+    m.points = m.points * n
+    rng = np.random.default_rng(3)
+    fields = {"den": np.full(m.n_cells, rho), "apu": 0.5 + rng.random(m.n_cells),
+              "apv": 0.5 + rng.random(m.n_cells), "apw": 0.5 + rng.random(m.n_cells)}
+    return m, fields, np.array(vel), np.array(grad)
 
 
 def pytest_configure(config):

@@ -8,7 +8,9 @@ Runs in the build container only (it needs /root/reference and ROCm flang):
 For each case it writes the mesh in polyMesh form, runs oracle/_ref/ref_driver
 (the reference's own mesh_geometry, create_CSR_matrix_from_mesh_data,
 laplacian and iccg/dpcg/bicgstab) and stores inputs and outputs as a small
-.npz.  The 5x5 solver transcript fixture is parsed from the reference's
+.npz.  The pressure-correction cases also run oracle/_ref/ref_flux_driver (the
+reference's facefluxmass over every inner face, scattered as calcp does) on
+synthetic density, 1/a_P, velocity and pressure fields and their gradients.  The 5x5 solver transcript fixture is parsed from the reference's
 tests/output.txt and the matrices of tests/test_sparse_solvers.f90:605-655
 (default-REAL literals, widened to float64 as the reference does).
 """
@@ -33,6 +35,7 @@ from freecappuccino_amd import workloads as W  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+FLUX_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_flux_driver")
 REF_TESTS = "/root/reference/tests"
 
 SOLVERS = {"none": 0, "iccg": 1, "dpcg": 2, "bicgstab": 3}
@@ -161,6 +164,62 @@ def convdiff_case(name, mesh, seed, sor=1e-8):
     save_case(name, mesh, inputs, {"bicgstab": out})
 
 
+def run_reference_flux(mesh, fields, fmt="native"):
+    """cap, can, flmass (inner faces) and the calcp face-loop a, su from the reference."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    with tempfile.TemporaryDirectory() as d:
+        M.write_polymesh(mesh, os.path.join(d, "polyMesh"), fmt=fmt)
+        with open(os.path.join(d, "flux_in.bin"), "wb") as f:
+            for k in ("den", "apu", "apv", "apw", "u", "v", "w", "p"):
+                f.write(np.ascontiguousarray(fields[k], np.float64).tobytes())
+            for k in ("dudx", "dvdx", "dwdx", "dpdx"):
+                f.write(np.ascontiguousarray(fields[k], np.float64).reshape(n, 3).tobytes())
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        subprocess.run([FLUX_DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
+        raw = open(os.path.join(d, "flux_out.bin"), "rb").read()
+    hdr = np.frombuffer(raw, np.int32, 3)
+    assert tuple(hdr) == (n, nif, mesh.nnz), (hdr, n, nif, mesh.nnz)
+    vals = np.frombuffer(raw, np.float64, offset=12)
+    assert vals.size == 3 * nif + mesh.nnz + n
+    return {"cap": vals[:nif].copy(), "can": vals[nif:2 * nif].copy(), "flmass": vals[2 * nif:3 * nif].copy(),
+            "a": vals[3 * nif:3 * nif + mesh.nnz].copy(), "su": vals[3 * nif + mesh.nnz:].copy()}
+
+
+def calcp_fields(g, seed):
+    """Density, 1/a_P, velocity and pressure with perturbed gradients, all varying in space."""
+    rng = np.random.default_rng(seed)
+    x, y, z = g.xc, g.yc, g.zc
+    n = x.size
+    tp = 2.0 * np.pi
+    u, v, w = W.taylor_green(x, y, z)
+    p = np.cos(tp * x) * np.sin(tp * y) + 0.5 * z
+    dpdx = np.stack([-tp * np.sin(tp * x) * np.sin(tp * y), tp * np.cos(tp * x) * np.cos(tp * y),
+                     np.full(n, 0.5)], axis=1)
+    f = {"den": 1.0 + 0.2 * np.sin(tp * x + 1.0) * np.cos(tp * z),
+         "apu": (0.5 + rng.random(n)) / g.vol, "apv": (0.5 + rng.random(n)) / g.vol,
+         "apw": (0.5 + rng.random(n)) / g.vol, "u": u, "v": v, "w": w, "p": p,
+         "dpdx": dpdx + 0.1 * rng.standard_normal((n, 3))}
+    for k in ("dudx", "dvdx", "dwdx"):
+        f[k] = rng.standard_normal((n, 3))
+    return f
+
+
+def calcp_case(name, mesh, seed):
+    n = mesh.n_cells
+    base = run_reference(mesh, np.full(n, -1.0), np.zeros(n + mesh.n_boundary), np.zeros(n), np.zeros(n),
+                         solver="none")
+    g = O.Geometry(*(base[k] for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                       "facint")))
+    fields = calcp_fields(g, seed)
+    out = run_reference_flux(mesh, fields)
+    for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint", "ioffset", "ja",
+              "diag", "ij_slot", "ji_slot"):
+        out[k] = base[k]
+    inputs = {k: np.asarray(v) for k, v in fields.items()}
+    inputs["fmt"] = np.array("native")
+    save_case(name, mesh, inputs, {"calcp": out})
+
+
 def transcript_fixture():
     """The 5x5 systems of tests/test_sparse_solvers.f90:605-655 and the expected transcript."""
     f32 = lambda xs: [float(np.float32(x)) for x in xs]  # noqa: E731  default-REAL literals
@@ -197,7 +256,19 @@ def main():
     convdiff_case("tet4_convdiff", M.tet_box(4, grade_axis=1, grade_ratio=1.1, grade_max_aspect=10.0,
                                              permute_seed=8), seed=16)
     convdiff_case("hex8_convdiff", M.hex_box(8, perturb=0.2, seed=21), seed=17)
+    calcp_only()
+
+
+def calcp_only():
+    """Regenerate only the pressure-correction fixtures."""
+    calcp_case("hex6_calcp", M.hex_box(6, 5, 4, perturb=0.2, seed=31, grade_axis=0, grade_ratio=1.2,
+                                       grade_max_aspect=10.0), seed=18)
+    calcp_case("tet3_calcp", M.tet_box(3, grade_axis=2, grade_ratio=1.15, grade_max_aspect=10.0,
+                                       permute_seed=6), seed=19)
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "calcp":
+        calcp_only()
+    else:
+        main()

@@ -9,7 +9,7 @@ a solution within 1e-10 relative FP64 residual of the reference's.
 import numpy as np
 import pytest
 
-from conftest import CONVDIFF_CASES, MESH_CASES, load_transcript
+from conftest import CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
 from freecappuccino_amd import fcap
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
@@ -197,3 +197,73 @@ def test_structure_errors_are_reported(ctx):
     assert e.value.status == -5
     with pytest.raises(fcap.FcError):
         fcap.create_csr_matrix_from_mesh_data(3, np.array([1], np.int32), np.array([7], np.int32), ctx)
+
+
+# ------------------------------------------------------- pressure correction
+
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_calcp_bitwise_vs_reference(ctx, golden_loader, case):
+    """fc_calcp_assemble against the reference's facefluxmass + calcp face loop."""
+    g = golden_loader(case)
+    geo = fcap.Geometry(*(g.out("calcp", k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc",
+                                                      "zc", "facint")))
+    csr = fcap.CSR(*(g.out("calcp", k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    out = fcap.calcp_assemble(g.mesh, geo, csr, {k: g.inp(k) for k in CALCP_FIELDS}, ctx=ctx)
+    np.testing.assert_array_equal(out["flmass"], g.out("calcp", "flmass"))
+    np.testing.assert_array_equal(out["a"], g.out("calcp", "a"))
+    np.testing.assert_array_equal(out["su"], g.out("calcp", "su"))
+
+
+def _mirror_cuts(m, og, seed):
+    """O-C cuts on the boundary faces of the x = min plane, each joining the face's cell to the
+    cell mirrored across the box (periodic-style pairs, S . d != 0)."""
+    nif = m.n_inner
+    nb = m.n_faces - nif
+    xf = og.xf[nif:]
+    faces = np.nonzero(np.abs(xf - xf.min()) < 1e-9)[0]
+    # the cut faces must be consecutive face numbers: take the leading run
+    run = faces[: np.argmax(np.diff(np.r_[faces, -10]) != 1) + 1]
+    assert run.size >= 2 and nb > 0
+    cells = m.owner[nif + run]
+    xmax = og.xc.max() + og.xc.min()
+    ctr = np.stack([og.xc, og.yc, og.zc], axis=1)
+    tgt = np.stack([xmax - og.xc[cells], og.yc[cells], og.zc[cells]], axis=1)
+    partner = np.array([int(np.argmin(((ctr - t) ** 2).sum(axis=1))) for t in tgt])
+    rng = np.random.default_rng(seed)
+    return {"ijl": cells, "ijr": partner, "foc": 0.3 + 0.4 * rng.random(run.size), "faces": nif + int(run[0])}
+
+
+def test_calcp_oc_cuts_match_oracle(ctx):
+    m = M.hex_box(5, perturb=0.2, seed=12)
+    og = O.mesh_geometry(m)
+    oc_csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    rng = np.random.default_rng(4)
+    n = m.n_cells
+    f = {k: 0.5 + rng.random(n) for k in ("den", "apu", "apv", "apw")}
+    f.update({k: rng.standard_normal(n) for k in ("u", "v", "w", "p")})
+    f.update({k: rng.standard_normal((n, 3)) for k in ("dudx", "dvdx", "dwdx", "dpdx")})
+    cuts = _mirror_cuts(m, og, 5)
+    ref = O.calcp_assemble(m, og, oc_csr, f, oc=cuts)
+    assert np.all(np.isfinite(ref["fmoc"])) and ref["fmoc"].size >= 2
+    geo = fcap.Geometry(og.arx, og.ary, og.arz, og.xf, og.yf, og.zf, og.vol, og.xc, og.yc, og.zc, og.facint)
+    csr = fcap.CSR(oc_csr.ioffset, oc_csr.ja, oc_csr.diag, oc_csr.ij_slot, oc_csr.ji_slot)
+    out = fcap.calcp_assemble(m, geo, csr, f, oc=cuts, ctx=ctx)
+    for k in ("a", "su", "flmass", "al", "ar", "fmoc"):
+        np.testing.assert_array_equal(out[k], ref[k], err_msg=k)
+
+
+def test_calcp_uniform_flow_closed_form(ctx):
+    m, f, vel, grad = calcp_uniform_case()
+    geo = fcap.mesh_geometry(m, ctx)
+    csr = fcap.create_csr_matrix_from_mesh_data(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1, ctx)
+    n, nif = m.n_cells, m.n_inner
+    for k, val in zip(("u", "v", "w"), vel):
+        f[k] = np.full(n, val)
+    for k in ("dudx", "dvdx", "dwdx"):
+        f[k] = np.zeros((n, 3))
+    f["p"] = grad[0] * geo.xc + grad[1] * geo.yc + grad[2] * geo.zc
+    f["dpdx"] = np.tile(grad, (n, 1))
+    out = fcap.calcp_assemble(m, geo, csr, f, ctx=ctx)
+    S = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], axis=1)
+    np.testing.assert_allclose(out["flmass"], f["den"][0] * S @ vel, rtol=0, atol=1e-12)
+    assert abs(out["su"].sum()) < 1e-12

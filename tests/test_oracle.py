@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import CONVDIFF_CASES, MESH_CASES, load_transcript
+from conftest import CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
 
@@ -292,3 +292,55 @@ def test_face_flux_and_divergence():
     su1 = O.rhs_divergence(m, O.face_flux_central(m, geo, one, 0 * one, 0 * one), np.zeros(m.n_cells))
     interior = np.setdiff1d(np.arange(m.n_cells), np.unique(m.owner[nif:]))
     assert np.abs(su1[interior]).max() < 1e-15
+
+
+# ------------------------------------------------------- pressure correction
+
+def _calcp_inputs(g):
+    geo = O.Geometry(*(g.out("calcp", k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                                   "facint")))
+    c = O.CSR(*(g.out("calcp", k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    return geo, c, {k: g.inp(k) for k in CALCP_FIELDS}
+
+
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_calcp_bitwise_vs_reference(golden_loader, case):
+    """facefluxmass and calcp's face loop of the reference (oracle/_ref/ref_flux_driver)."""
+    g = golden_loader(case)
+    geo, c, f = _calcp_inputs(g)
+    # the geometry the reference computed is the oracle's too
+    og = _geom(g)
+    for k in ("arx", "xc", "vol", "facint"):
+        np.testing.assert_array_equal(getattr(og, k), getattr(geo, k), err_msg=k)
+    out = O.calcp_assemble(g.mesh, geo, c, f)
+    np.testing.assert_array_equal(out["cap"], g.out("calcp", "cap"))
+    np.testing.assert_array_equal(g.out("calcp", "can"), g.out("calcp", "cap"))
+    np.testing.assert_array_equal(out["flmass"], g.out("calcp", "flmass"))
+    np.testing.assert_array_equal(out["a"], g.out("calcp", "a"))
+    np.testing.assert_array_equal(out["su"], g.out("calcp", "su"))
+
+
+def test_calcp_uniform_flow_closed_form():
+    m, f, vel, grad = calcp_uniform_case()
+    geo = O.mesh_geometry(m)
+    c = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    n, nif = m.n_cells, m.n_inner
+    for k, val in zip(("u", "v", "w"), vel):
+        f[k] = np.full(n, val)
+    for k in ("dudx", "dvdx", "dwdx"):
+        f[k] = np.zeros((n, 3))
+    f["p"] = grad[0] * geo.xc + grad[1] * geo.yc + grad[2] * geo.zc
+    f["dpdx"] = np.tile(grad, (n, 1))
+    out = O.calcp_assemble(m, geo, c, f)
+    S = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], axis=1)
+    rho = f["den"][0]
+    np.testing.assert_allclose(out["flmass"], rho * S @ vel, rtol=0, atol=1e-12)
+    o, q = m.owner[:nif], m.neighbour
+    lam = geo.facint
+    dpu = lam * geo.vol[q] * f["apu"][q] + (1 - lam) * geo.vol[o] * f["apu"][o]
+    np.testing.assert_allclose(out["cap"], -rho * dpu * 1.0, rtol=1e-14)      # |S| = |d| = 1
+    # a row sums to zero (pure Neumann) and su is the divergence of a uniform field:
+    # -(boundary outflow) on boundary cells, zero inside, total = -sum over inner faces of 0
+    A = sp.csr_matrix((out["a"], c.ja - 1, c.ioffset - 1), shape=(n, n))
+    np.testing.assert_allclose(A @ np.ones(n), 0.0, atol=1e-12)
+    assert abs(out["su"].sum()) < 1e-12
