@@ -45,6 +45,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned")
 
+# HBM bytes per launch of the hidden network layer, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
+# 192^3 graded box; summary and commands in profiles/gnn_layer_hex192_graded.md
+LAYER_TRAFFIC = {(5, 192): 1_161_003_008 + 463_671_296}
+
 
 def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
     """Algorithmic bytes of one SpMV: values + column indices + row pointers + x once + y."""
@@ -210,7 +215,8 @@ def main():
         lb = gnn_layer_bytes(n)
         lgbs = lb / (layer_ms * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": round(lgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(lgbs / HBM_PEAK_GBS, 4), "traffic": None,
+                            "frac": round(lgbs / HBM_PEAK_GBS, 4),
+                            "traffic": LAYER_TRAFFIC.get((args.case, args.n)),
                             "kernel": "k_gnn_layer_pipe<false,false,6> (hidden message-passing layer)",
                             "launch_ms": layer_ms, "bytes_per_launch": lb}
     else:
