@@ -196,6 +196,8 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
         if (const char *w = std::getenv("FCAP_FCG_STAG")) ctx->fcg_stag_window = std::max(0, std::atoi(w));
+        if (const char *m = std::getenv("FCAP_FCG_M")) ctx->fcg_m = std::atoi(m);
+        if (const char *e = std::getenv("FCAP_INNER_ETA")) ctx->inner_eta = std::atof(e);
         if (const char *g = std::getenv("FCAP_GNN_XCD")) ctx->gnn_xcd = std::atoi(g) != 0;
         if (const char *o = std::getenv("FCAP_GNN_ORDER")) ctx->gnn_order = std::atoi(o) != 0;
         if (const char *pp = std::getenv("FCAP_GNN_PIPE")) ctx->gnn_pipe = std::max(-1, std::atoi(pp));
@@ -249,6 +251,18 @@ extern "C" fc_status fc_set_report(fc_context *ctx, int on, const char *name) {
 extern "C" fc_status fc_set_variant(fc_context *ctx, int variant) {
     if (!ctx || (variant != FC_VARIANT_SRC && variant != FC_VARIANT_TESTS)) return FC_ERR_ARGUMENT;
     ctx->variant = variant;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_inner_tolerance(fc_context *ctx, double eta) {
+    if (!ctx || !(eta >= 0.0 && eta < 1.0)) return FC_ERR_ARGUMENT;
+    ctx->inner_eta = eta;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_fcg_window(fc_context *ctx, int m) {
+    if (!ctx || m < 1 || m > kFcgM) return FC_ERR_ARGUMENT;
+    ctx->fcg_m = m;
     return FC_OK;
 }
 
@@ -830,6 +844,10 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         args.nsw = nsw;
         args.variant = FC_VARIANT_SRC;
         args.stop_rule = stop_rule;
+        // inner tolerance of the FP32 solves: the context's setting, else 1e-2 for flexible CG
+        // (restarts help the nonlinear preconditioner) and SolveArgs' 1e-3 for the others
+        if (ctx->inner_eta > 0.0 && ctx->inner_eta < 1.0) args.inner_eta = ctx->inner_eta;
+        else if (method == FC_METHOD_FCG_LEARNED) args.inner_eta = kFcgInnerEta;
         fc_solve_info local;
         std::memset(&local, 0, sizeof local);
         fc_status s;

@@ -147,6 +147,11 @@ struct fc_context {
     int gnn_unroll = 1;
     // flexible CG stagnation window in iterations (FCAP_FCG_STAG; 0: no stagnation check)
     int fcg_stag_window = 64;
+    // flexible CG directions kept, rounded up to 4, 8, 12 or 16 (FCAP_FCG_M, fc_set_fcg_window)
+    int fcg_m = 12;
+    // inner tolerance of the FP32 solves in FP64 refinement (FCAP_INNER_ETA,
+    // fc_set_inner_tolerance; 0: the method's default)
+    double inner_eta = 0.0;
     // network layer kernel: XCD-contiguous cell ranges (FCAP_GNN_XCD=0: natural block order)
     int gnn_xcd = 1;
     // network layer kernel: 64-cell blocks in Morton order of their centroids

@@ -21,7 +21,8 @@ constexpr int kSpmvRows = kWave * kSpmvWaves;    // rows per SpMV block
 constexpr int kSpmvCap = 512;                    // LDS entries staged per wave (8 per row on average)
 constexpr int kRedBlock = 256;                   // block size of the reduction kernels
 constexpr int kMaxRedGrid = 2048;                // grid cap of grid-stride reductions
-constexpr int kFcgM = 8;                         // directions kept by truncated flexible CG
+constexpr int kFcgM = 16;                        // most directions truncated flexible CG can keep
+constexpr double kFcgInnerEta = 1e-2;            // default inner tolerance of flexible CG
 constexpr int kXcds = 8;                         // XCDs of an MI355X (workgroups are dealt round-robin)
 constexpr int kFinBlock = 1024;                  // block size of the finalize kernels
 
@@ -38,6 +39,7 @@ struct Scalars {
     double fcg_pqh[kFcgM], fcg_bet[kFcgM];     // flexible CG: p.q and beta per history slot
     double stag_ref, stag_min;                 // flexible CG: best ||r||^2 before / within the window
     int stag_every;                            // flexible CG: stagnation window (0: no check)
+    int fcg_m;                                 // flexible CG: directions kept (1..kFcgM)
     int iters, done, status, nsw;
     int stop_rule;     // 0: L1 rsm < resmax; 1: L2 rel < resmax; 2: rnorm2 <= inner_tol2
     int early_exit;    // 1: stop before iterating when res0 < 1e-13 (src/iccg.f90:67-71)

@@ -14,7 +14,7 @@ struct Workspace {
     DevBuf r32, z32, p32, q32, e32, a32, d32;
     // reduction partials, summed scalars, device solver state, error flag
     DevBuf pa, pb, sums, sc, err, bar;
-    // flexible CG: the last kFcgM directions p and their images A p (FP32, kFcgM x n each)
+    // flexible CG: the last m directions p and their images A p (FP32, m x n each)
     DevBuf ph, qh;
     // the current solve's factor in level-packed order (PackedFactor), FP64 and FP32
     DevBuf pk_af, pk_df, pk_ab, pk_db, pk_af32, pk_df32, pk_ab32, pk_db32;

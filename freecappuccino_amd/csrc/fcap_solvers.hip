@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
     if (sc->done) return;
     if (step == kFinFcgBeta) {
         // partials of slot s at pa[s * nparts + block]
-        const int nwin = min(sc->iters, kFcgM);
+        const int nwin = min(sc->iters, sc->fcg_m);
         for (int s = 0; s < nwin; ++s) {
             double v = 0.0;
             for (int i = threadIdx.x; i < nparts; i += kFinBlock) v += pa[(size_t)s * nparts + i];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
     case kFinFcgAlf:
         sc->pkapk = a;
         sc->alf = sc->fcg_pr / a;
-        sc->fcg_pqh[sc->iters % kFcgM] = a;
+        sc->fcg_pqh[sc->iters % sc->fcg_m] = a;
         break;
     case kFinFcgEnd:
         sc->resl = a;
@@ -163,29 +163,39 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
     }
 }
 
-// Flexible CG with the last kFcgM directions.  Direction of iteration t lives
-// in slot t % kFcgM of ph (and A p in qh); at iteration k (= sc->iters) the
-// slots s < min(k, kFcgM) hold the directions of iterations k-1, ..., k-min(k, kFcgM).
+// Flexible CG with the last m = sc->fcg_m <= kFcgM directions.  Direction of
+// iteration t lives in slot t % m of ph (and A p in qh); at iteration k
+// (= sc->iters) the slots s < min(k, m) hold the directions of iterations
+// k-1, ..., k-min(k, m).
 
-// partials of z.q_s for every stored slot s, at pa[s * gridDim.x + block]
+// partials of z.q_s for every stored slot s, at pa[s * gridDim.x + block]; M = sc->fcg_m
+template <int M>
 __global__ __launch_bounds__(kRedBlock) void k_fcg_dots(int n, const float *__restrict__ z,
                                                         const float *__restrict__ qh, const Scalars *__restrict__ sc,
                                                         double *__restrict__ pa) {
     __shared__ double sh[kRedBlock / kWave];
     if (sc->done) return;
-    const int nwin = min(sc->iters, kFcgM);
+    const int nwin = min(sc->iters, M);
     if (nwin == 0) return;
-    double acc[kFcgM];
+    double acc[M];
 #pragma unroll
-    for (int s = 0; s < kFcgM; ++s) acc[s] = 0.0;
-    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-        const double zi = (double)z[i];
+    for (int s = 0; s < M; ++s) acc[s] = 0.0;
+    if (nwin == M) {
+        for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+            const double zi = (double)z[i];
 #pragma unroll
-        for (int s = 0; s < kFcgM; ++s)
-            if (s < nwin) acc[s] += zi * (double)qh[(size_t)s * n + i];
+            for (int s = 0; s < M; ++s) acc[s] += zi * (double)qh[(size_t)s * n + i];
+        }
+    } else {
+        for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+            const double zi = (double)z[i];
+#pragma unroll
+            for (int s = 0; s < M; ++s)
+                if (s < nwin) acc[s] += zi * (double)qh[(size_t)s * n + i];
+        }
     }
 #pragma unroll
-    for (int s = 0; s < kFcgM; ++s) {
+    for (int s = 0; s < M; ++s) {
         if (s < nwin) {
             const double v = block_sum<kRedBlock>(acc[s], sh);
             if (threadIdx.x == 0) pa[(size_t)s * gridDim.x + blockIdx.x] = v;
@@ -193,24 +203,35 @@ __global__ __launch_bounds__(kRedBlock) void k_fcg_dots(int n, const float *__re
     }
 }
 
-// p = z + sum_s bet_s p_s and the partial of p.r
+// p = z + sum_s bet_s p_s and the partial of p.r; M = sc->fcg_m
+template <int M>
 __global__ __launch_bounds__(kRedBlock) void k_fcg_p(int n, float *__restrict__ p, const float *__restrict__ z,
                                                      const float *__restrict__ r, const float *__restrict__ ph,
                                                      const Scalars *__restrict__ sc, double *__restrict__ pa) {
     __shared__ double sh[kRedBlock / kWave];
     if (sc->done) return;
-    const int nwin = min(sc->iters, kFcgM);
-    float bet[kFcgM];
+    const int nwin = min(sc->iters, M);
+    float bet[M];
 #pragma unroll
-    for (int s = 0; s < kFcgM; ++s) bet[s] = s < nwin ? (float)sc->fcg_bet[s] : 0.f;
+    for (int s = 0; s < M; ++s) bet[s] = s < nwin ? (float)sc->fcg_bet[s] : 0.f;
     double acc = 0.0;
-    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-        float pv = z[i];
+    if (nwin == M) {
+        for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+            float pv = z[i];
 #pragma unroll
-        for (int s = 0; s < kFcgM; ++s)
-            if (s < nwin) pv = pv + bet[s] * ph[(size_t)s * n + i];
-        p[i] = pv;
-        acc += (double)pv * (double)r[i];
+            for (int s = 0; s < M; ++s) pv = pv + bet[s] * ph[(size_t)s * n + i];
+            p[i] = pv;
+            acc += (double)pv * (double)r[i];
+        }
+    } else {
+        for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+            float pv = z[i];
+#pragma unroll
+            for (int s = 0; s < M; ++s)
+                if (s < nwin) pv = pv + bet[s] * ph[(size_t)s * n + i];
+            p[i] = pv;
+            acc += (double)pv * (double)r[i];
+        }
     }
     acc = block_sum<kRedBlock>(acc, sh);
     if (threadIdx.x == 0) pa[blockIdx.x] = acc;
@@ -225,7 +246,7 @@ __global__ __launch_bounds__(kRedBlock) void k_fcg_xr(int n, float *__restrict__
     __shared__ double sh[kRedBlock / kWave];
     if (sc->done) return;
     const float alf = (float)sc->alf;
-    const size_t slot = (size_t)(sc->iters % kFcgM) * n;
+    const size_t slot = (size_t)(sc->iters % sc->fcg_m) * n;
     double a = 0.0, b = 0.0;
     for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
         const float pv = p[i], qv = q[i];
@@ -828,6 +849,14 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     return status;
 }
 
+// Directions kept by flexible CG: the context's setting rounded up to a window
+// the kernels are compiled for (4, 8, 12 or 16).  The ring buffers hold exactly
+// this many slots, and every kernel and finaliser uses the same count.
+static int fcg_window(const fc_context *ctx) {
+    const int m = ctx->fcg_m;
+    return m <= 4 ? 4 : m <= 8 ? 8 : m <= 12 ? 12 : 16;
+}
+
 // Truncated FCG in FP32 on A e = r32 (unit-norm r32), preconditioner P.
 // Returns the inner status (FC_OK, FC_NOT_CONVERGED, FC_FALLBACK_USED on
 // stagnation, FC_ERR_NONFINITE) and adds the iterations to `iters`.
@@ -849,14 +878,32 @@ static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double
     h.stag_ref = 1e300;    // set by the first window
     h.stag_min = 1e300;
     h.stag_every = c.ctx->fcg_stag_window;
+    h.fcg_m = fcg_window(c.ctx);
     h.status = kStatusRunning;
     upload_scalars(c, h);
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
+    // the window is compiled in (h.fcg_m is one of 4, 8, 12, 16: fcg_window)
+    auto dots = [&]() {
+        switch (h.fcg_m) {
+        case 4: k_fcg_dots<4><<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa); break;
+        case 8: k_fcg_dots<8><<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa); break;
+        case 12: k_fcg_dots<12><<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa); break;
+        default: k_fcg_dots<16><<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa); break;
+        }
+    };
+    auto pupd = [&]() {
+        switch (h.fcg_m) {
+        case 4: k_fcg_p<4><<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa); break;
+        case 8: k_fcg_p<8><<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa); break;
+        case 12: k_fcg_p<12><<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa); break;
+        default: k_fcg_p<16><<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa); break;
+        }
+    };
     auto iter = [&]() {
         P.apply(P.obj, c.s, r, z, c.sc);
-        k_fcg_dots<<<c.gv, kRedBlock, 0, c.s>>>(n, z, qh, c.sc, c.pa);
+        dots();
         fin(c, kFinFcgBeta, c.gv, false);
-        k_fcg_p<<<c.gv, kRedBlock, 0, c.s>>>(n, p, z, r, ph, c.sc, c.pa);
+        pupd();
         fin(c, kFinFcgPr, c.gv, false);
         spmv_launch32(c.s, 2, n, ptr, ja, a32, p, nullptr, p, q, c.pa, nullptr, c.sc);
         fin(c, kFinFcgAlf, gs, false);
@@ -876,8 +923,9 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
     Ctx c = make_ctx(ctx, st.n);
     const int n = st.n;
     double *r = ws.r.as<double>(), *xgood = ws.tmp.as<double>();
-    ws.ph.ensure(sizeof(float) * (size_t)kFcgM * (size_t)n);
-    ws.qh.ensure(sizeof(float) * (size_t)kFcgM * (size_t)n);
+    const int m = fcg_window(ctx);
+    ws.ph.ensure(sizeof(float) * (size_t)m * (size_t)n);
+    ws.qh.ensure(sizeof(float) * (size_t)m * (size_t)n);
     ctx->timer.start(c.s);
     k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
     double setup_ms = ctx->timer.stop_ms(c.s);

@@ -72,6 +72,7 @@ def library() -> C.CDLL:
         "fc_init": ([C.POINTER(vp), i], i), "fc_finalize": ([vp], i), "fc_set_stream": ([vp, vp], i),
         "fc_last_error": ([vp], C.c_char_p), "fc_set_report": ([vp, i, C.c_char_p], i),
         "fc_set_variant": ([vp, i], i),
+        "fc_set_inner_tolerance": ([vp, d], i), "fc_set_fcg_window": ([vp, i], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
@@ -180,6 +181,14 @@ class Context:
 
     def set_variant(self, variant):
         self.check(self.L.fc_set_variant(self.h, variant))
+
+    def set_inner_tolerance(self, eta: float):
+        """Inner FP32 tolerance of the refinement solvers (0: the method's default)."""
+        self.check(self.L.fc_set_inner_tolerance(self.h, float(eta)))
+
+    def set_fcg_window(self, m: int):
+        """Directions kept by flexible CG (1..16, rounded up to a multiple of 4)."""
+        self.check(self.L.fc_set_fcg_window(self.h, int(m)))
 
     def set_report(self, on: bool, name: str = "p"):
         self.check(self.L.fc_set_report(self.h, 1 if on else 0, name.encode()))

@@ -204,6 +204,14 @@ fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *ioffset, const
 enum { FC_VARIANT_SRC = 0, FC_VARIANT_TESTS = 1 };
 fc_status fc_set_variant(fc_context *ctx, int variant);
 
+/* Tuning of the FP32-in-FP64 solvers; no counterpart in the reference.
+ * fc_set_inner_tolerance: relative residual reduction of each inner FP32 solve
+ * (0 = the method's default: 1e-2 for FC_METHOD_FCG_LEARNED, 1e-3 otherwise).
+ * fc_set_fcg_window: directions kept by flexible CG, 1..16, rounded up to a
+ * multiple of 4 (default 12). */
+fc_status fc_set_inner_tolerance(fc_context *ctx, double eta);
+fc_status fc_set_fcg_window(fc_context *ctx, int m);
+
 /* ------------------------------------------------ device-resident system */
 
 enum {
