@@ -48,7 +48,7 @@ METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned")
 # HBM bytes per launch of the hidden network layer, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
 # 192^3 graded box; summary and commands in profiles/gnn_layer_hex192_graded.md
-LAYER_TRAFFIC = {(5, 192): 1_161_003_008 + 463_671_296}
+LAYER_TRAFFIC = {(5, 192): 1_171_501_056 + 452_984_832}
 
 
 def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
@@ -217,7 +217,7 @@ def main():
         line["roofline"] = {"bound": "hbm", "achieved": round(lgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(lgbs / HBM_PEAK_GBS, 4),
                             "traffic": LAYER_TRAFFIC.get((args.case, args.n)),
-                            "kernel": "k_gnn_layer_pipe<false,false,6> (hidden message-passing layer)",
+                            "kernel": "k_gnn_layer_quad<false,false,6> (hidden message-passing layer)",
                             "launch_ms": layer_ms, "bytes_per_launch": lb}
     else:
         line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

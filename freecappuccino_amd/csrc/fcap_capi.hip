@@ -201,6 +201,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *g = std::getenv("FCAP_GNN_XCD")) ctx->gnn_xcd = std::atoi(g) != 0;
         if (const char *o = std::getenv("FCAP_GNN_ORDER")) ctx->gnn_order = std::atoi(o) != 0;
         if (const char *pp = std::getenv("FCAP_GNN_PIPE")) ctx->gnn_pipe = std::max(-1, std::atoi(pp));
+        if (const char *qd = std::getenv("FCAP_GNN_QUAD")) ctx->gnn_quad = std::atoi(qd) != 0;
         return FC_OK;
     });
     if (s != FC_OK) {

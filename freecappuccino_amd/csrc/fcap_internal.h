@@ -160,6 +160,8 @@ struct fc_context {
     // network layer kernel: persistent software-pipelined waves, this many per XCD
     // (FCAP_GNN_PIPE; -1: as many as are resident; 0: one 64-cell block per workgroup)
     int gnn_pipe = -1;
+    // persistent layer kernel gathers rows with quads of lanes (FCAP_GNN_QUAD=0: one lane per cell)
+    int gnn_quad = 1;
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -13,7 +13,7 @@ matrix, for a residual r:
     u_i   = (r_i / a_ii) / s
     h0_i  = relu(w_enc * u_i + b_enc)
     for l in 0..L-1:
-        c_k      = g0 + g1 * e1_k + g2 * e2_k         (k: off-diagonal entry (i, j) of row i)
+        c_k      = bf16(g0 + g1 * e1_k + g2 * e2_k)   (k: off-diagonal entry (i, j) of row i)
         agg_i    = sum_k c_k * h_j
         out_i    = Ws h_i + Wm agg_i + b
         h_i     <- h_i + relu(out_i)
@@ -21,9 +21,9 @@ matrix, for a residual r:
     z_i   = s * (u_i + y_i)
 Edge features of entry k in row i (face f between cells i and j):
     e1_k = (|S_f| / |x_j - x_i|) / a_ii,   e2_k = 1 - |S_f . d| / (|S_f| |d|)
-BF16 points (as on the GPU): weights, stored h, and agg before the GEMM are
-rounded to bfloat16 (round to nearest even); the rest is float64 here
-(FP32 on the GPU).  Rounding is identity in the backward pass.
+BF16 points (as on the GPU): weights, the edge gates c_k, stored h, and agg
+before the GEMM are rounded to bfloat16 (round to nearest even); the rest is
+float64 here (FP32 on the GPU).  Rounding is identity in the backward pass.
 
 Loss of a (residual, error) pair: L = (z - e)^T A (z - e) / (e^T A e).
 
@@ -93,7 +93,7 @@ def forward(theta, graph: Graph, r, bf=True, keep=False):
     h = rnd(np.maximum(pre0, 0.0))
     cache = {"u": u, "s": s, "pre0": pre0, "h": [h], "agg": [], "out": [], "c": []}
     for lp in p["layers"]:
-        c = lp["g"][0] + lp["g"][1] * graph.e1 + lp["g"][2] * graph.e2
+        c = rnd(lp["g"][0] + lp["g"][1] * graph.e1 + lp["g"][2] * graph.e2)
         agg = np.zeros_like(h)
         np.add.at(agg, graph.row, c[:, None] * h[graph.col])
         agg = rnd(agg)
