@@ -344,6 +344,26 @@ fc_status fc_polymesh_arrays(const fc_polymesh *mesh, double *xyz, int *face_ptr
 fc_status fc_polymesh_patch(const fc_polymesh *mesh, int index, char *bctype, int len, int *nFaces, int *startFace);
 void fc_polymesh_free(fc_polymesh *mesh);
 
+/* ------------------------------------------------------------- restart */
+
+/* Restart records of src/write_restart_files.f90:22-47 (read back by
+ * src/readfiles.f90:19-43): one unformatted sequential record per Fortran
+ * write, each framed by its 4-byte byte count.  Records: (itime, time),
+ * gradpcmf when has_gradpcmf (constant mass flow), then nrec float64 arrays of
+ * len[k] values -- the reference's order is flmass (numInnerFaces); u, v, w, p,
+ * te, ed, t, vis (numTotal); uu, vv, ww, uv, uw, vw (numCells); uo, vo, wo,
+ * teo, edo (numTotal).  Host only; errors go to err (errlen bytes). */
+fc_status fc_restart_write(const char *path, int itime, double time, int has_gradpcmf, double gradpcmf, int nrec,
+                           const long long *len, const double *const *rec, char *err, int errlen);
+/* itime, time, gradpcmf and the number and lengths of the field records
+ * (len has room for max_rec; *nrec is the count in the file). */
+fc_status fc_restart_probe(const char *path, int has_gradpcmf, int *itime, double *time, double *gradpcmf,
+                           int max_rec, int *nrec, long long *len, char *err, int errlen);
+/* The first nrec field records into rec[k] (len[k] values each; a null rec[k]
+ * skips that record); a length that differs from the file is an error. */
+fc_status fc_restart_read(const char *path, int has_gradpcmf, int nrec, const long long *len, double *const *rec,
+                          char *err, int errlen);
+
 /* No counterpart in the reference: a plain device-to-device copy kernel of
  * `bytes` bytes, `reps` times (calibration of the HBM traffic counters and the
  * achievable-bandwidth figure); returns the average milliseconds per copy. */

@@ -173,6 +173,34 @@ def test_solver_transcript_5x5(ctx):
     assert [f"{v:5.2f}" for v in r1.fi] == [f"{v:5.2f}" for v in t["xa1"]]
 
 
+@pytest.mark.parametrize("solver", ["iccg", "dpcg", "bicgstab"])
+def test_report_line_matches_reference(ctx, golden_loader, capfd, solver):
+    """The one-line report (src/iccg.f90:174-175 and its dpcg/bicgstab twins): same text as the
+    reference's line, character for character, up to the iteration count (+-1) and the last digit
+    of the final residual."""
+    import re
+
+    g = golden_loader("hex8_perturbed")
+    ref = str(g.out(solver, "report_line"))
+    a, su = g.out(solver, "a"), g.out(solver, "su")
+    ctx.set_report(True, "p")
+    try:
+        capfd.readouterr()
+        getattr(fcap, solver)(_csr_of(g), a, su, g.inp("fi0"), sor=float(g.inp("sor")[0]),
+                              nsw=int(g.inp("nsw")[0]), ctx=ctx)
+        out = capfd.readouterr().out
+    finally:
+        ctx.set_report(False, "p")
+    line = [ln for ln in out.splitlines() if "Solving for" in ln][-1]
+    pat = r"^(.*Solving for p, Initial residual = )(.{10}), Final residual = (.{10}), No Iterations (\d+)$"
+    mg, mr = re.match(pat, line), re.match(pat, ref)
+    assert mg and mr, (line, ref)
+    assert mg.group(1) == mr.group(1)                     # solver tag and spacing
+    assert mg.group(2) == mr.group(2)                     # initial residual, %10.3E
+    assert abs(int(mg.group(4)) - int(mr.group(4))) <= 1
+    assert mg.group(3)[-4:] == mr.group(3)[-4:] or abs(float(mg.group(3)) / float(mr.group(3)) - 1) < 0.5
+
+
 def test_early_exit_and_status_codes(ctx, golden_loader):
     g = golden_loader("hex4_uniform")
     csr = _csr_of(g)
