@@ -337,6 +337,26 @@ __global__ void k_calcp_rows(int n, const int *__restrict__ ptr, const int *__re
     su[i] = s;
 }
 
+// One velocity component's rows of calcuvw (src/calcuvw.f90:417-437): the
+// diagonal slot counts as zero in the row sum (the reference resets it), then
+// a(diag) = (sp - (sum(row) - 0)) * urfr, su = src + (urfm a(diag)) phi,
+// ap = 1 / (a(diag) + small), in the reference's order.
+__global__ void k_momentum_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                double *__restrict__ a, const double *__restrict__ sp, const double *__restrict__ src,
+                                const double *__restrict__ phi, double urfr, double urfm, double small,
+                                double *__restrict__ su, double *__restrict__ ap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dp = diagpos[i];
+    double s = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s = s + (k == dp ? 0.0 : a[k]);
+    const double off = s - 0.0;
+    const double d = (sp[i] - off) * urfr;
+    a[dp] = d;
+    su[i] = src[i] + (urfm * d) * phi[i];
+    ap[i] = 1.0 / (d + small);
+}
+
 __global__ void k_kface(int nif, const int *__restrict__ ij, const int *__restrict__ ji, int *__restrict__ kface) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nif) return;
@@ -581,6 +601,14 @@ void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
         st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface, ij_slot, flmass,
         oc.noc ? oc.co_ptr.as<int>() : nullptr, oc.noc ? oc.co_idx.as<int>() : nullptr,
         oc.noc ? oc.al.as<double>() : nullptr, oc.noc ? oc.ar.as<double>() : nullptr, oc.noc ? fmoc : nullptr, a, su);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,
+                       const double *phi, double urf, double *su, double *ap) {
+    const double urfr = 1.0 / urf, urfm = 1.0 - urf;
+    k_momentum_rows<<<grid_for(st.n), kBlock, 0, ctx->stream>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(), a, sp,
+                                                                src, phi, urfr, urfm, kSmallSrc, su, ap);
     FC_HIP_CHECK(hipGetLastError());
 }
 

@@ -806,6 +806,50 @@ extern "C" fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis,
     });
 }
 
+extern "C" fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *sp, const double *const *src,
+                                           const double *urf, double *const *phi, double *const *ap, int method,
+                                           double sor, int nsw, fc_solve_info *info) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(sp && src && urf && phi && nsw > 0 && sor > 0.0, FC_ERR_ARGUMENT, "arguments");
+        FC_REQUIRE(method == FC_METHOD_BICGSTAB || method == FC_METHOD_BICGSTAB_MIXED, FC_ERR_ARGUMENT,
+                   "method must be FC_METHOD_BICGSTAB or FC_METHOD_BICGSTAB_MIXED");
+        const int n = sys->st.n;
+        DevBuf dsp, dsrc, dap;
+        dap.alloc(sizeof(double) * n);
+        for (int c = 0; c < 3; ++c) {
+            FC_REQUIRE(sp[c] && src[c] && phi[c] && urf[c] > 0.0, FC_ERR_ARGUMENT, "component arrays / urf");
+            upload_dbl(ctx, dsp, sp[c], n);
+            upload_dbl(ctx, dsrc, src[c], n);
+            h2d(sys->fi.get(), phi[c], sizeof(double) * n, ctx->stream);
+            momentum_rows_dev(ctx, sys->st, sys->a.as<double>(), dsp.as<double>(), dsrc.as<double>(),
+                              sys->fi.as<double>(), urf[c], sys->su.as<double>(), dap.as<double>());
+            sys->assembly_version += 1;
+            SolveArgs args;
+            args.st = &sys->st;
+            args.a = sys->a.as<double>();
+            args.su = sys->su.as<double>();
+            args.fi = sys->fi.as<double>();
+            args.sor = sor;
+            args.nsw = nsw;
+            args.variant = FC_VARIANT_SRC;
+            args.stop_rule = FC_STOP_L1;
+            if (ctx->inner_eta > 0.0 && ctx->inner_eta < 1.0) args.inner_eta = ctx->inner_eta;
+            fc_solve_info local;
+            std::memset(&local, 0, sizeof local);
+            const fc_status s = method == FC_METHOD_BICGSTAB ? solve_bicgstab64(ctx, args, &local)
+                                                             : solve_mixed(ctx, args, true, &local);
+            local.rel_l2 = true_rel_l2(ctx, args);
+            if (info) info[c] = local;
+            d2h(phi[c], sys->fi.get(), sizeof(double) * n, ctx->stream);
+            if (ap && ap[c]) d2h(ap[c], dap.get(), sizeof(double) * n, ctx->stream);
+            if (s != FC_OK && s != FC_EARLY_EXIT) return s;
+        }
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_sys_set_rhs(fc_system *sys, const double *su) {
     if (!sys || !su) return FC_ERR_ARGUMENT;
     return guarded(sys->ctx, [&]() {

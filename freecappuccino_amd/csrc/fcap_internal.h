@@ -219,6 +219,9 @@ struct RhieChowDev {
     const double *xc, *yc, *zc, *vol, *den, *apu, *apv, *apw, *u, *v, *w, *p;
     const double *dudx, *dvdx, *dwdx, *dpdx;
 };
+// one velocity component's diagonal, source and 1/a_P (calcuvw), a's diagonal slots updated
+void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,
+                       const double *phi, double urf, double *su, double *ap);
 // calcp's face loops (a and su overwritten, flmass of the inner faces, O-C al/ar/fmoc)
 void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
                const int *ij_slot, const int *ji_slot, const int *kface, const int *kface_order, const double *xf,

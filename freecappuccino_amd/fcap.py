@@ -92,6 +92,8 @@ def library() -> C.CDLL:
         "fc_sys_assemble_laplacian_dev": ([vp], i),
         "fc_sys_assemble_convdiff": ([vp, dp, dp, d, dp], i),
         "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
+        "fc_sys_solve_momentum": ([vp, C.POINTER(dp), C.POINTER(dp), dp, C.POINTER(dp), C.POINTER(dp), i, d, i,
+                                   si], i),
         "fc_sys_get_solution": ([vp, dp], i),
         "fc_sys_solve": ([vp, i, i, d, i, si], i),
         "fc_sys_get": ([vp, i, vp, C.c_size_t], i),
@@ -510,6 +512,23 @@ class System:
 
     def set_rhs(self, su):
         self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))
+
+    def solve_momentum(self, sp, src, urf, phi, method=METHOD_BICGSTAB, sor=1e-8, nsw=1000):
+        """calcuvw's three component solves on the last convection-diffusion assembly
+        (src/calcuvw.f90:414-556).  sp, src, phi: three arrays each (u, v, w); urf: three
+        factors.  Returns (status, [phi_u, phi_v, phi_w], [apu, apv, apw], [info x 3])."""
+        n = self.n
+        sp_ = [_f64(x) for x in sp]
+        src_ = [_f64(x) for x in src]
+        phi_ = [_f64(x).copy() for x in phi]
+        ap_ = [np.zeros(n) for _ in range(3)]
+        urf_ = _f64(urf)
+        arr = lambda xs: (C.POINTER(C.c_double) * 3)(*[_dp(x) for x in xs])  # noqa: E731
+        infos = (SolveInfo * 3)()
+        rc = self.ctx.L.fc_sys_solve_momentum(self.h, arr(sp_), arr(src_), _dp(urf_), arr(phi_), arr(ap_),
+                                              int(method), float(sor), int(nsw), infos)
+        self.ctx.check(rc, allow=(FC_OK, FC_NOT_CONVERGED, FC_EARLY_EXIT))
+        return rc, phi_, ap_, [x.as_dict() for x in infos]
 
     def set_solution(self, fi):
         self.ctx.check(self.ctx.L.fc_sys_set_solution(self.h, _dp(_f64(fi))))

@@ -246,6 +246,19 @@ fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis, const doub
                                    double urf, const double *su0);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
+/* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal
+ * coefficients of the last fc_sys_assemble_convdiff are shared by the three
+ * components; for c = u, v, w in turn the row kernel rebuilds a(diag) =
+ * (sp[c] - sum of the row's off-diagonals) / urf[c], su = src[c] + (1 -
+ * urf[c]) a(diag) phi[c] and ap[c] = 1/(a(diag) + small) (apu/apv/apw of the
+ * reference), then phi[c] (numCells, in: initial guess, out: solution) is
+ * solved with BiCGStab-DILU (method FC_METHOD_BICGSTAB, the reference's L1
+ * rule with sor/nsw) or its FP32-in-FP64 form (FC_METHOD_BICGSTAB_MIXED).
+ * ap may be NULL; info (3 entries) may be NULL.  Stops at the first
+ * component that does not converge and returns its status. */
+fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *sp, const double *const *src, const double *urf,
+                                double *const *phi, double *const *ap, int method, double sor, int nsw,
+                                fc_solve_info *info);
 /* Upload / download the solution vector (numCells). */
 fc_status fc_sys_set_solution(fc_system *sys, const double *fi);
 fc_status fc_sys_get_solution(fc_system *sys, double *fi);

@@ -22,6 +22,9 @@
  *                         evaluation and the divergence-theorem invariant
  *                         (sum of su over cells equals the net boundary flux,
  *                         zero for inner-face-only fluxes).
+ *   orc_momentum_rows  -- closed forms: with urf = 1 every row sums to sp and
+ *                         su = s; for any urf, urf a_PP = sp - (sum of the row's
+ *                         off-diagonals) and su - s = (1 - urf) a_PP phi.
  *   orc_calcp_assemble -- bitwise vs the reference's facefluxmass and calcp
  *                         face loop (oracle/_ref/ref_flux_driver) on perturbed,
  *                         graded and permuted tet meshes; a uniform-flow case
@@ -790,4 +793,32 @@ int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
         I1(su, ijn) = I1(su, ijn) + fmoc[i];
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Momentum component rows of calcuvw (src/calcuvw.f90:417-437, and the v/w
+ * twins :468-489, :522-543): the diagonal slot is reset to zero, then
+ *   a(diag) = (sp - (sum(a(row)) - a(diag))) * urfr
+ *   su      = s + urfm * a(diag) * phi
+ *   ap      = 1 / (a(diag) + small)
+ * with urfr = 1/urf and urfm = 1 - urf (src/init.f90:80-81).               */
+/* ------------------------------------------------------------------------ */
+
+void orc_momentum_rows(int n, const int *ioffset, const int *diag, double *a, const double *sp, const double *src,
+                       const double *phi, double urf, double *su, double *ap)
+{
+    double urfr = 1.0 / urf;
+    double urfm = 1.0 - urf;
+    double small = orc_small();
+    for (int i = 1; i <= n; ++i) {
+        int dp = I1(diag, i);
+        I1(a, dp) = 0.0;
+        double s = 0.0;
+        for (int k = I1(ioffset, i); k < I1(ioffset, i + 1); ++k) s = s + I1(a, k);
+        double off = s - I1(a, dp);
+        I1(a, dp) = I1(sp, i) - off;
+        I1(a, dp) = I1(a, dp) * urfr;
+        I1(su, i) = I1(src, i) + urfm * I1(a, dp) * I1(phi, i);
+        I1(ap, i) = 1.0 / (I1(a, dp) + small);
+    }
 }

@@ -89,6 +89,12 @@ int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
                        double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
                        double *fmoc);
 
+/* Momentum component rows of calcuvw (src/calcuvw.f90:417-437): diagonal reset to zero, then
+ * a(diag) = (sp - sum of the row's off-diagonals) / urf, su = src + (1 - urf) a(diag) phi,
+ * ap = 1/(a(diag) + small).  a is updated in place (its diagonal slots). */
+void orc_momentum_rows(int n, const int *ioffset, const int *diag, double *a, const double *sp, const double *src,
+                       const double *phi, double urf, double *su, double *ap);
+
 /* su(o) -= F, su(n) += F in face order (src/calcp-multiple_correction_SIMPLE.f90:74-75). */
 void orc_rhs_divergence(int nif, const int *owner1, const int *neighbour1,
                         const double *flux, double *su);

@@ -51,6 +51,7 @@ def lib():
                                       i, i, ip, ip, ip, ip, i, d, dp]
         _lib.orc_face_flux_central.argtypes = [i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
         _lib.orc_rhs_divergence.argtypes = [i, ip, ip, dp, dp]
+        _lib.orc_momentum_rows.argtypes = [i, ip, ip, dp, dp, dp, dp, d, dp, dp]
         _lib.orc_calcp_assemble.argtypes = ([i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, i]
                                             + [dp] * 7)
         _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
@@ -196,6 +197,16 @@ def face_flux_central(mesh, g: Geometry, u, v, w):
     lib().orc_face_flux_central(nif, _ip(own1), _ip(nei1), _dp(g.facint), _dp(g.arx), _dp(g.ary),
                                 _dp(g.arz), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)), _dp(flux))
     return flux
+
+
+def momentum_rows(csr: CSR, a, sp, src, phi, urf):
+    """One velocity component's rows of calcuvw (src/calcuvw.f90:417-437): returns (a, su, ap)."""
+    n = csr.diag.shape[0]
+    a = _f64(a).copy()
+    su, ap = np.zeros(n), np.zeros(n)
+    lib().orc_momentum_rows(n, _ip(csr.ioffset), _ip(csr.diag), _dp(a), _dp(_f64(sp)), _dp(_f64(src)),
+                            _dp(_f64(phi)), float(urf), _dp(su), _dp(ap))
+    return a, su, ap
 
 
 RC_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p")

@@ -166,3 +166,36 @@ def test_spmv_bench_runs(case):
     sys_.assemble_laplacian(np.full(m.n_cells, -1.0))
     assert sys_.spmv_bench(64, 3) > 0
     assert sys_.spmv_bench(32, 3) > 0
+
+
+def test_momentum_three_components_match_oracle(ctx):
+    """calcuvw's u, v, w solves on one assembly (src/calcuvw.f90:414-556): 1/a_P of every
+    component bitwise the oracle's rows, the last component's matrix and source bitwise, and
+    each BiCGStab-DILU solution the oracle's within its tolerance (iterations +-2)."""
+    m = M.tet_box(6, grade_axis=1, grade_ratio=1.1, permute_seed=9)
+    geo = O.mesh_geometry(m)
+    csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    flux = O.face_flux_central(m, geo, u, v, w)
+    vis = np.full(m.n_cells, 1e-3)
+    n = m.n_cells
+    rng = np.random.default_rng(11)
+    sp = [rng.random(n) * geo.vol * 10 for _ in range(3)]
+    src = [W.blob_rhs(geo.xc, geo.yc, geo.zc, geo.vol, 4, seed=20 + c) for c in range(3)]
+    phi0 = [0.1 * rng.standard_normal(n) for _ in range(3)]
+    urf = [0.7, 0.6, 0.8]
+    s = fcap.System(m, ctx)
+    s.assemble_convdiff(vis, flux, 0.7, None)
+    rc, phi, ap, infos = s.solve_momentum(sp, src, urf, phi0, fcap.METHOD_BICGSTAB, 1e-8, 1000)
+    assert rc == 0, infos
+    a_base = O.convdiff(m, geo, csr, vis, flux, 0.7)
+    for c in range(3):
+        a_c, su_c, ap_c = O.momentum_rows(csr, a_base, sp[c], src[c], phi0[c], urf[c])
+        np.testing.assert_array_equal(ap[c], ap_c, err_msg=f"ap component {c}")
+        if c == 2:
+            np.testing.assert_array_equal(s.get("a"), a_c)
+            np.testing.assert_array_equal(s.get("su"), su_c)
+        ref = O.bicgstab(csr, a_c, su_c, phi0[c], sor=1e-8, nsw=1000)
+        assert abs(infos[c]["iters"] - ref.iters) <= 2, (c, infos[c]["iters"], ref.iters)
+        assert np.linalg.norm(phi[c] - ref.fi) / np.linalg.norm(ref.fi) < 1e-6, c
+    s.close()

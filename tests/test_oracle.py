@@ -344,3 +344,39 @@ def test_calcp_uniform_flow_closed_form():
     A = sp.csr_matrix((out["a"], c.ja - 1, c.ioffset - 1), shape=(n, n))
     np.testing.assert_allclose(A @ np.ones(n), 0.0, atol=1e-12)
     assert abs(out["su"].sum()) < 1e-12
+
+
+# ---------------------------------------------------------------- momentum rows
+
+def _momentum_case(seed=3):
+    from freecappuccino_amd import workloads as W
+    m = M.tet_box(4, grade_axis=1, grade_ratio=1.1, permute_seed=5)
+    geo = O.mesh_geometry(m)
+    c = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    a = O.convdiff(m, geo, c, np.full(m.n_cells, 1e-3), O.face_flux_central(m, geo, u, v, w), 0.7)
+    rng = np.random.default_rng(seed)
+    n = m.n_cells
+    return c, a, rng.random(n) * geo.vol, rng.standard_normal(n), rng.standard_normal(n)
+
+
+def test_momentum_rows_closed_forms():
+    """calcuvw's rows (src/calcuvw.f90:417-437): with urf = 1 every row sums to sp and su = src;
+    for any urf, urf a_PP = sp - sum(off-diagonals) and su - src = (1 - urf) a_PP phi; the
+    off-diagonals are left alone and ap = 1/(a_PP + small)."""
+    c, a, sp, src, phi = _momentum_case()
+    n = c.diag.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(c.ioffset))
+    d0 = c.diag - 1
+    off = np.ones(a.shape[0], bool)
+    off[d0] = False
+    offsum = np.bincount(rows[off], weights=a[off], minlength=n)
+    a1, su1, ap1 = O.momentum_rows(c, a, sp, src, phi, 1.0)
+    np.testing.assert_allclose(np.bincount(rows, weights=a1, minlength=n), sp, rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(su1, src)
+    np.testing.assert_array_equal(a1[off], a[off])
+    urf = 0.7
+    a2, su2, ap2 = O.momentum_rows(c, a, sp, src, phi, urf)
+    np.testing.assert_allclose(urf * a2[d0], sp - offsum, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(su2 - src, (1 - urf) * a2[d0] * phi, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(ap2, 1.0 / (a2[d0] + O.small()), rtol=0, atol=0)
