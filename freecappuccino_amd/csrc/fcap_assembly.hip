@@ -357,6 +357,147 @@ __global__ void k_momentum_rows(int n, const int *__restrict__ ptr, const int *_
     ap[i] = 1.0 / (d + small);
 }
 
+// --------------------------------------------------------------- Gauss gradient
+
+// item q of the boundary sequence (groups in the reference's order): its face and value index
+__device__ inline void grad_group_item(const GradGroups &G, int q, int &face, int &val) {
+    int g = 0;
+    while (g < 4 && q >= G.qstart[g + 1]) ++g;
+    face = G.fstart[g] + (q - G.qstart[g]);
+    val = G.vstart[g] + (q - G.qstart[g]);
+}
+
+__global__ void k_bnd_count(int nq, GradGroups G, const int *__restrict__ owner, int n, int *__restrict__ cnt,
+                            int *__restrict__ err) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int f, v;
+    grad_group_item(G, q, f, v);
+    const int c = owner[f];
+    if (c < 0 || c >= n) { atomicOr(err, 1); return; }
+    atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_bnd_fill(int nq, GradGroups G, const int *__restrict__ owner, int *__restrict__ cursor,
+                           int *__restrict__ idx) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    int f, v;
+    grad_group_item(G, q, f, v);
+    idx[atomicAdd(&cursor[owner[f]], 1)] = q;
+}
+
+// gradco (src/grad_gauss.f90:128-190): value at the face centre, extrapolated with the
+// previous gradient interpolated to the point of the P-N line, times S
+__device__ inline void grad_face(int p, int q, double xfc, double yfc, double zfc, double sx, double sy, double sz,
+                                 double fif, const double *__restrict__ fi, const double *__restrict__ xc,
+                                 const double *__restrict__ yc, const double *__restrict__ zc,
+                                 const double *__restrict__ ox, const double *__restrict__ oy,
+                                 const double *__restrict__ oz, double *out) {
+    const double fxn = fif;
+    const double fxp = 1.0 - fxn;
+    const double xi = xc[p] * fxp + xc[q] * fxn;
+    const double yi = yc[p] * fxp + yc[q] * fxn;
+    const double zi = zc[p] * fxp + zc[q] * fxn;
+    const double dfxi = ox[p] * fxp + ox[q] * fxn;
+    const double dfyi = oy[p] * fxp + oy[q] * fxn;
+    const double dfzi = oz[p] * fxp + oz[q] * fxn;
+    const double fie = (((fi[p] * fxp + fi[q] * fxn) + dfxi * (xfc - xi)) + dfyi * (yfc - yi)) + dfzi * (zfc - zi);
+    out[0] = fie * sx;
+    out[1] = fie * sy;
+    out[2] = fie * sz;
+}
+
+__global__ void k_grad_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                             const double *__restrict__ xf, const double *__restrict__ yf,
+                             const double *__restrict__ zf, const double *__restrict__ arx,
+                             const double *__restrict__ ary, const double *__restrict__ arz,
+                             const double *__restrict__ facint, const double *__restrict__ xc,
+                             const double *__restrict__ yc, const double *__restrict__ zc,
+                             const double *__restrict__ u, const double *__restrict__ ox,
+                             const double *__restrict__ oy, const double *__restrict__ oz, double *__restrict__ gf) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    grad_face(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], u, xc, yc, zc, ox, oy, oz,
+              gf + 3 * (size_t)f);
+}
+
+__global__ void k_grad_oc(int noc, const int *__restrict__ ocface, const int *__restrict__ ijl,
+                          const int *__restrict__ ijr, const double *__restrict__ foc, const double *__restrict__ xf,
+                          const double *__restrict__ yf, const double *__restrict__ zf,
+                          const double *__restrict__ arx, const double *__restrict__ ary,
+                          const double *__restrict__ arz, const double *__restrict__ xc,
+                          const double *__restrict__ yc, const double *__restrict__ zc,
+                          const double *__restrict__ u, const double *__restrict__ ox, const double *__restrict__ oy,
+                          const double *__restrict__ oz, double *__restrict__ gc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noc) return;
+    const int f = ocface[i];
+    grad_face(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], u, xc, yc, zc, ox, oy, oz,
+              gc + 3 * (size_t)i);
+}
+
+// One cell's gradient, summed as grad_gauss sums it: inner faces in face order (+ on the
+// owner side, - on the neighbour side), then O-C cuts, then boundary faces in group order,
+// then times 1/vol.
+__global__ void k_grad_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                            const int *__restrict__ kord, const int *__restrict__ kface, const int *__restrict__ ij,
+                            const double *__restrict__ gf, const int *__restrict__ co_ptr,
+                            const int *__restrict__ co_idx, const double *__restrict__ gc,
+                            const int *__restrict__ cb_ptr, const int *__restrict__ cb_q, GradGroups G,
+                            const double *__restrict__ arx, const double *__restrict__ ary,
+                            const double *__restrict__ arz, const double *__restrict__ u,
+                            const double *__restrict__ vol, double *__restrict__ dudx, double *__restrict__ dudy,
+                            double *__restrict__ dudz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dp = diagpos[i];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+        const int k = kord ? kord[p] : p;
+        if (k == dp) continue;
+        const int f = kface[k];
+        const double *g = gf + 3 * (size_t)f;
+        if (ij[f] == k) {   // row i is the owner of face f
+            sx = sx + g[0];
+            sy = sy + g[1];
+            sz = sz + g[2];
+        } else {
+            sx = sx - g[0];
+            sy = sy - g[1];
+            sz = sz - g[2];
+        }
+    }
+    if (co_ptr) {
+        for (int q = co_ptr[i]; q < co_ptr[i + 1]; ++q) {
+            const int code = co_idx[q];
+            const double *g = gc + 3 * (size_t)(code >> 1);
+            if (code & 1) {   // right cell of the cut
+                sx = sx - g[0];
+                sy = sy - g[1];
+                sz = sz - g[2];
+            } else {
+                sx = sx + g[0];
+                sy = sy + g[1];
+                sz = sz + g[2];
+            }
+        }
+    }
+    if (cb_ptr) {
+        for (int q = cb_ptr[i]; q < cb_ptr[i + 1]; ++q) {
+            int f, v;
+            grad_group_item(G, cb_q[q], f, v);
+            sx = sx + u[v] * arx[f];
+            sy = sy + u[v] * ary[f];
+            sz = sz + u[v] * arz[f];
+        }
+    }
+    const double volr = 1.0 / vol[i];
+    dudx[i] = sx * volr;
+    dudy[i] = sy * volr;
+    dudz[i] = sz * volr;
+}
+
 __global__ void k_kface(int nif, const int *__restrict__ ij, const int *__restrict__ ji, int *__restrict__ kface) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nif) return;
@@ -602,6 +743,63 @@ void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
         oc.noc ? oc.co_ptr.as<int>() : nullptr, oc.noc ? oc.co_idx.as<int>() : nullptr,
         oc.noc ? oc.al.as<double>() : nullptr, oc.noc ? oc.ar.as<double>() : nullptr, oc.noc ? fmoc : nullptr, a, su);
     FC_HIP_CHECK(hipGetLastError());
+}
+
+void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner0_all, int n, DevBuf &cb_ptr,
+                          DevBuf &cb_q) {
+    const int nq = G.qstart[5];
+    if (nq <= 0) { cb_ptr.release(); cb_q.release(); return; }
+    hipStream_t s = ctx->stream;
+    DevBuf cnt(sizeof(int) * (n + 1)), cursor(sizeof(int) * (n + 1)), err(sizeof(int));
+    FC_HIP_CHECK(hipMemsetAsync(cnt.get(), 0, sizeof(int) * (n + 1), s));
+    FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), s));
+    k_bnd_count<<<grid_for(nq), kBlock, 0, s>>>(nq, G, owner0_all, n, cnt.as<int>(), err.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    int herr = 0;
+    d2h(&herr, err.get(), sizeof(int), s);
+    FC_REQUIRE(herr == 0, FC_ERR_STRUCTURE, "boundary face owner out of range");
+    cb_ptr.alloc(sizeof(int) * (n + 1));
+    scan_dev(ctx, cnt.as<int>(), cb_ptr.as<int>(), n + 1);
+    cb_q.alloc(sizeof(int) * nq);
+    k_copy<<<grid_for(n), kBlock, 0, s>>>(cb_ptr.as<int>(), cursor.as<int>(), n);
+    k_bnd_fill<<<grid_for(nq), kBlock, 0, s>>>(nq, G, owner0_all, cursor.as<int>(), cb_q.as<int>());
+    sort_segments_dev(ctx, cb_ptr.as<int>(), cb_q.as<int>(), n);
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                    const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                    const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
+                    const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
+                    const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
+                    int nigrad, double *dudx, double *dudy, double *dudz) {
+    hipStream_t s = ctx->stream;
+    const int n = st.n;
+    DevBuf old(sizeof(double) * 3 * (size_t)n), gf(sizeof(double) * 3 * (size_t)std::max(nif, 1)),
+        gc(sizeof(double) * 3 * (size_t)std::max(oc.noc, 1));
+    double *ox = old.as<double>(), *oy = ox + n, *oz = oy + n;
+    FC_HIP_CHECK(hipMemsetAsync(ox, 0, sizeof(double) * 3 * (size_t)n, s));
+    for (int lc = 1; lc <= nigrad; ++lc) {
+        if (nif)
+            k_grad_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, xf, yf, zf, arx, ary, arz, facint, xc,
+                                                          yc, zc, u, ox, oy, oz, gf.as<double>());
+        if (oc.noc)
+            k_grad_oc<<<grid_for(oc.noc), kBlock, 0, s>>>(oc.noc, ocface, oc.ijl.as<int>(), oc.ijr.as<int>(),
+                                                          oc.foc.as<double>(), xf, yf, zf, arx, ary, arz, xc, yc, zc,
+                                                          u, ox, oy, oz, gc.as<double>());
+        k_grad_rows<<<grid_for(n), kBlock, 0, s>>>(n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface,
+                                                   ij_slot, gf.as<double>(), oc.noc ? oc.co_ptr.as<int>() : nullptr,
+                                                   oc.noc ? oc.co_idx.as<int>() : nullptr, gc.as<double>(), cb_ptr,
+                                                   cb_q, G, arx, ary, arz, u, vol, dudx, dudy, dudz);
+        if (lc != nigrad) {
+            FC_HIP_CHECK(hipMemcpyAsync(ox, dudx, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            FC_HIP_CHECK(hipMemcpyAsync(oy, dudy, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            FC_HIP_CHECK(hipMemcpyAsync(oz, dudz, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,

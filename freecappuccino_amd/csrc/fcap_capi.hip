@@ -524,6 +524,77 @@ extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInn
     });
 }
 
+extern "C" fc_status fc_grad_gauss(fc_context *ctx, int numCells, int numFaces, int numInnerFaces, const int *owner,
+                                   const int *neighbour, const double *arx, const double *ary, const double *arz,
+                                   const double *facint, const double *xc, const double *yc, const double *zc,
+                                   const double *xf, const double *yf, const double *zf, const double *vol,
+                                   const double *u, int numTotal, const int *bnd, int noc, const int *ijlFace,
+                                   const int *ijl, const int *ijr, const double *foc, int nigrad, double *dudx,
+                                   double *dudy, double *dudz) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces, nf = numFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nf >= nif && numTotal >= n, FC_ERR_ARGUMENT, "sizes");
+        FC_REQUIRE(owner && neighbour && u && vol && dudx && dudy && dudz && bnd && nigrad >= 1, FC_ERR_ARGUMENT,
+                   "arguments");
+        GradGroups G;
+        int q = 0, v = n;
+        for (int g = 0; g < 5; ++g) {
+            const int cnt = bnd[2 * g], fs = bnd[2 * g + 1];
+            FC_REQUIRE(cnt >= 0 && (cnt == 0 || (fs >= nif && fs + cnt <= nf)), FC_ERR_ARGUMENT,
+                       "boundary group outside the boundary faces");
+            G.qstart[g] = q;
+            G.fstart[g] = fs;
+            G.vstart[g] = v;
+            q += cnt;
+            v += cnt;
+        }
+        G.qstart[5] = q;
+        FC_REQUIRE(v <= numTotal, FC_ERR_ARGUMENT, "numTotal smaller than cells + boundary values");
+        DevBuf own, nei, ij, ji, kface, kord, own_all;
+        upload_idx(ctx, own, owner, nif);
+        upload_idx(ctx, nei, neighbour, nif);
+        upload_idx(ctx, own_all, owner, nf);
+        ij.alloc(sizeof(int) * std::max(nif, 1));
+        ji.alloc(sizeof(int) * std::max(nif, 1));
+        Structure st;
+        build_structure_from_faces_dev(ctx, st, n, nif, own.as<int>(), nei.as<int>(), ij.as<int>(), ji.as<int>());
+        bool needs = false;
+        build_kface(ctx, st, nif, ij.as<int>(), ji.as<int>(), kface, kord, needs);
+        DevBuf dx[6], dc[4], dfi, du, dg(sizeof(double) * 3 * (size_t)n), cb_ptr, cb_q, ocf;
+        const double *fa[6] = {arx, ary, arz, xf, yf, zf};
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nf);
+        const double *ca[4] = {xc, yc, zc, vol};
+        for (int k = 0; k < 4; ++k) upload_dbl(ctx, dc[k], ca[k], n);
+        upload_dbl(ctx, dfi, facint, nif);
+        upload_dbl(ctx, du, u, numTotal);
+        build_boundary_lists(ctx, G, own_all.as<int>(), n, cb_ptr, cb_q);
+        OcDev oc;
+        oc.noc = std::max(noc, 0);
+        if (noc > 0) {
+            FC_REQUIRE(ijlFace && ijl && ijr && foc, FC_ERR_ARGUMENT, "O-C arrays");
+            upload_idx(ctx, oc.ijl, ijl, noc);
+            upload_idx(ctx, oc.ijr, ijr, noc);
+            upload_idx(ctx, ocf, ijlFace, noc);
+            upload_dbl(ctx, oc.foc, foc, noc);
+            build_oc_lists(ctx, oc, n);
+            std::vector<int> hf(ijlFace, ijlFace + noc);
+            for (int f : hf) FC_REQUIRE(f >= 1 && f <= nf, FC_ERR_ARGUMENT, "O-C face number out of range");
+        }
+        double *g = dg.as<double>();
+        grad_gauss_dev(ctx, st, nif, own.as<int>(), nei.as<int>(), ij.as<int>(), kface.as<int>(),
+                       needs ? kord.as<int>() : nullptr, dx[3].as<double>(), dx[4].as<double>(), dx[5].as<double>(),
+                       dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), dfi.as<double>(),
+                       dc[0].as<double>(), dc[1].as<double>(), dc[2].as<double>(), dc[3].as<double>(),
+                       du.as<double>(), G, cb_ptr.get() ? cb_ptr.as<int>() : nullptr,
+                       cb_q.get() ? cb_q.as<int>() : nullptr, oc, noc > 0 ? ocf.as<int>() : nullptr, nigrad, g,
+                       g + n, g + 2 * (size_t)n);
+        download_dbl(ctx, dudx, g, n);
+        download_dbl(ctx, dudy, g + n, n);
+        download_dbl(ctx, dudz, g + 2 * (size_t)n, n);
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_face_flux_central(fc_context *ctx, int numInnerFaces, const int *owner, const int *neighbour,
                                           const double *facint, const double *arx, const double *ary,
                                           const double *arz, const double *u, const double *v, const double *w,

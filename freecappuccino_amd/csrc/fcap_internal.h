@@ -219,6 +219,24 @@ struct RhieChowDev {
     const double *xc, *yc, *zc, *vol, *den, *apu, *apv, *apw, *u, *v, *w, *p;
     const double *dudx, *dvdx, *dwdx, *dpdx;
 };
+// Boundary groups of the Gauss gradient in the reference's order (inlet, outlet,
+// symmetry, wall, pressure outlet): item q of the sequence over all groups is
+// face fstart[g] + (q - qstart[g]) with its value at vstart[g] + (q - qstart[g])
+// (0-based); qstart[5] is the number of items.
+struct GradGroups {
+    int qstart[6];
+    int fstart[5], vstart[5];
+};
+// per-cell lists of boundary items, each in sequence order
+void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner0_all, int n, DevBuf &cb_ptr,
+                          DevBuf &cb_q);
+// grad_gauss: nigrad iterations of the Gauss gradient of u (cells, then boundary values)
+void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                    const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                    const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
+                    const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
+                    const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
+                    int nigrad, double *dudx, double *dudy, double *dudz);
 // one velocity component's diagonal, source and 1/a_P (calcuvw), a's diagonal slots updated
 void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,
                        const double *phi, double urf, double *su, double *ap);

@@ -79,6 +79,7 @@ def library() -> C.CDLL:
                                                           dp, dp, dp, dp], i),
         "fc_convdiff": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, ip, ip, ip, ip, ip, i, d, dp], i),
         "fc_face_flux_central": ([vp, i, ip, ip, dp, dp, dp, dp, dp, dp, dp, i, dp], i),
+        "fc_grad_gauss": ([vp, i, i, i, ip, ip] + [dp] * 12 + [i, ip, i, ip, ip, ip, dp, i, dp, dp, dp], i),
         "fc_calcp_assemble": ([vp, i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
                               + [dp] * 6, i),
         "fc_rhs_divergence": ([vp, i, i, ip, ip, dp, dp], i),
@@ -317,6 +318,47 @@ def convdiff(mesh, geo: Geometry, csr: CSR, vis, flux, urf=0.7, ctx: Optional[Co
                                 _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag), _ip(_i32(csr.ij_slot)),
                                 _ip(_i32(csr.ji_slot)), mesh.nnz, float(urf), _dp(a)))
     return a
+
+
+def boundary_groups(mesh):
+    """(count, faces before the group) of the inlet, outlet, symmetry, wall and pressure-outlet
+    groups, as src/mesh_geometry_and_topology.f90:409-470 derives them from the boundary file."""
+    groups = ("inlet", "outlet", "symmetry", "wall", "prOutlet")
+    walls = ("wall", "wallIsoth", "wallAdiab", "wallQFlux")
+    bnd = np.zeros(10, np.int32)
+    for t, nf, st in mesh.patches:
+        g = 3 if t in walls else groups.index(t) if t in groups else -1
+        if g < 0:
+            continue
+        if bnd[2 * g] == 0:
+            bnd[2 * g + 1] = st
+        bnd[2 * g] += nf
+    return bnd
+
+
+def grad_gauss(mesh, geo: Geometry, u, nigrad=1, oc=None, ctx: Optional[Context] = None):
+    """Gauss gradient on the GPU (src/grad_gauss.f90) of u (cells, then boundary values in group
+    order); returns (dudx, dudy, dudz).  oc: optional O-C cuts {ijl, ijr (0-based cells), foc,
+    faces (0-based face of the first cut, the others following)}."""
+    ctx = ctx or default_context()
+    n, nif, nf = mesh.n_cells, mesh.n_inner, mesh.n_faces
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    u = _f64(u)
+    out = [np.zeros(n) for _ in range(3)]
+    zi, zd = np.zeros(1, np.int32), np.zeros(1)
+    noc = 0 if oc is None else len(oc["ijl"])
+    if noc:
+        ijlf = _i32(int(oc["faces"]) + np.arange(noc) + 1)
+        ijl, ijr, foc = _i32(np.asarray(oc["ijl"]) + 1), _i32(np.asarray(oc["ijr"]) + 1), _f64(oc["foc"])
+    else:
+        ijlf, ijl, ijr, foc = zi, zi, zi, zd
+    g = [_f64(getattr(geo, k)) for k in ("arx", "ary", "arz", "facint", "xc", "yc", "zc", "xf", "yf", "zf", "vol")]
+    ctx.check(ctx.L.fc_grad_gauss(ctx.h, n, nf, nif, _ip(own1), _ip(nei1), *[_dp(x) for x in g], _dp(u), u.shape[0],
+                                  _ip(boundary_groups(mesh)), noc, _ip(ijlf), _ip(ijl), _ip(ijr), _dp(foc),
+                                  int(nigrad),
+                                  *[_dp(x) for x in out]))
+    return tuple(out)
 
 
 CALCP_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p")

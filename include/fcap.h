@@ -149,6 +149,24 @@ fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, co
                             const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
                             double *su, double *flmass, double *al, double *ar, double *fmoc);
 
+/* Gauss gradient (src/grad_gauss.f90:1-124 with gradco :128-190 and gradbc
+ * :194-211): nigrad iterations, each summing per cell the inner faces' value
+ * times S (the value at the face centre extrapolated with the previous
+ * iteration's gradient), then the O-C cuts (faces ijlFace, 1-based), then the
+ * boundary faces with their boundary values, then dividing by the volume.
+ * u has numTotal entries: cells, then the boundary values in the reference's
+ * group order.  bnd (10 ints): (count, faces before the group) for the inlet,
+ * outlet, symmetry, wall and pressure-outlet groups (ninl, iInletFacesStart,
+ * nout, iOutletFacesStart, nsym, iSymmetryFacesStart, nwal, iWallFacesStart,
+ * npru, iPressOutletFacesStart).  owner and the face arrays cover numFaces. */
+fc_status fc_grad_gauss(fc_context *ctx, int numCells, int numFaces, int numInnerFaces, const int *owner,
+                        const int *neighbour, const double *arx, const double *ary, const double *arz,
+                        const double *facint, const double *xc, const double *yc, const double *zc,
+                        const double *xf, const double *yf, const double *zf, const double *vol,
+                        const double *u, int numTotal, const int *bnd, int noc, const int *ijlFace,
+                        const int *ijl, const int *ijr, const double *foc, int nigrad, double *dudx,
+                        double *dudy, double *dudz);
+
 /* Face mass flux F_f = u_f . S_f, u_f = (1-facint) u_P + facint u_N, and the
  * continuity source su(o) -= F, su(n) += F of
  * src/calcp-multiple_correction_SIMPLE.f90:74-75 (accumulated in face order). */

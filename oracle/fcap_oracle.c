@@ -22,6 +22,10 @@
  *                         evaluation and the divergence-theorem invariant
  *                         (sum of su over cells equals the net boundary flux,
  *                         zero for inner-face-only fluxes).
+ *   orc_grad_gauss     -- bitwise vs the reference's grad_gauss compiled on its
+ *                         own (oracle/_ref/ref_grad_driver) on a box with every
+ *                         boundary group and on permuted tets, 1 and 2
+ *                         iterations; exact for linear fields on a uniform box.
  *   orc_momentum_rows  -- closed forms: with urf = 1 every row sums to sp and
  *                         su = s; for any urf, urf a_PP = sp - (sum of the row's
  *                         off-diagonals) and su - s = (1 - urf) a_PP phi.
@@ -821,4 +825,83 @@ void orc_momentum_rows(int n, const int *ioffset, const int *diag, double *a, co
         I1(su, i) = I1(src, i) + urfm * I1(a, dp) * I1(phi, i);
         I1(ap, i) = 1.0 / (I1(a, dp) + small);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gauss gradient: src/grad_gauss.f90:1-124, gradco :128-190, gradbc :194-211. */
+/* ------------------------------------------------------------------------ */
+
+static void gradco(int ijp, int ijn, double xfc, double yfc, double zfc, double sx, double sy, double sz,
+                   double fif, const double *fi, const double *xc, const double *yc, const double *zc,
+                   const double *dfxo, const double *dfyo, const double *dfzo, double *dfx, double *dfy, double *dfz)
+{
+    double fxn = fif;
+    double fxp = 1.0 - fxn;
+    double xi = I1(xc, ijp) * fxp + I1(xc, ijn) * fxn;
+    double yi = I1(yc, ijp) * fxp + I1(yc, ijn) * fxn;
+    double zi = I1(zc, ijp) * fxp + I1(zc, ijn) * fxn;
+    double dfxi = I1(dfxo, ijp) * fxp + I1(dfxo, ijn) * fxn;
+    double dfyi = I1(dfyo, ijp) * fxp + I1(dfyo, ijn) * fxn;
+    double dfzi = I1(dfzo, ijp) * fxp + I1(dfzo, ijn) * fxn;
+    double fie = I1(fi, ijp) * fxp + I1(fi, ijn) * fxn + dfxi * (xfc - xi) + dfyi * (yfc - yi) + dfzi * (zfc - zi);
+    double dfxe = fie * sx, dfye = fie * sy, dfze = fie * sz;
+    I1(dfx, ijp) = I1(dfx, ijp) + dfxe;
+    I1(dfy, ijp) = I1(dfy, ijp) + dfye;
+    I1(dfz, ijp) = I1(dfz, ijp) + dfze;
+    I1(dfx, ijn) = I1(dfx, ijn) - dfxe;
+    I1(dfy, ijn) = I1(dfy, ijn) - dfye;
+    I1(dfz, ijn) = I1(dfz, ijn) - dfze;
+}
+
+int orc_grad_gauss(int n, int nif, const int *owner1, const int *neighbour1,
+                   const double *arx, const double *ary, const double *arz, const double *facint,
+                   const double *xc, const double *yc, const double *zc,
+                   const double *xf, const double *yf, const double *zf, const double *vol,
+                   const double *u, const int *bnd, int noc, const int *ijlface1, const int *ijl1,
+                   const int *ijr1, const double *foc, int nigrad, double *dudx, double *dudy, double *dudz)
+{
+    double *o = (double *)calloc(3 * (size_t)n, sizeof(double));
+    if (!o) return -1;
+    double *dfxo = o, *dfyo = o + n, *dfzo = o + 2 * n;
+    for (int lc = 1; lc <= nigrad; ++lc) {
+        memset(dudx, 0, sizeof(double) * n);
+        memset(dudy, 0, sizeof(double) * n);
+        memset(dudz, 0, sizeof(double) * n);
+        for (int i = 0; i < nif; ++i)
+            gradco(owner1[i], neighbour1[i], xf[i], yf[i], zf[i], arx[i], ary[i], arz[i], facint[i], u, xc, yc, zc,
+                   dfxo, dfyo, dfzo, dudx, dudy, dudz);
+        for (int i = 0; i < noc; ++i) {
+            int iface = ijlface1[i];
+            gradco(ijl1[i], ijr1[i], I1(xf, iface), I1(yf, iface), I1(zf, iface), I1(arx, iface), I1(ary, iface),
+                   I1(arz, iface), foc[i], u, xc, yc, zc, dfxo, dfyo, dfzo, dudx, dudy, dudz);
+        }
+        /* boundary groups in the reference's order: inlet, outlet, symmetry, wall, pressure
+         * outlet; values of group g follow those of the groups before it, after the cells */
+        int vstart = n;
+        for (int g = 0; g < 5; ++g) {
+            int cnt = bnd[2 * g], fstart = bnd[2 * g + 1];
+            for (int i = 1; i <= cnt; ++i) {
+                int iface = fstart + i;
+                int ijp = I1(owner1, iface);
+                int ijb = vstart + i;
+                I1(dudx, ijp) = I1(dudx, ijp) + I1(u, ijb) * I1(arx, iface);
+                I1(dudy, ijp) = I1(dudy, ijp) + I1(u, ijb) * I1(ary, iface);
+                I1(dudz, ijp) = I1(dudz, ijp) + I1(u, ijb) * I1(arz, iface);
+            }
+            vstart += cnt;
+        }
+        for (int i = 0; i < n; ++i) {
+            double volr = 1.0 / vol[i];
+            dudx[i] = dudx[i] * volr;
+            dudy[i] = dudy[i] * volr;
+            dudz[i] = dudz[i] * volr;
+        }
+        if (lc != nigrad) {
+            memcpy(dfxo, dudx, sizeof(double) * n);
+            memcpy(dfyo, dudy, sizeof(double) * n);
+            memcpy(dfzo, dudz, sizeof(double) * n);
+        }
+    }
+    free(o);
+    return 0;
 }

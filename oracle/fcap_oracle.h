@@ -89,6 +89,18 @@ int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
                        double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
                        double *fmoc);
 
+/* Gauss gradient of u (numTotal values: cells, then the boundary groups' values) as
+ * src/grad_gauss.f90 computes it, nigrad iterations.  owner1 covers every face; the
+ * face arrays are 0-based per face.  bnd: (count, face start) of the inlet, outlet,
+ * symmetry, wall and pressure-outlet groups (face start = faces before the group);
+ * O-C cuts through 1-based face numbers ijlface1. */
+int orc_grad_gauss(int n, int nif, const int *owner1, const int *neighbour1,
+                   const double *arx, const double *ary, const double *arz, const double *facint,
+                   const double *xc, const double *yc, const double *zc,
+                   const double *xf, const double *yf, const double *zf, const double *vol,
+                   const double *u, const int *bnd, int noc, const int *ijlface1, const int *ijl1,
+                   const int *ijr1, const double *foc, int nigrad, double *dudx, double *dudy, double *dudz);
+
 /* Momentum component rows of calcuvw (src/calcuvw.f90:417-437): diagonal reset to zero, then
  * a(diag) = (sp - sum of the row's off-diagonals) / urf, su = src + (1 - urf) a(diag) phi,
  * ap = 1/(a(diag) + small).  a is updated in place (its diagonal slots). */

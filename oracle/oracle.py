@@ -52,6 +52,7 @@ def lib():
         _lib.orc_face_flux_central.argtypes = [i, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp]
         _lib.orc_rhs_divergence.argtypes = [i, ip, ip, dp, dp]
         _lib.orc_momentum_rows.argtypes = [i, ip, ip, dp, dp, dp, dp, d, dp, dp]
+        _lib.orc_grad_gauss.argtypes = [i, i, ip, ip] + [dp] * 12 + [ip, i, ip, ip, ip, dp, i, dp, dp, dp]
         _lib.orc_calcp_assemble.argtypes = ([i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, i]
                                             + [dp] * 7)
         _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
@@ -197,6 +198,49 @@ def face_flux_central(mesh, g: Geometry, u, v, w):
     lib().orc_face_flux_central(nif, _ip(own1), _ip(nei1), _dp(g.facint), _dp(g.arx), _dp(g.ary),
                                 _dp(g.arz), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)), _dp(flux))
     return flux
+
+
+GROUPS = ("inlet", "outlet", "symmetry", "wall", "prOutlet")
+_WALLS = ("wall", "wallIsoth", "wallAdiab", "wallQFlux")
+
+
+def boundary_groups(mesh):
+    """(count, face start) per group in the reference's order (src/mesh_geometry_and_topology.f90:
+    409-470): the first patch of a type sets its start, every patch of it adds its faces."""
+    bnd = np.zeros(10, np.int32)
+    for t, nf, st in mesh.patches:
+        g = 3 if t in _WALLS else GROUPS.index(t) if t in GROUPS else -1
+        if g < 0:
+            continue
+        if bnd[2 * g] == 0:
+            bnd[2 * g + 1] = st
+        bnd[2 * g] += nf
+    return bnd
+
+
+def grad_gauss(mesh, g: Geometry, u, nigrad=1, oc=None):
+    """Gauss gradient (src/grad_gauss.f90) of u (cells then boundary values): (dudx, dudy, dudz).
+    oc: optional O-C cuts {ijl, ijr (0-based cells), foc, faces (0-based face of the first cut,
+    the others following)}."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    out = [np.zeros(n) for _ in range(3)]
+    z = np.zeros(1, np.int32)
+    noc = 0 if oc is None else len(oc["ijl"])
+    if noc:
+        ijlf = _i32(int(oc["faces"]) + np.arange(noc) + 1)
+        ijl, ijr, foc = _i32(np.asarray(oc["ijl"]) + 1), _i32(np.asarray(oc["ijr"]) + 1), _f64(oc["foc"])
+    else:
+        ijlf, ijl, ijr, foc = z, z, z, np.zeros(1)
+    rc = lib().orc_grad_gauss(n, nif, _ip(own1), _ip(nei1), _dp(_f64(g.arx)), _dp(_f64(g.ary)), _dp(_f64(g.arz)),
+                              _dp(_f64(g.facint)), _dp(_f64(g.xc)), _dp(_f64(g.yc)), _dp(_f64(g.zc)),
+                              _dp(_f64(g.xf)), _dp(_f64(g.yf)), _dp(_f64(g.zf)), _dp(_f64(g.vol)), _dp(_f64(u)),
+                              _ip(boundary_groups(mesh)), noc, _ip(ijlf), _ip(ijl), _ip(ijr), _dp(foc), int(nigrad),
+                              *[_dp(x) for x in out])
+    if rc != 0:
+        raise MemoryError
+    return tuple(out)
 
 
 def momentum_rows(csr: CSR, a, sp, src, phi, urf):

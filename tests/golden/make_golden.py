@@ -35,6 +35,7 @@ from freecappuccino_amd import workloads as W  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+GRAD_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_grad_driver")
 FLUX_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_flux_driver")
 REF_TESTS = "/root/reference/tests"
 
@@ -220,6 +221,63 @@ def calcp_case(name, mesh, seed):
     save_case(name, mesh, inputs, {"calcp": out})
 
 
+def run_reference_grad(mesh, u, nigrad, fmt="native"):
+    """dudx, dudy, dudz of the reference's grad_gauss on u (numTotal values)."""
+    n = mesh.n_cells
+    with tempfile.TemporaryDirectory() as d:
+        M.write_polymesh(mesh, os.path.join(d, "polyMesh"), fmt=fmt)
+        with open(os.path.join(d, "grad.txt"), "w") as f:
+            f.write(f"{nigrad}\n")
+        with open(os.path.join(d, "grad_in.bin"), "wb") as f:
+            f.write(np.ascontiguousarray(u, np.float64).tobytes())
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        subprocess.run([GRAD_DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
+        raw = open(os.path.join(d, "grad_out.bin"), "rb").read()
+    hdr = np.frombuffer(raw, np.int32, 2)
+    assert hdr[0] == n and hdr[1] == u.size, (hdr, n, u.size)
+    g = np.frombuffer(raw, np.float64, offset=8)
+    assert g.size == 3 * n
+    return g[:n].copy(), g[n:2 * n].copy(), g[2 * n:].copy()
+
+
+def grad_case(name, mesh, seed):
+    """Gauss gradients (1 and 2 iterations) of a smooth field plus noise, boundary values included."""
+    n = mesh.n_cells
+    base = run_reference(mesh, np.full(n, -1.0), np.zeros(n + mesh.n_boundary), np.zeros(n), np.zeros(n),
+                         solver="none")
+    rng = np.random.default_rng(seed)
+    xc, yc, zc = base["xc"], base["yc"], base["zc"]
+    nb = mesh.n_boundary
+    fb = slice(mesh.n_inner, mesh.n_faces)
+    cell = np.sin(2.0 * xc) + yc * zc + 0.05 * rng.standard_normal(n)
+    bval = np.sin(2.0 * base["xf"][fb]) + base["yf"][fb] * base["zf"][fb]
+    # the boundary values sit in the reference's group order (inlet, outlet, symmetry, wall,
+    # prOutlet); here every patch type is one contiguous run in that order, so face order works
+    u = np.concatenate([cell, bval + 0.05 * rng.standard_normal(nb)])
+    solves = {}
+    for nig in (1, 2):
+        gx, gy, gz = run_reference_grad(mesh, u, nig)
+        out = {"dudx": gx, "dudy": gy, "dudz": gz}
+        for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint"):
+            out[k] = base[k]
+        solves[f"grad{nig}"] = out
+    save_case(name, mesh, {"u": u, "fmt": np.array("native")}, solves)
+
+
+def grad_meshes():
+    m = M.hex_box(5, 4, 3, perturb=0.2, seed=41)
+    # patch types in the reference's group order, each type one contiguous run of faces
+    types = ["inlet", "outlet", "symmetry", "symmetry", "wall", "prOutlet"]
+    m.patches = [(t, nf, st) for t, (_, nf, st) in zip(types, m.patches)]
+    t = M.tet_box(3, grade_axis=0, grade_ratio=1.2, grade_max_aspect=10.0, permute_seed=3)
+    return [("hex5_grad", m, 23), ("tet3_grad", t, 24)]
+
+
+def grad_only():
+    for name, mesh, seed in grad_meshes():
+        grad_case(name, mesh, seed)
+
+
 def transcript_fixture():
     """The 5x5 systems of tests/test_sparse_solvers.f90:605-655 and the expected transcript."""
     f32 = lambda xs: [float(np.float32(x)) for x in xs]  # noqa: E731  default-REAL literals
@@ -257,6 +315,7 @@ def main():
                                              permute_seed=8), seed=16)
     convdiff_case("hex8_convdiff", M.hex_box(8, perturb=0.2, seed=21), seed=17)
     calcp_only()
+    grad_only()
 
 
 def calcp_only():
@@ -270,5 +329,7 @@ def calcp_only():
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "calcp":
         calcp_only()
+    elif len(sys.argv) > 1 and sys.argv[1] == "grad":
+        grad_only()
     else:
         main()

@@ -9,7 +9,8 @@ a solution within 1e-10 relative FP64 residual of the reference's.
 import numpy as np
 import pytest
 
-from conftest import CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
+from conftest import (CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, GRAD_CASES, MESH_CASES, calcp_uniform_case,
+                      load_transcript)
 from freecappuccino_amd import fcap
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
@@ -295,3 +296,31 @@ def test_calcp_uniform_flow_closed_form(ctx):
     S = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], axis=1)
     np.testing.assert_allclose(out["flmass"], f["den"][0] * S @ vel, rtol=0, atol=1e-12)
     assert abs(out["su"].sum()) < 1e-12
+
+
+# ---------------------------------------------------------------- Gauss gradient
+
+@pytest.mark.parametrize("case", GRAD_CASES)
+@pytest.mark.parametrize("nigrad", [1, 2])
+def test_grad_gauss_bitwise_vs_reference(ctx, golden_loader, case, nigrad):
+    g = golden_loader(case)
+    s = f"grad{nigrad}"
+    geo = fcap.Geometry(*(g.out(s, k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                                "facint")))
+    out = fcap.grad_gauss(g.mesh, geo, g.inp("u"), nigrad, ctx=ctx)
+    for a, k in zip(out, ("dudx", "dudy", "dudz")):
+        np.testing.assert_array_equal(a, g.out(s, k), err_msg=k)
+
+
+def test_grad_gauss_oc_cuts_match_oracle(ctx):
+    m = M.hex_box(5, perturb=0.2, seed=12)
+    og = O.mesh_geometry(m)
+    cuts = _mirror_cuts(m, og, 7)
+    rng = np.random.default_rng(8)
+    u = rng.standard_normal(m.n_cells + m.n_boundary)
+    geo = fcap.Geometry(og.arx, og.ary, og.arz, og.xf, og.yf, og.zf, og.vol, og.xc, og.yc, og.zc, og.facint)
+    for nig in (1, 3):
+        ref = O.grad_gauss(m, og, u, nig, oc=cuts)
+        out = fcap.grad_gauss(m, geo, u, nig, oc=cuts, ctx=ctx)
+        for a, b, k in zip(out, ref, ("dudx", "dudy", "dudz")):
+            np.testing.assert_array_equal(a, b, err_msg=f"{k} nigrad={nig}")

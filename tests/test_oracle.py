@@ -380,3 +380,37 @@ def test_momentum_rows_closed_forms():
     np.testing.assert_allclose(urf * a2[d0], sp - offsum, rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(su2 - src, (1 - urf) * a2[d0] * phi, rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(ap2, 1.0 / (a2[d0] + O.small()), rtol=0, atol=0)
+
+
+# ---------------------------------------------------------------- Gauss gradient
+
+@pytest.mark.parametrize("case", ["hex5_grad", "tet3_grad"])
+@pytest.mark.parametrize("nigrad", [1, 2])
+def test_grad_gauss_bitwise_vs_reference(golden_loader, case, nigrad):
+    """grad_gauss of the reference, compiled on its own (oracle/_ref/ref_grad_driver); the hex
+    case has inlet, outlet, symmetry, wall and prOutlet patches."""
+    g = golden_loader(case)
+    s = f"grad{nigrad}"
+    geo = O.Geometry(*(g.out(s, k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                             "facint")))
+    gx, gy, gz = O.grad_gauss(g.mesh, geo, g.inp("u"), nigrad)
+    for a, k in zip((gx, gy, gz), ("dudx", "dudy", "dudz")):
+        np.testing.assert_array_equal(a, g.out(s, k), err_msg=k)
+
+
+def test_grad_gauss_linear_field_exact():
+    """On a uniform box the face-centre interpolation is exact for a linear field, so with the
+    exact boundary values the Gauss gradient is the field's gradient (to rounding)."""
+    m = M.hex_box(5, 4, 3)
+    geo = O.mesh_geometry(m)
+    grad = np.array([0.7, -1.3, 2.1])
+    lin = lambda x, y, z: 0.4 + grad[0] * x + grad[1] * y + grad[2] * z  # noqa: E731
+    # boundary values in the reference's group order (here symmetry before wall, unlike the faces)
+    bnd = O.boundary_groups(m)
+    bv = [lin(geo.xf[fs:fs + c], geo.yf[fs:fs + c], geo.zf[fs:fs + c]) for c, fs in bnd.reshape(5, 2)]
+    u = np.concatenate([lin(geo.xc, geo.yc, geo.zc)] + bv)
+    assert u.size == m.n_cells + m.n_boundary
+    for nig in (1, 2):
+        out = O.grad_gauss(m, geo, u, nig)
+        for a, ex in zip(out, grad):
+            np.testing.assert_allclose(a, ex, rtol=0, atol=1e-12)
