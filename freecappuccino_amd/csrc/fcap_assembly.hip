@@ -357,6 +357,182 @@ __global__ void k_momentum_rows(int n, const int *__restrict__ ptr, const int *_
     ap[i] = 1.0 / (d + small);
 }
 
+// ------------------------------------------- corrector stage of the pressure correction
+
+// fluxmc (src/facefluxmass.f90:520-607): the non-orthogonal mass flux correction
+__device__ inline double fluxmc(int p, int q, double xf, double yf, double zf, double arx, double ary, double arz,
+                               double lambda, const double *__restrict__ xc, const double *__restrict__ yc,
+                               const double *__restrict__ zc, const double *__restrict__ vol,
+                               const double *__restrict__ den, const double *__restrict__ apu,
+                               const double *__restrict__ dpdx) {
+    const double fxn = lambda;
+    const double fxp = 1.0 - lambda;
+    const double xcp = xc[p], ycp = yc[p], zcp = zc[p], xcn = xc[q], ycn = yc[q], zcn = zc[q];
+    const double xpn = xcn - xcp, ypn = ycn - ycp, zpn = zcn - zcp;
+    const double are = sqrt((arx * arx + ary * ary) + arz * arz);
+    const double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+    const double dppnnr = 1.0 / (((xpn * nxx) + (ypn * nyy)) + (zpn * nzz));
+    const double xpp = (xf - (xf - xcp) * nxx) - xcp, ypp = (yf - (yf - ycp) * nyy) - ycp,
+                 zpp = (zf - (zf - zcp) * nzz) - zcp;
+    const double xep = (xf - (xf - xcn) * nxx) - xcn, yep = (yf - (yf - ycn) * nyy) - ycn,
+                 zep = (zf - (zf - zcn) * nzz) - zcn;
+    const double rapr = ((apu[p] * den[p]) * vol[p]) * fxp + ((apu[q] * den[q]) * vol[q]) * fxn;
+    const double *gp = dpdx + 3 * (size_t)p, *gn = dpdx + 3 * (size_t)q;
+    return ((rapr * are) * (((gn[0] * xep - gp[0] * xpp) + (gn[1] * yep - gp[1] * ypp)) + (gn[2] * zep - gp[2] * zpp))) *
+           dppnnr;
+}
+
+__global__ void k_nonorth_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                                const double *__restrict__ xf, const double *__restrict__ yf,
+                                const double *__restrict__ zf, const double *__restrict__ arx,
+                                const double *__restrict__ ary, const double *__restrict__ arz,
+                                const double *__restrict__ facint, const double *__restrict__ xc,
+                                const double *__restrict__ yc, const double *__restrict__ zc,
+                                const double *__restrict__ vol, const double *__restrict__ den,
+                                const double *__restrict__ apu, const double *__restrict__ dpdx,
+                                double *__restrict__ fmcor, double *__restrict__ flmass) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    const double fm = fluxmc(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], xc, yc, zc, vol,
+                             den, apu, dpdx);
+    fmcor[f] = fm;
+    flmass[f] = flmass[f] + fm;
+}
+
+__global__ void k_nonorth_oc(int noc, int oc_face0, const int *__restrict__ ijl, const int *__restrict__ ijr,
+                             const double *__restrict__ foc, const double *__restrict__ xf,
+                             const double *__restrict__ yf, const double *__restrict__ zf,
+                             const double *__restrict__ arx, const double *__restrict__ ary,
+                             const double *__restrict__ arz, const double *__restrict__ xc,
+                             const double *__restrict__ yc, const double *__restrict__ zc,
+                             const double *__restrict__ vol, const double *__restrict__ den,
+                             const double *__restrict__ apu, const double *__restrict__ dpdx,
+                             double *__restrict__ fmcor_oc, double *__restrict__ fmoc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noc) return;
+    const int f = oc_face0 + i;
+    const double fm = fluxmc(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], xc, yc, zc, vol,
+                             den, apu, dpdx);
+    fmcor_oc[i] = fm;
+    fmoc[i] = fmoc[i] + fm;
+}
+
+// su(i) = 0 -/+ fmcor over the row's faces in face order, then the O-C cuts
+__global__ void k_face_source_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
+                                   const int *__restrict__ kord, const int *__restrict__ kface,
+                                   const int *__restrict__ ij, const double *__restrict__ fm,
+                                   const int *__restrict__ co_ptr, const int *__restrict__ co_idx,
+                                   const double *__restrict__ fm_oc, double *__restrict__ su) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int dp = diagpos[i];
+    double s = 0.0;
+    for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+        const int k = kord ? kord[q] : q;
+        if (k == dp) continue;
+        const int f = kface[k];
+        if (ij[f] == k) s = s - fm[f];
+        else s = s + fm[f];
+    }
+    if (co_ptr) {
+        for (int q = co_ptr[i]; q < co_ptr[i + 1]; ++q) {
+            const int code = co_idx[q];
+            if (code & 1) s = s + fm_oc[code >> 1];
+            else s = s - fm_oc[code >> 1];
+        }
+    }
+    su[i] = s;
+}
+
+// adjustMassFlow (src/adjustMassFlow.f90:29-47): outlet flux den (u.S) from the owner's
+// velocity, the value the boundary velocity is extrapolated to
+__global__ void k_outlet_flux(int nout, int out_face0, const int *__restrict__ owner, const double *__restrict__ arx,
+                              const double *__restrict__ ary, const double *__restrict__ arz,
+                              const double *__restrict__ den, const double *__restrict__ u,
+                              const double *__restrict__ v, const double *__restrict__ w,
+                              double *__restrict__ fmo) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const int f = out_face0 + i, c = owner[f];
+    fmo[i] = den[c] * ((u[c] * arx[f] + v[c] * ary[f]) + w[c] * arz[f]);
+}
+
+// flowo = sum of fmo in outlet order (one wave: 64 values loaded at a time, added in
+// order through lane broadcasts), fac = flomas / (flowo + small)
+__global__ void __launch_bounds__(64) k_outlet_factor(int nout, const double *__restrict__ fmo, double flomas,
+                                                      double *__restrict__ fac) {
+    const int lane = threadIdx.x;
+    double flowo = 0.0;
+    for (int base = 0; base < nout; base += 64) {
+        const double mine = base + lane < nout ? fmo[base + lane] : 0.0;
+        const int cnt = min(64, nout - base);
+        for (int j = 0; j < cnt; ++j) flowo = flowo + __shfl(mine, j, 64);
+    }
+    if (lane == 0) fac[0] = flomas / (flowo + kSmallSrc);
+}
+
+// :52-69 without the source: fmo *= fac, boundary velocities u(ijb) = u(owner) fac
+__global__ void k_outlet_scale(int nout, int out_face0, int out_val0, const int *__restrict__ owner,
+                               const double *__restrict__ fac, double *__restrict__ fmo, double *__restrict__ u,
+                               double *__restrict__ v, double *__restrict__ w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const double k = fac[0];
+    const int c = owner[out_face0 + i], b = out_val0 + i;
+    fmo[i] = fmo[i] * k;
+    u[b] = u[c] * k;
+    v[b] = v[c] * k;
+    w[b] = w[c] * k;
+}
+
+// su(i) -= fmi over the cell's inlet faces in order, then -= fmo over its outlet faces
+__global__ void k_adjust_rows(int n, int ninl, const int *__restrict__ cb_ptr, const int *__restrict__ cb_q,
+                              const double *__restrict__ fmi, const double *__restrict__ fmo,
+                              double *__restrict__ su) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q0 = cb_ptr[i], q1 = cb_ptr[i + 1];
+    if (q0 == q1) return;
+    double s = su[i];
+    for (int q = q0; q < q1; ++q) {
+        const int k = cb_q[q];
+        s = s - (k < ninl ? fmi[k] : fmo[k - ninl]);
+    }
+    su[i] = s;
+}
+
+// flmass += a(ij) (pp(N) - pp(P))
+__global__ void k_correct_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
+                                const int *__restrict__ ij, const double *__restrict__ a,
+                                const double *__restrict__ pp, double *__restrict__ flmass) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nif) return;
+    flmass[f] = flmass[f] + a[ij[f]] * (pp[nei[f]] - pp[own[f]]);
+}
+
+__global__ void k_correct_oc(int noc, const int *__restrict__ ijl, const int *__restrict__ ijr,
+                             const double *__restrict__ ar, const double *__restrict__ pp, double *__restrict__ fmoc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= noc) return;
+    fmoc[i] = fmoc[i] + ar[i] * (pp[ijr[i]] - pp[ijl[i]]);
+}
+
+// u -= dPdx Vol apu (v, w alike); p += urf (pp - pp(pref))
+__global__ void k_correct_cells(int n, const double *__restrict__ pp, const double *__restrict__ dpdx,
+                                const double *__restrict__ vol, const double *__restrict__ apu,
+                                const double *__restrict__ apv, const double *__restrict__ apw, double urf, int pref,
+                                double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                                double *__restrict__ p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ppref = pp[pref];
+    const double *g = dpdx + 3 * (size_t)i;
+    u[i] = u[i] - (g[0] * vol[i]) * apu[i];
+    v[i] = v[i] - (g[1] * vol[i]) * apv[i];
+    w[i] = w[i] - (g[2] * vol[i]) * apw[i];
+    p[i] = p[i] + urf * (pp[i] - ppref);
+}
+
 // --------------------------------------------------------------- Gauss gradient
 
 // item q of the boundary sequence (groups in the reference's order): its face and value index
@@ -800,6 +976,67 @@ void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *ow
     }
     FC_HIP_CHECK(hipGetLastError());
     FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                       const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                       const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
+                       const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
+                       const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
+                       double *fmoc) {
+    hipStream_t s = ctx->stream;
+    DevBuf fm(sizeof(double) * (size_t)std::max(nif, 1)), fmo(sizeof(double) * (size_t)std::max(oc.noc, 1));
+    if (nif)
+        k_nonorth_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, xf, yf, zf, arx, ary, arz, facint, xc,
+                                                         yc, zc, vol, den, apu, dpdx, fm.as<double>(), flmass);
+    if (oc.noc)
+        k_nonorth_oc<<<grid_for(oc.noc), kBlock, 0, s>>>(oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(),
+                                                         oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx, ary,
+                                                         arz, xc, yc, zc, vol, den, apu, dpdx, fmo.as<double>(), fmoc);
+    k_face_source_rows<<<grid_for(st.n), kBlock, 0, s>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order,
+                                                         kface, ij_slot, fm.as<double>(),
+                                                         oc.noc ? oc.co_ptr.as<int>() : nullptr,
+                                                         oc.noc ? oc.co_idx.as<int>() : nullptr, fmo.as<double>(), su);
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void adjust_mass_flow_dev(fc_context *ctx, int n, const int *owner0_all, const double *arx, const double *ary,
+                          const double *arz, int ninl, int inlet_face0, const double *fmi, int nout,
+                          int outlet_face0, int outlet_val0, double flomas, const double *den, double *u, double *v,
+                          double *w, double *su, double *fmo) {
+    hipStream_t s = ctx->stream;
+    GradGroups G{};
+    G.qstart[0] = 0;
+    G.qstart[1] = ninl;
+    for (int g = 2; g < 6; ++g) G.qstart[g] = ninl + nout;
+    G.fstart[0] = inlet_face0;
+    G.fstart[1] = outlet_face0;
+    DevBuf cb_ptr, cb_q, fac(sizeof(double));
+    build_boundary_lists(ctx, G, owner0_all, n, cb_ptr, cb_q);
+    if (nout > 0) {
+        k_outlet_flux<<<grid_for(nout), kBlock, 0, s>>>(nout, outlet_face0, owner0_all, arx, ary, arz, den, u, v, w,
+                                                        fmo);
+        k_outlet_factor<<<1, 64, 0, s>>>(nout, fmo, flomas, fac.as<double>());
+        k_outlet_scale<<<grid_for(nout), kBlock, 0, s>>>(nout, outlet_face0, outlet_val0, owner0_all,
+                                                         fac.as<double>(), fmo, u, v, w);
+    }
+    if (ninl + nout > 0)
+        k_adjust_rows<<<grid_for(n), kBlock, 0, s>>>(n, ninl, cb_ptr.as<int>(), cb_q.as<int>(), fmi, fmo, su);
+    FC_HIP_CHECK(hipGetLastError());
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void calcp_correct_dev(fc_context *ctx, int n, int nif, const int *owner0, const int *neigh0, const int *ij_slot,
+                       const double *a, const double *pp, const double *dpdx, const double *vol, const double *apu,
+                       const double *apv, const double *apw, double urf, int pref0, int noc, const int *ijl,
+                       const int *ijr, const double *ar, double *fmoc, double *flmass, double *u, double *v,
+                       double *w, double *p) {
+    hipStream_t s = ctx->stream;
+    if (nif) k_correct_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, a, pp, flmass);
+    if (noc) k_correct_oc<<<grid_for(noc), kBlock, 0, s>>>(noc, ijl, ijr, ar, pp, fmoc);
+    k_correct_cells<<<grid_for(n), kBlock, 0, s>>>(n, pp, dpdx, vol, apu, apv, apw, urf, pref0, u, v, w, p);
+    FC_HIP_CHECK(hipGetLastError());
 }
 
 void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,

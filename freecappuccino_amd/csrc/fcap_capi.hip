@@ -524,6 +524,142 @@ extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInn
     });
 }
 
+extern "C" fc_status fc_calcp_nonorth(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                     const int *neighbour, const double *xc, const double *yc, const double *zc,
+                                     const double *vol, const double *xf, const double *yf, const double *zf,
+                                     const double *arx, const double *ary, const double *arz, const double *facint,
+                                     const double *den, const double *apu, const double *dPdxi, int noc,
+                                     int iOCFacesStart, const int *ijl, const int *ijr, const double *foc,
+                                     const int *ioffset, const int *ja, const int *diag, const int *ij_slot,
+                                     const int *ji_slot, int nnz, double *su, double *flmass, double *fmoc) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n, FC_ERR_ARGUMENT, "nnz must be 2*numInnerFaces+numCells");
+        FC_REQUIRE(su && (flmass || nif == 0), FC_ERR_ARGUMENT, "su/flmass");
+        const double *cf[6] = {xc, yc, zc, vol, den, apu};
+        for (const double *c : cf) FC_REQUIRE(c != nullptr, FC_ERR_ARGUMENT, "cell fields");
+        FC_REQUIRE(dPdxi != nullptr, FC_ERR_ARGUMENT, "dPdxi");
+        const int nfaces_needed = std::max(nif, noc > 0 ? iOCFacesStart + noc : 0);
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        FaceSet fs;
+        upload_faces(ctx, fs, st, nif, owner, neighbour, ij_slot, ji_slot);
+        DevBuf dx[6], dc[6], dg, dfi, dsu(sizeof(double) * n), dfl, dfm;
+        const double *fa[6] = {arx, ary, arz, xf, yf, zf};
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nfaces_needed);
+        for (int k = 0; k < 6; ++k) upload_dbl(ctx, dc[k], cf[k], n);
+        upload_dbl(ctx, dg, dPdxi, 3 * (size_t)n);
+        upload_dbl(ctx, dfi, facint, nif);
+        upload_dbl(ctx, dfl, flmass, nif);
+        OcDev oc;
+        oc.noc = std::max(noc, 0);
+        oc.ioc_faces_start = iOCFacesStart;
+        if (noc > 0) {
+            FC_REQUIRE(ijl && ijr && foc && fmoc, FC_ERR_ARGUMENT, "O-C arrays");
+            upload_idx(ctx, oc.ijl, ijl, noc);
+            upload_idx(ctx, oc.ijr, ijr, noc);
+            upload_dbl(ctx, oc.foc, foc, noc);
+            upload_dbl(ctx, dfm, fmoc, noc);
+            build_oc_lists(ctx, oc, n);
+        }
+        calcp_nonorth_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.kface.as<int>(),
+                          fs.needs_order ? fs.kord.as<int>() : nullptr, dx[3].as<double>(), dx[4].as<double>(),
+                          dx[5].as<double>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(),
+                          dfi.as<double>(), dc[0].as<double>(), dc[1].as<double>(), dc[2].as<double>(),
+                          dc[3].as<double>(), dc[4].as<double>(), dc[5].as<double>(), dg.as<double>(), oc,
+                          dsu.as<double>(), dfl.as<double>(), noc > 0 ? dfm.as<double>() : nullptr);
+        download_dbl(ctx, su, dsu.get(), n);
+        if (nif) download_dbl(ctx, flmass, dfl.get(), nif);
+        if (noc > 0) download_dbl(ctx, fmoc, dfm.get(), noc);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_adjust_mass_flow(fc_context *ctx, int numCells, int numTotal, int numFaces,
+                                        const int *owner, const double *arx, const double *ary, const double *arz,
+                                        int ninl, int iInletFacesStart, const double *fmi, int nout,
+                                        int iOutletFacesStart, int iOutletStart, double flomas, const double *den,
+                                        double *u, double *v, double *w, double *su, double *fmo) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nf = numFaces;
+        FC_REQUIRE(n > 0 && numTotal >= n && nf >= 0 && ninl >= 0 && nout >= 0, FC_ERR_ARGUMENT, "sizes");
+        FC_REQUIRE(ninl == 0 || (iInletFacesStart >= 0 && iInletFacesStart + ninl <= nf), FC_ERR_ARGUMENT,
+                   "inlet faces outside the faces");
+        FC_REQUIRE(nout == 0 || (iOutletFacesStart >= 0 && iOutletFacesStart + nout <= nf), FC_ERR_ARGUMENT,
+                   "outlet faces outside the faces");
+        FC_REQUIRE(nout == 0 || (iOutletStart >= n && iOutletStart + nout <= numTotal), FC_ERR_ARGUMENT,
+                   "outlet values outside numCells+1..numTotal");
+        FC_REQUIRE(owner && arx && ary && arz && den && u && v && w && su && (fmi || ninl == 0) && (fmo || nout == 0),
+                   FC_ERR_ARGUMENT, "arguments");
+        DevBuf own, dx[3], dden, duvw[3], dsu, dfmi, dfmo(sizeof(double) * (size_t)std::max(nout, 1));
+        upload_idx(ctx, own, owner, nf);
+        const double *fa[3] = {arx, ary, arz};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dx[k], fa[k], nf);
+        upload_dbl(ctx, dden, den, n);
+        double *vel[3] = {u, v, w};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, duvw[k], vel[k], numTotal);
+        upload_dbl(ctx, dsu, su, n);
+        upload_dbl(ctx, dfmi, fmi, ninl);
+        adjust_mass_flow_dev(ctx, n, own.as<int>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), ninl,
+                             iInletFacesStart, dfmi.as<double>(), nout, iOutletFacesStart, iOutletStart,
+                             flomas, dden.as<double>(), duvw[0].as<double>(), duvw[1].as<double>(),
+                             duvw[2].as<double>(), dsu.as<double>(), dfmo.as<double>());
+        download_dbl(ctx, su, dsu.get(), n);
+        for (int k = 0; k < 3; ++k) download_dbl(ctx, vel[k], duvw[k].get(), numTotal);
+        download_dbl(ctx, fmo, dfmo.get(), nout);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_calcp_correct(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                     const int *neighbour, const int *ioffset, const int *ja, const int *diag,
+                                     const int *ij_slot, const int *ji_slot, int nnz, const double *a,
+                                     const double *pp, const double *dPdxi, const double *vol, const double *apu,
+                                     const double *apv, const double *apw, double urf, int pRefCell, int noc,
+                                     const int *ijl, const int *ijr, const double *ar, double *fmoc,
+                                     double *flmass, double *u, double *v, double *w, double *p) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nif = numInnerFaces;
+        FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n, FC_ERR_ARGUMENT, "nnz must be 2*numInnerFaces+numCells");
+        FC_REQUIRE(pRefCell >= 1 && pRefCell <= n, FC_ERR_ARGUMENT, "pRefCell outside 1..numCells");
+        FC_REQUIRE(a && pp && dPdxi && vol && apu && apv && apw && u && v && w && p && (flmass || nif == 0),
+                   FC_ERR_ARGUMENT, "arguments");
+        Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
+        FaceSet fs;
+        upload_faces(ctx, fs, st, nif, owner, neighbour, ij_slot, ji_slot);
+        DevBuf da, dpp, dg, dvol, dap[3], duvwp[4], dfl, dar, dfm;
+        upload_dbl(ctx, da, a, nnz);
+        upload_dbl(ctx, dpp, pp, n);
+        upload_dbl(ctx, dg, dPdxi, 3 * (size_t)n);
+        upload_dbl(ctx, dvol, vol, n);
+        const double *aps[3] = {apu, apv, apw};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dap[k], aps[k], n);
+        double *flds[4] = {u, v, w, p};
+        for (int k = 0; k < 4; ++k) upload_dbl(ctx, duvwp[k], flds[k], n);
+        upload_dbl(ctx, dfl, flmass, nif);
+        OcDev oc;
+        oc.noc = std::max(noc, 0);
+        if (noc > 0) {
+            FC_REQUIRE(ijl && ijr && ar && fmoc, FC_ERR_ARGUMENT, "O-C arrays");
+            upload_idx(ctx, oc.ijl, ijl, noc);
+            upload_idx(ctx, oc.ijr, ijr, noc);
+            build_oc_lists(ctx, oc, n);   // range check of ijl/ijr
+            upload_dbl(ctx, dar, ar, noc);
+            upload_dbl(ctx, dfm, fmoc, noc);
+        }
+        calcp_correct_dev(ctx, n, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), da.as<double>(),
+                          dpp.as<double>(), dg.as<double>(), dvol.as<double>(), dap[0].as<double>(),
+                          dap[1].as<double>(), dap[2].as<double>(), urf, pRefCell - 1, oc.noc,
+                          noc > 0 ? oc.ijl.as<int>() : nullptr, noc > 0 ? oc.ijr.as<int>() : nullptr,
+                          noc > 0 ? dar.as<double>() : nullptr, noc > 0 ? dfm.as<double>() : nullptr,
+                          dfl.as<double>(), duvwp[0].as<double>(), duvwp[1].as<double>(), duvwp[2].as<double>(),
+                          duvwp[3].as<double>());
+        if (nif) download_dbl(ctx, flmass, dfl.get(), nif);
+        if (noc > 0) download_dbl(ctx, fmoc, dfm.get(), noc);
+        for (int k = 0; k < 4; ++k) download_dbl(ctx, flds[k], duvwp[k].get(), n);
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_grad_gauss(fc_context *ctx, int numCells, int numFaces, int numInnerFaces, const int *owner,
                                    const int *neighbour, const double *arx, const double *ary, const double *arz,
                                    const double *facint, const double *xc, const double *yc, const double *zc,

@@ -219,6 +219,24 @@ struct RhieChowDev {
     const double *xc, *yc, *zc, *vol, *den, *apu, *apv, *apw, *u, *v, *w, *p;
     const double *dudx, *dvdx, *dwdx, *dpdx;
 };
+// calcp's non-orthogonal corrector source (su overwritten, flmass and fmoc += fmcor)
+void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
+                       const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                       const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
+                       const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
+                       const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
+                       double *fmoc);
+// adjustMassFlow: inlet sources, outlet fluxes scaled to flomas (0-based faces and values)
+void adjust_mass_flow_dev(fc_context *ctx, int n, const int *owner0_all, const double *arx, const double *ary,
+                          const double *arz, int ninl, int inlet_face0, const double *fmi, int nout,
+                          int outlet_face0, int outlet_val0, double flomas, const double *den, double *u, double *v,
+                          double *w, double *su, double *fmo);
+// calcp's flux, velocity and pressure corrections (pref0: 0-based reference cell)
+void calcp_correct_dev(fc_context *ctx, int n, int nif, const int *owner0, const int *neigh0, const int *ij_slot,
+                       const double *a, const double *pp, const double *dpdx, const double *vol, const double *apu,
+                       const double *apv, const double *apw, double urf, int pref0, int noc, const int *ijl,
+                       const int *ijr, const double *ar, double *fmoc, double *flmass, double *u, double *v,
+                       double *w, double *p);
 // Boundary groups of the Gauss gradient in the reference's order (inlet, outlet,
 // symmetry, wall, pressure outlet): item q of the sequence over all groups is
 // face fstart[g] + (q - qstart[g]) with its value at vstart[g] + (q - qstart[g])

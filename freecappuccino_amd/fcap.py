@@ -82,6 +82,11 @@ def library() -> C.CDLL:
         "fc_grad_gauss": ([vp, i, i, i, ip, ip] + [dp] * 12 + [i, ip, i, ip, ip, ip, dp, i, dp, dp, dp], i),
         "fc_calcp_assemble": ([vp, i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
                               + [dp] * 6, i),
+        "fc_calcp_nonorth": ([vp, i, i, ip, ip] + [dp] * 14 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
+                             + [dp] * 3, i),
+        "fc_adjust_mass_flow": ([vp, i, i, i, ip, dp, dp, dp, i, i, dp, i, i, i, d] + [dp] * 6, i),
+        "fc_calcp_correct": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip, i] + [dp] * 7 + [d, i, i, ip, ip]
+                             + [dp] * 7, i),
         "fc_rhs_divergence": ([vp, i, i, ip, ip, dp, dp], i),
         "fc_spmv": ([vp, i, i, ip, ip, dp, dp, dp], i),
         "fc_dic_pivots": ([vp, i, i, ip, ip, ip, dp, dp], i),
@@ -394,6 +399,83 @@ def calcp_assemble(mesh, geo: Geometry, csr: CSR, fields, oc=None, ctx: Optional
     for k in ("al", "ar", "fmoc"):
         out[k] = out[k][:noc]
     return out
+
+
+def _oc_host(oc):
+    noc = 0 if oc is None else len(oc["ijl"])
+    ijl = _i32(np.asarray(oc["ijl"]) + 1) if noc else np.zeros(1, np.int32)
+    ijr = _i32(np.asarray(oc["ijr"]) + 1) if noc else np.zeros(1, np.int32)
+    foc = _f64(oc["foc"]) if noc else np.zeros(1)
+    return noc, (int(oc["faces"]) if noc else 0), ijl, ijr, foc
+
+
+def calcp_nonorth(mesh, geo: Geometry, csr: CSR, den, apu, dpdx, flmass, oc=None, fmoc=None,
+                  ctx: Optional[Context] = None):
+    """Non-orthogonal corrector source of calcp on the GPU
+    (src/calcp-multiple_correction_SIMPLE.f90:185-229, fluxmc src/facefluxmass.f90:520-607).
+    dpdx: gradient of the pressure correction (n x 3).  Returns (su, flmass + fmcor,
+    fmoc + fmcor); su starts from zero."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    noc, oc0, ijl, ijr, foc = _oc_host(oc)
+    su = np.zeros(n)
+    fl = np.zeros(max(nif, 1))
+    fl[:nif] = _f64(flmass)[:nif]
+    fm = _f64(fmoc).copy() if noc else np.zeros(1)
+    g = [_f64(getattr(geo, k)) for k in ("xc", "yc", "zc", "vol", "xf", "yf", "zf", "arx", "ary", "arz", "facint")]
+    ctx.check(ctx.L.fc_calcp_nonorth(ctx.h, n, nif, _ip(own1), _ip(nei1), *[_dp(x) for x in g], _dp(_f64(den)),
+                                     _dp(_f64(apu)), _dp(_f64(np.asarray(dpdx).reshape(n, 3))), noc, oc0, _ip(ijl),
+                                     _ip(ijr), _dp(foc), _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                     _ip(_i32(csr.ij_slot)), _ip(_i32(csr.ji_slot)), mesh.nnz, _dp(su), _dp(fl),
+                                     _dp(fm)))
+    return su, fl[:nif], fm[:noc]
+
+
+def adjust_mass_flow(mesh, geo: Geometry, flomas, den, u, v, w, su, fmi, ctx: Optional[Context] = None):
+    """adjustMassFlow on the GPU (src/adjustMassFlow.f90:1-71): inlet fluxes fmi into su, outlet
+    fluxes from the owner cells scaled so that they carry flomas.  u, v, w hold the cells then the
+    boundary values in the reference's group order.  Returns (su, u, v, w, fmo)."""
+    ctx = ctx or default_context()
+    n = mesh.n_cells
+    bnd = boundary_groups(mesh)
+    ninl, inl0, nout, out0 = (int(x) for x in bnd[:4])
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    vel = [_f64(x).copy() for x in (u, v, w)]
+    ntot = vel[0].shape[0]
+    s = _f64(su).copy()
+    fmo = np.zeros(max(nout, 1))
+    fmi_ = _f64(fmi) if ninl else np.zeros(1)
+    ctx.check(ctx.L.fc_adjust_mass_flow(ctx.h, n, ntot, mesh.n_faces, _ip(own1), _dp(_f64(geo.arx)),
+                                        _dp(_f64(geo.ary)), _dp(_f64(geo.arz)), ninl, inl0, _dp(fmi_), nout, out0,
+                                        n + ninl, float(flomas), _dp(_f64(den)), *[_dp(x) for x in vel], _dp(s),
+                                        _dp(fmo)))
+    return (s, *vel, fmo[:nout])
+
+
+def calcp_correct(mesh, csr: CSR, a, pp, dpdx, vol, apu, apv, apw, urf, pref_cell, flmass, u, v, w, p,
+                  oc=None, ar=None, fmoc=None, ctx: Optional[Context] = None):
+    """Corrections of calcp on the GPU (src/calcp-multiple_correction_SIMPLE.f90:145-183): face mass
+    fluxes by a(ij) (pp_N - pp_P), O-C fluxes by ar (pp_r - pp_l), u, v, w by -grad(pp) Vol/a_P and
+    p by urf (pp - pp[pref_cell]); pref_cell is 0-based.  Returns (flmass, fmoc, u, v, w, p)."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    noc, _, ijl, ijr, _ = _oc_host(oc)
+    ar_ = _f64(ar) if noc else np.zeros(1)
+    fm = _f64(fmoc).copy() if noc else np.zeros(1)
+    fl = np.zeros(max(nif, 1))
+    fl[:nif] = _f64(flmass)[:nif]
+    cells = [_f64(x).copy() for x in (u, v, w, p)]
+    ctx.check(ctx.L.fc_calcp_correct(ctx.h, n, nif, _ip(own1), _ip(nei1), _ip(csr.ioffset), _ip(csr.ja),
+                                     _ip(csr.diag), _ip(_i32(csr.ij_slot)), _ip(_i32(csr.ji_slot)), mesh.nnz,
+                                     _dp(_f64(a)), _dp(_f64(pp)), _dp(_f64(np.asarray(dpdx).reshape(n, 3))),
+                                     _dp(_f64(vol)), _dp(_f64(apu)), _dp(_f64(apv)), _dp(_f64(apw)), float(urf),
+                                     int(pref_cell) + 1, noc, _ip(ijl), _ip(ijr), _dp(ar_), _dp(fm), _dp(fl),
+                                     *[_dp(x) for x in cells]))
+    return (fl[:nif], fm[:noc], *cells)
 
 
 def face_flux_central(mesh, geo: Geometry, u, v, w, ctx: Optional[Context] = None):

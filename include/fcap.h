@@ -149,6 +149,50 @@ fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, co
                             const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
                             double *su, double *flmass, double *al, double *ar, double *fmoc);
 
+/* Non-orthogonal corrector source of calcp
+ * (src/calcp-multiple_correction_SIMPLE.f90:185-229) with fluxmc
+ * (src/facefluxmass.f90:520-607): per inner face, then per O-C cut,
+ * fmcor = rho_f (Vol/a_P)_f |S| ((grad p_N . r_N') - (grad p_P . r_P')) / (d_PN . n),
+ * r' the offset of the cell centre's projection onto the face normal line;
+ * flmass (fmoc) += fmcor, and su, overwritten with zero first, gets
+ * su(P) -= fmcor, su(N) += fmcor in face order.  dPdxi is the gradient of the
+ * pressure correction (3, numCells). */
+fc_status fc_calcp_nonorth(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                           const int *neighbour, const double *xc, const double *yc, const double *zc,
+                           const double *vol, const double *xf, const double *yf, const double *zf,
+                           const double *arx, const double *ary, const double *arz, const double *facint,
+                           const double *den, const double *apu, const double *dPdxi, int noc,
+                           int iOCFacesStart, const int *ijl, const int *ijr, const double *foc,
+                           const int *ioffset, const int *ja, const int *diag, const int *ij_slot,
+                           const int *ji_slot, int nnz, double *su, double *flmass, double *fmoc);
+
+/* adjustMassFlow (src/adjustMassFlow.f90:1-71), called by calcp between the
+ * face loops and the solve (src/calcp-multiple_correction_SIMPLE.f90:107):
+ * su(owner) -= fmi over the inlet faces iInletFacesStart+1..+ninl; over the
+ * outlet faces iOutletFacesStart+1..+nout, fmo = den (u.S) of the owner cell,
+ * flowo = their sum in outlet order, fac = flomas / (flowo + small); then
+ * fmo *= fac, the outlet boundary values u(iOutletStart+i) = u(owner) fac
+ * (v and w alike) and su(owner) -= fmo.  owner and the face areas cover
+ * numFaces; u, v and w hold numTotal values. */
+fc_status fc_adjust_mass_flow(fc_context *ctx, int numCells, int numTotal, int numFaces, const int *owner,
+                              const double *arx, const double *ary, const double *arz, int ninl,
+                              int iInletFacesStart, const double *fmi, int nout, int iOutletFacesStart,
+                              int iOutletStart, double flomas, const double *den, double *u, double *v,
+                              double *w, double *su, double *fmo);
+
+/* Corrections after a pressure-correction solve
+ * (src/calcp-multiple_correction_SIMPLE.f90:145-183): ppref = pp(pRefCell);
+ * flmass += a(ij) (pp(N) - pp(P)) per inner face; fmoc += ar (pp(ijr) - pp(ijl))
+ * per O-C cut; u -= dPdxi(1) Vol apu, v and w alike; p += urf (pp - ppref).
+ * a is the assembled pressure-correction matrix, dPdxi the gradient of pp. */
+fc_status fc_calcp_correct(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                           const int *neighbour, const int *ioffset, const int *ja, const int *diag,
+                           const int *ij_slot, const int *ji_slot, int nnz, const double *a,
+                           const double *pp, const double *dPdxi, const double *vol, const double *apu,
+                           const double *apv, const double *apw, double urf, int pRefCell, int noc,
+                           const int *ijl, const int *ijr, const double *ar, double *fmoc, double *flmass,
+                           double *u, double *v, double *w, double *p);
+
 /* Gauss gradient (src/grad_gauss.f90:1-124 with gradco :128-190 and gradbc
  * :194-211): nigrad iterations, each summing per cell the inner faces' value
  * times S (the value at the face centre extrapolated with the previous

@@ -22,6 +22,15 @@
  *                         evaluation and the divergence-theorem invariant
  *                         (sum of su over cells equals the net boundary flux,
  *                         zero for inner-face-only fluxes).
+ *   orc_calcp_nonorth  -- fmcor bitwise vs the reference's fluxmc (the flux
+ *                         harness); zero on an orthogonal box.
+ *   orc_calcp_correct  -- closed forms: with pp linear and its exact gradient on
+ *                         a uniform box the flux change is a_ij (pp_N - pp_P), the
+ *                         velocity change -grad(pp) Vol ap, p shifts by urf (pp - pp_ref).
+ *   orc_adjust_mass_flow -- bitwise vs the reference's adjustMassFlow compiled on
+ *                         its own (oracle/_ref/ref_adjust_driver) on a hex box and a
+ *                         tet box with inlet and outlet patches; global conservation
+ *                         (outlet mass flow = flomas) to rounding.
  *   orc_grad_gauss     -- bitwise vs the reference's grad_gauss compiled on its
  *                         own (oracle/_ref/ref_grad_driver) on a box with every
  *                         boundary group and on permuted tets, 1 and 2
@@ -904,4 +913,125 @@ int orc_grad_gauss(int n, int nif, const int *owner1, const int *neighbour1,
     }
     free(o);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Corrector stage of calcp (src/calcp-multiple_correction_SIMPLE.f90:145-231):
+ * fluxmc (src/facefluxmass.f90:520-607) and the flux, velocity and pressure
+ * corrections.                                                               */
+/* ------------------------------------------------------------------------ */
+
+static double fluxmc(int ijp, int ijn, double xf, double yf, double zf, double arx, double ary, double arz,
+                     double lambda, const double *xc, const double *yc, const double *zc, const double *vol,
+                     const double *den, const double *apu, const double *dpdx)
+{
+    double fxn = lambda;
+    double fxp = 1.0 - lambda;
+    double xpn = I1(xc, ijn) - I1(xc, ijp);
+    double ypn = I1(yc, ijn) - I1(yc, ijp);
+    double zpn = I1(zc, ijn) - I1(zc, ijp);
+    double are = sqrt(arx * arx + ary * ary + arz * arz);
+    double nxx = arx / are, nyy = ary / are, nzz = arz / are;
+    double dppnnr = 1.0 / ((xpn * nxx) + (ypn * nyy) + (zpn * nzz));
+    double xpp = xf - (xf - I1(xc, ijp)) * nxx, ypp = yf - (yf - I1(yc, ijp)) * nyy, zpp = zf - (zf - I1(zc, ijp)) * nzz;
+    double xep = xf - (xf - I1(xc, ijn)) * nxx, yep = yf - (yf - I1(yc, ijn)) * nyy, zep = zf - (zf - I1(zc, ijn)) * nzz;
+    xpp = xpp - I1(xc, ijp);
+    ypp = ypp - I1(yc, ijp);
+    zpp = zpp - I1(zc, ijp);
+    xep = xep - I1(xc, ijn);
+    yep = yep - I1(yc, ijn);
+    zep = zep - I1(zc, ijn);
+    double rapr = I1(apu, ijp) * I1(den, ijp) * I1(vol, ijp) * fxp + I1(apu, ijn) * I1(den, ijn) * I1(vol, ijn) * fxn;
+    return rapr * are * ((G1(dpdx, ijn, 0) * xep - G1(dpdx, ijp, 0) * xpp) + (G1(dpdx, ijn, 1) * yep - G1(dpdx, ijp, 1) * ypp) +
+                         (G1(dpdx, ijn, 2) * zep - G1(dpdx, ijp, 2) * zpp)) * dppnnr;
+}
+
+/* the non-orthogonal corrector's source (:190-229): su = 0, then per inner face and
+ * per O-C cut flmass/fmoc += fmcor, su(P) -= fmcor, su(N) += fmcor */
+void orc_calcp_nonorth(int n, int nif, const int *owner1, const int *neighbour1,
+                       const double *xc, const double *yc, const double *zc, const double *vol,
+                       const double *xf, const double *yf, const double *zf,
+                       const double *arx, const double *ary, const double *arz, const double *facint,
+                       const double *den, const double *apu, const double *dpdx,
+                       int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                       double *su, double *flmass, double *fmoc, double *fmcor_out)
+{
+    memset(su, 0, sizeof(double) * n);
+    for (int i = 0; i < nif; ++i) {
+        int ijp = owner1[i], ijn = neighbour1[i];
+        double fm = fluxmc(ijp, ijn, xf[i], yf[i], zf[i], arx[i], ary[i], arz[i], facint[i], xc, yc, zc, vol, den,
+                           apu, dpdx);
+        if (fmcor_out) fmcor_out[i] = fm;
+        flmass[i] = flmass[i] + fm;
+        I1(su, ijp) = I1(su, ijp) - fm;
+        I1(su, ijn) = I1(su, ijn) + fm;
+    }
+    for (int i = 0; i < noc; ++i) {
+        int iface = ioc_faces_start + i;
+        int ijp = ijl1[i], ijn = ijr1[i];
+        double fm = fluxmc(ijp, ijn, xf[iface], yf[iface], zf[iface], arx[iface], ary[iface], arz[iface], foc[i],
+                           xc, yc, zc, vol, den, apu, dpdx);
+        fmoc[i] = fmoc[i] + fm;
+        I1(su, ijp) = I1(su, ijp) - fm;
+        I1(su, ijn) = I1(su, ijn) + fm;
+    }
+}
+
+/* adjustMassFlow (src/adjustMassFlow.f90:1-71): su(owner) -= fmi over the inlet faces; outlet
+ * boundary velocities extrapolated from the owner cells, fmo = den (u.S) summed in outlet order
+ * into flowo, scaled by fac = flomas / (flowo + small) together with the outlet velocities, and
+ * su(owner) -= fmo.  Faces and boundary values are 1-based positions after the given starts. */
+void orc_adjust_mass_flow(const int *owner1, const double *arx, const double *ary, const double *arz,
+                          int ninl, int inlet_faces_start, const double *fmi, int nout,
+                          int outlet_faces_start, int outlet_start, double flomas, const double *den,
+                          double *u, double *v, double *w, double *su, double *fmo)
+{
+    for (int i = 1; i <= ninl; ++i) {
+        int ijp = owner1[inlet_faces_start + i - 1];
+        I1(su, ijp) = I1(su, ijp) - I1(fmi, i);
+    }
+    double flowo = 0.0;
+    for (int i = 1; i <= nout; ++i) {
+        int iface = outlet_faces_start + i;
+        int ijp = I1(owner1, iface);
+        int ijb = outlet_start + i;
+        I1(u, ijb) = I1(u, ijp);
+        I1(v, ijb) = I1(v, ijp);
+        I1(w, ijb) = I1(w, ijp);
+        I1(fmo, i) = I1(den, ijp) * (I1(u, ijb) * I1(arx, iface) + I1(v, ijb) * I1(ary, iface) + I1(w, ijb) * I1(arz, iface));
+        flowo = flowo + I1(fmo, i);
+    }
+    double fac = flomas / (flowo + orc_small());
+    for (int i = 1; i <= nout; ++i) {
+        int iface = outlet_faces_start + i;
+        int ijp = I1(owner1, iface);
+        int ijb = outlet_start + i;
+        I1(fmo, i) = I1(fmo, i) * fac;
+        I1(u, ijb) = I1(u, ijb) * fac;
+        I1(v, ijb) = I1(v, ijb) * fac;
+        I1(w, ijb) = I1(w, ijb) * fac;
+        I1(su, ijp) = I1(su, ijp) - I1(fmo, i);
+    }
+}
+
+/* the corrections (:145-183): ppref = pp(pRefCell); flmass += a(ij) (pp(N) - pp(P));
+ * fmoc += ar (pp(ijr) - pp(ijl)); u -= dPdx Vol apu (v, w alike); p += urf (pp - ppref) */
+void orc_calcp_correct(int n, int nif, const int *owner1, const int *neighbour1, const int *ij_slot,
+                       const double *a, const double *pp, const double *dpdx, const double *vol,
+                       const double *apu, const double *apv, const double *apw, double urf, int pref_cell,
+                       int noc, const int *ijl1, const int *ijr1, const double *ar, double *fmoc,
+                       double *flmass, double *u, double *v, double *w, double *p)
+{
+    double ppref = I1(pp, pref_cell);
+    for (int i = 0; i < nif; ++i) {
+        int ijp = owner1[i], ijn = neighbour1[i];
+        flmass[i] = flmass[i] + I1(a, ij_slot[i]) * (I1(pp, ijn) - I1(pp, ijp));
+    }
+    for (int i = 0; i < noc; ++i) fmoc[i] = fmoc[i] + ar[i] * (I1(pp, ijr1[i]) - I1(pp, ijl1[i]));
+    for (int i = 1; i <= n; ++i) {
+        I1(u, i) = I1(u, i) - G1(dpdx, i, 0) * I1(vol, i) * I1(apu, i);
+        I1(v, i) = I1(v, i) - G1(dpdx, i, 1) * I1(vol, i) * I1(apv, i);
+        I1(w, i) = I1(w, i) - G1(dpdx, i, 2) * I1(vol, i) * I1(apw, i);
+        I1(p, i) = I1(p, i) + urf * (I1(pp, i) - ppref);
+    }
 }

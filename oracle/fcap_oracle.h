@@ -101,6 +101,29 @@ int orc_grad_gauss(int n, int nif, const int *owner1, const int *neighbour1,
                    const double *u, const int *bnd, int noc, const int *ijlface1, const int *ijl1,
                    const int *ijr1, const double *foc, int nigrad, double *dudx, double *dudy, double *dudz);
 
+/* Non-orthogonal corrector source of calcp (src/calcp-multiple_correction_SIMPLE.f90:190-229)
+ * with fluxmc (src/facefluxmass.f90:520-607): su = 0; per inner face and O-C cut
+ * flmass/fmoc += fmcor, su(P) -= fmcor, su(N) += fmcor.  fmcor_out (nif) may be NULL. */
+void orc_calcp_nonorth(int n, int nif, const int *owner1, const int *neighbour1,
+                       const double *xc, const double *yc, const double *zc, const double *vol,
+                       const double *xf, const double *yf, const double *zf,
+                       const double *arx, const double *ary, const double *arz, const double *facint,
+                       const double *den, const double *apu, const double *dpdx,
+                       int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                       double *su, double *flmass, double *fmoc, double *fmcor_out);
+
+/* adjustMassFlow (src/adjustMassFlow.f90:1-71); owner1 holds every face's owner (1-based). */
+void orc_adjust_mass_flow(const int *owner1, const double *arx, const double *ary, const double *arz,
+                          int ninl, int inlet_faces_start, const double *fmi, int nout,
+                          int outlet_faces_start, int outlet_start, double flomas, const double *den,
+                          double *u, double *v, double *w, double *su, double *fmo);
+/* Corrections of calcp (src/calcp-multiple_correction_SIMPLE.f90:145-183). */
+void orc_calcp_correct(int n, int nif, const int *owner1, const int *neighbour1, const int *ij_slot,
+                       const double *a, const double *pp, const double *dpdx, const double *vol,
+                       const double *apu, const double *apv, const double *apw, double urf, int pref_cell,
+                       int noc, const int *ijl1, const int *ijr1, const double *ar, double *fmoc,
+                       double *flmass, double *u, double *v, double *w, double *p);
+
 /* Momentum component rows of calcuvw (src/calcuvw.f90:417-437): diagonal reset to zero, then
  * a(diag) = (sp - sum of the row's off-diagonals) / urf, su = src + (1 - urf) a(diag) phi,
  * ap = 1/(a(diag) + small).  a is updated in place (its diagonal slots). */

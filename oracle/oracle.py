@@ -53,6 +53,10 @@ def lib():
         _lib.orc_rhs_divergence.argtypes = [i, ip, ip, dp, dp]
         _lib.orc_momentum_rows.argtypes = [i, ip, ip, dp, dp, dp, dp, d, dp, dp]
         _lib.orc_grad_gauss.argtypes = [i, i, ip, ip] + [dp] * 12 + [ip, i, ip, ip, ip, dp, i, dp, dp, dp]
+        _lib.orc_calcp_nonorth.argtypes = [i, i, ip, ip] + [dp] * 14 + [i, i, ip, ip, dp, dp, dp, dp, dp]
+        _lib.orc_adjust_mass_flow.argtypes = [ip, dp, dp, dp, i, i, dp, i, i, i, d, dp, dp, dp, dp, dp, dp]
+        _lib.orc_calcp_correct.argtypes = ([i, i, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp, d, i, i, ip, ip, dp, dp]
+                                           + [dp] * 5)
         _lib.orc_calcp_assemble.argtypes = ([i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, i]
                                             + [dp] * 7)
         _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
@@ -198,6 +202,70 @@ def face_flux_central(mesh, g: Geometry, u, v, w):
     lib().orc_face_flux_central(nif, _ip(own1), _ip(nei1), _dp(g.facint), _dp(g.arx), _dp(g.ary),
                                 _dp(g.arz), _dp(_f64(u)), _dp(_f64(v)), _dp(_f64(w)), _dp(flux))
     return flux
+
+
+def _oc_args(oc):
+    noc = 0 if oc is None else len(oc["ijl"])
+    if not noc:
+        z = np.zeros(1, np.int32)
+        return 0, 0, z, z, np.zeros(1)
+    return (noc, int(oc["faces"]), _i32(np.asarray(oc["ijl"]) + 1), _i32(np.asarray(oc["ijr"]) + 1),
+            _f64(oc["foc"]))
+
+
+def calcp_nonorth(mesh, g: Geometry, den, apu, dpdx, flmass, oc=None, fmoc=None):
+    """The non-orthogonal corrector of calcp (src/calcp-multiple_correction_SIMPLE.f90:190-229,
+    fluxmc src/facefluxmass.f90:520-607): returns (su, flmass + fmcor, fmoc + fmcor, fmcor)."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    noc, oc0, ijl, ijr, foc = _oc_args(oc)
+    su = np.zeros(n)
+    fl = _f64(flmass).copy()
+    fm = np.zeros(max(noc, 1)) if fmoc is None else _f64(fmoc).copy()
+    fmcor = np.zeros(max(nif, 1))
+    lib().orc_calcp_nonorth(n, nif, _ip(own1), _ip(nei1), _dp(_f64(g.xc)), _dp(_f64(g.yc)), _dp(_f64(g.zc)),
+                            _dp(_f64(g.vol)), _dp(_f64(g.xf)), _dp(_f64(g.yf)), _dp(_f64(g.zf)), _dp(_f64(g.arx)),
+                            _dp(_f64(g.ary)), _dp(_f64(g.arz)), _dp(_f64(g.facint)), _dp(_f64(den)), _dp(_f64(apu)),
+                            _dp(_f64(np.asarray(dpdx).reshape(n, 3))), noc, oc0, _ip(ijl), _ip(ijr), _dp(foc),
+                            _dp(su), _dp(fl), _dp(fm), _dp(fmcor))
+    return su, fl, fm[:noc], fmcor[:nif]
+
+
+def calcp_correct(mesh, csr: CSR, a, pp, dpdx, vol, apu, apv, apw, urf, pref_cell, flmass, u, v, w, p,
+                  oc=None, ar=None, fmoc=None):
+    """The corrections of calcp (src/calcp-multiple_correction_SIMPLE.f90:145-183); pref_cell is
+    0-based.  Returns (flmass, fmoc, u, v, w, p) corrected."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    noc, _, ijl, ijr, _ = _oc_args(oc)
+    ar_ = _f64(ar) if noc else np.zeros(1)
+    fm = _f64(fmoc).copy() if noc else np.zeros(1)
+    out = [_f64(x).copy() for x in (flmass, u, v, w, p)]
+    lib().orc_calcp_correct(n, nif, _ip(own1), _ip(nei1), _ip(_i32(csr.ij_slot)), _dp(_f64(a)), _dp(_f64(pp)),
+                            _dp(_f64(np.asarray(dpdx).reshape(n, 3))), _dp(_f64(vol)), _dp(_f64(apu)),
+                            _dp(_f64(apv)), _dp(_f64(apw)), float(urf), int(pref_cell) + 1, noc, _ip(ijl), _ip(ijr),
+                            _dp(ar_), _dp(fm), *[_dp(x) for x in out])
+    fl, u_, v_, w_, p_ = out
+    return fl, fm[:noc], u_, v_, w_, p_
+
+
+def adjust_mass_flow(mesh, g: Geometry, flomas, den, u, v, w, su, fmi):
+    """adjustMassFlow (src/adjustMassFlow.f90:1-71): returns (su, u, v, w, fmo); u, v, w hold the
+    cells then the boundary values in the reference's group order (outlet values after the
+    inlet ones)."""
+    bnd = boundary_groups(mesh)
+    ninl, inl0, nout, out0 = (int(x) for x in bnd[:4])
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    outs = [_f64(x).copy() for x in (u, v, w, su)]
+    fmo = np.zeros(max(nout, 1))
+    fmi_ = _f64(fmi) if ninl else np.zeros(1)
+    lib().orc_adjust_mass_flow(_ip(own1), _dp(_f64(g.arx)), _dp(_f64(g.ary)), _dp(_f64(g.arz)), ninl, inl0,
+                               _dp(fmi_), nout, out0, mesh.n_cells + ninl, float(flomas), _dp(_f64(den)),
+                               *[_dp(x) for x in outs], _dp(fmo))
+    uu, vv, ww, ss = outs
+    return ss, uu, vv, ww, fmo[:nout]
 
 
 GROUPS = ("inlet", "outlet", "symmetry", "wall", "prOutlet")

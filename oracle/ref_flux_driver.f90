@@ -11,10 +11,11 @@
 !               component fastest)
 ! Writes flux_out.bin (stream):
 !   int32   numCells, numInnerFaces, nnz
-!   float64 cap(nif), can(nif), flmass(nif), a(nnz), su(numCells)
+!   float64 cap(nif), can(nif), flmass(nif), a(nnz), su(numCells), fmcor(nif)
 ! where a and su come from the face loop of the pressure-correction assembly
 ! (a = 0 and su = 0, then per inner face in order: a(ij) = can, a(ji) = cap,
-! a(diag(P)) -= can, a(diag(N)) -= cap, su(P) -= F, su(N) += F).
+! a(diag(P)) -= can, a(diag(N)) -= cap, su(P) -= F, su(N) += F), and fmcor of
+! the reference's fluxmc (non-orthogonal mass flux correction) for every face.
 program ref_flux_driver
   use types
   use parameters
@@ -25,14 +26,14 @@ program ref_flux_driver
   implicit none
   integer :: un, i, k, ijp, ijn
   real(dp) :: capf, canf
-  real(dp), allocatable :: capv(:), canv(:)
+  real(dp), allocatable :: capv(:), canv(:), fmc(:)
 
   call mesh_geometry
   call create_CSR_matrix_from_mesh_data
 
   allocate(den(numTotal), u(numTotal), v(numTotal), w(numTotal), p(numTotal))
   allocate(dUdxi(3,numCells), dVdxi(3,numCells), dWdxi(3,numCells), dPdxi(3,numCells))
-  allocate(flmass(numInnerFaces), capv(numInnerFaces), canv(numInnerFaces))
+  allocate(flmass(numInnerFaces), capv(numInnerFaces), canv(numInnerFaces), fmc(numInnerFaces))
   den = 0.0_dp
   u = 0.0_dp
   v = 0.0_dp
@@ -74,8 +75,12 @@ program ref_flux_driver
     su(ijn) = su(ijn) + flmass(i)
   end do
 
+  do i = 1, numInnerFaces
+    call fluxmc(owner(i), neighbour(i), xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), facint(i), fmc(i))
+  end do
+
   open(newunit=un, file='flux_out.bin', access='stream', form='unformatted', status='replace')
   write(un) numCells, numInnerFaces, nnz
-  write(un) capv, canv, flmass, a, su
+  write(un) capv, canv, flmass, a, su, fmc
   close(un)
 end program ref_flux_driver

@@ -15,6 +15,7 @@ MESH_CASES = ["hex4_uniform", "hex8_perturbed", "hex6x5x4_graded", "tet3_permute
 CONVDIFF_CASES = ["tet4_convdiff", "hex8_convdiff"]
 CALCP_CASES = ["hex6_calcp", "tet3_calcp"]
 GRAD_CASES = ["hex5_grad", "tet3_grad"]
+ADJUST_CASES = ["hex5_adjust", "tet3_adjust"]
 CALCP_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p", "dudx", "dvdx", "dwdx", "dpdx")
 
 

@@ -36,6 +36,7 @@ from oracle import oracle as O  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 GRAD_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_grad_driver")
+ADJUST_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_adjust_driver")
 FLUX_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_flux_driver")
 REF_TESTS = "/root/reference/tests"
 
@@ -181,9 +182,10 @@ def run_reference_flux(mesh, fields, fmt="native"):
     hdr = np.frombuffer(raw, np.int32, 3)
     assert tuple(hdr) == (n, nif, mesh.nnz), (hdr, n, nif, mesh.nnz)
     vals = np.frombuffer(raw, np.float64, offset=12)
-    assert vals.size == 3 * nif + mesh.nnz + n
+    assert vals.size == 4 * nif + mesh.nnz + n
+    o = 3 * nif + mesh.nnz
     return {"cap": vals[:nif].copy(), "can": vals[nif:2 * nif].copy(), "flmass": vals[2 * nif:3 * nif].copy(),
-            "a": vals[3 * nif:3 * nif + mesh.nnz].copy(), "su": vals[3 * nif + mesh.nnz:].copy()}
+            "a": vals[3 * nif:o].copy(), "su": vals[o:o + n].copy(), "fmcor": vals[o + n:].copy()}
 
 
 def calcp_fields(g, seed):
@@ -238,6 +240,64 @@ def run_reference_grad(mesh, u, nigrad, fmt="native"):
     g = np.frombuffer(raw, np.float64, offset=8)
     assert g.size == 3 * n
     return g[:n].copy(), g[n:2 * n].copy(), g[2 * n:].copy()
+
+
+def run_reference_adjust(mesh, flomas, den, u, v, w, su, fmi, fmt="native"):
+    """su, u, v, w and fmo after the reference's adjustMassFlow."""
+    n, ntot = mesh.n_cells, mesh.n_cells + mesh.n_boundary
+    with tempfile.TemporaryDirectory() as d:
+        M.write_polymesh(mesh, os.path.join(d, "polyMesh"), fmt=fmt)
+        with open(os.path.join(d, "adjust_in.bin"), "wb") as f:
+            for arr in (np.array([flomas]), den, u, v, w, su, fmi):
+                f.write(np.ascontiguousarray(arr, np.float64).tobytes())
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        subprocess.run([ADJUST_DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
+        raw = open(os.path.join(d, "adjust_out.bin"), "rb").read()
+    hdr = np.frombuffer(raw, np.int32, 4)
+    assert hdr[0] == n and hdr[1] == ntot and hdr[2] == fmi.size, hdr
+    vals = np.frombuffer(raw, np.float64, offset=16)
+    nout = int(hdr[3])
+    assert vals.size == n + 3 * ntot + nout
+    out, o = {}, 0
+    for k, sz in (("su", n), ("u", ntot), ("v", ntot), ("w", ntot), ("fmo", nout)):
+        out[k] = vals[o:o + sz].copy()
+        o += sz
+    return out
+
+
+def adjust_case(name, mesh, seed):
+    """adjustMassFlow on a mesh with inlet and outlet patches: random density, velocities (cells
+    and boundary values), continuity source and inlet fluxes."""
+    n, ntot = mesh.n_cells, mesh.n_cells + mesh.n_boundary
+    bnd = O.boundary_groups(mesh)
+    ninl = int(bnd[0])
+    assert ninl > 0 and bnd[2] > 0
+    base = run_reference(mesh, np.full(n, -1.0), np.zeros(ntot), np.zeros(n), np.zeros(n), solver="none")
+    rng = np.random.default_rng(seed)
+    inputs = {"flomas": np.array([0.75 + rng.random()]), "den": 1.0 + 0.2 * rng.random(ntot),
+              "u": 1.0 + 0.3 * rng.standard_normal(ntot), "v": 0.3 * rng.standard_normal(ntot),
+              "w": 0.3 * rng.standard_normal(ntot), "su": rng.standard_normal(n),
+              "fmi": 0.01 + 0.02 * rng.random(ninl), "fmt": np.array("native")}
+    out = run_reference_adjust(mesh, inputs["flomas"][0], inputs["den"], inputs["u"], inputs["v"], inputs["w"],
+                               inputs["su"], inputs["fmi"])
+    for k in ("arx", "ary", "arz"):
+        out[k] = base[k]
+    save_case(name, mesh, inputs, {"adjust": out})
+
+
+def adjust_meshes():
+    m = M.hex_box(5, 4, 3, perturb=0.2, seed=43)
+    types = ["inlet", "outlet", "symmetry", "symmetry", "wall", "prOutlet"]
+    m.patches = [(t, nf, st) for t, (_, nf, st) in zip(types, m.patches)]
+    t = M.tet_box(3, grade_axis=0, grade_ratio=1.2, grade_max_aspect=10.0, permute_seed=7)
+    t.patches = [(ty, nf, st) for ty, (_, nf, st) in zip(["inlet", "outlet", "wall", "wall", "wall", "wall"],
+                                                          t.patches)]
+    return [("hex5_adjust", m, 25), ("tet3_adjust", t, 26)]
+
+
+def adjust_only():
+    for name, mesh, seed in adjust_meshes():
+        adjust_case(name, mesh, seed)
 
 
 def grad_case(name, mesh, seed):
@@ -316,6 +376,7 @@ def main():
     convdiff_case("hex8_convdiff", M.hex_box(8, perturb=0.2, seed=21), seed=17)
     calcp_only()
     grad_only()
+    adjust_only()
 
 
 def calcp_only():
@@ -331,5 +392,7 @@ if __name__ == "__main__":
         calcp_only()
     elif len(sys.argv) > 1 and sys.argv[1] == "grad":
         grad_only()
+    elif len(sys.argv) > 1 and sys.argv[1] == "adjust":
+        adjust_only()
     else:
         main()

@@ -9,7 +9,7 @@ a solution within 1e-10 relative FP64 residual of the reference's.
 import numpy as np
 import pytest
 
-from conftest import (CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, GRAD_CASES, MESH_CASES, calcp_uniform_case,
+from conftest import (ADJUST_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, GRAD_CASES, MESH_CASES, calcp_uniform_case,
                       load_transcript)
 from freecappuccino_amd import fcap
 from freecappuccino_amd import mesh as M
@@ -298,6 +298,146 @@ def test_calcp_uniform_flow_closed_form(ctx):
     assert abs(out["su"].sum()) < 1e-12
 
 
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_calcp_nonorth_bitwise_vs_reference(ctx, golden_loader, case):
+    """fc_calcp_nonorth: fmcor against the reference's fluxmc, su against the oracle."""
+    g = golden_loader(case)
+    geo = fcap.Geometry(*(g.out("calcp", k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc",
+                                                      "zc", "facint")))
+    csr = fcap.CSR(*(g.out("calcp", k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    nif = g.mesh.n_inner
+    su, fl, _ = fcap.calcp_nonorth(g.mesh, geo, csr, g.inp("den"), g.inp("apu"), g.inp("dpdx"), np.zeros(nif),
+                                   ctx=ctx)
+    np.testing.assert_array_equal(fl, g.out("calcp", "fmcor"))
+    og = O.Geometry(*(g.out("calcp", k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                                  "facint")))
+    fl0 = g.out("calcp", "flmass")
+    osu, ofl, _, _ = O.calcp_nonorth(g.mesh, og, g.inp("den"), g.inp("apu"), g.inp("dpdx"), fl0)
+    su2, fl2, _ = fcap.calcp_nonorth(g.mesh, geo, csr, g.inp("den"), g.inp("apu"), g.inp("dpdx"), fl0, ctx=ctx)
+    np.testing.assert_array_equal(su, osu)
+    np.testing.assert_array_equal(su2, osu)
+    np.testing.assert_array_equal(fl2, ofl)
+
+
+def test_calcp_nonorth_oc_cuts_match_oracle(ctx):
+    m = M.hex_box(5, perturb=0.2, seed=12)
+    og = O.mesh_geometry(m)
+    oc_csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    rng = np.random.default_rng(9)
+    n, nif = m.n_cells, m.n_inner
+    den, apu, gpp = 0.5 + rng.random(n), 0.5 + rng.random(n), rng.standard_normal((n, 3))
+    fl0 = rng.standard_normal(nif)
+    cuts = _mirror_cuts(m, og, 5)
+    fm0 = rng.standard_normal(len(cuts["ijl"]))
+    osu, ofl, ofm, _ = O.calcp_nonorth(m, og, den, apu, gpp, fl0, oc=cuts, fmoc=fm0)
+    assert np.abs(ofm - fm0).max() > 1e-6
+    geo = fcap.Geometry(og.arx, og.ary, og.arz, og.xf, og.yf, og.zf, og.vol, og.xc, og.yc, og.zc, og.facint)
+    csr = fcap.CSR(oc_csr.ioffset, oc_csr.ja, oc_csr.diag, oc_csr.ij_slot, oc_csr.ji_slot)
+    su, fl, fm = fcap.calcp_nonorth(m, geo, csr, den, apu, gpp, fl0, oc=cuts, fmoc=fm0, ctx=ctx)
+    np.testing.assert_array_equal(su, osu)
+    np.testing.assert_array_equal(fl, ofl)
+    np.testing.assert_array_equal(fm, ofm)
+
+
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_calcp_correct_matches_oracle(ctx, golden_loader, case):
+    """fc_calcp_correct with the reference's assembled matrix and random pp and gradient, plus
+    O-C cuts on a perturbed box."""
+    g = golden_loader(case)
+    csr = fcap.CSR(*(g.out("calcp", k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    n = g.mesh.n_cells
+    rng = np.random.default_rng(13)
+    pp, gpp = rng.standard_normal(n), rng.standard_normal((n, 3))
+    args = (g.out("calcp", "a"), pp, gpp, g.out("calcp", "vol"), g.inp("apu"), g.inp("apv"), g.inp("apw"), 0.2,
+            n // 3, g.out("calcp", "flmass"), g.inp("u"), g.inp("v"), g.inp("w"), g.inp("p"))
+    ref = O.calcp_correct(g.mesh, O.CSR(csr.ioffset, csr.ja, csr.diag, csr.ij_slot, csr.ji_slot), *args)
+    out = fcap.calcp_correct(g.mesh, csr, *args, ctx=ctx)
+    for k, (x, y) in enumerate(zip(out, ref)):
+        np.testing.assert_array_equal(x, y, err_msg=str(k))
+
+
+def test_calcp_correct_oc_cuts_and_errors(ctx):
+    m = M.hex_box(5, perturb=0.2, seed=12)
+    og = O.mesh_geometry(m)
+    oc_csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    csr = fcap.CSR(oc_csr.ioffset, oc_csr.ja, oc_csr.diag, oc_csr.ij_slot, oc_csr.ji_slot)
+    rng = np.random.default_rng(14)
+    n, nif = m.n_cells, m.n_inner
+    cuts = _mirror_cuts(m, og, 5)
+    noc = len(cuts["ijl"])
+    a = rng.standard_normal(m.nnz)
+    cell = [rng.standard_normal(n) for _ in range(4)]
+    aps = [0.5 + rng.random(n) for _ in range(3)]
+    args = (a, rng.standard_normal(n), rng.standard_normal((n, 3)), og.vol, *aps, 0.8, 0, rng.standard_normal(nif),
+            *cell)
+    kw = dict(oc=cuts, ar=rng.standard_normal(noc), fmoc=rng.standard_normal(noc))
+    ref = O.calcp_correct(m, oc_csr, *args, **kw)
+    out = fcap.calcp_correct(m, csr, *args, **kw, ctx=ctx)
+    for k, (x, y) in enumerate(zip(out, ref)):
+        np.testing.assert_array_equal(x, y, err_msg=str(k))
+    bad = list(args)
+    bad[8] = n                      # reference cell outside the mesh (0-based n)
+    with pytest.raises(fcap.FcError):
+        fcap.calcp_correct(m, csr, *bad, ctx=ctx)
+
+
+def test_calcp_correct_consistent_with_velocity_correction(ctx):
+    """On unit cubes with apu = apv = apw and pp linear: the flux correction a_ij (pp_N - pp_P) of
+    the GPU-assembled matrix is rho times the interpolated velocity correction dotted with S."""
+    m, f, _, grad = calcp_uniform_case()
+    f["apv"] = f["apw"] = f["apu"]
+    geo = fcap.mesh_geometry(m, ctx)
+    csr = fcap.create_csr_matrix_from_mesh_data(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1, ctx)
+    n, nif = m.n_cells, m.n_inner
+    rng = np.random.default_rng(6)
+    f.update({k: rng.standard_normal(n) for k in ("u", "v", "w", "p")})
+    f.update({k: np.zeros((n, 3)) for k in ("dudx", "dvdx", "dwdx", "dpdx")})
+    a = fcap.calcp_assemble(m, geo, csr, f, ctx=ctx)["a"]
+    pp = grad[0] * geo.xc + grad[1] * geo.yc + grad[2] * geo.zc
+    fl0 = rng.standard_normal(nif)
+    fl, _, u, v, w, p = fcap.calcp_correct(m, csr, a, pp, np.tile(grad, (n, 1)), geo.vol, f["apu"], f["apv"],
+                                           f["apw"], 0.3, 5, fl0, f["u"], f["v"], f["w"], f["p"], ctx=ctx)
+    o, q, lam = m.owner[:nif], m.neighbour, geo.facint
+    du = np.stack([u - f["u"], v - f["v"], w - f["w"]], axis=1)
+    duf = (1 - lam)[:, None] * du[o] + lam[:, None] * du[q]
+    S = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], axis=1)
+    np.testing.assert_allclose(fl - fl0, f["den"][0] * (duf * S).sum(axis=1), rtol=1e-12, atol=1e-14)
+    assert p[5] == f["p"][5]
+
+
+@pytest.mark.parametrize("case", ADJUST_CASES)
+def test_adjust_mass_flow_bitwise_vs_reference(ctx, golden_loader, case):
+    """fc_adjust_mass_flow against the reference's adjustMassFlow."""
+    g = golden_loader(case)
+    geo = fcap.Geometry(g.out("adjust", "arx"), g.out("adjust", "ary"), g.out("adjust", "arz"), *[None] * 8)
+    args = (g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"), g.inp("w"), g.inp("su"), g.inp("fmi"))
+    out = fcap.adjust_mass_flow(g.mesh, geo, *args, ctx=ctx)
+    for k, x in zip(("su", "u", "v", "w", "fmo"), out):
+        np.testing.assert_array_equal(x, g.out("adjust", k), err_msg=k)
+
+
+def test_adjust_mass_flow_long_outlet_matches_oracle(ctx):
+    """An outlet of 900 faces (the ordered sum runs over 15 loads of 64) and cells owning an
+    inlet and an outlet face at once (a box one cell long in x)."""
+    m = M.hex_box(1, 30, 30, perturb=0.1, seed=3)
+    m.patches = [(t, nf, st) for t, (_, nf, st) in zip(["inlet", "outlet", "wall", "wall", "wall", "wall"],
+                                                       m.patches)]
+    og = O.mesh_geometry(m)
+    n, ntot = m.n_cells, m.n_cells + m.n_boundary
+    ninl = int(O.boundary_groups(m)[0])
+    rng = np.random.default_rng(21)
+    args = (1.7, 1 + 0.1 * rng.random(ntot), 1 + 0.2 * rng.standard_normal(ntot), rng.standard_normal(ntot),
+            rng.standard_normal(ntot), rng.standard_normal(n), rng.random(ninl))
+    ref = O.adjust_mass_flow(m, og, *args)
+    assert ref[4].size == 900
+    geo = fcap.Geometry(og.arx, og.ary, og.arz, og.xf, og.yf, og.zf, og.vol, og.xc, og.yc, og.zc, og.facint)
+    out = fcap.adjust_mass_flow(m, geo, *args, ctx=ctx)
+    for k, (x, y) in enumerate(zip(out, ref)):
+        np.testing.assert_array_equal(x, y, err_msg=str(k))
+
+
+# ---------------------------------------------------------------- Gauss gradient
+# ---------------------------------------------------------------- Gauss gradient
 # ---------------------------------------------------------------- Gauss gradient
 
 @pytest.mark.parametrize("case", GRAD_CASES)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
+from conftest import ADJUST_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
 
@@ -346,6 +346,117 @@ def test_calcp_uniform_flow_closed_form():
     assert abs(out["su"].sum()) < 1e-12
 
 
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_calcp_nonorth_bitwise_vs_reference(golden_loader, case):
+    """fmcor of the reference's fluxmc for every inner face (oracle/_ref/ref_flux_driver)."""
+    g = golden_loader(case)
+    geo, c, f = _calcp_inputs(g)
+    nif = g.mesh.n_inner
+    su, fl, _, fmcor = O.calcp_nonorth(g.mesh, geo, f["den"], f["apu"], f["dpdx"], np.zeros(nif))
+    ref = g.out("calcp", "fmcor")
+    assert np.abs(ref).max() > 1e-3          # the meshes are non-orthogonal
+    np.testing.assert_array_equal(fmcor, ref)
+    np.testing.assert_array_equal(fl, ref)
+    # su is the divergence of the correction flux: it sums to zero and matches a scatter
+    chk = np.zeros(g.mesh.n_cells)
+    np.add.at(chk, g.mesh.owner[:nif], -ref)
+    np.add.at(chk, g.mesh.neighbour, ref)
+    np.testing.assert_allclose(su, chk, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert abs(su.sum()) < 1e-12 * np.abs(ref).sum()
+
+
+def test_calcp_nonorth_vanishes_on_orthogonal_box():
+    m, f, _, _ = calcp_uniform_case()
+    geo = O.mesh_geometry(m)
+    rng = np.random.default_rng(8)
+    su, fl, _, fmcor = O.calcp_nonorth(m, geo, f["den"], f["apu"], rng.standard_normal((m.n_cells, 3)),
+                                       np.ones(m.n_inner))
+    # the cell centres lie on the face normals: zero up to the rounding of the centroids
+    np.testing.assert_allclose(fmcor, 0.0, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(fl, 1.0, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(su, 0.0, rtol=0, atol=1e-12)
+
+
+def _correct_case(seed=6):
+    """Uniform box of unit cubes, apu = apv = apw, pp linear with its exact gradient."""
+    m, f, _, grad = calcp_uniform_case()
+    f["apv"] = f["apw"] = f["apu"]
+    geo = O.mesh_geometry(m)
+    c = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    n = m.n_cells
+    rng = np.random.default_rng(seed)
+    f.update({k: rng.standard_normal(n) for k in ("u", "v", "w", "p")})
+    f.update({k: np.zeros((n, 3)) for k in ("dudx", "dvdx", "dwdx", "dpdx")})
+    pp = grad[0] * geo.xc + grad[1] * geo.yc + grad[2] * geo.zc
+    return m, geo, c, f, pp, np.tile(grad, (n, 1))
+
+
+def test_calcp_correct_consistent_with_velocity_correction():
+    """SIMPLE's consistency: the face flux correction a_ij (pp_N - pp_P) equals rho times the
+    face-interpolated velocity correction -grad(pp) Vol/a_P dotted with S; the pressure moves by
+    urf (pp - pp_ref) and keeps its value at the reference cell."""
+    m, geo, c, f, pp, gpp = _correct_case()
+    a = O.calcp_assemble(m, geo, c, f)["a"]
+    nif, n = m.n_inner, m.n_cells
+    fl0 = np.random.default_rng(1).standard_normal(nif)
+    fl, _, u, v, w, p = O.calcp_correct(m, c, a, pp, gpp, geo.vol, f["apu"], f["apv"], f["apw"], 0.3, 5, fl0,
+                                        f["u"], f["v"], f["w"], f["p"])
+    o, q, lam = m.owner[:nif], m.neighbour, geo.facint
+    du = np.stack([u - f["u"], v - f["v"], w - f["w"]], axis=1)
+    duf = (1 - lam)[:, None] * du[o] + lam[:, None] * du[q]
+    S = np.stack([geo.arx[:nif], geo.ary[:nif], geo.arz[:nif]], axis=1)
+    rho = f["den"][0]
+    np.testing.assert_allclose(fl - fl0, rho * (duf * S).sum(axis=1), rtol=1e-12, atol=1e-14)
+    assert np.abs(fl - fl0).max() > 0.1
+    assert p[5] == f["p"][5]
+    np.testing.assert_allclose(p - f["p"], 0.3 * (pp - pp[5]), rtol=1e-14, atol=1e-15)
+
+
+ADJUST_KEYS = ("su", "u", "v", "w", "fmo")
+
+
+def _adjust_args(g):
+    return (g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"), g.inp("w"), g.inp("su"), g.inp("fmi"))
+
+
+@pytest.mark.parametrize("case", ADJUST_CASES)
+def test_adjust_mass_flow_bitwise_vs_reference(golden_loader, case):
+    """adjustMassFlow of the reference compiled on its own (oracle/_ref/ref_adjust_driver)."""
+    g = golden_loader(case)
+    geo = _geom(g)
+    for k in ("arx", "ary", "arz"):
+        np.testing.assert_array_equal(getattr(geo, k), g.out("adjust", k), err_msg=k)
+    out = O.adjust_mass_flow(g.mesh, geo, *_adjust_args(g))
+    for k, x in zip(ADJUST_KEYS, out):
+        np.testing.assert_array_equal(x, g.out("adjust", k), err_msg=k)
+
+
+@pytest.mark.parametrize("case", ADJUST_CASES)
+def test_adjust_mass_flow_conserves_mass(golden_loader, case):
+    """The outlet carries flomas; su falls by the inlet and outlet fluxes; the outlet velocities
+    are the owners' scaled by one common factor; cells and other boundary values are untouched."""
+    g = golden_loader(case)
+    geo = _geom(g)
+    flomas, den, u, v, w, su, fmi = _adjust_args(g)
+    su2, u2, v2, w2, fmo = O.adjust_mass_flow(g.mesh, geo, flomas, den, u, v, w, su, fmi)
+    n = g.mesh.n_cells
+    bnd = O.boundary_groups(g.mesh)
+    ninl, nout, out0 = int(bnd[0]), int(bnd[2]), int(bnd[3])
+    assert abs(fmo.sum() - flomas) < 1e-14 * flomas * nout
+    np.testing.assert_allclose(su.sum() - su2.sum(), fmi.sum() + flomas, rtol=1e-13)
+    own = g.mesh.owner[out0:out0 + nout]
+    fac = u2[n + ninl:n + ninl + nout] / u[own]
+    np.testing.assert_allclose(fac, fac[0], rtol=1e-14)
+    np.testing.assert_allclose(fac[0] * den[own] * (u[own] * geo.arx[out0:out0 + nout] + v[own] *
+                                                  geo.ary[out0:out0 + nout] + w[own] * geo.arz[out0:out0 + nout]),
+                               fmo, rtol=1e-13)
+    keep = np.ones(u.size, bool)
+    keep[n + ninl:n + ninl + nout] = False
+    np.testing.assert_array_equal(u2[keep], u[keep])
+
+
+# ---------------------------------------------------------------- momentum rows
+# ---------------------------------------------------------------- momentum rows
 # ---------------------------------------------------------------- momentum rows
 
 def _momentum_case(seed=3):
