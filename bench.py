@@ -31,6 +31,7 @@ number of iterations and scaled to the GPU's FP64 IC(0)-PCG iteration count.
 from __future__ import annotations
 
 import argparse
+import inspect
 import json
 import os
 import platform
@@ -69,6 +70,182 @@ def gnn_bytes(n, width=6, layers=4):
     return n * (8 + 8 + 4 + 64) + (layers - 1) * gnn_layer_bytes(n, width) + n * (64 + 8 * width + 4 + 4 + 4)
 
 
+def calcp_step_bytes(n, nif, nnz, nb):
+    """Algorithmic bytes per launch of each kernel of the pressure-correction step: every index,
+    face value and cell value the kernel needs read once, every output written once (4-byte
+    indices, 8-byte values; a face's geometry is xf, yf, zf, arx, ary, arz, facint = 56 bytes)."""
+    off = nnz - n
+    return {
+        # own, nei, ij, ji, geometry; a(ij), a(ji), flmass out; 12 cell fields + 4 gradients (3 each)
+        "k_calcp_faces": nif * (16 + 56 + 24) + n * 24 * 8,
+        # ptr, diag, a and su out; per off-diagonal slot kface + a; per face ij + flmass
+        "k_calcp_rows": n * (8 + 16) + off * (4 + 8) + nif * (4 + 8),
+        # own, nei, geometry, (x, y, z)-products out; u, centre and previous gradient per cell
+        "k_grad_faces": nif * (8 + 56 + 24) + n * 7 * 8,
+        # ptr, diag, vol, gradient out; kface per slot; ij and products per face; boundary faces
+        # (owner list entry, S, value)
+        "k_grad_rows": n * (8 + 8 + 24) + off * 4 + nif * (4 + 24) + nb * (4 + 24 + 8),
+        # own, nei, ij, a(ij), flmass in and out; pp per cell
+        "k_correct_faces": nif * (12 + 8 + 16) + n * 8,
+        # pp, grad pp, vol, 1/a_P (3), u, v, w, p in and out
+        "k_correct_cells": n * (8 + 24 + 8 + 24 + 64),
+        # own, nei, geometry, fmcor out, flmass in and out; centre, vol, den, apu, grad pp per cell
+        "k_nonorth_faces": nif * (8 + 56 + 8 + 16) + n * 9 * 8,
+        # ptr, diag, su out; kface per slot; ij and fmcor per face
+        "k_face_source_rows": n * (8 + 8) + off * 4 + nif * (4 + 8),
+    }
+
+
+def _inlet_outlet(m):
+    """The mesh with its x = 0 patch as inlet, x = 1 as outlet and walls elsewhere."""
+    import dataclasses
+    types = ["inlet", "outlet", "wall", "wall", "wall", "wall"]
+    return dataclasses.replace(m, patches=[(t, nf, st) for t, (_, nf, st) in zip(types, m.patches)])
+
+
+def _calcp_inputs(api, m, geo):
+    """Fields of the pressure-correction step (cells, and cells + boundary values extrapolated
+    from the owners), the pressure correction pp and the inlet mass fluxes."""
+    from freecappuccino_amd import workloads as W
+    n, nif = m.n_cells, m.n_inner
+    bown = m.owner[nif:]
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    rng = np.random.default_rng(7)
+    f = {"den": np.ones(n), "u": u, "v": v, "w": w, "p": np.sin(3.0 * geo.xc) * geo.yc + geo.zc}
+    for k in ("apu", "apv", "apw"):
+        f[k] = (0.5 + rng.random(n)) / geo.vol
+    pp = 1e-2 * np.cos(2.0 * geo.yc) * geo.xc
+    bnd = api.boundary_groups(m)
+    s_in = slice(int(bnd[1]), int(bnd[1]) + int(bnd[0]))
+    o = m.owner[s_in]
+    fmi = -(u[o] * geo.arx[s_in] + v[o] * geo.ary[s_in] + w[o] * geo.arz[s_in])
+    fb = {k: np.concatenate([f[k], f[k][bown]]) for k in ("u", "v", "w", "p")}
+    return f, fb, pp, np.concatenate([pp, pp[bown]]), fmi
+
+
+def _calcp_pass(api, m, geo, csr, f, fb, pp, ppb, fmi, timed, **kw):
+    """One pass of the step's calls; wall ms per call name are appended to timed."""
+    def clock(name, fn, *a):
+        t0 = time.perf_counter()
+        r = fn(*a, **kw)
+        timed.setdefault(name, []).append((time.perf_counter() - t0) * 1e3)
+        return r
+
+    g = {}
+    for k, src in (("dudx", "u"), ("dvdx", "v"), ("dwdx", "w"), ("dpdx", "p")):
+        g[k] = np.stack(clock("grad_gauss", api.grad_gauss, m, geo, fb[src]), axis=1)
+    out = clock("calcp_assemble", api.calcp_assemble, m, geo, csr, dict(f, **g))
+    su = clock("adjust_mass_flow", api.adjust_mass_flow, m, geo, float(fmi.sum()), f["den"], fb["u"], fb["v"],
+               fb["w"], out["su"], fmi)[0]
+    gpp = np.stack(clock("grad_gauss", api.grad_gauss, m, geo, ppb), axis=1)
+    fl = clock("calcp_correct", api.calcp_correct, m, csr, out["a"], pp, gpp, geo.vol, f["apu"], f["apv"],
+               f["apw"], 0.2, 0, out["flmass"], f["u"], f["v"], f["w"], f["p"])[0]
+    # the library's call takes the CSR structure (it checks the face slots), the oracle's does not
+    with_csr = [csr] if "csr" in inspect.signature(api.calcp_nonorth).parameters else []
+    su2 = clock("calcp_nonorth", api.calcp_nonorth, m, geo, *with_csr, f["den"], f["apu"], gpp, fl)[0]
+    return float(np.abs(su).sum()), float(np.abs(su2).sum())
+
+
+def profile_calcp(mesh, ctx, reps, cpu_n):
+    """The pressure-correction step's calls on the workload mesh with inlet and outlet patches:
+    Gauss gradients, assembly, adjustMassFlow, corrections and the non-orthogonal corrector's
+    source, each through the host-array C ABI (wall times include the PCIe copies; the kernels'
+    own times come from a kernel trace of the same command).  The oracle runs the same calls on
+    a cpu_n^3 mesh of the same family for the CPU baseline."""
+    from freecappuccino_amd import fcap
+    from freecappuccino_amd import workloads as W
+
+    m = _inlet_outlet(mesh)
+    geo = fcap.mesh_geometry(m, ctx)
+    csr = fcap.create_csr_matrix_from_mesh_data(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1, ctx)
+    data = _calcp_inputs(fcap, m, geo)
+    timed = {}
+    for _ in range(reps):
+        check = _calcp_pass(fcap, m, geo, csr, *data, timed, ctx=ctx)
+        print(f"calcp step pass: {({k: round(v[-1], 1) for k, v in timed.items()})}", file=sys.stderr, flush=True)
+    res = {"cells": m.n_cells, "inner_faces": m.n_inner, "nnz": m.nnz, "reps": reps,
+           "wall_ms_incl_pcie": {k: float(np.median(v)) for k, v in timed.items()},
+           "kernel_bytes": calcp_step_bytes(m.n_cells, m.n_inner, m.nnz, m.n_boundary), "check": check}
+    if cpu_n:
+        from oracle import oracle as O
+        cm = _inlet_outlet(W.case_mesh(5, cpu_n))
+        cgeo = O.mesh_geometry(cm)
+        ccsr = O.csr_build(cm.n_cells, cm.owner[:cm.n_inner] + 1, cm.neighbour + 1)
+        cdata = _calcp_inputs(O, cm, cgeo)
+        ct = {}
+        _calcp_pass(O, cm, cgeo, ccsr, *cdata, ct)
+        total = sum(sum(v) for v in ct.values())
+        res["cpu_baseline"] = {
+            "value": total * m.n_cells / cm.n_cells, "unit": "ms", "cores": 1, "kind": "port",
+            "per_pass_ms": {k: float(sum(v)) for k, v in ct.items()},
+            "sample": f"the oracle (C restatement, gcc -O2, 1 thread) on one pass of the same calls over a "
+                      f"{cpu_n}^3 mesh of the same family ({cm.n_cells} cells, {total / 1e3:.2f} s), "
+                      f"scaled by cells to {m.n_cells}"}
+    return res
+
+
+def _momentum_inputs(api, m, geo, ctx_kw):
+    from freecappuccino_amd import workloads as W
+    n = m.n_cells
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    flux = api.face_flux_central(m, geo, u, v, w, **ctx_kw)
+    rng = np.random.default_rng(11)
+    sp = [rng.random(n) * geo.vol * 10 for _ in range(3)]
+    src = [W.blob_rhs(geo.xc, geo.yc, geo.zc, geo.vol, 8, seed=20 + c) for c in range(3)]
+    phi0 = [0.1 * rng.standard_normal(n) for _ in range(3)]
+    return np.full(n, 1e-3), flux, sp, src, phi0, [0.7, 0.7, 0.7]
+
+
+def profile_momentum(mesh, ctx, reps, cpu_n, sor=1e-8):
+    """calcuvw's three component solves (BiCGStab-DILU, FP64) on one convection-diffusion
+    assembly of the workload mesh: a central-difference flux of a Taylor-Green field, vis = 1e-3,
+    random sp, blob sources, urf 0.7.  Device times from the solver's own events; the oracle's
+    BiCGStab on a cpu_n^3 mesh of the same family, scaled by cells x iterations, is the CPU
+    baseline."""
+    from freecappuccino_amd import fcap
+    from freecappuccino_amd import workloads as W
+    geo = fcap.mesh_geometry(mesh, ctx)
+    vis, flux, sp, src, phi0, urf = _momentum_inputs(fcap, mesh, geo, {"ctx": ctx})
+    s = fcap.System(mesh, ctx)
+    runs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.assemble_convdiff(vis, flux, 0.7, None)
+        rc, _, _, infos = s.solve_momentum(sp, src, urf, phi0, fcap.METHOD_BICGSTAB, sor, 2000)
+        wall = (time.perf_counter() - t0) * 1e3
+        runs.append({"wall_ms_incl_pcie": wall, "status": fcap.STATUS_NAMES.get(rc, rc),
+                     "iters": [x["iters"] for x in infos],
+                     "solve_device_ms": [x["solve_ms"] + x["setup_ms"] for x in infos]})
+        print(f"momentum pass: {runs[-1]}", file=sys.stderr, flush=True)
+    s.close()
+    res = {"cells": mesh.n_cells, "nnz": mesh.nnz, "sor": sor, "runs": runs}
+    if cpu_n:
+        from oracle import oracle as O
+        cm = W.case_mesh(5, cpu_n)
+        cgeo = O.mesh_geometry(cm)
+        csr = O.csr_build(cm.n_cells, cm.owner[:cm.n_inner] + 1, cm.neighbour + 1)
+        cvis, cflux, csp, csrc, cphi, curf = _momentum_inputs(O, cm, cgeo, {})
+        t0 = time.perf_counter()
+        a_base = O.convdiff(cm, cgeo, csr, cvis, cflux, 0.7)
+        per_iter, its = [], []
+        for c in range(3):
+            a_c, su_c, _ = O.momentum_rows(csr, a_base, csp[c], csrc[c], cphi[c], curf[c])
+            t1 = time.perf_counter()
+            r = O.bicgstab(csr, a_c, su_c, cphi[c], sor=sor, nsw=2000)
+            per_iter.append((time.perf_counter() - t1) / max(r.iters, 1))
+            its.append(r.iters)
+        dt = time.perf_counter() - t0
+        gpu_iters = runs[-1]["iters"]
+        scale = mesh.n_cells / cm.n_cells
+        res["cpu_baseline"] = {
+            "value": 1e3 * scale * sum(p * g for p, g in zip(per_iter, gpu_iters)), "unit": "ms", "cores": 1,
+            "kind": "port",
+            "sample": f"the oracle's assembly, rows and BiCGStab-DILU (C restatement, gcc -O2, 1 thread) on a "
+                      f"{cpu_n}^3 mesh of the same family ({cm.n_cells} cells, iterations {its}, {dt:.1f} s), "
+                      f"time per iteration scaled by cells and by the GPU's iterations {gpu_iters}"}
+    return res
+
+
 def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
     """The oracle's sequential DIC-PCG on one host core, `budget_iters` iterations, scaled."""
     from oracle import oracle as O
@@ -102,6 +279,14 @@ def main():
                          "(the calibration kernel of the HBM counters), then exit")
     ap.add_argument("--profile-gnn", type=int, default=0, metavar="REPS",
                     help="only assemble and run REPS applications of the network, then exit")
+    ap.add_argument("--profile-calcp", type=int, default=0, metavar="REPS",
+                    help="only run REPS passes of the pressure-correction step's calls (gradients, "
+                         "assembly, adjustMassFlow, corrections, non-orthogonal source), then exit")
+    ap.add_argument("--profile-momentum", type=int, default=0, metavar="REPS",
+                    help="only run REPS passes of calcuvw's three component solves, then exit")
+    ap.add_argument("--calcp-cpu-n", type=int, default=128,
+                    help="cells per direction of the CPU baseline sample of --profile-calcp and "
+                         "--profile-momentum (0: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -117,6 +302,14 @@ def main():
     args.n = args.n or {1: 40, 2: 128, 5: 192}[args.case]
     mesh = W.case_mesh(args.case, args.n)
     ctx = fcap.Context(0)
+    if args.profile_calcp or args.profile_momentum:
+        cpu_n = 0 if args.no_cpu else args.calcp_cpu_n
+        if args.profile_calcp:
+            print(json.dumps(profile_calcp(mesh, ctx, args.profile_calcp, cpu_n)))
+        else:
+            print(json.dumps(profile_momentum(mesh, ctx, args.profile_momentum, cpu_n)))
+        ctx.close()
+        return
     s = fcap.System(mesh, ctx)
     n, nnz = mesh.n_cells, mesh.nnz
     xc, yc, zc, vol = s.get("xc"), s.get("yc"), s.get("zc"), s.get("vol")

@@ -457,17 +457,31 @@ __global__ void k_outlet_flux(int nout, int out_face0, const int *__restrict__ o
     fmo[i] = den[c] * ((u[c] * arx[f] + v[c] * ary[f]) + w[c] * arz[f]);
 }
 
-// flowo = sum of fmo in outlet order (one wave: 64 values loaded at a time, added in
-// order through lane broadcasts), fac = flomas / (flowo + small)
+// lane j's value, read into scalar registers (j uniform)
+__device__ inline double lane_value(double x, int j) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __hiloint2double(hi, lo);
+}
+
+// flowo = sum of fmo in outlet order, fac = flomas / (flowo + small).  One wave: each lane
+// holds one of 64 consecutive values, the next 64 are loaded while these are added, and
+// the adds form one chain in outlet order (every lane computes the same sum).
 __global__ void __launch_bounds__(64) k_outlet_factor(int nout, const double *__restrict__ fmo, double flomas,
                                                       double *__restrict__ fac) {
     const int lane = threadIdx.x;
     double flowo = 0.0;
-    for (int base = 0; base < nout; base += 64) {
-        const double mine = base + lane < nout ? fmo[base + lane] : 0.0;
-        const int cnt = min(64, nout - base);
-        for (int j = 0; j < cnt; ++j) flowo = flowo + __shfl(mine, j, 64);
+    double cur = lane < nout ? fmo[lane] : 0.0;
+    int base = 0;
+    for (; base + 64 <= nout; base += 64) {
+        const int nx = base + 64 + lane;
+        const double next = nx < nout ? fmo[nx] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 64; ++j) flowo = flowo + lane_value(cur, j);
+        cur = next;
     }
+    for (int j = 0; j < nout - base; ++j) flowo = flowo + lane_value(cur, j);
     if (lane == 0) fac[0] = flomas / (flowo + kSmallSrc);
 }
 

@@ -192,6 +192,8 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         ctx->num_cus = prop.multiProcessorCount;
         if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
         if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
+        if (const char *m = std::getenv("FCAP_SF_WAVES")) ctx->sf_waves = std::atoi(m);
+        if (const char *m = std::getenv("FCAP_GNN_FUSE")) ctx->gnn_fuse_input = std::atoi(m) != 0;
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
@@ -264,6 +266,12 @@ extern "C" fc_status fc_set_inner_tolerance(fc_context *ctx, double eta) {
 extern "C" fc_status fc_set_fcg_window(fc_context *ctx, int m) {
     if (!ctx || m < 1 || m > kFcgM) return FC_ERR_ARGUMENT;
     ctx->fcg_m = m;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_sweep_mode(fc_context *ctx, int mode) {
+    if (!ctx || mode < FC_SWEEP_LEVELS || mode > FC_SWEEP_DEPENDENT) return FC_ERR_ARGUMENT;
+    ctx->sweep_mode = mode;
     return FC_OK;
 }
 
@@ -856,9 +864,24 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
             pf.ab = ab.as<double>();
             pf.db = db.as<double>();
         }
+        SweepCfg cfg;
+        DevBuf sfs, sft;
+        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && pf.af && syncfree_sweeps(st)) {
+            sfs.alloc(sizeof(SyncFree));
+            sft.alloc(sizeof(double) * n);
+            FC_HIP_CHECK(hipMemsetAsync(sfs.get(), 0, sizeof(SyncFree), ctx->stream));
+            cfg.sf = sfs.as<SyncFree>();
+            cfg.tmp = sft.get();
+            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : 8 * ctx->num_cus;
+        }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
-                    nullptr, nullptr, pf.af ? &pf : nullptr);
+                    nullptr, &cfg, pf.af ? &pf : nullptr);
         FC_HIP_CHECK(hipGetLastError());
+        if (cfg.sf) {
+            SyncFree h;
+            d2h(&h, sfs.get(), sizeof(SyncFree), ctx->stream);
+            FC_REQUIRE(!h.abort, FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its spin limit");
+        }
         download_dbl(ctx, z, dz.get(), n);
         return FC_OK;
     });

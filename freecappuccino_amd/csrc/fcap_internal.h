@@ -141,6 +141,7 @@ struct fc_context {
     // triangular sweeps: 0 one launch per level, 1 persistent cooperative kernel
     // with a grid barrier per level (FCAP_SWEEP selects)
     int sweep_mode = 0;
+    int sf_waves = 0;   // persistent waves of the dependency-driven sweeps (0: 8 per CU)
     // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
     int packed = 1;
     // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)
@@ -162,6 +163,7 @@ struct fc_context {
     int gnn_pipe = -1;
     // persistent layer kernel gathers rows with quads of lanes (FCAP_GNN_QUAD=0: one lane per cell)
     int gnn_quad = 1;
+    int gnn_fuse_input = 1;   // inference: the first layer computes the encoder rows from u
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

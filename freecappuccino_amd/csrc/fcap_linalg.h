@@ -35,14 +35,30 @@ struct GridBar {
     int abort, pad;
 };
 
-// How the triangular sweeps run: one launch per level (bar == nullptr), or one
+// State of the dependency-driven sweeps: the next block of 64 packed rows to
+// hand out, the number of waves that have left the kernel (the last one resets
+// both for the next launch) and a sticky abort flag raised when a wait exceeds
+// its spin limit.
+struct SyncFree {
+    unsigned ticket, fin;
+    int abort, pad;
+};
+
+// How the triangular sweeps run: one launch per level (default), one
 // persistent cooperative kernel per sweep with `blocks` co-resident workgroups
-// and a grid barrier between levels.
+// and a grid barrier between levels (bar), or one launch per sweep in which
+// every row waits only for the rows it reads (sf; needs the packed factor;
+// `waves` persistent one-wave workgroups, `tmp` holds the forward result).
 struct SweepCfg {
     GridBar *bar = nullptr;
     int blocks = 0;
+    SyncFree *sf = nullptr;
+    int waves = 0;
+    void *tmp = nullptr;
 };
 SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar);
+// dependency-driven sweeps when the structure is packed at a width they handle
+bool syncfree_sweeps(const Structure &st);
 
 // The factor's values in level-packed order (Structure::kpos_f / kpos_b): strict
 // lower / upper entries as an ELL over the packed rows, and the pivots per

@@ -529,11 +529,188 @@ void pack_factor_t(hipStream_t s, const Structure &st, const T *a, const T *d, T
     FC_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------- dependency-driven sweeps
+// One launch per sweep.  Persistent one-wave workgroups take blocks of 64
+// consecutive packed rows (level order) from a ticket counter, so every row a
+// block reads lies in a block handed out earlier, to a wave that is running or
+// has finished: the waits always end.  The output starts filled with a
+// signalling-NaN pattern that arithmetic never produces; a row polls the
+// outputs of the rows it reads until none holds the pattern, computes exactly
+// as k_fwd_packed / k_bwd_packed (same operands, same order), and stores its
+// value with one coherent store, which is also its "ready" signal.  A wave
+// whose wait exceeds kSfSpin polls, or that sees the abort flag, raises the
+// flag and leaves; the host then reports the solve as failed.
+
+constexpr int kSfSpin = 1 << 20;
+
+template <class T> struct SfBits;
+template <> struct SfBits<double> {
+    using U = unsigned long long;
+    static constexpr U kMark = 0x7FF4DEADBEEF0001ull;   // signalling NaN
+};
+template <> struct SfBits<float> {
+    using U = unsigned;
+    static constexpr U kMark = 0x7FA0BEEFu;               // signalling NaN
+};
+
+template <class T>
+__device__ inline typename SfBits<T>::U sf_load_bits(const T *p) {
+    return __hip_atomic_load(reinterpret_cast<const typename SfBits<T>::U *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__device__ inline void sf_store(T *p, T v) {
+    using U = typename SfBits<T>::U;
+    U b;
+    __builtin_memcpy(&b, &v, sizeof(U));
+    __hip_atomic_store(reinterpret_cast<U *>(p), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class T>
+__global__ void k_sf_mark(int n, T *__restrict__ w, const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using U = typename SfBits<T>::U;
+    reinterpret_cast<U *>(w)[i] = SfBits<T>::kMark;
+}
+
+// BWD == false: out[i] = (rhs[i] - sum a w) d over the strict lower triangle;
+// BWD == true:  out[i] = (fwd[i] / (d + small) - sum a w) d over the strict upper one.
+template <class T, int W, bool BWD>
+__global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restrict__ rows,
+                                                        const int *__restrict__ cntp, const int *__restrict__ col,
+                                                        const T *__restrict__ av, const T *__restrict__ dv,
+                                                        const T *__restrict__ in, T small, T *out, SyncFree *sf,
+                                                        const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    using U = typename SfBits<T>::U;
+    const int lane = threadIdx.x;
+    const unsigned nblk = (unsigned)((n + 63) / 64);
+    for (;;) {
+        unsigned b = 0;
+        if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = __shfl(b, 0);
+        if (b >= nblk) break;
+        const int p = (int)b * 64 + lane;
+        bool done = p >= n;
+        int i = 0, c = 0;
+        int cj[W];
+        T ak[W], wk[W];
+        T acc = T(0), di = T(0);
+        unsigned ready = 0;
+        if (!done) {
+            i = rows[p];
+            c = cntp[p];
+            di = dv[p];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                cj[k] = col[(size_t)k * n + p];
+                ak[k] = av[(size_t)k * n + p];
+                wk[k] = T(0);
+            }
+            acc = BWD ? in[i] / (di + small) : in[i];
+        }
+        int spins = 0;
+        for (;;) {
+            if (!done) {
+                bool all = true;
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    if (k < c && !((ready >> k) & 1u)) {
+                        const U bits = sf_load_bits(out + cj[k]);
+                        if (bits != SfBits<T>::kMark) {
+                            T v;
+                            __builtin_memcpy(&v, &bits, sizeof(U));
+                            wk[k] = v;
+                            ready |= 1u << k;
+                        } else {
+                            all = false;
+                        }
+                    }
+                }
+                if (all) {
+                    T r = acc;
+#pragma unroll
+                    for (int k = 0; k < W; ++k)
+                        if (k < c) r = r - ak[k] * wk[k];
+                    sf_store(out + i, r * di);
+                    done = true;
+                }
+            }
+            if (__all(done)) break;
+            ++spins;
+            bool give_up = spins > kSfSpin;
+            if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                give_up = true;
+            if (__any(give_up)) {
+                if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    if (lane == 0) {
+        const unsigned f = __hip_atomic_fetch_add(&sf->fin, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (f == gridDim.x - 1) {
+            __hip_atomic_store(&sf->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sf->fin, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+bool syncfree_sweeps(const Structure &st) {
+    auto ok = [](int w) { return w == 3 || w == 4 || w == 8; };
+    return packed_sweeps(st) && ok(st.ell_f) && ok(st.ell_b);
+}
+
+template <class T, int WF, int WB>
+void lu_syncfree_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                   const Scalars *sc, const SweepCfg &cfg) {
+    const int n = st.n;
+    T *tmp = static_cast<T *>(cfg.tmp);
+    const int waves = std::max(1, std::min(cfg.waves, (n + 63) / 64));
+    k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, tmp, sc);
+    k_sweep_syncfree<T, WF, false><<<waves, 64, 0, s>>>(n, st.rows_f.as<int>(), st.cnt_f.as<int>(),
+                                                          st.col_f.as<int>(), pf.af, pf.df, rhs, small, tmp, cfg.sf,
+                                                          sc);
+    k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, w, sc);
+    k_sweep_syncfree<T, WB, true><<<waves, 64, 0, s>>>(n, st.rows_b.as<int>(), st.cnt_b.as<int>(),
+                                                         st.col_b.as<int>(), pf.ab, pf.db, tmp, small, w, cfg.sf, sc);
+}
+
+template <class T, int WF>
+void lu_syncfree_b(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                   const Scalars *sc, const SweepCfg &cfg) {
+    switch (st.ell_b) {
+    case 3: lu_syncfree_w<T, WF, 3>(s, st, pf, small, rhs, w, sc, cfg); break;
+    case 4: lu_syncfree_w<T, WF, 4>(s, st, pf, small, rhs, w, sc, cfg); break;
+    default: lu_syncfree_w<T, WF, 8>(s, st, pf, small, rhs, w, sc, cfg); break;
+    }
+}
+
+template <class T>
+void lu_syncfree(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                 const Scalars *sc, const SweepCfg &cfg) {
+    FC_REQUIRE(syncfree_sweeps(st) && cfg.tmp && cfg.sf, FC_ERR_ARGUMENT, "dependency-driven sweeps not set up");
+    switch (st.ell_f) {
+    case 3: lu_syncfree_b<T, 3>(s, st, pf, small, rhs, w, sc, cfg); break;
+    case 4: lu_syncfree_b<T, 4>(s, st, pf, small, rhs, w, sc, cfg); break;
+    default: lu_syncfree_b<T, 8>(s, st, pf, small, rhs, w, sc, cfg); break;
+    }
+    FC_HIP_CHECK(hipGetLastError());
+}
+
 template <class T>
 void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
                  const Scalars *sc, const SweepCfg *cfg, const PackedFactor<T> *pf) {
     if (cfg && cfg->bar) {
         lu_persistent<T>(s, st, a, d, small, rhs, w, sc, *cfg);
+        return;
+    }
+    if (cfg && cfg->sf && pf && pf->af) {
+        lu_syncfree<T>(s, st, *pf, small, rhs, w, sc, *cfg);
         return;
     }
     if (pf && pf->af) {

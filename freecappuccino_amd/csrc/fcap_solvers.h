@@ -14,6 +14,8 @@ struct Workspace {
     DevBuf r32, z32, p32, q32, e32, a32, d32;
     // reduction partials, summed scalars, device solver state, error flag
     DevBuf pa, pb, sums, sc, err, bar;
+    // dependency-driven sweeps: their ticket state and the forward sweep's result
+    DevBuf sf, sf_tmp;
     // flexible CG: the last m directions p and their images A p (FP32, m x n each)
     DevBuf ph, qh;
     // the current solve's factor in level-packed order (PackedFactor), FP64 and FP32

@@ -512,6 +512,8 @@ void Workspace::ensure(int n_, int nnz_) {
     sc.ensure(sizeof(Scalars));
     err.ensure(sizeof(int) * 4);
     bar.ensure(sizeof(GridBar));
+    sf.ensure(sizeof(SyncFree));
+    sf_tmp.ensure(D);
     n = n_;
     nnz = nnz_;
 }
@@ -523,12 +525,25 @@ static Ctx make_ctx(fc_context *ctx, int n) {
 }
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
+    if (c.ctx->sweep_mode == 2 && c.ctx->packed && syncfree_sweeps(st)) {
+        FC_HIP_CHECK(hipMemsetAsync(c.ws.sf.get(), 0, sizeof(SyncFree), c.s));
+        c.sw.sf = c.ws.sf.as<SyncFree>();
+        c.sw.tmp = c.ws.sf_tmp.get();
+        c.sw.waves = c.ctx->sf_waves > 0 ? c.ctx->sf_waves : 8 * c.ctx->num_cus;
+        return;
+    }
     if (c.ctx->sweep_mode != 1) return;
     FC_HIP_CHECK(hipMemsetAsync(c.ws.bar.get(), 0, sizeof(GridBar), c.s));
     c.sw = make_sweep_cfg(c.ctx, st, c.ws.bar.as<GridBar>());
 }
 
 static void check_sweeps(Ctx &c) {
+    if (c.sw.sf) {
+        SyncFree h;
+        FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.sf, sizeof(SyncFree), hipMemcpyDeviceToHost, c.s));
+        FC_HIP_CHECK(hipStreamSynchronize(c.s));
+        if (h.abort) throw Error(FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its spin limit");
+    }
     if (!c.sw.bar) return;
     GridBar h;
     FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.bar, sizeof(GridBar), hipMemcpyDeviceToHost, c.s));

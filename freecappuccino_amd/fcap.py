@@ -35,6 +35,7 @@ VARIANT_SRC, VARIANT_TESTS = 0, 1
 METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
 METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
+SWEEP_LEVELS, SWEEP_BARRIER, SWEEP_DEPENDENT = 0, 1, 2
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
 
@@ -73,6 +74,7 @@ def library() -> C.CDLL:
         "fc_last_error": ([vp], C.c_char_p), "fc_set_report": ([vp, i, C.c_char_p], i),
         "fc_set_variant": ([vp, i], i),
         "fc_set_inner_tolerance": ([vp, d], i), "fc_set_fcg_window": ([vp, i], i),
+        "fc_set_sweep_mode": ([vp, i], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
@@ -197,6 +199,11 @@ class Context:
     def set_fcg_window(self, m: int):
         """Directions kept by flexible CG (1..16, rounded up to a multiple of 4)."""
         self.check(self.L.fc_set_fcg_window(self.h, int(m)))
+
+    def set_sweep_mode(self, mode: int):
+        """Schedule of the triangular sweeps: SWEEP_LEVELS, SWEEP_BARRIER or SWEEP_DEPENDENT
+        (bit-identical results)."""
+        self.check(self.L.fc_set_sweep_mode(self.h, int(mode)))
 
     def set_report(self, on: bool, name: str = "p"):
         self.check(self.L.fc_set_report(self.h, 1 if on else 0, name.encode()))

@@ -274,6 +274,18 @@ fc_status fc_set_variant(fc_context *ctx, int variant);
 fc_status fc_set_inner_tolerance(fc_context *ctx, double eta);
 fc_status fc_set_fcg_window(fc_context *ctx, int m);
 
+/* Schedule of the DIC/DILU triangular sweeps (src/iccg.f90:94-111,
+ * src/bicgstab.f90:107-124); every schedule computes each row with the same
+ * operations in the same order, so results are bit-identical.
+ *   FC_SWEEP_LEVELS    one launch per level (default);
+ *   FC_SWEEP_BARRIER   one cooperative launch per sweep, grid barrier per level;
+ *   FC_SWEEP_DEPENDENT one launch per sweep, each row waiting only for the rows
+ *                      it reads (packed structures of width 3, 4 or 8; other
+ *                      structures keep the level launches).
+ * No counterpart in the reference. */
+enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_BARRIER = 1, FC_SWEEP_DEPENDENT = 2 };
+fc_status fc_set_sweep_mode(fc_context *ctx, int mode);
+
 /* ------------------------------------------------ device-resident system */
 
 enum {

@@ -94,6 +94,34 @@ def test_forward_matches_oracle(ctx, small, pseed):
     net.close()
 
 
+@pytest.mark.parametrize("mesh", ["hex", "tet"])
+def test_fused_first_layer_matches_stored_encoder_rows(ctx, monkeypatch, mesh):
+    """Inference computes the encoder rows inside the first layer from u (4 bytes gathered per
+    neighbour); they are the BF16 values the separate encoder pass stores, so z is bitwise the
+    same.  ELL widths 6 (hex) and 4 (tet)."""
+    m = M.hex_box(6, perturb=0.2, seed=2) if mesh == "hex" else M.tet_box(5, grade_axis=1, grade_ratio=1.1,
+                                                                          permute_seed=3)
+    th = _params(4)
+    r = np.random.default_rng(21).standard_normal(m.n_cells).astype(np.float32)
+    s1 = _system(m, ctx)
+    net1 = gnn.Network(th, ctx)
+    z1 = net1.apply(s1, r)
+    net1.close()
+    s1.close()
+    monkeypatch.setenv("FCAP_GNN_FUSE", "0")
+    c2 = fcap.Context(0)
+    try:
+        s2 = _system(m, c2)
+        net2 = gnn.Network(th, c2)
+        z2 = net2.apply(s2, r)
+        net2.close()
+        s2.close()
+    finally:
+        c2.close()
+    assert np.abs(z1).max() > 0
+    np.testing.assert_array_equal(z1, z2)
+
+
 @pytest.mark.parametrize("pseed", [3, 5])
 def test_loss_and_gradient_match_oracle(ctx, small, pseed):
     m, A, graph, feats, s = small

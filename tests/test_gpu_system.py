@@ -104,6 +104,27 @@ def test_mixed_precision_reaches_fp64_tolerance(case, method):
     assert err < 1e-4, err
 
 
+@pytest.mark.parametrize("method", [fcap.METHOD_ICCG, fcap.METHOD_BICGSTAB, fcap.METHOD_ICCG_MIXED,
+                                    fcap.METHOD_BICGSTAB_MIXED])
+def test_dependent_sweeps_give_identical_solves(case, ctx, method):
+    """Every schedule of the triangular sweeps computes each row with the same operations in the
+    same order: iterations and solutions are bitwise those of the level launches."""
+    name, m, geo, csr, sys_ = case
+    sys_.assemble_laplacian(np.full(m.n_cells, -1.0), None, _rhs(geo, 8, 13))
+    out = []
+    try:
+        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT):
+            ctx.set_sweep_mode(mode)
+            sys_.set_solution(np.zeros(m.n_cells))
+            rc, info = sys_.solve(method, fcap.STOP_L2, 1e-8, 5000)
+            assert rc == 0, (mode, info)
+            out.append((info["iters"], info["outer"], sys_.solution()))
+    finally:
+        ctx.set_sweep_mode(fcap.SWEEP_LEVELS)
+    assert out[0][:2] == out[1][:2]
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
 def test_convdiff_resident_bitwise_and_bicgstab(ctx):
     m = M.tet_box(6, grade_axis=1, grade_ratio=1.1, permute_seed=9)
     geo = O.mesh_geometry(m)
