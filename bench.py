@@ -7,12 +7,15 @@ Krylov solve to 1e-8 FP64 relative residual ||su - A fi||_2 / ||su||_2.
 Default workload: 192^3 hex box (7,077,888 cells), interior vertices
 perturbed by +-20% (seeded), spacing graded along x by ratio 1.05 toward the
 wall (aspect ratio capped at 100), mu = -1, Dirichlet wall at x = 0, RHS = 32
-seeded Gaussian blobs with zero mean.  Three solvers run on the same system:
-  * ic_pcg_fp64  -- IC(0)-PCG in FP64 (the reference's DIC-PCG, level-scheduled sweeps);
-  * ic_pcg_mixed -- FP32 DIC-PCG inside FP64 iterative refinement;
+seeded Gaussian blobs with zero mean.  Four solves run on the same system:
+  * ic_pcg_fp64  -- IC(0)-PCG in FP64 (the reference's DIC-PCG; triangular sweeps
+                    with one launch per sweep, each row waiting for the rows it reads);
+  * ic_pcg_mixed -- FP32 DIC-PCG inside FP64 iterative refinement (same sweeps);
   * fcg_learned  -- flexible CG in FP32 preconditioned by the trained BF16
                     message-passing network, inside FP64 iterative refinement,
-                    IC(0)-PCG fallback on stagnation.
+                    IC(0)-PCG fallback on stagnation;
+  * ic_pcg_fp64_levels -- ic_pcg_fp64 with one sweep launch per level (the same
+                    bits, the schedule before the dependency-driven one).
 `value` is the time to 1e-8 of fcg_learned (ms per step, lower is better);
 every solver's time, iterations, FP64 residual and error against the FP64
 IC(0)-PCG solution are in `solvers`.
@@ -44,7 +47,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned")
+METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "ic_pcg_fp64_levels")
 
 # HBM bytes per launch of the hidden network layer, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
@@ -346,7 +349,7 @@ def main():
         net = gnn.Network.load(args.weights or gnn.DEFAULT_WEIGHTS, ctx)
         net.attach(s)
     code = {"ic_pcg_fp64": fcap.METHOD_ICCG, "ic_pcg_mixed": fcap.METHOD_ICCG_MIXED,
-            "fcg_learned": fcap.METHOD_FCG_LEARNED}
+            "fcg_learned": fcap.METHOD_FCG_LEARNED, "ic_pcg_fp64_levels": fcap.METHOD_ICCG}
 
     def step(method):
         s.set_solution(np.zeros(n))
@@ -357,6 +360,7 @@ def main():
 
     results, sols = {}, {}
     for name in methods:
+        ctx.set_sweep_mode(fcap.SWEEP_LEVELS if name == "ic_pcg_fp64_levels" else fcap.SWEEP_DEFAULT)
         for _ in range(args.warmup):
             step(code[name])
         times, infos = [], []
@@ -371,6 +375,10 @@ def main():
                          "solve_device_ms": info["solve_ms"] + info["setup_ms"]}
         sols[name] = s.solution()
         print(f"{name}: {results[name]}", file=sys.stderr, flush=True)
+    ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
+    if "ic_pcg_fp64" in sols and "ic_pcg_fp64_levels" in sols:
+        results["ic_pcg_fp64_levels"]["bitwise_equal_to_ic_pcg_fp64"] = bool(
+            np.array_equal(sols["ic_pcg_fp64"], sols["ic_pcg_fp64_levels"]))
     if net is not None:
         layer_ms = net.bench_layer(s, args.spmv_reps)
         net.detach(s)
@@ -396,7 +404,8 @@ def main():
         "metric": "pressure-solve time to 1e-8 rel. residual, learned-FCG vs IC-PCG; SpMV GB/s",
         "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"ic_pcg_fp64": "f64", "ic_pcg_mixed": "f32+f64", "fcg_learned": "bf16+f32+f64"}[head],
+        "dtype": {"ic_pcg_fp64": "f64", "ic_pcg_mixed": "f32+f64", "fcg_learned": "bf16+f32+f64",
+                  "ic_pcg_fp64_levels": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": workload, "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"],
                    "solver": head},

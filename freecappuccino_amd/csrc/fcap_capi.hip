@@ -872,7 +872,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
             FC_HIP_CHECK(hipMemsetAsync(sfs.get(), 0, sizeof(SyncFree), ctx->stream));
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
-            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : 8 * ctx->num_cus;
+            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;
         }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
                     nullptr, &cfg, pf.af ? &pf : nullptr);

@@ -140,8 +140,8 @@ struct fc_context {
     int use_graphs = 1;
     // triangular sweeps: 0 one launch per level, 1 persistent cooperative kernel
     // with a grid barrier per level (FCAP_SWEEP selects)
-    int sweep_mode = 0;
-    int sf_waves = 0;   // persistent waves of the dependency-driven sweeps (0: 8 per CU)
+    int sweep_mode = 2;   // FC_SWEEP_DEPENDENT
+    int sf_waves = 0;     // persistent waves of the dependency-driven sweeps (0: one per CU)
     // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
     int packed = 1;
     // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)

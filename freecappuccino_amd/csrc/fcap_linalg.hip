@@ -541,7 +541,11 @@ void pack_factor_t(hipStream_t s, const Structure &st, const T *a, const T *d, T
 // whose wait exceeds kSfSpin polls, or that sees the abort flag, raises the
 // flag and leaves; the host then reports the solve as failed.
 
-constexpr int kSfSpin = 1 << 20;
+// Waves in flight poll the coherent level of the memory system; the fewer there
+// are, the faster the chain of levels advances (192^3 box: 256 waves 1.87 s,
+// 512 1.96 s, 2048 2.55 s, 4096 3.61 s per FP64 IC(0)-PCG solve; at 8192 a wait
+// ran past 2^20 polls).  The limit only ends waits that cannot end otherwise.
+constexpr int kSfSpin = 1 << 22;
 
 template <class T> struct SfBits;
 template <> struct SfBits<double> {

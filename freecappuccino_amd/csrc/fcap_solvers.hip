@@ -529,7 +529,7 @@ static void setup_sweeps(Ctx &c, const Structure &st) {
         FC_HIP_CHECK(hipMemsetAsync(c.ws.sf.get(), 0, sizeof(SyncFree), c.s));
         c.sw.sf = c.ws.sf.as<SyncFree>();
         c.sw.tmp = c.ws.sf_tmp.get();
-        c.sw.waves = c.ctx->sf_waves > 0 ? c.ctx->sf_waves : 8 * c.ctx->num_cus;
+        c.sw.waves = c.ctx->sf_waves > 0 ? c.ctx->sf_waves : c.ctx->num_cus;
         return;
     }
     if (c.ctx->sweep_mode != 1) return;

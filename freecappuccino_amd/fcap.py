@@ -36,6 +36,7 @@ METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
 METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
 SWEEP_LEVELS, SWEEP_BARRIER, SWEEP_DEPENDENT = 0, 1, 2
+SWEEP_DEFAULT = SWEEP_DEPENDENT
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
 

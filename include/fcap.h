@@ -277,11 +277,12 @@ fc_status fc_set_fcg_window(fc_context *ctx, int m);
 /* Schedule of the DIC/DILU triangular sweeps (src/iccg.f90:94-111,
  * src/bicgstab.f90:107-124); every schedule computes each row with the same
  * operations in the same order, so results are bit-identical.
- *   FC_SWEEP_LEVELS    one launch per level (default);
+ *   FC_SWEEP_LEVELS    one launch per level;
  *   FC_SWEEP_BARRIER   one cooperative launch per sweep, grid barrier per level;
  *   FC_SWEEP_DEPENDENT one launch per sweep, each row waiting only for the rows
- *                      it reads (packed structures of width 3, 4 or 8; other
- *                      structures keep the level launches).
+ *                      it reads (default; packed structures of width 3, 4 or 8,
+ *                      other structures keep the level launches).  A wait that
+ *                      runs past its poll limit fails the call with FC_ERR_HIP.
  * No counterpart in the reference. */
 enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_BARRIER = 1, FC_SWEEP_DEPENDENT = 2 };
 fc_status fc_set_sweep_mode(fc_context *ctx, int mode);

@@ -106,20 +106,21 @@ def test_spmv_pivots_sweeps_bitwise(ctx, golden_loader, case):
 
 @pytest.mark.parametrize("case", MESH_CASES + CONVDIFF_CASES)
 def test_sweeps_dependent_schedule_bitwise(ctx, golden_loader, case):
-    """The dependency-driven sweeps (one launch per sweep, each row waiting for the rows it
-    reads) give the oracle's DIC and DILU sweeps bit for bit."""
+    """Both schedules of the sweeps, one launch per level and one launch per sweep with each row
+    waiting for the rows it reads, give the oracle's DIC and DILU sweeps bit for bit."""
     g = golden_loader(case)
     csr = _csr_of(g)
     oc = O.CSR(csr.ioffset, csr.ja, csr.diag, csr.ij_slot, csr.ji_slot)
     a = g.inp("a_convdiff") if case in CONVDIFF_CASES else g.out(g.first, "a")
     d = fcap.dilu_pivots(csr, a, ctx) if case in CONVDIFF_CASES else fcap.dic_pivots(csr, a, ctx)
     r = np.random.default_rng(2).standard_normal(csr.diag.shape[0])
+    ref = O.lu_apply(oc, a, d, r)
     try:
-        ctx.set_sweep_mode(fcap.SWEEP_DEPENDENT)
-        z = fcap.lu_apply(csr, a, d, r, ctx)
+        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT):
+            ctx.set_sweep_mode(mode)
+            np.testing.assert_array_equal(fcap.lu_apply(csr, a, d, r, ctx), ref, err_msg=str(mode))
     finally:
-        ctx.set_sweep_mode(fcap.SWEEP_LEVELS)
-    np.testing.assert_array_equal(z, O.lu_apply(oc, a, d, r))
+        ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
 
 
 @pytest.mark.parametrize("case", CONVDIFF_CASES)

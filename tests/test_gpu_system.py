@@ -120,7 +120,7 @@ def test_dependent_sweeps_give_identical_solves(case, ctx, method):
             assert rc == 0, (mode, info)
             out.append((info["iters"], info["outer"], sys_.solution()))
     finally:
-        ctx.set_sweep_mode(fcap.SWEEP_LEVELS)
+        ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
     assert out[0][:2] == out[1][:2]
     np.testing.assert_array_equal(out[0][2], out[1][2])
 
