@@ -249,6 +249,50 @@ def profile_momentum(mesh, ctx, reps, cpu_n, sor=1e-8):
     return res
 
 
+def profile_io(mesh, nthreads, workdir):
+    """Host-side rows of the step: the polyMesh reader on the workload mesh written in both
+    layouts (native 1-based, OpenFOAM ASCII), and a restart file of the mesh's fields written
+    and read back (FIELDS of freecappuccino_amd.restart, the reference's record list)."""
+    import shutil
+    import tempfile
+
+    from freecappuccino_amd import mesh as M
+    from freecappuccino_amd import polymesh, restart
+    res = {"cells": mesh.n_cells, "faces": mesh.n_faces, "threads": nthreads}
+    d = tempfile.mkdtemp(prefix="fcap_io_", dir=workdir)
+    try:
+        for fmt in ("native", "foam"):
+            pm = os.path.join(d, fmt)
+            M.write_polymesh(mesh, pm, fmt=fmt)
+            nbytes = sum(os.path.getsize(os.path.join(pm, f)) for f in os.listdir(pm))
+            for nt in sorted({1, nthreads}):
+                t0 = time.perf_counter()
+                got = polymesh.read_polymesh(pm, nthreads=nt)
+                dt = time.perf_counter() - t0
+                assert got.n_cells == mesh.n_cells and got.n_faces == mesh.n_faces
+                res[f"polymesh_{fmt}_{nt}t"] = {"s": dt, "MB": nbytes / 1e6, "MB_per_s": nbytes / dt / 1e6}
+                print(f"polyMesh {fmt} {nt} threads: {dt:.2f} s", file=sys.stderr, flush=True)
+            shutil.rmtree(pm)
+        n, nif, ntot = mesh.n_cells, mesh.n_inner, mesh.n_cells + mesh.n_boundary
+        sizes = {"flmass": nif, "uu": n, "vv": n, "ww": n, "uv": n, "uw": n, "vw": n}
+        rng = np.random.default_rng(5)
+        fields = {k: rng.standard_normal(sizes.get(k, ntot)) for k in restart.FIELDS}
+        nbytes = 8 * sum(a.size for a in fields.values())
+        path = os.path.join(d, "restart.bin")
+        t0 = time.perf_counter()
+        restart.write(path, fields, itime=100, time=1.5)
+        tw = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, _, back = restart.read(path)
+        tr = time.perf_counter() - t0
+        assert all(np.array_equal(back[k], fields[k]) for k in restart.FIELDS)
+        res["restart"] = {"MB": nbytes / 1e6, "write_s": tw, "read_s": tr, "write_MB_per_s": nbytes / tw / 1e6,
+                          "read_MB_per_s": nbytes / tr / 1e6}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
 def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
     """The oracle's sequential DIC-PCG on one host core, `budget_iters` iterations, scaled."""
     from oracle import oracle as O
@@ -287,6 +331,10 @@ def main():
                          "assembly, adjustMassFlow, corrections, non-orthogonal source), then exit")
     ap.add_argument("--profile-momentum", type=int, default=0, metavar="REPS",
                     help="only run REPS passes of calcuvw's three component solves, then exit")
+    ap.add_argument("--profile-io", action="store_true",
+                    help="only time the polyMesh reader and a restart write/read on the workload mesh, then exit")
+    ap.add_argument("--io-threads", type=int, default=16, help="reader threads of --profile-io")
+    ap.add_argument("--io-dir", type=str, default=None, help="scratch directory of --profile-io (default: TMPDIR)")
     ap.add_argument("--calcp-cpu-n", type=int, default=128,
                     help="cells per direction of the CPU baseline sample of --profile-calcp and "
                          "--profile-momentum (0: none)")
@@ -304,6 +352,9 @@ def main():
 
     args.n = args.n or {1: 40, 2: 128, 5: 192}[args.case]
     mesh = W.case_mesh(args.case, args.n)
+    if args.profile_io:
+        print(json.dumps(profile_io(mesh, args.io_threads, args.io_dir)))
+        return
     ctx = fcap.Context(0)
     if args.profile_calcp or args.profile_momentum:
         cpu_n = 0 if args.no_cpu else args.calcp_cpu_n

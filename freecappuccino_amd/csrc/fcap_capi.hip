@@ -819,7 +819,20 @@ static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const
         upload_dbl(ctx, da, a, nnz);
         dd.alloc(sizeof(double) * n);
         FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx->stream));
-        pivots_launch(ctx->stream, st, da.as<double>(), dd.as<double>(), dilu, err.as<int>());
+        SweepCfg cfg;
+        DevBuf sfs;
+        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && ctx->packed && syncfree_sweeps(st)) {
+            sfs.alloc(sizeof(SyncFree));
+            FC_HIP_CHECK(hipMemsetAsync(sfs.get(), 0, sizeof(SyncFree), ctx->stream));
+            cfg.sf = sfs.as<SyncFree>();
+            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;
+        }
+        pivots_launch(ctx->stream, st, da.as<double>(), dd.as<double>(), dilu, err.as<int>(), &cfg);
+        if (cfg.sf) {
+            SyncFree hs;
+            d2h(&hs, sfs.get(), sizeof(SyncFree), ctx->stream);
+            FC_REQUIRE(!hs.abort, FC_ERR_HIP, "dependency-driven pivots: a wait exceeded its spin limit");
+        }
         int h = 0;
         d2h(&h, err.get(), sizeof(int), ctx->stream);
         download_dbl(ctx, d, dd.get(), n);

@@ -74,7 +74,10 @@ void pack_factor64(hipStream_t s, const Structure &st, const double *a, const do
 void pack_factor32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
                    float *ab, float *db);
 
-void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err);
+// DIC (src/iccg.f90:77-83) / DILU (src/bicgstab.f90:68-79) pivots: one launch per
+// level, or with cfg->sf one dependency-driven launch (bit-identical).
+void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err,
+                   const SweepCfg *cfg = nullptr);
 // z = M^-1 rhs with the sweeps of src/iccg.f90:94-111: level-packed kernels when
 // `pf` is given, one launch per level over the CSR rows otherwise, or the
 // persistent cooperative kernels when cfg->bar is set.

@@ -267,7 +267,22 @@ __global__ void k_bwd_level(const int *__restrict__ rows, int beg, int cnt, cons
     w[i] = acc * di;
 }
 
-void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err) {
+template <int W>
+void pivots_syncfree_w(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err,
+                       const SweepCfg &cfg);
+
+void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err,
+                   const SweepCfg *cfg) {
+    if (cfg && cfg->sf && syncfree_sweeps(st)) {
+        FC_REQUIRE(!dilu || st.tpos.get(), FC_ERR_ARGUMENT, "DILU pivots need the transpose positions");
+        switch (st.ell_f) {
+        case 3: pivots_syncfree_w<3>(s, st, a, d, dilu, err, *cfg); break;
+        case 4: pivots_syncfree_w<4>(s, st, a, d, dilu, err, *cfg); break;
+        default: pivots_syncfree_w<8>(s, st, a, d, dilu, err, *cfg); break;
+        }
+        FC_HIP_CHECK(hipGetLastError());
+        return;
+    }
     const int *rows = st.rows_f.as<int>();
     for (int l = 0; l < st.nlev_f; ++l) {
         int beg = st.lvl_f[l], cnt = st.lvl_f[l + 1] - beg;
@@ -662,6 +677,113 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
             __hip_atomic_store(&sf->fin, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+// The pivot recurrences of k_dic_level / k_dilu_level with the same schedule:
+// d_i = 1 / (a_ii - sum over the strict lower row, in CSR order, of a_k^2 d_j
+// (DIC) or (a_k d_j) a_kT (DILU)); W bounds the strict lower row (the packed
+// width of the forward triangle).
+template <int W, bool DILU>
+__global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__restrict__ rows, const int *__restrict__ ptr,
+                                                        const int *__restrict__ ja, const int *__restrict__ diag,
+                                                        const int *__restrict__ tpos, const double *__restrict__ a,
+                                                        double *d, int *__restrict__ err, SyncFree *sf) {
+    using U = SfBits<double>::U;
+    const int lane = threadIdx.x;
+    const unsigned nblk = (unsigned)((n + 63) / 64);
+    for (;;) {
+        unsigned b = 0;
+        if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = __shfl(b, 0);
+        if (b >= nblk) break;
+        const int p = (int)b * 64 + lane;
+        bool done = p >= n;
+        int i = 0, c = 0;
+        int cj[W];
+        double a1[W], a2[W], dk[W];
+        double di = 0.0;
+        unsigned ready = 0;
+        if (!done) {
+            i = rows[p];
+            const int k0 = ptr[i], kd = diag[i];
+            c = kd - k0;
+            di = a[kd];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const bool in = k < c;
+                cj[k] = in ? ja[k0 + k] : 0;
+                a1[k] = in ? a[k0 + k] : 0.0;
+                a2[k] = in && DILU ? a[tpos[k0 + k]] : 0.0;
+                dk[k] = 0.0;
+            }
+            if (c > W) {   // not a structure these sweeps were set up for: report, never wait
+                atomicOr(err, 2);
+                sf_store(d + i, __builtin_nan(""));
+                done = true;
+            }
+        }
+        int spins = 0;
+        for (;;) {
+            if (!done) {
+                bool all = true;
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    if (k < c && !((ready >> k) & 1u)) {
+                        const U bits = sf_load_bits(d + cj[k]);
+                        if (bits != SfBits<double>::kMark) {
+                            __builtin_memcpy(&dk[k], &bits, sizeof(U));
+                            ready |= 1u << k;
+                        } else {
+                            all = false;
+                        }
+                    }
+                }
+                if (all) {
+                    double v = di;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        if (k < c) v = DILU ? v - (a1[k] * dk[k]) * a2[k] : v - (a1[k] * a1[k]) * dk[k];
+                    }
+                    if (!(v != 0.0) || !isfinite(v)) atomicOr(err, 1);
+                    sf_store(d + i, 1.0 / v);
+                    done = true;
+                }
+            }
+            if (__all(done)) break;
+            ++spins;
+            bool give_up = spins > kSfSpin;
+            if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                give_up = true;
+            if (__any(give_up)) {
+                if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    if (lane == 0) {
+        const unsigned f = __hip_atomic_fetch_add(&sf->fin, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (f == gridDim.x - 1) {
+            __hip_atomic_store(&sf->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sf->fin, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <int W>
+void pivots_syncfree_w(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err,
+                       const SweepCfg &cfg) {
+    const int n = st.n;
+    const int waves = std::max(1, std::min(cfg.waves, (n + 63) / 64));
+    k_sf_mark<double><<<grid_for(n, 256), 256, 0, s>>>(n, d, nullptr);
+    if (dilu)
+        k_pivots_syncfree<W, true><<<waves, 64, 0, s>>>(n, st.rows_f.as<int>(), st.ioffset.as<int>(),
+                                                         st.ja.as<int>(), st.diag.as<int>(), st.tpos.as<int>(), a, d,
+                                                         err, cfg.sf);
+    else
+        k_pivots_syncfree<W, false><<<waves, 64, 0, s>>>(n, st.rows_f.as<int>(), st.ioffset.as<int>(),
+                                                          st.ja.as<int>(), st.diag.as<int>(), nullptr, a, d, err,
+                                                          cfg.sf);
 }
 
 bool syncfree_sweeps(const Structure &st) {

@@ -643,7 +643,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
     initial_residual(c, args, r);
     PackedFactor<double> pf;
     if (dic) {
-        pivots_launch(c.s, st, args.a, d, false, ws.err.as<int>());
+        pivots_launch(c.s, st, args.a, d, false, ws.err.as<int>(), &c.sw);
         pf = pack_for_solve<double>(c, st, args.a, d);
     }
     double setup_ms = ctx->timer.stop_ms(c.s);
@@ -701,7 +701,7 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
     upload_scalars(c, h);
     FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
     initial_residual(c, args, res);
-    pivots_launch(c.s, st, args.a, d, true, ws.err.as<int>());
+    pivots_launch(c.s, st, args.a, d, true, ws.err.as<int>(), &c.sw);
     const PackedFactor<double> pf = pack_for_solve<double>(c, st, args.a, d);
     const PackedFactor<double> *ppf = pf.af ? &pf : nullptr;
     k_copy<double><<<c.gv, 256, 0, c.s>>>(n, res, reso);
@@ -813,7 +813,7 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     ctx->timer.start(c.s);
     FC_HIP_CHECK(hipMemsetAsync(ws.err.get(), 0, sizeof(int), c.s));
     if (bicg) build_tpos(ctx, const_cast<Structure &>(st));
-    pivots_launch(c.s, st, args.a, d, bicg, ws.err.as<int>());
+    pivots_launch(c.s, st, args.a, d, bicg, ws.err.as<int>(), &c.sw);
     k_cast<double, float><<<std::min(grid_for(st.nnz, 256), 8192), 256, 0, c.s>>>(st.nnz, args.a, ws.a32.as<float>());
     k_cast<double, float><<<c.gv, 256, 0, c.s>>>(n, d, ws.d32.as<float>());
     const PackedFactor<float> pf = pack_for_solve<float>(c, st, ws.a32.as<float>(), ws.d32.as<float>());

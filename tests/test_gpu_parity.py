@@ -106,19 +106,22 @@ def test_spmv_pivots_sweeps_bitwise(ctx, golden_loader, case):
 
 @pytest.mark.parametrize("case", MESH_CASES + CONVDIFF_CASES)
 def test_sweeps_dependent_schedule_bitwise(ctx, golden_loader, case):
-    """Both schedules of the sweeps, one launch per level and one launch per sweep with each row
-    waiting for the rows it reads, give the oracle's DIC and DILU sweeps bit for bit."""
+    """Both schedules of the pivots and sweeps, one launch per level and one launch per sweep with
+    each row waiting for the rows it reads, give the oracle's DIC and DILU values bit for bit."""
     g = golden_loader(case)
     csr = _csr_of(g)
     oc = O.CSR(csr.ioffset, csr.ja, csr.diag, csr.ij_slot, csr.ji_slot)
-    a = g.inp("a_convdiff") if case in CONVDIFF_CASES else g.out(g.first, "a")
-    d = fcap.dilu_pivots(csr, a, ctx) if case in CONVDIFF_CASES else fcap.dic_pivots(csr, a, ctx)
+    dilu = case in CONVDIFF_CASES
+    a = g.inp("a_convdiff") if dilu else g.out(g.first, "a")
+    d_ref = O.dilu_pivots(oc, a) if dilu else O.dic_pivots(oc, a)
     r = np.random.default_rng(2).standard_normal(csr.diag.shape[0])
-    ref = O.lu_apply(oc, a, d, r)
+    ref = O.lu_apply(oc, a, d_ref, r)
     try:
         for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT):
             ctx.set_sweep_mode(mode)
-            np.testing.assert_array_equal(fcap.lu_apply(csr, a, d, r, ctx), ref, err_msg=str(mode))
+            d = fcap.dilu_pivots(csr, a, ctx) if dilu else fcap.dic_pivots(csr, a, ctx)
+            np.testing.assert_array_equal(d, d_ref, err_msg=f"pivots, mode {mode}")
+            np.testing.assert_array_equal(fcap.lu_apply(csr, a, d, r, ctx), ref, err_msg=f"sweeps, mode {mode}")
     finally:
         ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
 
