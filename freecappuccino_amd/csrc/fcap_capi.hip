@@ -193,6 +193,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
         if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
         if (const char *m = std::getenv("FCAP_SF_WAVES")) ctx->sf_waves = std::atoi(m);
+        if (const char *m = std::getenv("FCAP_SF_SPIN")) ctx->sf_spin = std::atoi(m);
         if (const char *m = std::getenv("FCAP_GNN_FUSE")) ctx->gnn_fuse_input = std::atoi(m) != 0;
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
@@ -823,7 +824,7 @@ static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const
         DevBuf sfs;
         if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && ctx->packed && syncfree_sweeps(st)) {
             sfs.alloc(sizeof(SyncFree));
-            FC_HIP_CHECK(hipMemsetAsync(sfs.get(), 0, sizeof(SyncFree), ctx->stream));
+            syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;
         }
@@ -882,7 +883,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && pf.af && syncfree_sweeps(st)) {
             sfs.alloc(sizeof(SyncFree));
             sft.alloc(sizeof(double) * n);
-            FC_HIP_CHECK(hipMemsetAsync(sfs.get(), 0, sizeof(SyncFree), ctx->stream));
+            syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
             cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;

@@ -142,6 +142,7 @@ struct fc_context {
     // with a grid barrier per level (FCAP_SWEEP selects)
     int sweep_mode = 2;   // FC_SWEEP_DEPENDENT
     int sf_waves = 0;     // persistent waves of the dependency-driven sweeps (0: one per CU)
+    int sf_spin = 0;      // poll limit of their waits (0: the built-in one)
     // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
     int packed = 1;
     // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)

@@ -37,12 +37,14 @@ struct GridBar {
 
 // State of the dependency-driven sweeps: the next block of 64 packed rows to
 // hand out, the number of waves that have left the kernel (the last one resets
-// both for the next launch) and a sticky abort flag raised when a wait exceeds
-// its spin limit.
+// both for the next launch), a sticky abort flag raised when a wait exceeds
+// its poll limit, and that limit (0: the built-in one).
 struct SyncFree {
     unsigned ticket, fin;
-    int abort, pad;
+    int abort, spin;
 };
+// zeroed state with the context's poll limit, on the stream (not for graph capture)
+void syncfree_reset(fc_context *ctx, hipStream_t s, SyncFree *sf);
 
 // How the triangular sweeps run: one launch per level (default), one
 // persistent cooperative kernel per sweep with `blocks` co-resident workgroups

@@ -553,7 +553,7 @@ void pack_factor_t(hipStream_t s, const Structure &st, const T *a, const T *d, T
 // outputs of the rows it reads until none holds the pattern, computes exactly
 // as k_fwd_packed / k_bwd_packed (same operands, same order), and stores its
 // value with one coherent store, which is also its "ready" signal.  A wave
-// whose wait exceeds kSfSpin polls, or that sees the abort flag, raises the
+// whose wait exceeds the poll limit, or that sees the abort flag, raises the
 // flag and leaves; the host then reports the solve as failed.
 
 // Waves in flight poll the coherent level of the memory system; the fewer there
@@ -607,6 +607,7 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
     using U = typename SfBits<T>::U;
     const int lane = threadIdx.x;
     const unsigned nblk = (unsigned)((n + 63) / 64);
+    const int limit = sf->spin > 0 ? sf->spin : kSfSpin;
     for (;;) {
         unsigned b = 0;
         if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -660,7 +661,7 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
             }
             if (__all(done)) break;
             ++spins;
-            bool give_up = spins > kSfSpin;
+            bool give_up = spins > limit;
             if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 give_up = true;
             if (__any(give_up)) {
@@ -691,6 +692,7 @@ __global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__rest
     using U = SfBits<double>::U;
     const int lane = threadIdx.x;
     const unsigned nblk = (unsigned)((n + 63) / 64);
+    const int limit = sf->spin > 0 ? sf->spin : kSfSpin;
     for (;;) {
         unsigned b = 0;
         if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -751,7 +753,7 @@ __global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__rest
             }
             if (__all(done)) break;
             ++spins;
-            bool give_up = spins > kSfSpin;
+            bool give_up = spins > limit;
             if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 give_up = true;
             if (__any(give_up)) {
@@ -784,6 +786,11 @@ void pivots_syncfree_w(hipStream_t s, const Structure &st, const double *a, doub
         k_pivots_syncfree<W, false><<<waves, 64, 0, s>>>(n, st.rows_f.as<int>(), st.ioffset.as<int>(),
                                                           st.ja.as<int>(), st.diag.as<int>(), nullptr, a, d, err,
                                                           cfg.sf);
+}
+
+void syncfree_reset(fc_context *ctx, hipStream_t s, SyncFree *sf) {
+    FC_HIP_CHECK(hipMemsetAsync(sf, 0, sizeof(SyncFree), s));
+    if (ctx->sf_spin > 0) FC_HIP_CHECK(hipMemcpyAsync(&sf->spin, &ctx->sf_spin, sizeof(int), hipMemcpyHostToDevice, s));
 }
 
 bool syncfree_sweeps(const Structure &st) {

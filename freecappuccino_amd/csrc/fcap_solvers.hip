@@ -526,7 +526,7 @@ static Ctx make_ctx(fc_context *ctx, int n) {
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
     if (c.ctx->sweep_mode == 2 && c.ctx->packed && syncfree_sweeps(st)) {
-        FC_HIP_CHECK(hipMemsetAsync(c.ws.sf.get(), 0, sizeof(SyncFree), c.s));
+        syncfree_reset(c.ctx, c.s, c.ws.sf.as<SyncFree>());
         c.sw.sf = c.ws.sf.as<SyncFree>();
         c.sw.tmp = c.ws.sf_tmp.get();
         c.sw.waves = c.ctx->sf_waves > 0 ? c.ctx->sf_waves : c.ctx->num_cus;

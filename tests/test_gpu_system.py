@@ -125,6 +125,39 @@ def test_dependent_sweeps_give_identical_solves(case, ctx, method):
     np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
+def test_dependent_wait_limit_fails_the_solve_cleanly(monkeypatch):
+    """The waits of the dependency-driven schedule are bounded.  With the poll limit set to 1 from
+    the host, waves give up and leave their kernels, and the solve reports FC_ERR_HIP instead of
+    hanging; a context with the built-in limit then solves the same system."""
+    m = M.hex_box(16, perturb=0.2, seed=4)
+    geo = O.mesh_geometry(m)
+    su = _rhs(geo, 8, 3)
+
+    def solve(c):
+        s = fcap.System(m, c)
+        try:
+            s.assemble_laplacian(np.full(m.n_cells, -1.0), None, su)
+            return s.solve(fcap.METHOD_ICCG, fcap.STOP_L2, 1e-8, 200)
+        finally:
+            s.close()
+
+    monkeypatch.setenv("FCAP_SF_SPIN", "1")
+    c = fcap.Context(0)
+    try:
+        with pytest.raises(fcap.FcError) as e:
+            solve(c)
+        assert "spin limit" in str(e.value)
+    finally:
+        c.close()
+    monkeypatch.delenv("FCAP_SF_SPIN")
+    c = fcap.Context(0)
+    try:
+        rc, info = solve(c)
+        assert rc == 0 and info["rel_l2"] < 1e-8
+    finally:
+        c.close()
+
+
 def test_convdiff_resident_bitwise_and_bicgstab(ctx):
     m = M.tet_box(6, grade_axis=1, grade_ratio=1.1, permute_seed=9)
     geo = O.mesh_geometry(m)
