@@ -193,8 +193,7 @@ __global__ void k_face_flux(int nif, const int *__restrict__ own, const int *__r
 // su(o) -= F, su(n) += F accumulated per cell in face order
 __global__ void k_rhs_divergence(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
                                  const int *__restrict__ kord, const int *__restrict__ kface,
-                                 const int *__restrict__ ij, const double *__restrict__ flux,
-                                 double *__restrict__ su) {
+                                 const double *__restrict__ flux, double *__restrict__ su) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int dp = diagpos[i];
@@ -202,8 +201,8 @@ __global__ void k_rhs_divergence(int n, const int *__restrict__ ptr, const int *
     for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
         int k = kord ? kord[p] : p;
         if (k == dp) continue;
-        int f = kface[k];
-        if (ij[f] == k) s = s - flux[f];     // row i is the owner of face f
+        const int e = kface[k], f = e >> 1;
+        if (!(e & 1)) s = s - flux[f];     // row i is the owner of face f
         else s = s + flux[f];
     }
     su[i] = s;
@@ -303,7 +302,7 @@ __global__ void k_calcp_oc(int noc, int oc_face0, const int *__restrict__ ijl, c
 // its continuity source, 0 -/+ F in face order then the cuts' -/+ fmoc: the
 // values the reference's face loops leave in a(diag(i)) and su(i).
 __global__ void k_calcp_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
-                             const int *__restrict__ kord, const int *__restrict__ kface, const int *__restrict__ ij,
+                             const int *__restrict__ kord, const int *__restrict__ kface,
                              const double *__restrict__ flmass, const int *__restrict__ co_ptr,
                              const int *__restrict__ co_idx, const double *__restrict__ al,
                              const double *__restrict__ ar, const double *__restrict__ fmoc, double *__restrict__ a,
@@ -316,8 +315,8 @@ __global__ void k_calcp_rows(int n, const int *__restrict__ ptr, const int *__re
         const int k = kord ? kord[q] : q;
         if (k == dp) continue;
         d = d - a[k];
-        const int f = kface[k];
-        if (ij[f] == k) s = s - flmass[f];   // row i is the owner of face f
+        const int e = kface[k], f = e >> 1;
+        if (!(e & 1)) s = s - flmass[f];   // row i is the owner of face f
         else s = s + flmass[f];
     }
     if (co_ptr) {
@@ -420,7 +419,7 @@ __global__ void k_nonorth_oc(int noc, int oc_face0, const int *__restrict__ ijl,
 // su(i) = 0 -/+ fmcor over the row's faces in face order, then the O-C cuts
 __global__ void k_face_source_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
                                    const int *__restrict__ kord, const int *__restrict__ kface,
-                                   const int *__restrict__ ij, const double *__restrict__ fm,
+                                   const double *__restrict__ fm,
                                    const int *__restrict__ co_ptr, const int *__restrict__ co_idx,
                                    const double *__restrict__ fm_oc, double *__restrict__ su) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -430,8 +429,8 @@ __global__ void k_face_source_rows(int n, const int *__restrict__ ptr, const int
     for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
         const int k = kord ? kord[q] : q;
         if (k == dp) continue;
-        const int f = kface[k];
-        if (ij[f] == k) s = s - fm[f];
+        const int e = kface[k], f = e >> 1;
+        if (!(e & 1)) s = s - fm[f];
         else s = s + fm[f];
     }
     if (co_ptr) {
@@ -631,7 +630,7 @@ __global__ void k_grad_oc(int noc, const int *__restrict__ ocface, const int *__
 // owner side, - on the neighbour side), then O-C cuts, then boundary faces in group order,
 // then times 1/vol.
 __global__ void k_grad_rows(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
-                            const int *__restrict__ kord, const int *__restrict__ kface, const int *__restrict__ ij,
+                            const int *__restrict__ kord, const int *__restrict__ kface,
                             const double *__restrict__ gf, const int *__restrict__ co_ptr,
                             const int *__restrict__ co_idx, const double *__restrict__ gc,
                             const int *__restrict__ cb_ptr, const int *__restrict__ cb_q, GradGroups G,
@@ -646,9 +645,9 @@ __global__ void k_grad_rows(int n, const int *__restrict__ ptr, const int *__res
     for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
         const int k = kord ? kord[p] : p;
         if (k == dp) continue;
-        const int f = kface[k];
-        const double *g = gf + 3 * (size_t)f;
-        if (ij[f] == k) {   // row i is the owner of face f
+        const int e = kface[k];
+        const double *g = gf + 3 * (size_t)(e >> 1);
+        if (!(e & 1)) {   // row i is the owner of face f
             sx = sx + g[0];
             sy = sy + g[1];
             sz = sz + g[2];
@@ -688,11 +687,13 @@ __global__ void k_grad_rows(int n, const int *__restrict__ ptr, const int *__res
     dudz[i] = sz * volr;
 }
 
+// kface[k] = (f << 1) | side: the face of slot k, side 0 in the owner's row, 1 in the
+// neighbour's (diagonal slots: -1); the order of the values is the order of the faces
 __global__ void k_kface(int nif, const int *__restrict__ ij, const int *__restrict__ ji, int *__restrict__ kface) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nif) return;
-    kface[ij[f]] = f;
-    kface[ji[f]] = f;
+    kface[ij[f]] = f << 1;
+    kface[ji[f]] = (f << 1) | 1;
 }
 
 __global__ void k_face_order_check(int n, const int *__restrict__ ptr, const int *__restrict__ diagpos,
@@ -852,6 +853,7 @@ void build_oc_lists(fc_context *ctx, OcDev &o, int n) {
 
 void build_kface(fc_context *ctx, const Structure &st, int nif, const int *ij_slot, const int *ji_slot,
                  DevBuf &kface, DevBuf &kface_order, bool &needs_order) {
+    FC_REQUIRE(nif < (1 << 30), FC_ERR_ARGUMENT, "more than 2^30 inner faces");
     hipStream_t s = ctx->stream;
     kface.alloc(sizeof(int) * (size_t)st.nnz);
     FC_HIP_CHECK(hipMemsetAsync(kface.get(), 0xff, sizeof(int) * (size_t)st.nnz, s));
@@ -929,7 +931,7 @@ void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
                                                        oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx, ary,
                                                        arz, F, oc.al.as<double>(), oc.ar.as<double>(), fmoc);
     k_calcp_rows<<<grid_for(st.n), kBlock, 0, s>>>(
-        st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface, ij_slot, flmass,
+        st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface, flmass,
         oc.noc ? oc.co_ptr.as<int>() : nullptr, oc.noc ? oc.co_idx.as<int>() : nullptr,
         oc.noc ? oc.al.as<double>() : nullptr, oc.noc ? oc.ar.as<double>() : nullptr, oc.noc ? fmoc : nullptr, a, su);
     FC_HIP_CHECK(hipGetLastError());
@@ -959,7 +961,7 @@ void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner
 }
 
 void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
-                    const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                    const int *kface, const int *kface_order, const double *xf, const double *yf,
                     const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                     const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
                     const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
@@ -979,7 +981,7 @@ void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *ow
                                                           oc.foc.as<double>(), xf, yf, zf, arx, ary, arz, xc, yc, zc,
                                                           u, ox, oy, oz, gc.as<double>());
         k_grad_rows<<<grid_for(n), kBlock, 0, s>>>(n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface,
-                                                   ij_slot, gf.as<double>(), oc.noc ? oc.co_ptr.as<int>() : nullptr,
+                                                   gf.as<double>(), oc.noc ? oc.co_ptr.as<int>() : nullptr,
                                                    oc.noc ? oc.co_idx.as<int>() : nullptr, gc.as<double>(), cb_ptr,
                                                    cb_q, G, arx, ary, arz, u, vol, dudx, dudy, dudz);
         if (lc != nigrad) {
@@ -993,7 +995,7 @@ void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *ow
 }
 
 void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
-                       const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                       const int *kface, const int *kface_order, const double *xf, const double *yf,
                        const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                        const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
                        const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
@@ -1008,7 +1010,7 @@ void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int 
                                                          oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx, ary,
                                                          arz, xc, yc, zc, vol, den, apu, dpdx, fmo.as<double>(), fmoc);
     k_face_source_rows<<<grid_for(st.n), kBlock, 0, s>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order,
-                                                         kface, ij_slot, fm.as<double>(),
+                                                         kface, fm.as<double>(),
                                                          oc.noc ? oc.co_ptr.as<int>() : nullptr,
                                                          oc.noc ? oc.co_idx.as<int>() : nullptr, fmo.as<double>(), su);
     FC_HIP_CHECK(hipGetLastError());
@@ -1070,10 +1072,10 @@ void face_flux_dev(fc_context *ctx, int nif, const int *owner0, const int *neigh
     FC_HIP_CHECK(hipGetLastError());
 }
 
-void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *ij_slot, const int *kface,
-                        const int *kface_order, const double *flux, double *su) {
+void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *kface, const int *kface_order,
+                        const double *flux, double *su) {
     k_rhs_divergence<<<grid_for(st.n), kBlock, 0, ctx->stream>>>(st.n, st.ioffset.as<int>(), st.diag.as<int>(),
-                                                                 kface_order, kface, ij_slot, flux, su);
+                                                                 kface_order, kface, flux, su);
     FC_HIP_CHECK(hipGetLastError());
 }
 

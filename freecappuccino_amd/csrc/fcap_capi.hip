@@ -570,7 +570,7 @@ extern "C" fc_status fc_calcp_nonorth(fc_context *ctx, int numCells, int numInne
             upload_dbl(ctx, dfm, fmoc, noc);
             build_oc_lists(ctx, oc, n);
         }
-        calcp_nonorth_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.ij.as<int>(), fs.kface.as<int>(),
+        calcp_nonorth_dev(ctx, st, nif, fs.own.as<int>(), fs.nei.as<int>(), fs.kface.as<int>(),
                           fs.needs_order ? fs.kord.as<int>() : nullptr, dx[3].as<double>(), dx[4].as<double>(),
                           dx[5].as<double>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(),
                           dfi.as<double>(), dc[0].as<double>(), dc[1].as<double>(), dc[2].as<double>(),
@@ -726,7 +726,7 @@ extern "C" fc_status fc_grad_gauss(fc_context *ctx, int numCells, int numFaces, 
             for (int f : hf) FC_REQUIRE(f >= 1 && f <= nf, FC_ERR_ARGUMENT, "O-C face number out of range");
         }
         double *g = dg.as<double>();
-        grad_gauss_dev(ctx, st, nif, own.as<int>(), nei.as<int>(), ij.as<int>(), kface.as<int>(),
+        grad_gauss_dev(ctx, st, nif, own.as<int>(), nei.as<int>(), kface.as<int>(),
                        needs ? kord.as<int>() : nullptr, dx[3].as<double>(), dx[4].as<double>(), dx[5].as<double>(),
                        dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), dfi.as<double>(),
                        dc[0].as<double>(), dc[1].as<double>(), dc[2].as<double>(), dc[3].as<double>(),
@@ -781,7 +781,7 @@ extern "C" fc_status fc_rhs_divergence(fc_context *ctx, int numCells, int numInn
         build_kface(ctx, st, nif, ij.as<int>(), ji.as<int>(), kface, kord, needs);
         upload_dbl(ctx, dfl, flux, nif);
         upload_dbl(ctx, dsu, su, n);
-        rhs_divergence_dev(ctx, st, ij.as<int>(), kface.as<int>(), needs ? kord.as<int>() : nullptr,
+        rhs_divergence_dev(ctx, st, kface.as<int>(), needs ? kord.as<int>() : nullptr,
                            dfl.as<double>(), dsu.as<double>());
         download_dbl(ctx, su, dsu.get(), n);
         return FC_OK;

@@ -224,7 +224,7 @@ struct RhieChowDev {
 };
 // calcp's non-orthogonal corrector source (su overwritten, flmass and fmoc += fmcor)
 void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
-                       const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                       const int *kface, const int *kface_order, const double *xf, const double *yf,
                        const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                        const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
                        const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
@@ -253,7 +253,7 @@ void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner
                           DevBuf &cb_q);
 // grad_gauss: nigrad iterations of the Gauss gradient of u (cells, then boundary values)
 void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
-                    const int *ij_slot, const int *kface, const int *kface_order, const double *xf, const double *yf,
+                    const int *kface, const int *kface_order, const double *xf, const double *yf,
                     const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                     const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
                     const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
@@ -286,8 +286,8 @@ void convdiff_dev(fc_context *ctx, const Structure &st, int nif, const int *owne
 void face_flux_dev(fc_context *ctx, int nif, const int *owner0, const int *neigh0,
                    const double *facint, const double *arx, const double *ary, const double *arz,
                    const double *u, const double *v, const double *w, double *flux);
-void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *ij_slot, const int *kface,
-                        const int *kface_order, const double *flux, double *su);
+void rhs_divergence_dev(fc_context *ctx, const Structure &st, const int *kface, const int *kface_order,
+                        const double *flux, double *su);
 // face id of each off-diagonal CSR entry (-1 on the diagonal), and whether every
 // row lists its faces in ascending face order when read in column order
 void build_kface(fc_context *ctx, const Structure &st, int nif, const int *ij_slot,
