@@ -195,6 +195,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *m = std::getenv("FCAP_SF_WAVES")) ctx->sf_waves = std::atoi(m);
         if (const char *m = std::getenv("FCAP_SF_SPIN")) ctx->sf_spin = std::atoi(m);
         if (const char *m = std::getenv("FCAP_GNN_FUSE")) ctx->gnn_fuse_input = std::atoi(m) != 0;
+        if (const char *m = std::getenv("FCAP_GNN_LDS")) ctx->gnn_lds = std::atoi(m) != 0;
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
@@ -1185,6 +1186,7 @@ extern "C" void *fc_sys_device_ptr(fc_system *sys, int which) {
     case FC_ARR_ZF: return g.zf.get();
     case FC_ARR_RES: return sys->ctx->ws ? sys->ctx->ws->r.get() : nullptr;
     case FC_ARR_PIVOTS: return sys->ctx->ws ? sys->ctx->ws->d.get() : nullptr;
+    case FC_ARR_GNN_PERM: return sys->graph.lds ? sys->graph.perm.get() : nullptr;
     default: return nullptr;
     }
 }
@@ -1207,6 +1209,7 @@ extern "C" fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_
         case FC_ARR_JA: expect = sizeof(int) * nnz; index = true; break;
         case FC_ARR_DIAG: expect = sizeof(int) * n; index = true; break;
         case FC_ARR_IJ_SLOT: case FC_ARR_JI_SLOT: expect = sizeof(int) * nif; index = true; break;
+        case FC_ARR_GNN_PERM: expect = sizeof(int) * n; index = true; break;
         default: throw Error(FC_ERR_ARGUMENT, "unknown array");
         }
         FC_REQUIRE(bytes >= expect, FC_ERR_ARGUMENT, "output buffer too small");

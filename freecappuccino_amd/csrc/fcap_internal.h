@@ -165,6 +165,7 @@ struct fc_context {
     // persistent layer kernel gathers rows with quads of lanes (FCAP_GNN_QUAD=0: one lane per cell)
     int gnn_quad = 1;
     int gnn_fuse_input = 1;   // inference: the first layer computes the encoder rows from u
+    int gnn_lds = 0;          // inference: renumbered cells, a block's rows staged in LDS (FCAP_GNN_LDS=1)
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

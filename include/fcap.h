@@ -350,7 +350,10 @@ enum {
     FC_ARR_A = 1, FC_ARR_SU = 2, FC_ARR_IOFFSET = 3, FC_ARR_JA = 4, FC_ARR_DIAG = 5,
     FC_ARR_IJ_SLOT = 6, FC_ARR_JI_SLOT = 7, FC_ARR_VOL = 8, FC_ARR_XC = 9, FC_ARR_YC = 10,
     FC_ARR_ZC = 11, FC_ARR_FACINT = 12, FC_ARR_ARX = 13, FC_ARR_ARY = 14, FC_ARR_ARZ = 15,
-    FC_ARR_XF = 16, FC_ARR_YF = 17, FC_ARR_ZF = 18, FC_ARR_RES = 19, FC_ARR_PIVOTS = 20
+    FC_ARR_XF = 16, FC_ARR_YF = 17, FC_ARR_ZF = 18, FC_ARR_RES = 19, FC_ARR_PIVOTS = 20,
+    /* the network's cell order (new -> old, 1-based) once an attached network
+     * has run with its rows staged in LDS; no counterpart in the reference */
+    FC_ARR_GNN_PERM = 21
 };
 fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_t bytes);
 /* Device pointer of an array of the resident system (for callers that share

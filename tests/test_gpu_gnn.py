@@ -122,6 +122,44 @@ def test_fused_first_layer_matches_stored_encoder_rows(ctx, monkeypatch, mesh):
     np.testing.assert_array_equal(z1, z2)
 
 
+@pytest.mark.parametrize("mesh", ["hex12", "hex10_graded", "tet6"])
+def test_lds_layers_match_gathering_layers(monkeypatch, mesh):
+    """With FCAP_GNN_LDS=1 inference renumbers the cells (Morton order of their per-axis ranks) and
+    stages each 64-cell block's own and halo rows in LDS; every cell is computed with the same
+    slots, gates and sums, so z is bitwise that of the layers that gather rows from memory."""
+    m = {"hex12": lambda: M.hex_box(12, perturb=0.2, seed=2),
+         "hex10_graded": lambda: M.hex_box(10, perturb=0.2, seed=4, grade_axis=0, grade_ratio=1.05),
+         "tet6": lambda: M.tet_box(6, grade_axis=1, grade_ratio=1.1, permute_seed=3)}[mesh]()
+    th = _params(6)
+    r = np.random.default_rng(31).standard_normal(m.n_cells).astype(np.float32)
+    monkeypatch.setenv("FCAP_GNN_LDS", "1")
+    c1 = fcap.Context(0)
+    try:
+        s1 = _system(m, c1)
+        net1 = gnn.Network(th, c1)
+        z1 = net1.apply(s1, r)
+        perm = s1.get("gnn_perm") - 1
+        net1.close()
+        s1.close()
+    finally:
+        c1.close()
+    assert np.array_equal(np.sort(perm), np.arange(m.n_cells))
+    monkeypatch.setenv("FCAP_GNN_LDS", "0")
+    c2 = fcap.Context(0)
+    try:
+        s2 = _system(m, c2)
+        net2 = gnn.Network(th, c2)
+        z2 = net2.apply(s2, r)
+        with pytest.raises(fcap.FcError):
+            s2.get("gnn_perm")          # not built with the LDS layers off
+        net2.close()
+        s2.close()
+    finally:
+        c2.close()
+    assert np.abs(z1).max() > 0
+    np.testing.assert_array_equal(z1, z2)
+
+
 @pytest.mark.parametrize("pseed", [3, 5])
 def test_loss_and_gradient_match_oracle(ctx, small, pseed):
     m, A, graph, feats, s = small
