@@ -81,21 +81,21 @@ def calcp_step_bytes(n, nif, nnz, nb):
     return {
         # own, nei, ij, ji, geometry; a(ij), a(ji), flmass out; 12 cell fields + 4 gradients (3 each)
         "k_calcp_faces": nif * (16 + 56 + 24) + n * 24 * 8,
-        # ptr, diag, a and su out; per off-diagonal slot kface + a; per face ij + flmass
-        "k_calcp_rows": n * (8 + 16) + off * (4 + 8) + nif * (4 + 8),
+        # ptr, diag, a and su out; per off-diagonal slot kface (face and side) + a; per face flmass
+        "k_calcp_rows": n * (8 + 16) + off * (4 + 8) + nif * 8,
         # own, nei, geometry, (x, y, z)-products out; u, centre and previous gradient per cell
         "k_grad_faces": nif * (8 + 56 + 24) + n * 7 * 8,
-        # ptr, diag, vol, gradient out; kface per slot; ij and products per face; boundary faces
+        # ptr, diag, vol, gradient out; kface per slot; products per face; boundary faces
         # (owner list entry, S, value)
-        "k_grad_rows": n * (8 + 8 + 24) + off * 4 + nif * (4 + 24) + nb * (4 + 24 + 8),
+        "k_grad_rows": n * (8 + 8 + 24) + off * 4 + nif * 24 + nb * (4 + 24 + 8),
         # own, nei, ij, a(ij), flmass in and out; pp per cell
         "k_correct_faces": nif * (12 + 8 + 16) + n * 8,
         # pp, grad pp, vol, 1/a_P (3), u, v, w, p in and out
         "k_correct_cells": n * (8 + 24 + 8 + 24 + 64),
         # own, nei, geometry, fmcor out, flmass in and out; centre, vol, den, apu, grad pp per cell
         "k_nonorth_faces": nif * (8 + 56 + 8 + 16) + n * 9 * 8,
-        # ptr, diag, su out; kface per slot; ij and fmcor per face
-        "k_face_source_rows": n * (8 + 8) + off * 4 + nif * (4 + 8),
+        # ptr, diag, su out; kface per slot; fmcor per face
+        "k_face_source_rows": n * (8 + 8) + off * 4 + nif * 8,
     }
 
 
