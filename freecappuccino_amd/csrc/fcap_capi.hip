@@ -196,6 +196,8 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *m = std::getenv("FCAP_SF_SPIN")) ctx->sf_spin = std::atoi(m);
         if (const char *m = std::getenv("FCAP_GNN_FUSE")) ctx->gnn_fuse_input = std::atoi(m) != 0;
         if (const char *m = std::getenv("FCAP_GNN_LDS")) ctx->gnn_lds = std::atoi(m) != 0;
+        if (const char *m = std::getenv("FCAP_GNN_SMOOTH")) ctx->gnn_smooth = std::atoi(m);
+        if (const char *m = std::getenv("FCAP_GNN_OMEGA")) ctx->gnn_omega = (float)std::atof(m);
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
@@ -268,6 +270,13 @@ extern "C" fc_status fc_set_inner_tolerance(fc_context *ctx, double eta) {
 extern "C" fc_status fc_set_fcg_window(fc_context *ctx, int m) {
     if (!ctx || m < 1 || m > kFcgM) return FC_ERR_ARGUMENT;
     ctx->fcg_m = m;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega) {
+    if (!ctx || steps < 0 || steps > 8 || !(omega > 0.0 && omega < 2.0)) return FC_ERR_ARGUMENT;
+    ctx->gnn_smooth = steps;
+    ctx->gnn_omega = (float)omega;
     return FC_OK;
 }
 

@@ -166,6 +166,8 @@ struct fc_context {
     int gnn_quad = 1;
     int gnn_fuse_input = 1;   // inference: the first layer computes the encoder rows from u
     int gnn_lds = 0;          // inference: renumbered cells, a block's rows staged in LDS (FCAP_GNN_LDS=1)
+    int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG
+    float gnn_omega = 0.8f;   // their damping
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

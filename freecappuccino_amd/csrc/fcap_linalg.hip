@@ -47,7 +47,10 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
                                                     const T *__restrict__ b, const T *__restrict__ v,
                                                     T *__restrict__ y, float *__restrict__ y32,
                                                     double *__restrict__ partA, double *__restrict__ partB,
-                                                    const Scalars *__restrict__ sc, int xcd) {
+                                                    const Scalars *__restrict__ sc, int xcd, float omega) {
+    // MODE 0: y = A x; 1: y = b - A x with |y| and y^2 partials; 2: y = A x with y.v
+    // partials; 3: y = A x with y.v and y^2 partials; 4: y = x + omega (b - A x) / v
+    constexpr bool kRes = MODE == 1 || MODE == 4;
     __shared__ AT sa[kSpmvWaves][kSpmvCap];
     __shared__ int sj[kSpmvWaves][kSpmvCap];
     __shared__ double red[kSpmvWaves];
@@ -99,7 +102,7 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
     double pa = 0.0, pb = 0.0;
     if (valid) {
         const int rs = ptr[row], re = ptr[row + 1];
-        T acc = (MODE == 1) ? b[row] : T(0);
+        T acc = kRes ? b[row] : T(0);
         if (staged) {
             // gathers issued four at a time; the sum still runs in CSR order
             int k = rs - s0;
@@ -108,20 +111,23 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
                 const T x0 = x[sj[w][k]], x1 = x[sj[w][k + 1]], x2 = x[sj[w][k + 2]], x3 = x[sj[w][k + 3]];
                 const T p0 = (T)sa[w][k] * x0, p1 = (T)sa[w][k + 1] * x1;
                 const T p2 = (T)sa[w][k + 2] * x2, p3 = (T)sa[w][k + 3] * x3;
-                if (MODE == 1) { acc = acc - p0; acc = acc - p1; acc = acc - p2; acc = acc - p3; }
+                if (kRes) { acc = acc - p0; acc = acc - p1; acc = acc - p2; acc = acc - p3; }
                 else { acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3; }
             }
             for (; k < ke; ++k) {
                 T prod = (T)sa[w][k] * x[sj[w][k]];
-                acc = (MODE == 1) ? acc - prod : acc + prod;
+                acc = kRes ? acc - prod : acc + prod;
             }
         } else {
             for (int k = rs; k < re; ++k) {
                 T prod = (T)a[k] * x[ja[k]];
-                acc = (MODE == 1) ? acc - prod : acc + prod;
+                acc = kRes ? acc - prod : acc + prod;
             }
         }
-        y[row] = acc;
+        if (MODE == 4)
+            y[row] = x[row] + (T)omega * (acc / v[row]);
+        else
+            y[row] = acc;
         if (MODE == 1) {
             if (y32) y32[row] = (float)acc;
             pa = fabs((double)acc);
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
             pb = (double)acc * (double)acc;
         }
     }
-    if (MODE != 0) {
+    if (MODE != 0 && MODE != 4) {
         pa = wave_sum(pa);
         if (lane == 0) red[w] = pa;
         __syncthreads();
@@ -183,10 +189,10 @@ void spmv_launch64(hipStream_t s, int mode, int n, const int *ptr, const int *ja
     const int xo = g_spmv_xcd;
     const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
     switch (mode) {
-    case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
-    case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
-    case 2: k_spmv<double, double, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
-    default: k_spmv<double, double, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo); break;
+    case 0: k_spmv<double, double, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo, 0.f); break;
+    case 1: k_spmv<double, double, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo, 0.f); break;
+    case 2: k_spmv<double, double, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo, 0.f); break;
+    default: k_spmv<double, double, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, y32, pa, pb, sc, xo, 0.f); break;
     }
 }
 
@@ -195,11 +201,19 @@ void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja
     const int xo = g_spmv_xcd;
     const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
     switch (mode) {
-    case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
-    case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
-    case 2: k_spmv<float, float, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
-    default: k_spmv<float, float, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo); break;
+    case 0: k_spmv<float, float, 0><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo, 0.f); break;
+    case 1: k_spmv<float, float, 1><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo, 0.f); break;
+    case 2: k_spmv<float, float, 2><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo, 0.f); break;
+    default: k_spmv<float, float, 3><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, v, y, nullptr, pa, pb, sc, xo, 0.f); break;
     }
+}
+
+void jacobi_step32(hipStream_t s, int n, const int *ptr, const int *ja, const float *a, const float *x, const float *b,
+                   const float *d, float omega, float *y, const Scalars *sc) {
+    const int xo = g_spmv_xcd;
+    const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
+    k_spmv<float, float, 4><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, d, y, nullptr, nullptr, nullptr, sc, xo,
+                                                    omega);
 }
 
 void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc) {

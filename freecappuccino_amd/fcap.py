@@ -75,7 +75,7 @@ def library() -> C.CDLL:
         "fc_last_error": ([vp], C.c_char_p), "fc_set_report": ([vp, i, C.c_char_p], i),
         "fc_set_variant": ([vp, i], i),
         "fc_set_inner_tolerance": ([vp, d], i), "fc_set_fcg_window": ([vp, i], i),
-        "fc_set_sweep_mode": ([vp, i], i),
+        "fc_set_sweep_mode": ([vp, i], i), "fc_set_fcg_smoothing": ([vp, i, d], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
@@ -200,6 +200,10 @@ class Context:
     def set_fcg_window(self, m: int):
         """Directions kept by flexible CG (1..16, rounded up to a multiple of 4)."""
         self.check(self.L.fc_set_fcg_window(self.h, int(m)))
+
+    def set_fcg_smoothing(self, steps: int = 3, omega: float = 0.8):
+        """Damped-Jacobi steps after the network in the learned FCG's preconditioner."""
+        self.check(self.L.fc_set_fcg_smoothing(self.h, int(steps), float(omega)))
 
     def set_sweep_mode(self, mode: int):
         """Schedule of the triangular sweeps: SWEEP_LEVELS, SWEEP_BARRIER or SWEEP_DEPENDENT

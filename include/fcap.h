@@ -273,6 +273,10 @@ fc_status fc_set_variant(fc_context *ctx, int variant);
  * multiple of 4 (default 12). */
 fc_status fc_set_inner_tolerance(fc_context *ctx, double eta);
 fc_status fc_set_fcg_window(fc_context *ctx, int m);
+/* Preconditioner of FC_METHOD_FCG_LEARNED: the network, then `steps` damped-Jacobi
+ * steps z <- z + omega (r - A z) / a_ii (0..8 steps, 0 < omega < 2; default 3
+ * steps, omega 0.8).  No counterpart in the reference. */
+fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega);
 
 /* Schedule of the DIC/DILU triangular sweeps (src/iccg.f90:94-111,
  * src/bicgstab.f90:107-124); every schedule computes each row with the same

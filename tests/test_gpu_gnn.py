@@ -245,10 +245,14 @@ def test_fcg_learned_converges_and_beats_no_network(ctx, trained):
     net, _ = trained
     m = M.hex_box(16, perturb=0.2, seed=99)      # not a training mesh
     s = _system(m, ctx, seed=123)
-    rc0, info0 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
-    x0 = s.solution()
-    rc1, info1 = _fcg(ctx, s, net.params)
-    x1 = s.solution()
+    ctx.set_fcg_smoothing(0, 0.8)               # the network alone against Jacobi
+    try:
+        rc0, info0 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
+        x0 = s.solution()
+        rc1, info1 = _fcg(ctx, s, net.params)
+        x1 = s.solution()
+    finally:
+        ctx.set_fcg_smoothing()
     assert rc0 == fcap.FC_OK and rc1 == fcap.FC_OK, (rc0, rc1, info0, info1)
     assert info0["rel_l2"] < 1e-8 and info1["rel_l2"] < 1e-8
     assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-6
@@ -265,12 +269,73 @@ def test_fcg_with_zero_network_matches_jacobi_pcg(ctx):
     s.set_solution(np.zeros(s.n))
     rc_j, info_j = s.solve(fcap.METHOD_DPCG, fcap.STOP_L2, 1e-3, 5000)
     ctx.set_inner_tolerance(1e-3)       # the inner solve runs to the same 1e-3
+    ctx.set_fcg_smoothing(0, 0.8)       # the network alone
     try:
         rc_f, info_f = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
     finally:
         ctx.set_inner_tolerance(0.0)
+        ctx.set_fcg_smoothing()
     assert rc_j == fcap.FC_OK and rc_f == fcap.FC_OK, (info_j, info_f)
     assert abs(info_f["iters"] - info_j["iters"]) <= 0.1 * info_j["iters"] + 5, (info_f["iters"], info_j["iters"])
+    s.close()
+
+
+def _poly_jacobi_pcg(A, b, steps, omega, tol, maxit=5000):
+    """float64 PCG with M^-1 r = the Jacobi guess r/d followed by `steps` damped-Jacobi steps
+    (a polynomial in D^-1 A times D^-1: symmetric, and positive for these omegas)."""
+    d = A.diagonal()
+
+    def prec(r):
+        z = r / d
+        for _ in range(steps):
+            z = z + omega * (r - A @ z) / d
+        return z
+
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    nb = np.linalg.norm(b)
+    for it in range(1, maxit + 1):
+        q = A @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        if np.linalg.norm(r) / nb < tol:
+            return it
+        z = prec(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return maxit
+
+
+def test_fcg_smoothing_steps_match_polynomial_jacobi_pcg(ctx):
+    """A zero network followed by 3 damped-Jacobi steps (omega 0.8) is a fixed SPD polynomial
+    preconditioner, so flexible CG with it is PCG: the iteration count to 1e-3 matches a float64
+    PCG with the same preconditioner in numpy up to FP32 rounding, and the steps cut the count
+    against the network alone."""
+    import scipy.sparse as sp
+    m = M.hex_box(12, perturb=0.2, seed=6)
+    s = _system(m, ctx, seed=9)
+    n = s.n
+    A = sp.csr_matrix((s.get("a"), s.get("ja") - 1, s.get("ioffset") - 1), shape=(n, n))
+    # CG with a definite matrix and a preconditioner of the same sign takes the same steps
+    # whether the pair is positive or negative definite
+    ref = _poly_jacobi_pcg(A, s.get("su"), 3, 0.8, 1e-3)
+    ctx.set_inner_tolerance(1e-3)
+    try:
+        ctx.set_fcg_smoothing(3, 0.8)
+        rc3, info3 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
+        ctx.set_fcg_smoothing(0, 0.8)
+        rc0, info0 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
+    finally:
+        ctx.set_inner_tolerance(0.0)
+        ctx.set_fcg_smoothing()
+    assert rc3 == fcap.FC_OK and rc0 == fcap.FC_OK, (info3, info0)
+    assert abs(info3["iters"] - ref) <= 0.1 * ref + 5, (info3["iters"], ref)
+    assert info3["iters"] < 0.8 * info0["iters"], (info3["iters"], info0["iters"])
     s.close()
 
 
