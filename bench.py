@@ -12,7 +12,8 @@ seeded Gaussian blobs with zero mean.  Four solves run on the same system:
                     with one launch per sweep, each row waiting for the rows it reads);
   * ic_pcg_mixed -- FP32 DIC-PCG inside FP64 iterative refinement (same sweeps);
   * fcg_learned  -- flexible CG in FP32 preconditioned by the trained BF16
-                    message-passing network, inside FP64 iterative refinement,
+                    message-passing network followed by three damped-Jacobi
+                    steps (omega 0.8), inside FP64 iterative refinement,
                     IC(0)-PCG fallback on stagnation;
   * ic_pcg_fp64_levels -- ic_pcg_fp64 with one sweep launch per level (the same
                     bits, the schedule before the dependency-driven one).
@@ -459,7 +460,9 @@ def main():
                   "ic_pcg_fp64_levels": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": workload, "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"],
-                   "solver": head},
+                   "solver": head,
+                   "preconditioner": ("BF16 message-passing network (4 layers, width 32) + 3 damped-Jacobi "
+                                      "steps, omega 0.8" if head == "fcg_learned" else "IC(0)")},
         "solvers": results,
         "spmv": {"fp64_ms": spmv_ms, "fp64_GBps": gbs, "fp32_ms": spmv32_ms,
                  "fp32_GBps": b32 / (spmv32_ms * 1e-3) / 1e9, "bytes_fp64": b64, "bytes_fp32": b32},
