@@ -198,6 +198,14 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         if (const char *m = std::getenv("FCAP_GNN_LDS")) ctx->gnn_lds = std::atoi(m) != 0;
         if (const char *m = std::getenv("FCAP_GNN_SMOOTH")) ctx->gnn_smooth = std::atoi(m);
         if (const char *m = std::getenv("FCAP_GNN_OMEGA")) ctx->gnn_omega = (float)std::atof(m);
+        if (const char *m = std::getenv("FCAP_GNN_PRE")) ctx->gnn_pre = std::atoi(m);
+        if (const char *m = std::getenv("FCAP_GNN_CHEB")) {
+            float lo = 0.f, hi = 0.f;
+            if (std::sscanf(m, "%f,%f", &lo, &hi) == 2 && lo > 0.f && hi > lo) {
+                ctx->gnn_cheb_lo = lo;
+                ctx->gnn_cheb_hi = hi;
+            }
+        }
         if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
         if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
         if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;

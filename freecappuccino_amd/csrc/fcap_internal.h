@@ -168,6 +168,10 @@ struct fc_context {
     int gnn_lds = 0;          // inference: renumbered cells, a block's rows staged in LDS (FCAP_GNN_LDS=1)
     int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG
     float gnn_omega = 0.8f;   // their damping
+    // Chebyshev steps instead (lo < hi): step k damps with 1 / theta_k, theta_k the
+    // roots of the Chebyshev polynomial of degree gnn_smooth mapped to [lo, hi]
+    float gnn_cheb_lo = 0.f, gnn_cheb_hi = 0.f;
+    int gnn_pre = 0;          // a damped-Jacobi step before the network too
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -28,6 +28,9 @@ void spmv_launch32(hipStream_t s, int mode, int n, const int *ptr, const int *ja
 // y = x + omega (b - A x) / d (a damped-Jacobi step; y must not alias x)
 void jacobi_step32(hipStream_t s, int n, const int *ptr, const int *ja, const float *a, const float *x, const float *b,
                    const float *d, float omega, float *y, const Scalars *sc);
+// y = b - A x, no partial sums (y must not alias x)
+void residual32(hipStream_t s, int n, const int *ptr, const int *ja, const float *a, const float *x, const float *b,
+                float *y, const Scalars *sc);
 void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc);
 
 // Grid barrier state of the persistent sweeps.  `abort` is sticky: once a

@@ -49,8 +49,9 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
                                                     double *__restrict__ partA, double *__restrict__ partB,
                                                     const Scalars *__restrict__ sc, int xcd, float omega) {
     // MODE 0: y = A x; 1: y = b - A x with |y| and y^2 partials; 2: y = A x with y.v
-    // partials; 3: y = A x with y.v and y^2 partials; 4: y = x + omega (b - A x) / v
-    constexpr bool kRes = MODE == 1 || MODE == 4;
+    // partials; 3: y = A x with y.v and y^2 partials; 4: y = x + omega (b - A x) / v;
+    // 5: y = b - A x
+    constexpr bool kRes = MODE == 1 || MODE == 4 || MODE == 5;
     __shared__ AT sa[kSpmvWaves][kSpmvCap];
     __shared__ int sj[kSpmvWaves][kSpmvCap];
     __shared__ double red[kSpmvWaves];
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(kSpmvRows) void k_spmv(int n, const int *__restrict
             pb = (double)acc * (double)acc;
         }
     }
-    if (MODE != 0 && MODE != 4) {
+    if (MODE != 0 && MODE != 4 && MODE != 5) {
         pa = wave_sum(pa);
         if (lane == 0) red[w] = pa;
         __syncthreads();
@@ -214,6 +215,14 @@ void jacobi_step32(hipStream_t s, int n, const int *ptr, const int *ja, const fl
     const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
     k_spmv<float, float, 4><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, d, y, nullptr, nullptr, nullptr, sc, xo,
                                                     omega);
+}
+
+void residual32(hipStream_t s, int n, const int *ptr, const int *ja, const float *a, const float *x, const float *b,
+                float *y, const Scalars *sc) {
+    const int xo = g_spmv_xcd;
+    const int g = xo ? xcd_grid(spmv_grid(n)) : spmv_grid(n);
+    k_spmv<float, float, 5><<<g, kSpmvRows, 0, s>>>(n, ptr, ja, a, x, b, nullptr, y, nullptr, nullptr, nullptr, sc,
+                                                    xo, 0.f);
 }
 
 void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc) {
