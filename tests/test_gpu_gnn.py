@@ -339,6 +339,30 @@ def test_fcg_smoothing_steps_match_polynomial_jacobi_pcg(ctx):
     s.close()
 
 
+@pytest.mark.parametrize("env", [{}, {"FCAP_GNN_PRE": "1"}, {"FCAP_GNN_CHEB": "0.5,2.0", "FCAP_GNN_SMOOTH": "4"}])
+def test_learned_fcg_on_tets_reaches_fp64_tolerance(trained, monkeypatch, env):
+    """The learned flexible CG with the default Jacobi steps, with a Jacobi step before the network
+    as well, and with Chebyshev step sizes, on a graded tetrahedral mesh the network was not
+    trained on: 1e-8 in the recomputed FP64 residual, the solution that of FP64 IC(0)-PCG."""
+    net, _ = trained
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    c = fcap.Context(0)
+    try:
+        m = M.tet_box(8, grade_axis=0, grade_ratio=1.1, grade_max_aspect=4.0, permute_seed=2)
+        s = _system(m, c, seed=77)
+        s.set_solution(np.zeros(s.n))
+        rc, info = s.solve(fcap.METHOD_ICCG, fcap.STOP_L2, 1e-12, 5000)
+        x64 = s.solution()
+        rc1, info1 = _fcg(c, s, net.params)
+        assert rc == fcap.FC_OK and rc1 == fcap.FC_OK, (info, info1)
+        assert info1["rel_l2"] < 1e-8 and info1["fallback_iters"] == 0, info1
+        assert np.linalg.norm(s.solution() - x64) / np.linalg.norm(x64) < 1e-6
+        s.close()
+    finally:
+        c.close()
+
+
 def test_nonfinite_network_falls_back_to_ic0_pcg(ctx):
     m = M.hex_box(10, perturb=0.2, seed=5)
     s = _system(m, ctx, seed=8)
