@@ -25,6 +25,9 @@ void sort_segments_dev(fc_context *ctx, const int *ptr, int *v, int nseg);
 namespace {
 
 constexpr int kBlock = 256;
+// slots of one wave's 64 consecutive rows staged in LDS by the row kernels (a
+// wave whose rows hold more reads them from memory)
+constexpr int kRowStage = 512;
 
 inline int grid_for(long long n, int block = kBlock) {
     long long g = (n + block - 1) / block;
@@ -307,17 +310,38 @@ __global__ void k_calcp_rows(int n, const int *__restrict__ ptr, const int *__re
                              const int *__restrict__ co_idx, const double *__restrict__ al,
                              const double *__restrict__ ar, const double *__restrict__ fmoc, double *__restrict__ a,
                              double *__restrict__ su) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // The wave's 64 rows are consecutive, so their slots are one contiguous range:
+    // with no reordering it is staged in LDS (slot values, face codes and the
+    // gathered face fluxes) with coalesced loads, and each lane then walks its row
+    // there in the same order.
+    __shared__ double sa[kBlock / 64][kRowStage], sv[kBlock / 64][kRowStage];
+    __shared__ int se[kBlock / 64][kRowStage];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = (int)(blockIdx.x * blockDim.x) + 64 * w;
+    const int i = r0 + lane;
+    const int s0 = r0 < n ? ptr[r0] : 0;
+    const int cnt = r0 < n ? ptr[min(r0 + 64, n)] - s0 : 0;
+    const bool staged = !kord && cnt <= kRowStage;
+    if (staged) {
+        for (int t = lane; t < cnt; t += 64) {
+            const int e = kface[s0 + t];
+            sa[w][t] = a[s0 + t];
+            se[w][t] = e;
+            sv[w][t] = e >= 0 ? flmass[e >> 1] : 0.0;
+        }
+    }
+    __syncthreads();
     if (i >= n) return;
     const int dp = diagpos[i];
     double d = 0.0, s = 0.0;
     for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
         const int k = kord ? kord[q] : q;
         if (k == dp) continue;
-        d = d - a[k];
-        const int e = kface[k], f = e >> 1;
-        if (!(e & 1)) s = s - flmass[f];   // row i is the owner of face f
-        else s = s + flmass[f];
+        const int e = staged ? se[w][k - s0] : kface[k];
+        const double fv = staged ? sv[w][k - s0] : flmass[e >> 1];
+        d = d - (staged ? sa[w][k - s0] : a[k]);
+        if (!(e & 1)) s = s - fv;   // row i is the owner of face e >> 1
+        else s = s + fv;
     }
     if (co_ptr) {
         for (int q = co_ptr[i]; q < co_ptr[i + 1]; ++q) {
@@ -422,6 +446,8 @@ __global__ void k_face_source_rows(int n, const int *__restrict__ ptr, const int
                                    const double *__restrict__ fm,
                                    const int *__restrict__ co_ptr, const int *__restrict__ co_idx,
                                    const double *__restrict__ fm_oc, double *__restrict__ su) {
+    // (staging the slots in LDS as k_calcp_rows does made this kernel slower: with
+    // one gathered value per slot the strided index loads are not what bounds it)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int dp = diagpos[i];
@@ -638,6 +664,8 @@ __global__ void k_grad_rows(int n, const int *__restrict__ ptr, const int *__res
                             const double *__restrict__ arz, const double *__restrict__ u,
                             const double *__restrict__ vol, double *__restrict__ dudx, double *__restrict__ dudy,
                             double *__restrict__ dudz) {
+    // (staging the slots in LDS as k_calcp_rows does made this kernel slower: the
+    // 57 KB per block left 8 waves per CU for the scattered face-product gathers)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int dp = diagpos[i];
