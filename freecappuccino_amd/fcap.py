@@ -28,6 +28,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfcap.so")
 
 FC_OK, FC_NOT_CONVERGED, FC_EARLY_EXIT, FC_FALLBACK_USED = 0, 1, 2, 3
+FC_ERR_ARGUMENT, FC_ERR_HIP, FC_ERR_NONFINITE, FC_ERR_ZERO_PIVOT = -1, -2, -3, -4
+FC_ERR_STRUCTURE, FC_ERR_ALLOC, FC_ERR_NO_DEVICE = -5, -6, -7
 STATUS_NAMES = {0: "FC_OK", 1: "FC_NOT_CONVERGED", 2: "FC_EARLY_EXIT", 3: "FC_FALLBACK_USED",
                 -1: "FC_ERR_ARGUMENT", -2: "FC_ERR_HIP", -3: "FC_ERR_NONFINITE", -4: "FC_ERR_ZERO_PIVOT",
                 -5: "FC_ERR_STRUCTURE", -6: "FC_ERR_ALLOC", -7: "FC_ERR_NO_DEVICE"}

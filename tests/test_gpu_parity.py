@@ -458,6 +458,51 @@ def test_adjust_mass_flow_long_outlet_matches_oracle(ctx):
         np.testing.assert_array_equal(x, y, err_msg=str(k))
 
 
+def test_step_entry_points_reject_bad_arguments(ctx, golden_loader):
+    """The pressure-correction step's entry points check their sizes and ranges before any kernel
+    runs, and the tuning setters their values."""
+    import ctypes as C
+    g = golden_loader("hex5_adjust")
+    m = g.mesh
+    geo = fcap.Geometry(g.out("adjust", "arx"), g.out("adjust", "ary"), g.out("adjust", "arz"), *[None] * 8)
+    args = (g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"), g.inp("w"), g.inp("su"), g.inp("fmi"))
+    n, ntot = m.n_cells, m.n_cells + m.n_boundary
+    own1 = (m.owner.astype(np.int64) + 1).astype(np.int32)
+    dp = lambda x: np.ascontiguousarray(x, np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    ip = lambda x: np.ascontiguousarray(x, np.int32).ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+    u, v, w, su, fmo = (np.array(x, np.float64) for x in (g.inp("u"), g.inp("v"), g.inp("w"), g.inp("su"),
+                                                           np.zeros(64)))
+    bnd = fcap.boundary_groups(m)
+    # outlet values past numTotal
+    rc = ctx.L.fc_adjust_mass_flow(ctx.h, n, ntot, m.n_faces, ip(own1), dp(geo.arx), dp(geo.ary), dp(geo.arz),
+                                   int(bnd[0]), int(bnd[1]), dp(g.inp("fmi")), int(bnd[2]), int(bnd[3]), ntot - 1,
+                                   1.0, dp(g.inp("den")), dp(u), dp(v), dp(w), dp(su), dp(fmo))
+    assert rc == fcap.FC_ERR_ARGUMENT
+    # inlet faces past the faces
+    rc = ctx.L.fc_adjust_mass_flow(ctx.h, n, ntot, m.n_faces, ip(own1), dp(geo.arx), dp(geo.ary), dp(geo.arz),
+                                   int(bnd[0]), m.n_faces, dp(g.inp("fmi")), int(bnd[2]), int(bnd[3]), n,
+                                   1.0, dp(g.inp("den")), dp(u), dp(v), dp(w), dp(su), dp(fmo))
+    assert rc == fcap.FC_ERR_ARGUMENT
+    # the valid call still works afterwards
+    out = fcap.adjust_mass_flow(m, geo, *args, ctx=ctx)
+    np.testing.assert_array_equal(out[0], g.out("adjust", "su"))
+    # corrector source with a structure that does not match numInnerFaces
+    c = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    rc = ctx.L.fc_calcp_nonorth(ctx.h, n, m.n_inner - 1, ip(own1), ip(m.neighbour + 1), *[dp(np.zeros(m.n_faces))] * 11,
+                                dp(np.ones(n)), dp(np.ones(n)), dp(np.zeros(3 * n)), 0, 0, ip([0]), ip([0]),
+                                dp([0.0]), ip(c.ioffset), ip(c.ja), ip(c.diag), ip(c.ij_slot), ip(c.ji_slot),
+                                m.nnz, dp(np.zeros(n)), dp(np.zeros(m.n_inner)), dp([0.0]))
+    assert rc == fcap.FC_ERR_ARGUMENT
+    # reference cell outside the mesh is covered by test_calcp_correct_oc_cuts_and_errors; setters
+    for bad in ((-1, 0.8), (9, 0.8), (3, 0.0), (3, 2.0)):
+        with pytest.raises(fcap.FcError):
+            ctx.set_fcg_smoothing(*bad)
+    with pytest.raises(fcap.FcError):
+        ctx.set_sweep_mode(3)
+    ctx.set_fcg_smoothing()
+    ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
+
+
 # ---------------------------------------------------------------- Gauss gradient
 # ---------------------------------------------------------------- Gauss gradient
 # ---------------------------------------------------------------- Gauss gradient
