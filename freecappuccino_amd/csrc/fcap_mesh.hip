@@ -543,15 +543,18 @@ __global__ void k_tri_ell(const int *__restrict__ rows, const int *__restrict__ 
 
 // ---------------------------------------------------------------- host side
 
+// Copies between a caller's array and the library's device buffers.  The direction
+// is taken from the pointers (unified addressing), so a caller's array may also be
+// device memory: the copy then stays on the device.
 void h2d(void *dst, const void *src, size_t bytes, hipStream_t s) {
     if (!bytes) return;
-    FC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    FC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
     FC_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void d2h(void *dst, const void *src, size_t bytes, hipStream_t s) {
     if (!bytes) return;
-    FC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    FC_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
     FC_HIP_CHECK(hipStreamSynchronize(s));
 }
 

@@ -458,6 +458,45 @@ def test_adjust_mass_flow_long_outlet_matches_oracle(ctx):
         np.testing.assert_array_equal(x, y, err_msg=str(k))
 
 
+def test_adjust_mass_flow_with_device_resident_fields(ctx, golden_loader):
+    """Floating-point arrays may be device memory: the library copies them on the device.  The
+    fields of fc_adjust_mass_flow in hipMalloc'd buffers give the reference's bits as well."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    g = golden_loader("hex5_adjust")
+    m = g.mesh
+    n, ntot = m.n_cells, m.n_cells + m.n_boundary
+    bnd = fcap.boundary_groups(m)
+    host = {k: np.ascontiguousarray(g.inp(k), np.float64) for k in ("den", "u", "v", "w", "su", "fmi")}
+    for k in ("arx", "ary", "arz"):
+        host[k] = np.ascontiguousarray(g.out("adjust", k), np.float64)
+    host["fmo"] = np.zeros(int(bnd[2]))
+    dev = {}
+    try:
+        for k, a in host.items():
+            p = C.c_void_p()
+            assert hip.hipMalloc(C.byref(p), max(a.nbytes, 8)) == 0
+            dev[k] = p
+            assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0        # host to device
+        dp = lambda k: C.cast(dev[k], C.POINTER(C.c_double))  # noqa: E731
+        own1 = (m.owner.astype(np.int64) + 1).astype(np.int32)
+        rc = ctx.L.fc_adjust_mass_flow(ctx.h, n, ntot, m.n_faces, own1.ctypes.data_as(C.POINTER(C.c_int)),
+                                       dp("arx"), dp("ary"), dp("arz"), int(bnd[0]), int(bnd[1]), dp("fmi"),
+                                       int(bnd[2]), int(bnd[3]), n + int(bnd[0]), float(g.inp("flomas")[0]),
+                                       dp("den"), dp("u"), dp("v"), dp("w"), dp("su"), dp("fmo"))
+        assert rc == fcap.FC_OK, ctx.L.fc_last_error(ctx.h)
+        for k in ("su", "u", "v", "w", "fmo"):
+            out = np.zeros_like(host[k])
+            assert hip.hipMemcpy(out.ctypes.data, dev[k], out.nbytes, 2) == 0   # device to host
+            np.testing.assert_array_equal(out, g.out("adjust", k), err_msg=k)
+    finally:
+        for p in dev.values():
+            hip.hipFree(p)
+
+
 def test_step_entry_points_reject_bad_arguments(ctx, golden_loader):
     """The pressure-correction step's entry points check their sizes and ranges before any kernel
     runs, and the tuning setters their values."""
