@@ -150,6 +150,52 @@ def _calcp_pass(api, m, geo, csr, f, fb, pp, ppb, fmi, timed, **kw):
     return float(np.abs(su).sum()), float(np.abs(su2).sum())
 
 
+def _assemble_device_resident(fcap, ctx, m, geo, csr, f, reps):
+    """Wall ms of fc_calcp_assemble with every floating-point input and output in device memory
+    (hipMalloc'd once; index arrays stay on the host): the call copies them on the device."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    n, nif = m.n_cells, m.n_inner
+    host = [np.ascontiguousarray(getattr(geo, k), np.float64)
+            for k in ("xc", "yc", "zc", "vol", "xf", "yf", "zf", "arx", "ary", "arz", "facint")]
+    host += [np.ascontiguousarray(f[k], np.float64) for k in fcap.CALCP_FIELDS]
+    host += [np.ascontiguousarray(np.asarray(f[k]).reshape(n, 3), np.float64) for k in fcap.CALCP_GRADS]
+    outs = [np.zeros(m.nnz), np.zeros(n), np.zeros(nif)]
+    dev = []
+    try:
+        for a in host + outs:
+            p = C.c_void_p()
+            if hip.hipMalloc(C.byref(p), max(a.nbytes, 8)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            dev.append(p)
+            hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1)
+        dp = [C.cast(p, C.POINTER(C.c_double)) for p in dev]
+        ip = lambda x: np.ascontiguousarray(x, np.int32).ctypes.data_as(C.POINTER(C.c_int))  # noqa: E731
+        own1 = (m.owner[:nif].astype(np.int64) + 1).astype(np.int32)
+        nei1 = (m.neighbour.astype(np.int64) + 1).astype(np.int32)
+        z = np.zeros(1)
+        zi = np.zeros(1, np.int32)
+        times = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            # 11 geometry arrays, 8 fields, 4 gradients, then the outputs a, su, flmass
+            rc = ctx.L.fc_calcp_assemble(ctx.h, n, nif, ip(own1), ip(nei1), *dp[:23], 0, 0, ip(zi), ip(zi),
+                                         z.ctypes.data_as(C.POINTER(C.c_double)), ip(csr.ioffset), ip(csr.ja),
+                                         ip(csr.diag), ip(csr.ij_slot), ip(csr.ji_slot), m.nnz, dp[23], dp[24],
+                                         dp[25], *[z.ctypes.data_as(C.POINTER(C.c_double))] * 3)
+            hip.hipDeviceSynchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+            ctx.check(rc)
+        return float(np.median(times[1:]))
+    finally:
+        for p in dev:
+            hip.hipFree(p)
+
+
 def profile_calcp(mesh, ctx, reps, cpu_n):
     """The pressure-correction step's calls on the workload mesh with inlet and outlet patches:
     Gauss gradients, assembly, adjustMassFlow, corrections and the non-orthogonal corrector's
@@ -167,8 +213,14 @@ def profile_calcp(mesh, ctx, reps, cpu_n):
     for _ in range(reps):
         check = _calcp_pass(fcap, m, geo, csr, *data, timed, ctx=ctx)
         print(f"calcp step pass: {({k: round(v[-1], 1) for k, v in timed.items()})}", file=sys.stderr, flush=True)
+    f = dict(data[0])
+    for k, src in (("dudx", "u"), ("dvdx", "v"), ("dwdx", "w"), ("dpdx", "p")):
+        f[k] = np.stack(fcap.grad_gauss(m, geo, data[1][src], ctx=ctx), axis=1)
+    resident = _assemble_device_resident(fcap, ctx, m, geo, csr, f, reps)
+    print(f"calcp_assemble with device-resident fields: {resident:.1f} ms", file=sys.stderr, flush=True)
     res = {"cells": m.n_cells, "inner_faces": m.n_inner, "nnz": m.nnz, "reps": reps,
            "wall_ms_incl_pcie": {k: float(np.median(v)) for k, v in timed.items()},
+           "calcp_assemble_wall_ms_device_resident": resident,
            "kernel_bytes": calcp_step_bytes(m.n_cells, m.n_inner, m.nnz, m.n_boundary), "check": check}
     if cpu_n:
         from oracle import oracle as O
