@@ -196,6 +196,47 @@ def _assemble_device_resident(fcap, ctx, m, geo, csr, f, reps):
             hip.hipFree(p)
 
 
+def _assemble_resident_system(fcap, ctx, m, f, reps):
+    """fc_sys_calcp_assemble on a resident system of the mesh (geometry, structure and slot
+    maps already on the device): wall ms with host-array fields, wall ms with the fields and
+    flmass in device memory, and the device ms of the launches (HIP events)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = m.n_cells
+    host = [np.ascontiguousarray(f[k], np.float64) for k in fcap.CALCP_FIELDS]
+    host += [np.ascontiguousarray(np.asarray(f[k]).reshape(n, 3), np.float64) for k in fcap.CALCP_GRADS]
+    s = fcap.System(m, ctx)
+    dev = []
+    try:
+        t_host, t_dev, t_kern = [], [], []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            s.calcp_assemble(*host)
+            t_host.append((time.perf_counter() - t0) * 1e3)
+        for a in host + [np.zeros(max(m.n_inner, 1))]:
+            p = C.c_void_p()
+            if hip.hipMalloc(C.byref(p), max(a.nbytes, 8)) != 0:
+                raise RuntimeError("hipMalloc failed")
+            dev.append(p)
+            hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1)
+        ptrs = [p.value for p in dev]
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            s.calcp_assemble(*ptrs[:12], flmass=ptrs[12])   # returns after a stream synchronize
+            t_dev.append((time.perf_counter() - t0) * 1e3)
+            t_kern.append(s.last_ms(0))
+        return {"host_fields_wall_ms": float(np.median(t_host[1:])),
+                "device_fields_wall_ms": float(np.median(t_dev[1:])),
+                "device_ms": float(np.median(t_kern[1:]))}
+    finally:
+        for p in dev:
+            hip.hipFree(p)
+        s.close()
+
+
 def profile_calcp(mesh, ctx, reps, cpu_n):
     """The pressure-correction step's calls on the workload mesh with inlet and outlet patches:
     Gauss gradients, assembly, adjustMassFlow, corrections and the non-orthogonal corrector's
@@ -218,9 +259,12 @@ def profile_calcp(mesh, ctx, reps, cpu_n):
         f[k] = np.stack(fcap.grad_gauss(m, geo, data[1][src], ctx=ctx), axis=1)
     resident = _assemble_device_resident(fcap, ctx, m, geo, csr, f, reps)
     print(f"calcp_assemble with device-resident fields: {resident:.1f} ms", file=sys.stderr, flush=True)
+    on_system = _assemble_resident_system(fcap, ctx, m, f, reps)
+    print(f"sys_calcp_assemble on the resident system: {on_system}", file=sys.stderr, flush=True)
     res = {"cells": m.n_cells, "inner_faces": m.n_inner, "nnz": m.nnz, "reps": reps,
            "wall_ms_incl_pcie": {k: float(np.median(v)) for k, v in timed.items()},
            "calcp_assemble_wall_ms_device_resident": resident,
+           "sys_calcp_assemble": on_system,
            "kernel_bytes": calcp_step_bytes(m.n_cells, m.n_inner, m.nnz, m.n_boundary), "check": check}
     if cpu_n:
         from oracle import oracle as O

@@ -1068,6 +1068,84 @@ extern "C" fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis,
     });
 }
 
+// True when p is device memory of the current device (used in place); host memory of
+// any kind (pageable, pinned, unregistered) is staged through a copy instead.
+static bool device_resident(const void *p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    int dev = -1;
+    FC_HIP_CHECK(hipGetDevice(&dev));
+    return at.type == hipMemoryTypeDevice && at.device == dev;
+}
+
+// A field of count doubles as a device pointer: p itself when it is device memory,
+// else p copied into stage (which keeps its allocation across calls of one size).
+static const double *field_on_device(fc_context *ctx, DevBuf &stage, const double *p, size_t count) {
+    if (device_resident(p)) return p;
+    stage.ensure(sizeof(double) * std::max<size_t>(count, 1));
+    h2d(stage.get(), p, sizeof(double) * count, ctx->stream);
+    return stage.as<double>();
+}
+
+extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double *apu, const double *apv,
+                                           const double *apw, const double *u, const double *v, const double *w,
+                                           const double *p, const double *dUdxi, const double *dVdxi,
+                                           const double *dWdxi, const double *dPdxi, double *flmass) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        const int n = sys->st.n, nif = sys->mesh.nif;
+        FC_REQUIRE(flmass || nif == 0, FC_ERR_ARGUMENT, "flmass");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const double *cf[8] = {den, apu, apv, apw, u, v, w, p};
+        for (const double *c : cf) FC_REQUIRE(c != nullptr, FC_ERR_ARGUMENT, "cell fields");
+        const double *gf[4] = {dUdxi, dVdxi, dWdxi, dPdxi};
+        for (const double *g : gf) FC_REQUIRE(g != nullptr, FC_ERR_ARGUMENT, "gradients");
+        const double *dcf[8], *dgf[4];
+        for (int k = 0; k < 8; ++k) dcf[k] = field_on_device(ctx, sys->cp_stage[k], cf[k], n);
+        for (int k = 0; k < 4; ++k) dgf[k] = field_on_device(ctx, sys->cp_stage[8 + k], gf[k], 3 * (size_t)n);
+        const bool fl_dev = nif > 0 && device_resident(flmass);
+        double *dfl = flmass;
+        if (!fl_dev) {
+            sys->cp_flmass.ensure(sizeof(double) * std::max(nif, 1));
+            dfl = sys->cp_flmass.as<double>();
+        }
+        GeometryDev &g = sys->mesh.g;
+        RhieChowDev F;
+        F.xc = g.xc.as<double>();
+        F.yc = g.yc.as<double>();
+        F.zc = g.zc.as<double>();
+        F.vol = g.vol.as<double>();
+        F.den = dcf[0];
+        F.apu = dcf[1];
+        F.apv = dcf[2];
+        F.apw = dcf[3];
+        F.u = dcf[4];
+        F.v = dcf[5];
+        F.w = dcf[6];
+        F.p = dcf[7];
+        F.dudx = dgf[0];
+        F.dvdx = dgf[1];
+        F.dwdx = dgf[2];
+        F.dpdx = dgf[3];
+        ctx->timer.start(ctx->stream);
+        FC_HIP_CHECK(hipMemsetAsync(sys->a.get(), 0, sizeof(double) * sys->st.nnz, ctx->stream));   // calcp:34
+        calcp_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(), sys->ij.as<int>(),
+                  sys->ji.as<int>(), sys->kface.as<int>(), sys->needs_order ? sys->kord.as<int>() : nullptr,
+                  g.xf.as<double>(), g.yf.as<double>(), g.zf.as<double>(), g.arx.as<double>(), g.ary.as<double>(),
+                  g.arz.as<double>(), g.facint.as<double>(), F, sys->oc, sys->a.as<double>(), sys->su.as<double>(),
+                  dfl, nullptr);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        sys->assembly_version += 1;
+        if (nif && !fl_dev) download_dbl(ctx, flmass, dfl, nif);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *sp, const double *const *src,
                                            const double *urf, double *const *phi, double *const *ap, int method,
                                            double sor, int nsw, fc_solve_info *info) {

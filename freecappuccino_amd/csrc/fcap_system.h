@@ -17,6 +17,8 @@ struct fc_system {
     fcap::OcDev oc;
     int ntotal = 0;
     fcap::DevBuf a, su, fi, mu, phi, su0, vis, flux;
+    // staging of host-resident fields and flmass of the pressure-correction assembly
+    fcap::DevBuf cp_stage[12], cp_flmass;
     bool su0_zero = true;
     double last_ms[3] = {0, 0, 0};
     // bumped by every assembly; the network's graph data is rebuilt when stale

@@ -102,6 +102,7 @@ def library() -> C.CDLL:
         "fc_sys_assemble_laplacian": ([vp, dp, dp, dp], i),
         "fc_sys_assemble_laplacian_dev": ([vp], i),
         "fc_sys_assemble_convdiff": ([vp, dp, dp, d, dp], i),
+        "fc_sys_calcp_assemble": ([vp] + [vp] * 13, i),
         "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
         "fc_sys_solve_momentum": ([vp, C.POINTER(dp), C.POINTER(dp), dp, C.POINTER(dp), C.POINTER(dp), i, d, i,
                                    si], i),
@@ -647,6 +648,29 @@ class System:
     def assemble_convdiff(self, vis, flux, urf=0.7, su0=None):
         self.ctx.check(self.ctx.L.fc_sys_assemble_convdiff(self.h, _dp(_f64(vis)), _dp(_f64(flux)), float(urf),
                                                            _dp(None if su0 is None else _f64(su0))))
+
+    def calcp_assemble(self, den, apu, apv, apw, u, v, w, p, dudx, dvdx, dwdx, dpdx, flmass=None):
+        """calcp's pressure-correction assembly on the system's own mesh
+        (src/calcp-multiple_correction_SIMPLE.f90:34-107): a and su go into the system.
+        Each argument is a numpy array (copied) or an int device address (used in
+        place); gradients are (numCells, 3) arrays.  Returns flmass (numInnerFaces)
+        when it was not given as a device address."""
+        keep = []
+
+        def arg(x):
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.ascontiguousarray(x, dtype=np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        out = None
+        if flmass is None:
+            out = np.zeros(max(self.mesh.n_inner, 1))
+            flmass = out
+        args = [arg(x) for x in (den, apu, apv, apw, u, v, w, p, dudx, dvdx, dwdx, dpdx, flmass)]
+        self.ctx.check(self.ctx.L.fc_sys_calcp_assemble(self.h, *args))
+        return None if out is None else out[:self.mesh.n_inner]
 
     def set_rhs(self, su):
         self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))

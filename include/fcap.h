@@ -323,6 +323,19 @@ fc_status fc_sys_assemble_laplacian(fc_system *sys, const double *mu, const doub
                                     const double *su0);
 fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis, const double *flux,
                                    double urf, const double *su0);
+/* Pressure-correction assembly of calcp (src/calcp-multiple_correction_SIMPLE.f90:34-107,
+ * src/facefluxmass.f90:34-199) on the system's own mesh: as fc_calcp_assemble, with
+ * the geometry, CSR structure and face-to-slot maps the system already holds, and
+ * a and su written into the system, so that fc_sys_solve then solves for the
+ * pressure correction (calcp's iccg call, line 119).  Cell fields have numCells
+ * entries, gradients (3, numCells); flmass (numInnerFaces) is written.  Each of
+ * these pointers may be host or device memory: device pointers are read and
+ * written in place, host arrays pass through buffers the system keeps. */
+fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double *apu,
+                                const double *apv, const double *apw, const double *u,
+                                const double *v, const double *w, const double *p,
+                                const double *dUdxi, const double *dVdxi, const double *dWdxi,
+                                const double *dPdxi, double *flmass);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
 /* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal

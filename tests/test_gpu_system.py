@@ -253,3 +253,101 @@ def test_momentum_three_components_match_oracle(ctx):
         assert abs(infos[c]["iters"] - ref.iters) <= 2, (c, infos[c]["iters"], ref.iters)
         assert np.linalg.norm(phi[c] - ref.fi) / np.linalg.norm(ref.fi) < 1e-6, c
     s.close()
+
+
+# ------------------------------------------------- pressure correction on the resident system
+
+def _calcp_fields(m, geo, seed):
+    """Taylor-Green velocity, a smooth pressure, 1/a_P in [0.5, 1.5)/Vol, random gradients."""
+    n = m.n_cells
+    rng = np.random.default_rng(seed)
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    apu = (0.5 + rng.random(n)) / geo.vol
+    f = {"den": np.ones(n), "apu": apu, "apv": apu, "apw": apu, "u": u, "v": v, "w": w,
+         "p": np.sin(3 * geo.xc) * geo.yc + geo.zc}
+    for k in ("dudx", "dvdx", "dwdx", "dpdx"):
+        f[k] = rng.standard_normal((n, 3))
+    return f
+
+
+_CP_ORDER = ("den", "apu", "apv", "apw", "u", "v", "w", "p", "dudx", "dvdx", "dwdx", "dpdx")
+
+
+@pytest.mark.parametrize("name", ["hex6_calcp", "tet3_calcp"])
+def test_calcp_resident_bitwise_vs_reference(ctx, golden_loader, name):
+    """fc_sys_calcp_assemble on a system built from the mesh gives the reference's a, su and
+    flmass bit for bit (the system's own geometry and slot maps are the reference's too)."""
+    g = golden_loader(name)
+    s = fcap.System(g.mesh, ctx)
+    try:
+        for k in ("xc", "yc", "zc", "vol", "arx", "ary", "arz", "facint"):
+            np.testing.assert_array_equal(s.get(k)[:g.out("calcp", k).size], g.out("calcp", k), err_msg=k)
+        fl = s.calcp_assemble(*[g.inp(k) for k in _CP_ORDER])
+        np.testing.assert_array_equal(fl, g.out("calcp", "flmass"))
+        np.testing.assert_array_equal(s.get("a"), g.out("calcp", "a"))
+        np.testing.assert_array_equal(s.get("su"), g.out("calcp", "su"))
+    finally:
+        s.close()
+
+
+def test_calcp_resident_then_iccg_matches_oracle(case):
+    """Assembly into the resident system followed by fc_sys_solve (calcp's iccg call): the
+    matrix and source are the oracle's bits, and the pressure correction the oracle's DIC-PCG
+    solution (same iteration count within one, solutions within 1e-6).  On the 4:1 tet box
+    the coefficient |S|^2/(S . d) changes sign on some faces and the oracle's DIC pivots are
+    not all finite; there the library refuses the factor with FC_ERR_ZERO_PIVOT."""
+    name, m, geo, csr, s = case
+    f = _calcp_fields(m, geo, seed=7)
+    ref = O.calcp_assemble(m, geo, csr, f)
+    fl = s.calcp_assemble(*[f[k] for k in _CP_ORDER])
+    np.testing.assert_array_equal(fl, ref["flmass"])
+    np.testing.assert_array_equal(s.get("a"), ref["a"])
+    np.testing.assert_array_equal(s.get("su"), ref["su"])
+    s.set_solution(np.zeros(m.n_cells))
+    if not np.all(np.isfinite(O.dic_pivots(csr, ref["a"]))):
+        with pytest.raises(fcap.FcError) as e:
+            s.solve(fcap.METHOD_ICCG, fcap.STOP_L1, 1e-8, 2000)
+        assert e.value.status == fcap.FC_ERR_ZERO_PIVOT
+        return
+    o = O.iccg(csr, ref["a"], ref["su"], np.zeros(m.n_cells), sor=1e-8, nsw=2000)
+    rc, info = s.solve(fcap.METHOD_ICCG, fcap.STOP_L1, 1e-8, 2000)
+    assert rc == fcap.FC_OK and abs(info["iters"] - o.iters) <= 1, (info, o.iters)
+    pp = s.solution()
+    assert np.linalg.norm(pp - o.fi) <= 1e-6 * np.linalg.norm(o.fi)
+
+
+def test_calcp_resident_with_device_fields(ctx):
+    """Fields and flmass in hipMalloc'd buffers are read and written in place and give the same
+    bits as host arrays; a second call with other fields reuses the system's buffers."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    m = M.tet_box(6, grade_axis=1, grade_ratio=1.1, permute_seed=9)
+    geo = O.mesh_geometry(m)
+    s = fcap.System(m, ctx)
+    dev = []
+    try:
+        for seed in (1, 2):
+            f = _calcp_fields(m, geo, seed)
+            fl_host = s.calcp_assemble(*[f[k] for k in _CP_ORDER])
+            a_host, su_host = s.get("a"), s.get("su")
+            ptrs = []
+            for k in _CP_ORDER + ("flmass",):
+                x = np.ascontiguousarray(f[k] if k != "flmass" else np.zeros(m.n_inner), np.float64)
+                p = C.c_void_p()
+                assert hip.hipMalloc(C.byref(p), max(x.nbytes, 8)) == 0
+                dev.append(p)
+                assert hip.hipMemcpy(p, x.ctypes.data, x.nbytes, 1) == 0      # host to device
+                ptrs.append(p.value)
+            assert s.calcp_assemble(*ptrs[:12], flmass=ptrs[12]) is None
+            fl_dev = np.zeros(m.n_inner)
+            assert hip.hipMemcpy(fl_dev.ctypes.data, C.c_void_p(ptrs[12]), fl_dev.nbytes, 2) == 0
+            np.testing.assert_array_equal(fl_dev, fl_host)
+            np.testing.assert_array_equal(s.get("a"), a_host)
+            np.testing.assert_array_equal(s.get("su"), su_host)
+    finally:
+        for p in dev:
+            hip.hipFree(p)
+        s.close()
