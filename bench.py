@@ -228,9 +228,24 @@ def _assemble_resident_system(fcap, ctx, m, f, reps):
             s.calcp_assemble(*ptrs[:12], flmass=ptrs[12])   # returns after a stream synchronize
             t_dev.append((time.perf_counter() - t0) * 1e3)
             t_kern.append(s.last_ms(0))
+        # the Gauss gradient of u (boundary values zero) into a device (3, numCells) array
+        p = C.c_void_p()
+        if hip.hipMalloc(C.byref(p), 8 * s.ntotal) != 0:
+            raise RuntimeError("hipMalloc failed")
+        dev.append(p)
+        u_full = np.concatenate([host[4], np.zeros(s.ntotal - n)])
+        hip.hipMemcpy(p, u_full.ctypes.data, u_full.nbytes, 1)
+        t_grad, t_grad_kern = [], []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            s.grad_gauss(p.value, out=ptrs[8])
+            t_grad.append((time.perf_counter() - t0) * 1e3)
+            t_grad_kern.append(s.last_ms(0))
         return {"host_fields_wall_ms": float(np.median(t_host[1:])),
                 "device_fields_wall_ms": float(np.median(t_dev[1:])),
-                "device_ms": float(np.median(t_kern[1:]))}
+                "device_ms": float(np.median(t_kern[1:])),
+                "grad_gauss_device_wall_ms": float(np.median(t_grad[1:])),
+                "grad_gauss_device_ms": float(np.median(t_grad_kern[1:]))}
     finally:
         for p in dev:
             hip.hipFree(p)

@@ -988,6 +988,21 @@ void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner
     FC_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Component arrays (x, y, z; n each) into the reference's (3, numCells) layout, component fastest.
+__global__ void k_interleave3(int n, const double *__restrict__ gx, const double *__restrict__ gy,
+                              const double *__restrict__ gz, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[3 * (size_t)i] = gx[i];
+    out[3 * (size_t)i + 1] = gy[i];
+    out[3 * (size_t)i + 2] = gz[i];
+}
+
+void interleave3_dev(fc_context *ctx, int n, const double *gx, const double *gy, const double *gz, double *out) {
+    k_interleave3<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, gx, gy, gz, out);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
 void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
                     const int *kface, const int *kface_order, const double *xf, const double *yf,
                     const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,

@@ -1146,6 +1146,61 @@ extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, co
     });
 }
 
+extern "C" fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad,
+                                       double *dudxi) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(bnd && u && dudxi && nigrad >= 1, FC_ERR_ARGUMENT, "arguments");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const int n = sys->st.n, nif = sys->mesh.nif, nf = sys->mesh.nfaces;
+        GradGroups G;
+        int q = 0, v = n;
+        for (int g = 0; g < 5; ++g) {
+            const int cnt = bnd[2 * g], fs = bnd[2 * g + 1];
+            FC_REQUIRE(cnt >= 0 && (cnt == 0 || (fs >= nif && fs + cnt <= nf)), FC_ERR_ARGUMENT,
+                       "boundary group outside the boundary faces");
+            G.qstart[g] = q;
+            G.fstart[g] = fs;
+            G.vstart[g] = v;
+            q += cnt;
+            v += cnt;
+        }
+        G.qstart[5] = q;
+        FC_REQUIRE(v <= sys->ntotal, FC_ERR_ARGUMENT, "boundary values beyond numTotal of the system");
+        if (!std::equal(bnd, bnd + 10, sys->grad_bnd)) {
+            std::fill(sys->grad_bnd, sys->grad_bnd + 10, -1);
+            build_boundary_lists(ctx, G, sys->mesh.owner.as<int>(), n, sys->grad_cb_ptr, sys->grad_cb_q);
+            std::copy(bnd, bnd + 10, sys->grad_bnd);
+        }
+        const double *du = field_on_device(ctx, sys->grad_u, u, sys->ntotal);
+        sys->grad_xyz.ensure(sizeof(double) * 3 * (size_t)n);
+        double *g = sys->grad_xyz.as<double>();
+        const bool out_dev = device_resident(dudxi);
+        double *out = dudxi;
+        if (!out_dev) {
+            sys->grad_out.ensure(sizeof(double) * 3 * (size_t)n);
+            out = sys->grad_out.as<double>();
+        }
+        GeometryDev &geo = sys->mesh.g;
+        OcDev none;
+        ctx->timer.start(ctx->stream);
+        grad_gauss_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(),
+                       sys->kface.as<int>(), sys->needs_order ? sys->kord.as<int>() : nullptr, geo.xf.as<double>(),
+                       geo.yf.as<double>(), geo.zf.as<double>(), geo.arx.as<double>(), geo.ary.as<double>(),
+                       geo.arz.as<double>(), geo.facint.as<double>(), geo.xc.as<double>(), geo.yc.as<double>(),
+                       geo.zc.as<double>(), geo.vol.as<double>(), du, G,
+                       sys->grad_cb_ptr.get() ? sys->grad_cb_ptr.as<int>() : nullptr,
+                       sys->grad_cb_q.get() ? sys->grad_cb_q.as<int>() : nullptr, none, nullptr, nigrad, g, g + n,
+                       g + 2 * (size_t)n);
+        interleave3_dev(ctx, n, g, g + n, g + 2 * (size_t)n, out);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        if (!out_dev) download_dbl(ctx, dudxi, out, 3 * (size_t)n);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *sp, const double *const *src,
                                            const double *urf, double *const *phi, double *const *ap, int method,
                                            double sor, int nsw, fc_solve_info *info) {

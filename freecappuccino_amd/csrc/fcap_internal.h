@@ -258,6 +258,8 @@ struct GradGroups {
 // per-cell lists of boundary items, each in sequence order
 void build_boundary_lists(fc_context *ctx, const GradGroups &G, const int *owner0_all, int n, DevBuf &cb_ptr,
                           DevBuf &cb_q);
+// (x, y, z component arrays of n) -> out(3, n), component fastest
+void interleave3_dev(fc_context *ctx, int n, const double *gx, const double *gy, const double *gz, double *out);
 // grad_gauss: nigrad iterations of the Gauss gradient of u (cells, then boundary values)
 void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
                     const int *kface, const int *kface_order, const double *xf, const double *yf,

@@ -19,6 +19,10 @@ struct fc_system {
     fcap::DevBuf a, su, fi, mu, phi, su0, vis, flux;
     // staging of host-resident fields and flmass of the pressure-correction assembly
     fcap::DevBuf cp_stage[12], cp_flmass;
+    // Gauss gradient: per-cell boundary lists of the last boundary groups, staging of u,
+    // the component arrays and the (3, numCells) output
+    int grad_bnd[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    fcap::DevBuf grad_cb_ptr, grad_cb_q, grad_u, grad_xyz, grad_out;
     bool su0_zero = true;
     double last_ms[3] = {0, 0, 0};
     // bumped by every assembly; the network's graph data is rebuilt when stale

@@ -103,6 +103,7 @@ def library() -> C.CDLL:
         "fc_sys_assemble_laplacian_dev": ([vp], i),
         "fc_sys_assemble_convdiff": ([vp, dp, dp, d, dp], i),
         "fc_sys_calcp_assemble": ([vp] + [vp] * 13, i),
+        "fc_sys_grad_gauss": ([vp, ip, vp, i, vp], i),
         "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
         "fc_sys_solve_momentum": ([vp, C.POINTER(dp), C.POINTER(dp), dp, C.POINTER(dp), C.POINTER(dp), i, d, i,
                                    si], i),
@@ -671,6 +672,24 @@ class System:
         args = [arg(x) for x in (den, apu, apv, apw, u, v, w, p, dudx, dvdx, dwdx, dpdx, flmass)]
         self.ctx.check(self.ctx.L.fc_sys_calcp_assemble(self.h, *args))
         return None if out is None else out[:self.mesh.n_inner]
+
+    def grad_gauss(self, u, nigrad=1, out=None):
+        """Gauss gradient of u (cells, then boundary values in group order) on the system's own
+        mesh (src/grad_gauss.f90:1-124), in the reference's (3, numCells) layout: returns an
+        (numCells, 3) array, or None when out is an int device address (written in place).
+        u may be a numpy array or an int device address."""
+        bnd = _i32(boundary_groups(self.mesh))
+        u_arg = C.c_void_p(u) if isinstance(u, int) else _f64(u)
+        if not isinstance(u, int) and u_arg.shape[0] < self.ntotal:
+            raise ValueError(f"u needs {self.ntotal} entries (cells, then boundary values)")
+        res = None
+        if out is None:
+            res = np.zeros((self.n, 3))
+            out = res
+        o_arg = C.c_void_p(out) if isinstance(out, int) else out.ctypes.data_as(C.c_void_p)
+        u_ptr = u_arg if isinstance(u, int) else u_arg.ctypes.data_as(C.c_void_p)
+        self.ctx.check(self.ctx.L.fc_sys_grad_gauss(self.h, _ip(bnd), u_ptr, int(nigrad), o_arg))
+        return res
 
     def set_rhs(self, su):
         self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))

@@ -336,6 +336,15 @@ fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double 
                                 const double *v, const double *w, const double *p,
                                 const double *dUdxi, const double *dVdxi, const double *dWdxi,
                                 const double *dPdxi, double *flmass);
+/* Gauss gradient on the system's own mesh (src/grad_gauss.f90:1-124, the
+ * calcp calls grad(U,dUdxi) .. grad(pp,dPdxi) at
+ * src/calcp-multiple_correction_SIMPLE.f90:38-41,138): as fc_grad_gauss with the
+ * system's geometry and face order, written in the reference's (3, numCells)
+ * layout that fc_sys_calcp_assemble reads.  u has the system's numTotal
+ * entries (cells, then the boundary values in group order); bnd as for
+ * fc_grad_gauss (the per-cell boundary lists are kept until bnd changes).  u
+ * and dudxi may be host or device memory, as for fc_sys_calcp_assemble. */
+fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad, double *dudxi);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
 /* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal

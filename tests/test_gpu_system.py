@@ -351,3 +351,70 @@ def test_calcp_resident_with_device_fields(ctx):
         for p in dev:
             hip.hipFree(p)
         s.close()
+
+
+@pytest.mark.parametrize("name", ["hex5_grad", "tet3_grad"])
+@pytest.mark.parametrize("nigrad", [1, 2])
+def test_grad_resident_bitwise_vs_reference(ctx, golden_loader, name, nigrad):
+    """fc_sys_grad_gauss on a system built from the mesh gives the reference's grad_gauss bits
+    (all five boundary groups on hex5_grad), in the (3, numCells) layout."""
+    g = golden_loader(name)
+    s = fcap.System(g.mesh, ctx)
+    try:
+        out = s.grad_gauss(g.inp("u"), nigrad)
+        for c, k in enumerate(("dudx", "dudy", "dudz")):
+            np.testing.assert_array_equal(out[:, c], g.out(f"grad{nigrad}", k), err_msg=k)
+    finally:
+        s.close()
+
+
+def test_resident_gradients_feed_the_assembly(case):
+    """The step's chain on the resident system, grad(U) .. grad(P) then the assembly, against
+    the oracle's gradients and assembly bit for bit; then the same chain with every field and
+    output in device memory gives the same bits."""
+    import ctypes as C
+    name, m, geo, csr, s = case
+    n, nt = m.n_cells, m.n_cells + m.n_boundary
+    rng = np.random.default_rng(11)
+    f = _calcp_fields(m, geo, seed=5)
+    full = {k: np.concatenate([f[k], rng.standard_normal(nt - n)]) for k in ("u", "v", "w", "p")}
+    for k, src in (("dudx", "u"), ("dvdx", "v"), ("dwdx", "w"), ("dpdx", "p")):
+        ref = np.stack(O.grad_gauss(m, geo, full[src], 1), axis=1)
+        f[k] = s.grad_gauss(full[src])
+        np.testing.assert_array_equal(f[k], ref, err_msg=k)
+    ref = O.calcp_assemble(m, geo, csr, f)
+    fl = s.calcp_assemble(*[f[k] for k in _CP_ORDER])
+    np.testing.assert_array_equal(fl, ref["flmass"])
+    np.testing.assert_array_equal(s.get("a"), ref["a"])
+    np.testing.assert_array_equal(s.get("su"), ref["su"])
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    dev = {}
+
+    def put(key, x):
+        x = np.ascontiguousarray(x, np.float64)
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), max(x.nbytes, 8)) == 0
+        dev[key] = p
+        assert hip.hipMemcpy(p, x.ctypes.data, x.nbytes, 1) == 0
+        return p.value
+
+    try:
+        ptr = {k: put(k, f[k]) for k in ("den", "apu", "apv", "apw", "u", "v", "w", "p")}
+        for k, src in (("dudx", "u"), ("dvdx", "v"), ("dwdx", "w"), ("dpdx", "p")):
+            ptr[k] = put(k, np.zeros((n, 3)))
+            assert s.grad_gauss(put("full_" + src, full[src]), out=ptr[k]) is None
+        ptr["flmass"] = put("flmass", np.zeros(m.n_inner))
+        assert s.calcp_assemble(*[ptr[k] for k in _CP_ORDER], flmass=ptr["flmass"]) is None
+        for k in ("dpdx", "flmass"):
+            got = np.zeros_like(f[k] if k != "flmass" else fl)
+            assert hip.hipMemcpy(got.ctypes.data, C.c_void_p(ptr[k]), got.nbytes, 2) == 0
+            np.testing.assert_array_equal(got, f[k] if k != "flmass" else fl, err_msg=k)
+        np.testing.assert_array_equal(s.get("a"), ref["a"])
+        np.testing.assert_array_equal(s.get("su"), ref["su"])
+    finally:
+        for p in dev.values():
+            hip.hipFree(p)
