@@ -241,7 +241,18 @@ def _assemble_resident_system(fcap, ctx, m, f, reps):
             s.grad_gauss(p.value, out=ptrs[8])
             t_grad.append((time.perf_counter() - t0) * 1e3)
             t_grad_kern.append(s.last_ms(0))
+        # the corrections with the system's matrix and pp = its (zero) solution, fields in place
+        t_cor, t_cor_kern = [], []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            ctx.check(ctx.L.fc_sys_calcp_correct(s.h, None, C.c_void_p(ptrs[11]), *[C.c_void_p(x) for x in ptrs[1:4]],
+                                                 0.2, 1, *[C.c_void_p(x) for x in (ptrs[12], ptrs[4], ptrs[5],
+                                                                                   ptrs[6], ptrs[7])]))
+            t_cor.append((time.perf_counter() - t0) * 1e3)
+            t_cor_kern.append(s.last_ms(0))
         return {"host_fields_wall_ms": float(np.median(t_host[1:])),
+                "correct_device_wall_ms": float(np.median(t_cor[1:])),
+                "correct_device_ms": float(np.median(t_cor_kern[1:])),
                 "device_fields_wall_ms": float(np.median(t_dev[1:])),
                 "device_ms": float(np.median(t_kern[1:])),
                 "grad_gauss_device_wall_ms": float(np.median(t_grad[1:])),

@@ -1146,6 +1146,51 @@ extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, co
     });
 }
 
+// An in/out array of count doubles as a device pointer: p itself when it is device memory, else
+// p copied into stage; *copy_back tells the caller to copy stage back into p afterwards.
+static double *inout_on_device(fc_context *ctx, DevBuf &stage, double *p, size_t count, bool *copy_back) {
+    *copy_back = !device_resident(p);
+    if (!*copy_back) return p;
+    return const_cast<double *>(field_on_device(ctx, stage, p, count));
+}
+
+extern "C" fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, const double *dPdxi, const double *apu,
+                                          const double *apv, const double *apw, double urf, int pRefCell,
+                                          double *flmass, double *u, double *v, double *w, double *p) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        const int n = sys->st.n, nif = sys->mesh.nif;
+        FC_REQUIRE(pRefCell >= 1 && pRefCell <= n, FC_ERR_ARGUMENT, "pRefCell outside 1..numCells");
+        FC_REQUIRE(dPdxi && apu && apv && apw && u && v && w && p && (flmass || nif == 0), FC_ERR_ARGUMENT,
+                   "arguments");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const double *dpp = pp ? field_on_device(ctx, sys->cp_stage[0], pp, n) : sys->fi.as<double>();
+        const double *dg = field_on_device(ctx, sys->cp_stage[1], dPdxi, 3 * (size_t)n);
+        const double *dap[3] = {field_on_device(ctx, sys->cp_stage[2], apu, n),
+                                field_on_device(ctx, sys->cp_stage[3], apv, n),
+                                field_on_device(ctx, sys->cp_stage[4], apw, n)};
+        double *io[5] = {u, v, w, p, flmass};
+        const size_t cnt[5] = {(size_t)n, (size_t)n, (size_t)n, (size_t)n, (size_t)nif};
+        double *dio[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        bool back[5] = {false, false, false, false, false};
+        for (int k = 0; k < 5; ++k) {
+            if (cnt[k] == 0) continue;
+            dio[k] = inout_on_device(ctx, k < 4 ? sys->cp_stage[5 + k] : sys->cp_flmass, io[k], cnt[k], &back[k]);
+        }
+        ctx->timer.start(ctx->stream);
+        calcp_correct_dev(ctx, n, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(), sys->ij.as<int>(),
+                          sys->a.as<double>(), dpp, dg, sys->mesh.g.vol.as<double>(), dap[0], dap[1], dap[2], urf,
+                          pRefCell - 1, 0, nullptr, nullptr, nullptr, nullptr, dio[4], dio[0], dio[1], dio[2],
+                          dio[3]);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        for (int k = 0; k < 5; ++k)
+            if (back[k]) download_dbl(ctx, io[k], dio[k], cnt[k]);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad,
                                        double *dudxi) {
     if (!sys) return FC_ERR_ARGUMENT;

@@ -104,6 +104,7 @@ def library() -> C.CDLL:
         "fc_sys_assemble_convdiff": ([vp, dp, dp, d, dp], i),
         "fc_sys_calcp_assemble": ([vp] + [vp] * 13, i),
         "fc_sys_grad_gauss": ([vp, ip, vp, i, vp], i),
+        "fc_sys_calcp_correct": ([vp, vp, vp, vp, vp, vp, d, i, vp, vp, vp, vp, vp], i),
         "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
         "fc_sys_solve_momentum": ([vp, C.POINTER(dp), C.POINTER(dp), dp, C.POINTER(dp), C.POINTER(dp), i, d, i,
                                    si], i),
@@ -690,6 +691,29 @@ class System:
         u_ptr = u_arg if isinstance(u, int) else u_arg.ctypes.data_as(C.c_void_p)
         self.ctx.check(self.ctx.L.fc_sys_grad_gauss(self.h, _ip(bnd), u_ptr, int(nigrad), o_arg))
         return res
+
+    def calcp_correct(self, dpdx, apu, apv, apw, urf, pref_cell, flmass, u, v, w, p, pp=None):
+        """calcp's corrections (src/calcp-multiple_correction_SIMPLE.f90:145-183) with the
+        system's matrix and, when pp is None, its solution; pref_cell is 0-based.  numpy
+        arguments are copied (the updated flmass, u, v, w, p are returned as new arrays);
+        int device addresses are read and updated in place (then None is returned)."""
+        keep = []
+
+        def arg(x, copy=False):
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.array(x, dtype=np.float64, copy=True) if copy else np.ascontiguousarray(x, np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        on_device = [isinstance(x, int) for x in (flmass, u, v, w, p)]
+        if any(on_device) and not all(on_device):
+            raise ValueError("flmass, u, v, w, p: all numpy arrays or all device addresses")
+        ins = [C.c_void_p(None) if pp is None else arg(pp)] + [arg(x) for x in (dpdx, apu, apv, apw)]
+        nin = len(keep)
+        outs = [arg(x, copy=True) for x in (flmass, u, v, w, p)]
+        self.ctx.check(self.ctx.L.fc_sys_calcp_correct(self.h, *ins, float(urf), int(pref_cell) + 1, *outs))
+        return None if all(on_device) else tuple(keep[nin:])
 
     def set_rhs(self, su):
         self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))

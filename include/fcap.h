@@ -345,6 +345,17 @@ fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double 
  * fc_grad_gauss (the per-cell boundary lists are kept until bnd changes).  u
  * and dudxi may be host or device memory, as for fc_sys_calcp_assemble. */
 fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad, double *dudxi);
+/* Corrections of calcp (src/calcp-multiple_correction_SIMPLE.f90:145-183) on
+ * the resident system: as fc_calcp_correct with a the system's matrix (the
+ * last fc_sys_calcp_assemble) and the system's geometry and slot maps.  pp
+ * (numCells) may be NULL for the system's solution (the last fc_sys_solve);
+ * dPdxi is (3, numCells), e.g. from fc_sys_grad_gauss.  flmass
+ * (numInnerFaces), u, v, w, p (numCells) are updated in place.  Host or
+ * device memory, as for fc_sys_calcp_assemble. */
+fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, const double *dPdxi,
+                               const double *apu, const double *apv, const double *apw,
+                               double urf, int pRefCell, double *flmass, double *u, double *v,
+                               double *w, double *p);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
 /* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal

@@ -314,6 +314,14 @@ def test_calcp_resident_then_iccg_matches_oracle(case):
     assert rc == fcap.FC_OK and abs(info["iters"] - o.iters) <= 1, (info, o.iters)
     pp = s.solution()
     assert np.linalg.norm(pp - o.fi) <= 1e-6 * np.linalg.norm(o.fi)
+    # the corrections with the system's matrix and solution give the bits of the host-array
+    # entry point (itself pinned against the oracle) on the same pp
+    dpdx = s.grad_gauss(np.concatenate([pp, np.zeros(m.n_boundary)]))
+    args = (dpdx, f["apu"], f["apv"], f["apw"], 0.2, 3, fl, f["u"], f["v"], f["w"], f["p"])
+    got = s.calcp_correct(*args)
+    want = fcap.calcp_correct(m, csr, ref["a"], pp, dpdx, geo.vol, *args[1:], ctx=s.ctx)
+    for k, x, y in zip(("flmass", "u", "v", "w", "p"), got, (want[0],) + tuple(want[2:])):
+        np.testing.assert_array_equal(x, y, err_msg=k)
 
 
 def test_calcp_resident_with_device_fields(ctx):
