@@ -250,7 +250,17 @@ def _assemble_resident_system(fcap, ctx, m, f, reps):
                                                                                    ptrs[6], ptrs[7])]))
             t_cor.append((time.perf_counter() - t0) * 1e3)
             t_cor_kern.append(s.last_ms(0))
+        # the corrector's source into the system's su, flmass in place
+        t_no, t_no_kern = [], []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            ctx.check(ctx.L.fc_sys_calcp_nonorth(s.h, *[C.c_void_p(x) for x in (ptrs[0], ptrs[1], ptrs[11],
+                                                                                ptrs[12])]))
+            t_no.append((time.perf_counter() - t0) * 1e3)
+            t_no_kern.append(s.last_ms(0))
         return {"host_fields_wall_ms": float(np.median(t_host[1:])),
+                "nonorth_device_wall_ms": float(np.median(t_no[1:])),
+                "nonorth_device_ms": float(np.median(t_no_kern[1:])),
                 "correct_device_wall_ms": float(np.median(t_cor[1:])),
                 "correct_device_ms": float(np.median(t_cor_kern[1:])),
                 "device_fields_wall_ms": float(np.median(t_dev[1:])),

@@ -1191,6 +1191,35 @@ extern "C" fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, cons
     });
 }
 
+extern "C" fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, const double *apu,
+                                          const double *dPdxi, double *flmass) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        const int n = sys->st.n, nif = sys->mesh.nif;
+        FC_REQUIRE(den && apu && dPdxi && (flmass || nif == 0), FC_ERR_ARGUMENT, "arguments");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const double *dden = field_on_device(ctx, sys->cp_stage[0], den, n);
+        const double *dapu = field_on_device(ctx, sys->cp_stage[1], apu, n);
+        const double *dg = field_on_device(ctx, sys->cp_stage[2], dPdxi, 3 * (size_t)n);
+        bool back = false;
+        double *dfl = nif ? inout_on_device(ctx, sys->cp_flmass, flmass, nif, &back) : nullptr;
+        GeometryDev &g = sys->mesh.g;
+        OcDev none;
+        ctx->timer.start(ctx->stream);
+        calcp_nonorth_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(),
+                          sys->kface.as<int>(), sys->needs_order ? sys->kord.as<int>() : nullptr, g.xf.as<double>(),
+                          g.yf.as<double>(), g.zf.as<double>(), g.arx.as<double>(), g.ary.as<double>(),
+                          g.arz.as<double>(), g.facint.as<double>(), g.xc.as<double>(), g.yc.as<double>(),
+                          g.zc.as<double>(), g.vol.as<double>(), dden, dapu, dg, none, sys->su.as<double>(), dfl,
+                          nullptr);
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        if (back) download_dbl(ctx, flmass, dfl, nif);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
 extern "C" fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad,
                                        double *dudxi) {
     if (!sys) return FC_ERR_ARGUMENT;

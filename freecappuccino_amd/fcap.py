@@ -105,6 +105,7 @@ def library() -> C.CDLL:
         "fc_sys_calcp_assemble": ([vp] + [vp] * 13, i),
         "fc_sys_grad_gauss": ([vp, ip, vp, i, vp], i),
         "fc_sys_calcp_correct": ([vp, vp, vp, vp, vp, vp, d, i, vp, vp, vp, vp, vp], i),
+        "fc_sys_calcp_nonorth": ([vp, vp, vp, vp, vp], i),
         "fc_sys_set_rhs": ([vp, dp], i), "fc_sys_set_solution": ([vp, dp], i),
         "fc_sys_solve_momentum": ([vp, C.POINTER(dp), C.POINTER(dp), dp, C.POINTER(dp), C.POINTER(dp), i, d, i,
                                    si], i),
@@ -714,6 +715,24 @@ class System:
         outs = [arg(x, copy=True) for x in (flmass, u, v, w, p)]
         self.ctx.check(self.ctx.L.fc_sys_calcp_correct(self.h, *ins, float(urf), int(pref_cell) + 1, *outs))
         return None if all(on_device) else tuple(keep[nin:])
+
+    def calcp_nonorth(self, den, apu, dpdx, flmass):
+        """Source of calcp's non-orthogonal corrector (src/calcp-multiple_correction_SIMPLE.f90:
+        185-229) into the system's su, for the corrector's next solve.  numpy arguments are
+        copied and the updated flmass is returned; an int device address for flmass is
+        updated in place (then None is returned)."""
+        keep = []
+
+        def arg(x, copy=False):
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.array(x, dtype=np.float64, copy=True) if copy else np.ascontiguousarray(x, np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        args = [arg(den), arg(apu), arg(dpdx), arg(flmass, copy=True)]
+        self.ctx.check(self.ctx.L.fc_sys_calcp_nonorth(self.h, *args))
+        return None if isinstance(flmass, int) else keep[-1]
 
     def set_rhs(self, su):
         self.ctx.check(self.ctx.L.fc_sys_set_rhs(self.h, _dp(_f64(su))))

@@ -356,6 +356,14 @@ fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, const double *d
                                const double *apu, const double *apv, const double *apw,
                                double urf, int pRefCell, double *flmass, double *u, double *v,
                                double *w, double *p);
+/* Source of calcp's non-orthogonal corrector
+ * (src/calcp-multiple_correction_SIMPLE.f90:185-229, fluxmc
+ * src/facefluxmass.f90:520-607) on the resident system: as fc_calcp_nonorth,
+ * with su written into the system for the corrector's next fc_sys_solve and
+ * flmass (numInnerFaces) updated in place.  Host or device memory, as for
+ * fc_sys_calcp_assemble. */
+fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, const double *apu,
+                               const double *dPdxi, double *flmass);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
 /* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal

@@ -322,6 +322,12 @@ def test_calcp_resident_then_iccg_matches_oracle(case):
     want = fcap.calcp_correct(m, csr, ref["a"], pp, dpdx, geo.vol, *args[1:], ctx=s.ctx)
     for k, x, y in zip(("flmass", "u", "v", "w", "p"), got, (want[0],) + tuple(want[2:])):
         np.testing.assert_array_equal(x, y, err_msg=k)
+    # the corrector's source into the system's su, against the oracle (pinned by the reference's
+    # fluxmc) on the corrected fluxes
+    fl2 = s.calcp_nonorth(f["den"], f["apu"], dpdx, got[0])
+    su_o, fl_o, _, _ = O.calcp_nonorth(m, geo, f["den"], f["apu"], dpdx, got[0])
+    np.testing.assert_array_equal(fl2, fl_o)
+    np.testing.assert_array_equal(s.get("su"), su_o)
 
 
 def test_calcp_resident_with_device_fields(ctx):
