@@ -432,3 +432,38 @@ def test_resident_gradients_feed_the_assembly(case):
     finally:
         for p in dev.values():
             hip.hipFree(p)
+
+
+def test_resident_step_entry_points_reject_bad_arguments(ctx):
+    """The resident step calls check their arguments before any kernel runs and leave the
+    system usable afterwards."""
+    import ctypes as C
+    m = M.hex_box(4, perturb=0.1, seed=2)
+    geo = O.mesh_geometry(m)
+    s = fcap.System(m, ctx)
+    try:
+        n = m.n_cells
+        f = _calcp_fields(m, geo, seed=3)
+        L, h = ctx.L, s.h
+        x = np.zeros(3 * n)
+        px = x.ctypes.data_as(C.c_void_p)
+        nul = C.c_void_p(None)
+        assert L.fc_sys_calcp_assemble(h, *[px] * 7, nul, *[px] * 5) == fcap.FC_ERR_ARGUMENT
+        assert L.fc_sys_calcp_assemble(None, *[px] * 13) == fcap.FC_ERR_ARGUMENT
+        assert L.fc_sys_calcp_correct(h, nul, px, px, px, px, 0.2, 0, px, px, px, px, px) == fcap.FC_ERR_ARGUMENT
+        assert L.fc_sys_calcp_correct(h, nul, px, px, px, px, 0.2, n + 1, px, px, px, px, px) == \
+            fcap.FC_ERR_ARGUMENT
+        assert L.fc_sys_calcp_nonorth(h, px, px, nul, px) == fcap.FC_ERR_ARGUMENT
+        bnd = np.asarray(fcap.boundary_groups(m), np.int32).copy()
+        bnd[0], bnd[1] = 5, m.n_inner - 1                      # a group starting among the inner faces
+        pb = bnd.ctypes.data_as(C.POINTER(C.c_int))
+        assert L.fc_sys_grad_gauss(h, pb, px, 1, px) == fcap.FC_ERR_ARGUMENT
+        good = np.asarray(fcap.boundary_groups(m), np.int32)
+        assert L.fc_sys_grad_gauss(h, good.ctypes.data_as(C.POINTER(C.c_int)), px, 0, px) == fcap.FC_ERR_ARGUMENT
+        # still usable: a good call after the rejected ones gives the oracle's bits
+        csr = O.csr_build(n, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+        ref = O.calcp_assemble(m, geo, csr, f)
+        np.testing.assert_array_equal(s.calcp_assemble(*[f[k] for k in _CP_ORDER]), ref["flmass"])
+        np.testing.assert_array_equal(s.get("a"), ref["a"])
+    finally:
+        s.close()
