@@ -1008,12 +1008,16 @@ void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *ow
                     const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                     const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
                     const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
-                    int nigrad, double *dudx, double *dudy, double *dudz) {
+                    int nigrad, double *dudx, double *dudy, double *dudz, GradScratch *scratch) {
     hipStream_t s = ctx->stream;
     const int n = st.n;
-    DevBuf old(sizeof(double) * 3 * (size_t)n), gf(sizeof(double) * 3 * (size_t)std::max(nif, 1)),
-        gc(sizeof(double) * 3 * (size_t)std::max(oc.noc, 1));
-    double *ox = old.as<double>(), *oy = ox + n, *oz = oy + n;
+    GradScratch local;
+    GradScratch &S = scratch ? *scratch : local;
+    S.old.ensure(sizeof(double) * 3 * (size_t)n);
+    S.gf.ensure(sizeof(double) * 3 * (size_t)std::max(nif, 1));
+    S.gc.ensure(sizeof(double) * 3 * (size_t)std::max(oc.noc, 1));
+    DevBuf &gf = S.gf, &gc = S.gc;
+    double *ox = S.old.as<double>(), *oy = ox + n, *oz = oy + n;
     FC_HIP_CHECK(hipMemsetAsync(ox, 0, sizeof(double) * 3 * (size_t)n, s));
     for (int lc = 1; lc <= nigrad; ++lc) {
         if (nif)
@@ -1042,9 +1046,13 @@ void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int 
                        const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                        const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
                        const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
-                       double *fmoc) {
+                       double *fmoc, GradScratch *scratch) {
     hipStream_t s = ctx->stream;
-    DevBuf fm(sizeof(double) * (size_t)std::max(nif, 1)), fmo(sizeof(double) * (size_t)std::max(oc.noc, 1));
+    GradScratch local;
+    GradScratch &S = scratch ? *scratch : local;   // gf holds fmcor per face, gc per cut
+    S.gf.ensure(sizeof(double) * (size_t)std::max(nif, 1));
+    S.gc.ensure(sizeof(double) * (size_t)std::max(oc.noc, 1));
+    DevBuf &fm = S.gf, &fmo = S.gc;
     if (nif)
         k_nonorth_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, xf, yf, zf, arx, ary, arz, facint, xc,
                                                          yc, zc, vol, den, apu, dpdx, fm.as<double>(), flmass);

@@ -1212,7 +1212,7 @@ extern "C" fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, con
                           g.yf.as<double>(), g.zf.as<double>(), g.arx.as<double>(), g.ary.as<double>(),
                           g.arz.as<double>(), g.facint.as<double>(), g.xc.as<double>(), g.yc.as<double>(),
                           g.zc.as<double>(), g.vol.as<double>(), dden, dapu, dg, none, sys->su.as<double>(), dfl,
-                          nullptr);
+                          nullptr, &sys->grad_scratch);
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         if (back) download_dbl(ctx, flmass, dfl, nif);
         FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1266,7 +1266,7 @@ extern "C" fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const dou
                        geo.zc.as<double>(), geo.vol.as<double>(), du, G,
                        sys->grad_cb_ptr.get() ? sys->grad_cb_ptr.as<int>() : nullptr,
                        sys->grad_cb_q.get() ? sys->grad_cb_q.as<int>() : nullptr, none, nullptr, nigrad, g, g + n,
-                       g + 2 * (size_t)n);
+                       g + 2 * (size_t)n, &sys->grad_scratch);
         interleave3_dev(ctx, n, g, g + n, g + 2 * (size_t)n, out);
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         if (!out_dev) download_dbl(ctx, dudxi, out, 3 * (size_t)n);

@@ -229,13 +229,18 @@ struct RhieChowDev {
     const double *xc, *yc, *zc, *vol, *den, *apu, *apv, *apw, *u, *v, *w, *p;
     const double *dudx, *dvdx, *dwdx, *dpdx;
 };
+// Scratch of grad_gauss_dev and calcp_nonorth_dev (previous iterate, per-face and per-cut
+// values); a caller that keeps one across calls saves the allocations
+struct GradScratch {
+    DevBuf old, gf, gc;
+};
 // calcp's non-orthogonal corrector source (su overwritten, flmass and fmoc += fmcor)
 void calcp_nonorth_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0, const int *neigh0,
                        const int *kface, const int *kface_order, const double *xf, const double *yf,
                        const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                        const double *xc, const double *yc, const double *zc, const double *vol, const double *den,
                        const double *apu, const double *dpdx, const OcDev &oc, double *su, double *flmass,
-                       double *fmoc);
+                       double *fmoc, GradScratch *scratch = nullptr);
 // adjustMassFlow: inlet sources, outlet fluxes scaled to flomas (0-based faces and values)
 void adjust_mass_flow_dev(fc_context *ctx, int n, const int *owner0_all, const double *arx, const double *ary,
                           const double *arz, int ninl, int inlet_face0, const double *fmi, int nout,
@@ -266,7 +271,8 @@ void grad_gauss_dev(fc_context *ctx, const Structure &st, int nif, const int *ow
                     const double *zf, const double *arx, const double *ary, const double *arz, const double *facint,
                     const double *xc, const double *yc, const double *zc, const double *vol, const double *u,
                     const GradGroups &G, const int *cb_ptr, const int *cb_q, const OcDev &oc, const int *ocface,
-                    int nigrad, double *dudx, double *dudy, double *dudz);
+                    int nigrad, double *dudx, double *dudy, double *dudz,
+                    GradScratch *scratch = nullptr);
 // one velocity component's diagonal, source and 1/a_P (calcuvw), a's diagonal slots updated
 void momentum_rows_dev(fc_context *ctx, const Structure &st, double *a, const double *sp, const double *src,
                        const double *phi, double urf, double *su, double *ap);

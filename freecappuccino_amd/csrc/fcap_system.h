@@ -23,6 +23,7 @@ struct fc_system {
     // the component arrays and the (3, numCells) output
     int grad_bnd[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     fcap::DevBuf grad_cb_ptr, grad_cb_q, grad_u, grad_xyz, grad_out;
+    fcap::GradScratch grad_scratch;
     bool su0_zero = true;
     double last_ms[3] = {0, 0, 0};
     // bumped by every assembly; the network's graph data is rebuilt when stale
