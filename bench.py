@@ -15,6 +15,10 @@ seeded Gaussian blobs with zero mean.  Four solves run on the same system:
                     message-passing network followed by three damped-Jacobi
                     steps (omega 0.8), inside FP64 iterative refinement,
                     IC(0)-PCG fallback on stagnation;
+  * fcg_jacobi_poly -- fcg_learned with the all-zero network: the preconditioner
+                    is then the Jacobi guess and the same three damped-Jacobi
+                    steps (a fixed polynomial), which shows what the trained
+                    network itself buys;
   * ic_pcg_fp64_levels -- ic_pcg_fp64 with one sweep launch per level (the same
                     bits, the schedule before the dependency-driven one).
 `value` is the time to 1e-8 of fcg_learned (ms per step, lower is better);
@@ -48,7 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "ic_pcg_fp64_levels")
+METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "fcg_jacobi_poly", "ic_pcg_fp64_levels")
 
 # HBM bytes per launch of the hidden network layer, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
@@ -453,6 +457,9 @@ def main():
     ap.add_argument("--weights", type=str, default=None, help="network weights (.npy); default: the packaged ones")
     ap.add_argument("--cpu-iters", type=int, default=40, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch each solver iteration's kernels one by one instead of replaying a "
+                         "captured HIP graph (for profilers that do not follow graph launches)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only assemble and run --spmv-reps FP64 SpMVs plus a 1 GiB device copy "
@@ -489,6 +496,8 @@ def main():
         print(json.dumps(profile_io(mesh, args.io_threads, args.io_dir)))
         return
     ctx = fcap.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     if args.profile_calcp or args.profile_momentum:
         cpu_n = 0 if args.no_cpu else args.calcp_cpu_n
         if args.profile_calcp:
@@ -527,13 +536,15 @@ def main():
         ctx.close()
         return
 
-    net = None
+    net = zero_net = None
     layer_ms = None
     if "fcg_learned" in methods:
         net = gnn.Network.load(args.weights or gnn.DEFAULT_WEIGHTS, ctx)
-        net.attach(s)
+    if "fcg_jacobi_poly" in methods:
+        zero_net = gnn.Network(np.zeros(gnn.N_PARAMS, np.float32), ctx)
     code = {"ic_pcg_fp64": fcap.METHOD_ICCG, "ic_pcg_mixed": fcap.METHOD_ICCG_MIXED,
-            "fcg_learned": fcap.METHOD_FCG_LEARNED, "ic_pcg_fp64_levels": fcap.METHOD_ICCG}
+            "fcg_learned": fcap.METHOD_FCG_LEARNED, "fcg_jacobi_poly": fcap.METHOD_FCG_LEARNED,
+            "ic_pcg_fp64_levels": fcap.METHOD_ICCG}
 
     def step(method):
         s.set_solution(np.zeros(n))
@@ -545,6 +556,10 @@ def main():
     results, sols = {}, {}
     for name in methods:
         ctx.set_sweep_mode(fcap.SWEEP_LEVELS if name == "ic_pcg_fp64_levels" else fcap.SWEEP_DEFAULT)
+        if name == "fcg_learned":
+            net.attach(s)
+        elif name == "fcg_jacobi_poly":
+            zero_net.attach(s)
         for _ in range(args.warmup):
             step(code[name])
         times, infos = [], []
@@ -565,8 +580,9 @@ def main():
             np.array_equal(sols["ic_pcg_fp64"], sols["ic_pcg_fp64_levels"]))
     if net is not None:
         layer_ms = net.bench_layer(s, args.spmv_reps)
-        net.detach(s)
         net.close()
+    if zero_net is not None:
+        zero_net.close()
     if "ic_pcg_fp64" in sols:
         x64 = sols["ic_pcg_fp64"]
         for name in results:
@@ -589,7 +605,7 @@ def main():
         "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": {"ic_pcg_fp64": "f64", "ic_pcg_mixed": "f32+f64", "fcg_learned": "bf16+f32+f64",
-                  "ic_pcg_fp64_levels": "f64"}[head],
+                  "fcg_jacobi_poly": "f32+f64", "ic_pcg_fp64_levels": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": workload, "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"],
                    "solver": head,

@@ -190,32 +190,9 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         hipDeviceProp_t prop;
         FC_HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->num_cus = prop.multiProcessorCount;
-        if (const char *g = std::getenv("FCAP_GRAPHS")) ctx->use_graphs = std::atoi(g) != 0;
-        if (const char *m = std::getenv("FCAP_SWEEP")) ctx->sweep_mode = std::atoi(m);
-        if (const char *m = std::getenv("FCAP_SF_WAVES")) ctx->sf_waves = std::atoi(m);
-        if (const char *m = std::getenv("FCAP_SF_SPIN")) ctx->sf_spin = std::atoi(m);
-        if (const char *m = std::getenv("FCAP_GNN_FUSE")) ctx->gnn_fuse_input = std::atoi(m) != 0;
-        if (const char *m = std::getenv("FCAP_GNN_LDS")) ctx->gnn_lds = std::atoi(m) != 0;
-        if (const char *m = std::getenv("FCAP_GNN_SMOOTH")) ctx->gnn_smooth = std::atoi(m);
-        if (const char *m = std::getenv("FCAP_GNN_OMEGA")) ctx->gnn_omega = (float)std::atof(m);
-        if (const char *m = std::getenv("FCAP_GNN_PRE")) ctx->gnn_pre = std::atoi(m);
-        if (const char *m = std::getenv("FCAP_GNN_CHEB")) {
-            float lo = 0.f, hi = 0.f;
-            if (std::sscanf(m, "%f,%f", &lo, &hi) == 2 && lo > 0.f && hi > lo) {
-                ctx->gnn_cheb_lo = lo;
-                ctx->gnn_cheb_hi = hi;
-            }
-        }
-        if (const char *p = std::getenv("FCAP_PACKED")) ctx->packed = std::atoi(p) != 0;
-        if (const char *x = std::getenv("FCAP_XCD")) set_spmv_xcd_order(std::atoi(x));
-        if (const char *u = std::getenv("FCAP_GNN_UNROLL")) ctx->gnn_unroll = std::atoi(u) != 0;
-        if (const char *w = std::getenv("FCAP_FCG_STAG")) ctx->fcg_stag_window = std::max(0, std::atoi(w));
-        if (const char *m = std::getenv("FCAP_FCG_M")) ctx->fcg_m = std::atoi(m);
-        if (const char *e = std::getenv("FCAP_INNER_ETA")) ctx->inner_eta = std::atof(e);
-        if (const char *g = std::getenv("FCAP_GNN_XCD")) ctx->gnn_xcd = std::atoi(g) != 0;
-        if (const char *o = std::getenv("FCAP_GNN_ORDER")) ctx->gnn_order = std::atoi(o) != 0;
-        if (const char *pp = std::getenv("FCAP_GNN_PIPE")) ctx->gnn_pipe = std::max(-1, std::atoi(pp));
-        if (const char *qd = std::getenv("FCAP_GNN_QUAD")) ctx->gnn_quad = std::atoi(qd) != 0;
+        int khz = 0;
+        FC_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+        if (khz > 0) ctx->wall_clock_khz = khz;
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -281,6 +258,13 @@ extern "C" fc_status fc_set_fcg_window(fc_context *ctx, int m) {
     return FC_OK;
 }
 
+extern "C" fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor) {
+    if (!ctx || window < 0 || !(factor >= 0.0 && factor <= 1.0)) return FC_ERR_ARGUMENT;
+    ctx->fcg_stag_window = window;
+    ctx->fcg_stag_factor = factor;
+    return FC_OK;
+}
+
 extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega) {
     if (!ctx || steps < 0 || steps > 8 || !(omega > 0.0 && omega < 2.0)) return FC_ERR_ARGUMENT;
     ctx->gnn_smooth = steps;
@@ -289,8 +273,20 @@ extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double ome
 }
 
 extern "C" fc_status fc_set_sweep_mode(fc_context *ctx, int mode) {
-    if (!ctx || mode < FC_SWEEP_LEVELS || mode > FC_SWEEP_DEPENDENT) return FC_ERR_ARGUMENT;
+    if (!ctx || (mode != FC_SWEEP_LEVELS && mode != FC_SWEEP_DEPENDENT)) return FC_ERR_ARGUMENT;
     ctx->sweep_mode = mode;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_sweep_wait_limit(fc_context *ctx, int microseconds) {
+    if (!ctx || microseconds < 0) return FC_ERR_ARGUMENT;
+    ctx->sf_wait_us = microseconds;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_graphs(fc_context *ctx, int on) {
+    if (!ctx) return FC_ERR_ARGUMENT;
+    ctx->use_graphs = on ? 1 : 0;
     return FC_OK;
 }
 
@@ -840,17 +836,17 @@ static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const
         FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx->stream));
         SweepCfg cfg;
         DevBuf sfs;
-        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && ctx->packed && syncfree_sweeps(st)) {
+        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && syncfree_sweeps(st)) {
             sfs.alloc(sizeof(SyncFree));
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
-            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;
+            cfg.waves = ctx->num_cus;
         }
         pivots_launch(ctx->stream, st, da.as<double>(), dd.as<double>(), dilu, err.as<int>(), &cfg);
         if (cfg.sf) {
             SyncFree hs;
             d2h(&hs, sfs.get(), sizeof(SyncFree), ctx->stream);
-            FC_REQUIRE(!hs.abort, FC_ERR_HIP, "dependency-driven pivots: a wait exceeded its spin limit");
+            FC_REQUIRE(!hs.abort, FC_ERR_HIP, "dependency-driven pivots: a wait exceeded its time limit");
         }
         int h = 0;
         d2h(&h, err.get(), sizeof(int), ctx->stream);
@@ -884,7 +880,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         // the solvers' sweeps: level-packed operands when the structure has them
         DevBuf af, df, ab, db;
         PackedFactor<double> pf;
-        if (ctx->packed && packed_sweeps(st)) {
+        if (packed_sweeps(st)) {
             af.alloc(sizeof(double) * (size_t)st.ell_f * n);
             ab.alloc(sizeof(double) * (size_t)st.ell_b * n);
             df.alloc(sizeof(double) * n);
@@ -904,7 +900,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
-            cfg.waves = ctx->sf_waves > 0 ? ctx->sf_waves : ctx->num_cus;
+            cfg.waves = ctx->num_cus;
         }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
                     nullptr, &cfg, pf.af ? &pf : nullptr);
@@ -912,7 +908,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         if (cfg.sf) {
             SyncFree h;
             d2h(&h, sfs.get(), sizeof(SyncFree), ctx->stream);
-            FC_REQUIRE(!h.abort, FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its spin limit");
+            FC_REQUIRE(!h.abort, FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its time limit");
         }
         download_dbl(ctx, z, dz.get(), n);
         return FC_OK;
@@ -1138,6 +1134,9 @@ extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, co
                   g.xf.as<double>(), g.yf.as<double>(), g.zf.as<double>(), g.arx.as<double>(), g.ary.as<double>(),
                   g.arz.as<double>(), g.facint.as<double>(), F, sys->oc, sys->a.as<double>(), sys->su.as<double>(),
                   dfl, nullptr);
+        // pp = 0 before the pressure-correction solve (calcp:112): the solution vector is the
+        // initial guess of the next fc_sys_solve, and the L1 rule's res0 depends on it
+        FC_HIP_CHECK(hipMemsetAsync(sys->fi.get(), 0, sizeof(double) * n, ctx->stream));
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         sys->assembly_version += 1;
         if (nif && !fl_dev) download_dbl(ctx, flmass, dfl, nif);
@@ -1213,6 +1212,8 @@ extern "C" fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, con
                           g.arz.as<double>(), g.facint.as<double>(), g.xc.as<double>(), g.yc.as<double>(),
                           g.zc.as<double>(), g.vol.as<double>(), dden, dapu, dg, none, sys->su.as<double>(), dfl,
                           nullptr, &sys->grad_scratch);
+        // the corrector's next pass starts again from pp = 0 (calcp:112)
+        FC_HIP_CHECK(hipMemsetAsync(sys->fi.get(), 0, sizeof(double) * n, ctx->stream));
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         if (back) download_dbl(ctx, flmass, dfl, nif);
         FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1285,10 +1286,14 @@ extern "C" fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *
         FC_REQUIRE(method == FC_METHOD_BICGSTAB || method == FC_METHOD_BICGSTAB_MIXED, FC_ERR_ARGUMENT,
                    "method must be FC_METHOD_BICGSTAB or FC_METHOD_BICGSTAB_MIXED");
         const int n = sys->st.n;
+        for (int c = 0; c < 3; ++c)
+            FC_REQUIRE(sp[c] && src[c] && phi[c] && urf[c] > 0.0, FC_ERR_ARGUMENT, "component arrays / urf");
         DevBuf dsp, dsrc, dap;
         dap.alloc(sizeof(double) * n);
+        // calcuvw solves u, v and w whatever each solve's outcome (src/calcuvw.f90:441,498,556):
+        // a component that runs out of nsw does not stop the next one; errors do.
+        fc_status worst = FC_OK;
         for (int c = 0; c < 3; ++c) {
-            FC_REQUIRE(sp[c] && src[c] && phi[c] && urf[c] > 0.0, FC_ERR_ARGUMENT, "component arrays / urf");
             upload_dbl(ctx, dsp, sp[c], n);
             upload_dbl(ctx, dsrc, src[c], n);
             h2d(sys->fi.get(), phi[c], sizeof(double) * n, ctx->stream);
@@ -1313,9 +1318,10 @@ extern "C" fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *
             if (info) info[c] = local;
             d2h(phi[c], sys->fi.get(), sizeof(double) * n, ctx->stream);
             if (ap && ap[c]) d2h(ap[c], dap.get(), sizeof(double) * n, ctx->stream);
-            if (s != FC_OK && s != FC_EARLY_EXIT) return s;
+            if (s < 0) return s;
+            if (s == FC_NOT_CONVERGED) worst = FC_NOT_CONVERGED;
         }
-        return FC_OK;
+        return worst;
     });
 }
 
@@ -1410,7 +1416,6 @@ extern "C" void *fc_sys_device_ptr(fc_system *sys, int which) {
     case FC_ARR_ZF: return g.zf.get();
     case FC_ARR_RES: return sys->ctx->ws ? sys->ctx->ws->r.get() : nullptr;
     case FC_ARR_PIVOTS: return sys->ctx->ws ? sys->ctx->ws->d.get() : nullptr;
-    case FC_ARR_GNN_PERM: return sys->graph.lds ? sys->graph.perm.get() : nullptr;
     default: return nullptr;
     }
 }
@@ -1433,7 +1438,6 @@ extern "C" fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_
         case FC_ARR_JA: expect = sizeof(int) * nnz; index = true; break;
         case FC_ARR_DIAG: expect = sizeof(int) * n; index = true; break;
         case FC_ARR_IJ_SLOT: case FC_ARR_JI_SLOT: expect = sizeof(int) * nif; index = true; break;
-        case FC_ARR_GNN_PERM: expect = sizeof(int) * n; index = true; break;
         default: throw Error(FC_ERR_ARGUMENT, "unknown array");
         }
         FC_REQUIRE(bytes >= expect, FC_ERR_ARGUMENT, "output buffer too small");

@@ -35,20 +35,8 @@ struct GnnGraph {
     // separates two neighbours).
     int nblk = 0;
     DevBuf border;
-    // The cells renumbered by the Morton code of their per-axis ranks, for the
-    // inference layer kernels that stage a block's rows in LDS: new cell i' is
-    // old cell perm[i'], so a 64-cell block is a compact cluster.  Each block
-    // lists the out-of-block rows its cells read (halo, at most kGnnHaloMax, in
-    // new numbering) and every ELL slot its row in the block's LDS tile (lidx:
-    // 0..63 the block's own cells, 64 + t halo row t).  pcnt and pef are the
-    // counts and features of the ELL in the new numbering.  Built once per
-    // structure; pef again after each assembly.
-    bool lds = false;   // built, and every block's halo fits
-    int lds_n = -1;     // cells the layout was built for (-1: not tried)
-    DevBuf perm, pcnt, pef, halo, hcnt, lidx;
     bool ready = false;
 };
-constexpr int kGnnHaloMax = 160;
 
 // Build the graph data of an assembled system (device arrays of the system).
 void gnn_prepare_graph(fc_context *ctx, GnnGraph &g, const Structure &st, const int *kface, const int *owner0,

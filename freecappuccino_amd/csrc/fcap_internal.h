@@ -136,42 +136,26 @@ struct fc_context {
     std::unique_ptr<fcap::Workspace> ws;
     fcap::Timer timer;
     int num_cus = 256;
-    // replay solver iterations as HIP graphs (FCAP_GRAPHS=0 launches kernels directly)
+    // replay solver iterations as HIP graphs (fc_set_graphs; off: launch kernels directly)
     int use_graphs = 1;
-    // triangular sweeps: 0 one launch per level, 1 persistent cooperative kernel
-    // with a grid barrier per level (FCAP_SWEEP selects)
-    int sweep_mode = 2;   // FC_SWEEP_DEPENDENT
-    int sf_waves = 0;     // persistent waves of the dependency-driven sweeps (0: one per CU)
-    int sf_spin = 0;      // poll limit of their waits (0: the built-in one)
-    // per-level sweeps over level-packed operands (FCAP_PACKED=0: over the CSR rows)
-    int packed = 1;
-    // network layers gather all neighbour rows of a cell at once (FCAP_GNN_UNROLL=0: one at a time)
-    int gnn_unroll = 1;
-    // flexible CG stagnation window in iterations (FCAP_FCG_STAG; 0: no stagnation check)
+    // triangular sweeps: FC_SWEEP_LEVELS or FC_SWEEP_DEPENDENT (fc_set_sweep_mode)
+    int sweep_mode = 2;
+    // wall-clock limit of one wait of the dependency-driven sweeps, microseconds
+    // (fc_set_sweep_wait_limit; 0: the built-in 2 s)
+    int sf_wait_us = 0;
+    int wall_clock_khz = 100000;   // rate of the device's wall clock (wall_clock64())
+    // flexible CG stagnation test (fc_set_fcg_stagnation): after the first window of
+    // fcg_stag_window iterations, every window's best ||r||^2 must fall below
+    // fcg_stag_factor x the best before it (window 0: no test)
     int fcg_stag_window = 64;
-    // flexible CG directions kept, rounded up to 4, 8, 12 or 16 (FCAP_FCG_M, fc_set_fcg_window)
+    double fcg_stag_factor = 0.81;
+    // flexible CG directions kept, rounded up to 4, 8, 12 or 16 (fc_set_fcg_window)
     int fcg_m = 12;
-    // inner tolerance of the FP32 solves in FP64 refinement (FCAP_INNER_ETA,
-    // fc_set_inner_tolerance; 0: the method's default)
+    // inner tolerance of the FP32 solves in FP64 refinement (fc_set_inner_tolerance;
+    // 0: the method's default)
     double inner_eta = 0.0;
-    // network layer kernel: XCD-contiguous cell ranges (FCAP_GNN_XCD=0: natural block order)
-    int gnn_xcd = 1;
-    // network layer kernel: 64-cell blocks in Morton order of their centroids
-    // (FCAP_GNN_ORDER=0: block index order)
-    int gnn_order = 1;
-    // network layer kernel: persistent software-pipelined waves, this many per XCD
-    // (FCAP_GNN_PIPE; -1: as many as are resident; 0: one 64-cell block per workgroup)
-    int gnn_pipe = -1;
-    // persistent layer kernel gathers rows with quads of lanes (FCAP_GNN_QUAD=0: one lane per cell)
-    int gnn_quad = 1;
-    int gnn_fuse_input = 1;   // inference: the first layer computes the encoder rows from u
-    int gnn_lds = 0;          // inference: renumbered cells, a block's rows staged in LDS (FCAP_GNN_LDS=1)
-    int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG
+    int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG (fc_set_fcg_smoothing)
     float gnn_omega = 0.8f;   // their damping
-    // Chebyshev steps instead (lo < hi): step k damps with 1 / theta_k, theta_k the
-    // roots of the Chebyshev polynomial of degree gnn_smooth mapped to [lo, hi]
-    float gnn_cheb_lo = 0.f, gnn_cheb_hi = 0.f;
-    int gnn_pre = 0;          // a damped-Jacobi step before the network too
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

@@ -38,6 +38,7 @@ struct Scalars {
     double fcg_pr;                             // flexible CG: p.r of this step
     double fcg_pqh[kFcgM], fcg_bet[kFcgM];     // flexible CG: p.q and beta per history slot
     double stag_ref, stag_min;                 // flexible CG: best ||r||^2 before / within the window
+    double stag_factor;                        // flexible CG: a window's best must beat stag_factor x stag_ref
     int stag_every;                            // flexible CG: stagnation window (0: no check)
     int fcg_m;                                 // flexible CG: directions kept (1..kFcgM)
     int iters, done, status, nsw;

@@ -16,7 +16,6 @@ struct OcView {
 int red_grid(int n);
 int spmv_grid(int n);
 // SpMV block order for the whole process: 0 natural (default), 1 XCD-aware
-void set_spmv_xcd_order(int on);
 
 // mode 0: y = A x; 1: y = b - A x with L1 (pa) and L2^2 (pb) partials and an
 // optional FP32 copy y32; 2: y = A x with partial y.v (pa); 3: as 2 plus y.y (pb)
@@ -33,38 +32,28 @@ void residual32(hipStream_t s, int n, const int *ptr, const int *ja, const float
                 float *y, const Scalars *sc);
 void oc_launch64(hipStream_t s, int n, const OcView &oc, const double *x, double *y, int sign, const Scalars *sc);
 
-// Grid barrier state of the persistent sweeps.  `abort` is sticky: once a
-// barrier wait exceeds its spin limit every block leaves and the host reports
-// the solve as failed.
-struct GridBar {
-    unsigned count, gen;
-    int abort, pad;
-};
-
 // State of the dependency-driven sweeps: the next block of 64 packed rows to
 // hand out, the number of waves that have left the kernel (the last one resets
 // both for the next launch), a sticky abort flag raised when a wait exceeds
-// its poll limit, and that limit (0: the built-in one).
+// its time limit, that limit in microseconds (0: the built-in one) and the
+// rate of the device's wall clock in kHz.
 struct SyncFree {
     unsigned ticket, fin;
-    int abort, spin;
+    int abort, wait_us;
+    unsigned rate_khz, pad;
 };
-// zeroed state with the context's poll limit, on the stream (not for graph capture)
+// zeroed state with the context's wait limit, on the stream (not for graph capture)
 void syncfree_reset(fc_context *ctx, hipStream_t s, SyncFree *sf);
 
-// How the triangular sweeps run: one launch per level (default), one
-// persistent cooperative kernel per sweep with `blocks` co-resident workgroups
-// and a grid barrier between levels (bar), or one launch per sweep in which
-// every row waits only for the rows it reads (sf; needs the packed factor;
-// `waves` persistent one-wave workgroups, `tmp` holds the forward result).
+// How the triangular sweeps run: one launch per level (default without sf), or
+// one launch per sweep in which every row waits only for the rows it reads (sf;
+// needs the packed factor; `waves` persistent one-wave workgroups, `tmp` holds
+// the forward result).
 struct SweepCfg {
-    GridBar *bar = nullptr;
-    int blocks = 0;
     SyncFree *sf = nullptr;
     int waves = 0;
     void *tmp = nullptr;
 };
-SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar);
 // dependency-driven sweeps when the structure is packed at a width they handle
 bool syncfree_sweeps(const Structure &st);
 
@@ -87,8 +76,8 @@ void pack_factor32(hipStream_t s, const Structure &st, const float *a, const flo
 void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *d, bool dilu, int *err,
                    const SweepCfg *cfg = nullptr);
 // z = M^-1 rhs with the sweeps of src/iccg.f90:94-111: level-packed kernels when
-// `pf` is given, one launch per level over the CSR rows otherwise, or the
-// persistent cooperative kernels when cfg->bar is set.
+// `pf` is given (one dependency-driven launch per sweep with cfg->sf), one
+// launch per level over the CSR rows otherwise.
 void lu_launch64(hipStream_t s, const Structure &st, const double *a, const double *d, double small,
                  const double *rhs, double *w, const Scalars *sc, const SweepCfg *cfg = nullptr,
                  const PackedFactor<double> *pf = nullptr);

@@ -33,11 +33,10 @@ int spmv_grid(int n) { return grid_for(n, kSpmvRows); }
 // launch grid of a kernel with an XCD-aware block order: `blocks` rounded up to a multiple of kXcds
 static int xcd_grid(int blocks) { return (blocks + kXcds - 1) / kXcds * kXcds; }
 
-// SpMV block order, process-wide: 0 natural (workgroup b takes row block b),
-// 1 XCD-aware.  Measured at 192^3 the natural order is faster (5.10 vs
-// 4.86 TB/s), so it is the default; FCAP_XCD=1 selects the other.
-static int g_spmv_xcd = 0;
-void set_spmv_xcd_order(int on) { g_spmv_xcd = on ? 1 : 0; }
+// SpMV block order: workgroup b takes row block b.  (An XCD-aware order, each
+// XCD sweeping one contiguous eighth of the rows, measured slower at 192^3:
+// 4.86 against 5.10 TB/s.)
+constexpr int g_spmv_xcd = 0;
 
 // ------------------------------------------------------------------- SpMV
 
@@ -319,117 +318,6 @@ void pivots_launch(hipStream_t s, const Structure &st, const double *a, double *
     FC_HIP_CHECK(hipGetLastError());
 }
 
-// ----------------------------------------------- persistent level sweeps
-// One cooperative launch per sweep; the blocks walk the levels in order and
-// meet at a grid barrier between levels.  Co-residency of every block is
-// guaranteed by hipLaunchCooperativeKernel.  A barrier wait gives up after a
-// bounded number of polls and raises the sticky abort flag, so every block
-// always reaches the end of the kernel.
-
-constexpr long kSpinLimit = 4000000;
-
-__device__ inline bool grid_sync(GridBar *bar, unsigned nblocks) {
-    __shared__ int s_ok;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            long it = 0;
-            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (++it > kSpinLimit) {
-                    __hip_atomic_store(&bar->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        __threadfence();
-        s_ok = __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_fwd_persistent(const int *__restrict__ rows, const int *__restrict__ lvl,
-                                                        int nlev, const int *__restrict__ ptr,
-                                                        const int *__restrict__ ja, const int *__restrict__ diag,
-                                                        const T *__restrict__ a, const T *__restrict__ d,
-                                                        const T *__restrict__ rhs, T *w,
-                                                        const Scalars *__restrict__ sc, GridBar *bar) {
-    if (sc && sc->done) return;
-    const int stride = gridDim.x * blockDim.x;
-    for (int l = 0; l < nlev; ++l) {
-        const int beg = lvl[l], end = lvl[l + 1];
-        for (int p = beg + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += stride) {
-            int i = rows[p];
-            T acc = rhs[i];
-            for (int k = ptr[i]; k < diag[i]; ++k) acc = acc - a[k] * w[ja[k]];
-            w[i] = acc * d[i];
-        }
-        if (l + 1 < nlev && !grid_sync(bar, gridDim.x)) return;
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_bwd_persistent(const int *__restrict__ rows, const int *__restrict__ lvl,
-                                                        int nlev, const int *__restrict__ ptr,
-                                                        const int *__restrict__ ja, const int *__restrict__ diag,
-                                                        const T *__restrict__ a, const T *__restrict__ d, T small,
-                                                        T *w, const Scalars *__restrict__ sc, GridBar *bar) {
-    if (sc && sc->done) return;
-    const int stride = gridDim.x * blockDim.x;
-    for (int l = 0; l < nlev; ++l) {
-        const int beg = lvl[l], end = lvl[l + 1];
-        for (int p = beg + blockIdx.x * blockDim.x + threadIdx.x; p < end; p += stride) {
-            int i = rows[p];
-            T di = d[i];
-            T acc = w[i] / (di + small);
-            for (int k = diag[i] + 1; k < ptr[i + 1]; ++k) acc = acc - a[k] * w[ja[k]];
-            w[i] = acc * di;
-        }
-        if (l + 1 < nlev && !grid_sync(bar, gridDim.x)) return;
-    }
-}
-
-SweepCfg make_sweep_cfg(fc_context *ctx, const Structure &st, GridBar *bar) {
-    SweepCfg c;
-    if (ctx->sweep_mode != 1 || !bar) return c;
-    int occ = 0;
-    FC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)k_fwd_persistent<double>, 256, 0));
-    int occ2 = 0;
-    FC_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ2, (const void *)k_bwd_persistent<double>, 256, 0));
-    occ = std::min(std::min(occ, occ2), 2);
-    int want = std::max(1, (std::max(st.max_level_f, st.max_level_b) + 255) / 256);
-    c.blocks = std::max(1, std::min(occ * ctx->num_cus, want));
-    c.bar = bar;
-    return c;
-}
-
-template <class T>
-void lu_persistent(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
-                   const Scalars *sc, const SweepCfg &cfg) {
-    const int *ptr = st.ioffset.as<int>(), *ja = st.ja.as<int>(), *dg = st.diag.as<int>();
-    const int *rf = st.rows_f.as<int>(), *rb = st.rows_b.as<int>();
-    const int *lf = st.lvl_f_d.as<int>(), *lb = st.lvl_b_d.as<int>();
-    int nf = st.nlev_f, nb = st.nlev_b;
-    GridBar *bar = cfg.bar;
-    void *fargs[] = {(void *)&rf, (void *)&lf, (void *)&nf, (void *)&ptr, (void *)&ja, (void *)&dg,
-                     (void *)&a, (void *)&d, (void *)&rhs, (void *)&w, (void *)&sc, (void *)&bar};
-    FC_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_fwd_persistent<T>, dim3(cfg.blocks), dim3(256), fargs, 0,
-                                            s));
-    void *bargs[] = {(void *)&rb, (void *)&lb, (void *)&nb, (void *)&ptr, (void *)&ja, (void *)&dg,
-                     (void *)&a, (void *)&d, (void *)&small, (void *)&w, (void *)&sc, (void *)&bar};
-    FC_HIP_CHECK(hipLaunchCooperativeKernel((const void *)k_bwd_persistent<T>, dim3(cfg.blocks), dim3(256), bargs, 0,
-                                            s));
-}
-
 // ------------------------------------------------- level-packed sweeps
 
 // packed row p of a level: operands at [k * n + p] (coalesced across the
@@ -576,14 +464,22 @@ void pack_factor_t(hipStream_t s, const Structure &st, const T *a, const T *d, T
 // outputs of the rows it reads until none holds the pattern, computes exactly
 // as k_fwd_packed / k_bwd_packed (same operands, same order), and stores its
 // value with one coherent store, which is also its "ready" signal.  A wave
-// whose wait exceeds the poll limit, or that sees the abort flag, raises the
+// whose wait exceeds the time limit, or that sees the abort flag, raises the
 // flag and leaves; the host then reports the solve as failed.
 
 // Waves in flight poll the coherent level of the memory system; the fewer there
 // are, the faster the chain of levels advances (192^3 box: 256 waves 1.87 s,
-// 512 1.96 s, 2048 2.55 s, 4096 3.61 s per FP64 IC(0)-PCG solve; at 8192 a wait
-// ran past 2^20 polls).  The limit only ends waits that cannot end otherwise.
-constexpr int kSfSpin = 1 << 22;
+// 512 1.96 s, 2048 2.55 s, 4096 3.61 s per FP64 IC(0)-PCG solve).  A wait is
+// bounded in wall-clock time (wall_clock64(), a constant-rate counter), not in
+// polls, so the bound does not shrink with a deeper chain or a faster clock:
+// it only ends waits that cannot end otherwise.
+constexpr unsigned long long kSfWaitUs = 2000000ull;   // 2 s
+
+// the wait limit in wall-clock ticks (sf->wait_us microseconds, 0: kSfWaitUs)
+__device__ inline unsigned long long sf_wait_ticks(const SyncFree *sf, unsigned long long rate_khz) {
+    const unsigned long long us = sf->wait_us > 0 ? (unsigned long long)sf->wait_us : kSfWaitUs;
+    return us * rate_khz / 1000ull + 1ull;
+}
 
 template <class T> struct SfBits;
 template <> struct SfBits<double> {
@@ -630,7 +526,7 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
     using U = typename SfBits<T>::U;
     const int lane = threadIdx.x;
     const unsigned nblk = (unsigned)((n + 63) / 64);
-    const int limit = sf->spin > 0 ? sf->spin : kSfSpin;
+    const unsigned long long limit = sf_wait_ticks(sf, sf->rate_khz);
     for (;;) {
         unsigned b = 0;
         if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -656,6 +552,7 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
             acc = BWD ? in[i] / (di + small) : in[i];
         }
         int spins = 0;
+        const unsigned long long t0 = wall_clock64();
         for (;;) {
             if (!done) {
                 bool all = true;
@@ -684,9 +581,11 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
             }
             if (__all(done)) break;
             ++spins;
-            bool give_up = spins > limit;
-            if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                give_up = true;
+            bool give_up = false;
+            if ((spins & 63) == 0) {
+                give_up = wall_clock64() - t0 > limit ||
+                          __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+            }
             if (__any(give_up)) {
                 if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -715,7 +614,7 @@ __global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__rest
     using U = SfBits<double>::U;
     const int lane = threadIdx.x;
     const unsigned nblk = (unsigned)((n + 63) / 64);
-    const int limit = sf->spin > 0 ? sf->spin : kSfSpin;
+    const unsigned long long limit = sf_wait_ticks(sf, sf->rate_khz);
     for (;;) {
         unsigned b = 0;
         if (lane == 0) b = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -748,6 +647,7 @@ __global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__rest
             }
         }
         int spins = 0;
+        const unsigned long long t0 = wall_clock64();
         for (;;) {
             if (!done) {
                 bool all = true;
@@ -776,9 +676,11 @@ __global__ __launch_bounds__(64) void k_pivots_syncfree(int n, const int *__rest
             }
             if (__all(done)) break;
             ++spins;
-            bool give_up = spins > limit;
-            if ((spins & 255) == 0 && __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                give_up = true;
+            bool give_up = false;
+            if ((spins & 63) == 0) {
+                give_up = wall_clock64() - t0 > limit ||
+                          __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+            }
             if (__any(give_up)) {
                 if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -812,8 +714,11 @@ void pivots_syncfree_w(hipStream_t s, const Structure &st, const double *a, doub
 }
 
 void syncfree_reset(fc_context *ctx, hipStream_t s, SyncFree *sf) {
-    FC_HIP_CHECK(hipMemsetAsync(sf, 0, sizeof(SyncFree), s));
-    if (ctx->sf_spin > 0) FC_HIP_CHECK(hipMemcpyAsync(&sf->spin, &ctx->sf_spin, sizeof(int), hipMemcpyHostToDevice, s));
+    SyncFree h{};
+    h.wait_us = ctx->sf_wait_us;
+    h.rate_khz = (unsigned)ctx->wall_clock_khz;
+    FC_HIP_CHECK(hipMemcpyAsync(sf, &h, sizeof(SyncFree), hipMemcpyHostToDevice, s));
+    FC_HIP_CHECK(hipStreamSynchronize(s));   // h lives on this stack frame
 }
 
 bool syncfree_sweeps(const Structure &st) {
@@ -861,10 +766,6 @@ void lu_syncfree(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, 
 template <class T>
 void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
                  const Scalars *sc, const SweepCfg *cfg, const PackedFactor<T> *pf) {
-    if (cfg && cfg->bar) {
-        lu_persistent<T>(s, st, a, d, small, rhs, w, sc, *cfg);
-        return;
-    }
     if (cfg && cfg->sf && pf && pf->af) {
         lu_syncfree<T>(s, st, *pf, small, rhs, w, sc, *cfg);
         return;

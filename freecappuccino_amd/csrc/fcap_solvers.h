@@ -13,7 +13,7 @@ struct Workspace {
     // FP32 vectors and matrix copies
     DevBuf r32, z32, p32, q32, e32, a32, d32;
     // reduction partials, summed scalars, device solver state, error flag
-    DevBuf pa, pb, sums, sc, err, bar;
+    DevBuf pa, pb, sums, sc, err;
     // dependency-driven sweeps: their ticket state and the forward sweep's result
     DevBuf sf, sf_tmp;
     // flexible CG: the last m directions p and their images A p (FP32, m x n each)

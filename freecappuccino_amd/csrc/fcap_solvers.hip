@@ -43,7 +43,6 @@ enum FinStep : int {
 };
 
 // FCG stagnation: window length in fc_context::fcg_stag_window (64 by default)
-constexpr double kStagFactor = 0.81;    // a window's best ||r||^2 must beat 0.81x the best before it
 
 __device__ inline void converge_test(Scalars *sc) {
     sc->iters += 1;
@@ -152,7 +151,7 @@ __global__ __launch_bounds__(kFinBlock) void k_fin(int step, int nparts, const d
             // only sets the reference; every later window's best must beat the best
             // before it by the factor.
             const bool first = sc->iters == sc->stag_every;
-            if (!first && !(sc->stag_min < kStagFactor * sc->stag_ref)) {
+            if (!first && !(sc->stag_min < sc->stag_factor * sc->stag_ref)) {
                 sc->done = 1;
                 sc->status = FC_FALLBACK_USED;
             }
@@ -468,7 +467,7 @@ struct GraphExec {
 
 void run_batches(Ctx &c, int batch, const std::function<void()> &iter) {
     GraphExec ge;
-    if (c.ctx->use_graphs && c.s != nullptr && !c.sw.bar) {
+    if (c.ctx->use_graphs && c.s != nullptr) {
         hipGraph_t g = nullptr;
         FC_HIP_CHECK(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
         try {
@@ -511,7 +510,6 @@ void Workspace::ensure(int n_, int nnz_) {
     sums.ensure(sizeof(double) * 4);
     sc.ensure(sizeof(Scalars));
     err.ensure(sizeof(int) * 4);
-    bar.ensure(sizeof(GridBar));
     sf.ensure(sizeof(SyncFree));
     sf_tmp.ensure(D);
     n = n_;
@@ -525,39 +523,28 @@ static Ctx make_ctx(fc_context *ctx, int n) {
 }
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
-    if (c.ctx->sweep_mode == 2 && c.ctx->packed && syncfree_sweeps(st)) {
+    if (c.ctx->sweep_mode == FC_SWEEP_DEPENDENT && syncfree_sweeps(st)) {
         syncfree_reset(c.ctx, c.s, c.ws.sf.as<SyncFree>());
         c.sw.sf = c.ws.sf.as<SyncFree>();
         c.sw.tmp = c.ws.sf_tmp.get();
-        c.sw.waves = c.ctx->sf_waves > 0 ? c.ctx->sf_waves : c.ctx->num_cus;
-        return;
+        c.sw.waves = c.ctx->num_cus;
     }
-    if (c.ctx->sweep_mode != 1) return;
-    FC_HIP_CHECK(hipMemsetAsync(c.ws.bar.get(), 0, sizeof(GridBar), c.s));
-    c.sw = make_sweep_cfg(c.ctx, st, c.ws.bar.as<GridBar>());
 }
 
 static void check_sweeps(Ctx &c) {
-    if (c.sw.sf) {
-        SyncFree h;
-        FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.sf, sizeof(SyncFree), hipMemcpyDeviceToHost, c.s));
-        FC_HIP_CHECK(hipStreamSynchronize(c.s));
-        if (h.abort) throw Error(FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its spin limit");
-    }
-    if (!c.sw.bar) return;
-    GridBar h;
-    FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.bar, sizeof(GridBar), hipMemcpyDeviceToHost, c.s));
+    if (!c.sw.sf) return;
+    SyncFree h;
+    FC_HIP_CHECK(hipMemcpyAsync(&h, c.sw.sf, sizeof(SyncFree), hipMemcpyDeviceToHost, c.s));
     FC_HIP_CHECK(hipStreamSynchronize(c.s));
-    if (h.abort) throw Error(FC_ERR_HIP, "persistent sweep: a grid barrier wait exceeded its spin limit");
+    if (h.abort) throw Error(FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its time limit");
 }
 
 // The factor (a, pivots d) in level-packed order for this solve's sweeps; empty
-// (sweeps over the CSR rows) when the structure is not packed, packing is
-// switched off, or the persistent sweeps run.
+// (sweeps over the CSR rows) when the structure is not packed.
 template <class T>
 static PackedFactor<T> pack_for_solve(Ctx &c, const Structure &st, const T *a, const T *d) {
     PackedFactor<T> pf;
-    if (!c.ctx->packed || c.sw.bar || !packed_sweeps(st)) return pf;
+    if (!packed_sweeps(st)) return pf;
     Workspace &ws = c.ws;
     constexpr bool f32 = sizeof(T) == sizeof(float);
     DevBuf &af = f32 ? ws.pk_af32 : ws.pk_af, &df = f32 ? ws.pk_df32 : ws.pk_df;
@@ -893,6 +880,7 @@ static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double
     h.stag_ref = 1e300;    // set by the first window
     h.stag_min = 1e300;
     h.stag_every = c.ctx->fcg_stag_window;
+    h.stag_factor = c.ctx->fcg_stag_factor;
     h.fcg_m = fcg_window(c.ctx);
     h.status = kStatusRunning;
     upload_scalars(c, h);
@@ -979,9 +967,13 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
             FC_HIP_CHECK(hipMemcpyAsync(args.fi, xgood, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
             fallback_ms = ctx->timer.stop_ms(c.s);
             fell_back = true;
+            // The fallback gets what is left of the caller's nsw, and keeps the caller's
+            // stopping rule: the L2 rule is relative to ||su|| already; the L1 rule is
+            // relative to the first refinement's ||r0||_1, so the fallback's tolerance is
+            // rescaled from its own start (resl: ||r||_1 at that iterate).
             SolveArgs fb = args;
-            fb.stop_rule = FC_STOP_L2;
-            fb.nsw = std::max(1, args.nsw);
+            fb.nsw = std::max(1, args.nsw - total_inner);
+            if (!l2) fb.sor = args.sor * (res0 + kSmallSrc) / (resl + kSmallSrc);
             fc_solve_info fi;
             std::memset(&fi, 0, sizeof fi);
             ensure_levels(ctx, const_cast<Structure &>(st));
@@ -989,7 +981,7 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
             fallback_iters = fi.iters;
             fallback_ms += fi.setup_ms + fi.solve_ms;
             resl = fi.resl;
-            status = (fs == FC_OK) ? FC_FALLBACK_USED : fs;
+            status = (fs == FC_OK || fs == FC_EARLY_EXIT) ? FC_FALLBACK_USED : fs;
             break;
         }
         k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);

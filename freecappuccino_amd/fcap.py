@@ -37,7 +37,7 @@ VARIANT_SRC, VARIANT_TESTS = 0, 1
 METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
 METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
-SWEEP_LEVELS, SWEEP_BARRIER, SWEEP_DEPENDENT = 0, 1, 2
+SWEEP_LEVELS, SWEEP_DEPENDENT = 0, 2
 SWEEP_DEFAULT = SWEEP_DEPENDENT
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20, gnn_perm=21)
@@ -78,6 +78,8 @@ def library() -> C.CDLL:
         "fc_set_variant": ([vp, i], i),
         "fc_set_inner_tolerance": ([vp, d], i), "fc_set_fcg_window": ([vp, i], i),
         "fc_set_sweep_mode": ([vp, i], i), "fc_set_fcg_smoothing": ([vp, i, d], i),
+        "fc_set_fcg_stagnation": ([vp, i, d], i),
+        "fc_set_sweep_wait_limit": ([vp, i], i), "fc_set_graphs": ([vp, i], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
@@ -211,10 +213,23 @@ class Context:
         """Damped-Jacobi steps after the network in the learned FCG's preconditioner."""
         self.check(self.L.fc_set_fcg_smoothing(self.h, int(steps), float(omega)))
 
+    def set_fcg_stagnation(self, window: int = 64, factor: float = 0.81):
+        """Stagnation test of the learned FCG's inner solves (window 0: off; factor 0 forces the
+        IC(0)-PCG fallback after the second window)."""
+        self.check(self.L.fc_set_fcg_stagnation(self.h, int(window), float(factor)))
+
     def set_sweep_mode(self, mode: int):
-        """Schedule of the triangular sweeps: SWEEP_LEVELS, SWEEP_BARRIER or SWEEP_DEPENDENT
+        """Schedule of the triangular sweeps: SWEEP_LEVELS or SWEEP_DEPENDENT
         (bit-identical results)."""
         self.check(self.L.fc_set_sweep_mode(self.h, int(mode)))
+
+    def set_sweep_wait_limit(self, microseconds: int = 0):
+        """Wall-clock limit of one wait of the dependency-driven sweeps (0: the built-in 2 s)."""
+        self.check(self.L.fc_set_sweep_wait_limit(self.h, int(microseconds)))
+
+    def set_graphs(self, on: bool = True):
+        """Replay solver iterations as HIP graphs (default) or launch kernels one by one."""
+        self.check(self.L.fc_set_graphs(self.h, 1 if on else 0))
 
     def set_report(self, on: bool, name: str = "p"):
         self.check(self.L.fc_set_report(self.h, 1 if on else 0, name.encode()))

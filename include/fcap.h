@@ -277,19 +277,35 @@ fc_status fc_set_fcg_window(fc_context *ctx, int m);
  * steps z <- z + omega (r - A z) / a_ii (0..8 steps, 0 < omega < 2; default 3
  * steps, omega 0.8).  No counterpart in the reference. */
 fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega);
+/* Stagnation test of FC_METHOD_FCG_LEARNED's inner solves: the first `window`
+ * iterations only set the reference; after every later window the window's
+ * smallest ||r||_2^2 must be below factor x the smallest before it, else plain
+ * FP64 IC(0)-PCG finishes the solve from the last refined iterate
+ * (FC_FALLBACK_USED) with what is left of nsw and the caller's stopping rule.
+ * window 0 switches the test off; 0 <= factor <= 1 (factor 0 forces the
+ * fallback after the second window).  Default 64 iterations, 0.81.  No
+ * counterpart in the reference. */
+fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
 
 /* Schedule of the DIC/DILU triangular sweeps (src/iccg.f90:94-111,
  * src/bicgstab.f90:107-124); every schedule computes each row with the same
  * operations in the same order, so results are bit-identical.
  *   FC_SWEEP_LEVELS    one launch per level;
- *   FC_SWEEP_BARRIER   one cooperative launch per sweep, grid barrier per level;
  *   FC_SWEEP_DEPENDENT one launch per sweep, each row waiting only for the rows
  *                      it reads (default; packed structures of width 3, 4 or 8,
  *                      other structures keep the level launches).  A wait that
- *                      runs past its poll limit fails the call with FC_ERR_HIP.
+ *                      runs past its time limit fails the call with FC_ERR_HIP.
  * No counterpart in the reference. */
-enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_BARRIER = 1, FC_SWEEP_DEPENDENT = 2 };
+enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_DEPENDENT = 2 };
 fc_status fc_set_sweep_mode(fc_context *ctx, int mode);
+/* Wall-clock limit of one wait of the dependency-driven sweeps, in
+ * microseconds (0: the built-in 2 s).  A wait that runs past it ends the
+ * sweep and the call returns FC_ERR_HIP.  No counterpart in the reference. */
+fc_status fc_set_sweep_wait_limit(fc_context *ctx, int microseconds);
+/* Replay each solver iteration as a captured HIP graph (1, default) or launch
+ * its kernels one by one (0; e.g. under a profiler).  Same results either way.
+ * No counterpart in the reference. */
+fc_status fc_set_graphs(fc_context *ctx, int on);
 
 /* ------------------------------------------------ device-resident system */
 
@@ -330,7 +346,9 @@ fc_status fc_sys_assemble_convdiff(fc_system *sys, const double *vis, const doub
  * pressure correction (calcp's iccg call, line 119).  Cell fields have numCells
  * entries, gradients (3, numCells); flmass (numInnerFaces) is written.  Each of
  * these pointers may be host or device memory: device pointers are read and
- * written in place, host arrays pass through buffers the system keeps. */
+ * written in place, host arrays pass through buffers the system keeps.  The
+ * system's solution vector (the pressure correction pp) is set to zero, as
+ * calcp sets pp = 0 before its solve (line 112). */
 fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double *apu,
                                 const double *apv, const double *apw, const double *u,
                                 const double *v, const double *w, const double *p,
@@ -361,7 +379,8 @@ fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, const double *d
  * src/facefluxmass.f90:520-607) on the resident system: as fc_calcp_nonorth,
  * with su written into the system for the corrector's next fc_sys_solve and
  * flmass (numInnerFaces) updated in place.  Host or device memory, as for
- * fc_sys_calcp_assemble. */
+ * fc_sys_calcp_assemble.  The system's solution vector is then set to zero
+ * for the next corrector pass (calcp:112). */
 fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, const double *apu,
                                const double *dPdxi, double *flmass);
 /* Upload a RHS (numCells) replacing su. */
@@ -374,8 +393,10 @@ fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
  * reference), then phi[c] (numCells, in: initial guess, out: solution) is
  * solved with BiCGStab-DILU (method FC_METHOD_BICGSTAB, the reference's L1
  * rule with sor/nsw) or its FP32-in-FP64 form (FC_METHOD_BICGSTAB_MIXED).
- * ap may be NULL; info (3 entries) may be NULL.  Stops at the first
- * component that does not converge and returns its status. */
+ * ap may be NULL; info (3 entries) may be NULL.  All three components are
+ * solved whatever each solve's outcome, as calcuvw does; the call returns
+ * FC_NOT_CONVERGED if any component ran out of nsw, and stops only on an
+ * error (status < 0). */
 fc_status fc_sys_solve_momentum(fc_system *sys, const double *const *sp, const double *const *src, const double *urf,
                                 double *const *phi, double *const *ap, int method, double sor, int nsw,
                                 fc_solve_info *info);
@@ -395,10 +416,7 @@ enum {
     FC_ARR_A = 1, FC_ARR_SU = 2, FC_ARR_IOFFSET = 3, FC_ARR_JA = 4, FC_ARR_DIAG = 5,
     FC_ARR_IJ_SLOT = 6, FC_ARR_JI_SLOT = 7, FC_ARR_VOL = 8, FC_ARR_XC = 9, FC_ARR_YC = 10,
     FC_ARR_ZC = 11, FC_ARR_FACINT = 12, FC_ARR_ARX = 13, FC_ARR_ARY = 14, FC_ARR_ARZ = 15,
-    FC_ARR_XF = 16, FC_ARR_YF = 17, FC_ARR_ZF = 18, FC_ARR_RES = 19, FC_ARR_PIVOTS = 20,
-    /* the network's cell order (new -> old, 1-based) once an attached network
-     * has run with its rows staged in LDS; no counterpart in the reference */
-    FC_ARR_GNN_PERM = 21
+    FC_ARR_XF = 16, FC_ARR_YF = 17, FC_ARR_ZF = 18, FC_ARR_RES = 19, FC_ARR_PIVOTS = 20
 };
 fc_status fc_sys_get(fc_system *sys, int which, void *host_out, size_t bytes);
 /* Device pointer of an array of the resident system (for callers that share

@@ -125,10 +125,11 @@ def test_dependent_sweeps_give_identical_solves(case, ctx, method):
     np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
-def test_dependent_wait_limit_fails_the_solve_cleanly(monkeypatch):
-    """The waits of the dependency-driven schedule are bounded.  With the poll limit set to 1 from
-    the host, waves give up and leave their kernels, and the solve reports FC_ERR_HIP instead of
-    hanging; a context with the built-in limit then solves the same system."""
+def test_dependent_wait_limit_fails_the_solve_cleanly():
+    """The waits of the dependency-driven schedule are bounded in wall-clock time.  With the limit
+    set to 1 microsecond from the host, waves give up and leave their kernels, and the solve
+    reports FC_ERR_HIP instead of hanging; the same context with the built-in limit then solves
+    the same system."""
     m = M.hex_box(16, perturb=0.2, seed=4)
     geo = O.mesh_geometry(m)
     su = _rhs(geo, 8, 3)
@@ -141,17 +142,13 @@ def test_dependent_wait_limit_fails_the_solve_cleanly(monkeypatch):
         finally:
             s.close()
 
-    monkeypatch.setenv("FCAP_SF_SPIN", "1")
     c = fcap.Context(0)
     try:
+        c.set_sweep_wait_limit(1)
         with pytest.raises(fcap.FcError) as e:
             solve(c)
-        assert "spin limit" in str(e.value)
-    finally:
-        c.close()
-    monkeypatch.delenv("FCAP_SF_SPIN")
-    c = fcap.Context(0)
-    try:
+        assert "time limit" in str(e.value)
+        c.set_sweep_wait_limit(0)
         rc, info = solve(c)
         assert rc == 0 and info["rel_l2"] < 1e-8
     finally:
@@ -255,6 +252,39 @@ def test_momentum_three_components_match_oracle(ctx):
     s.close()
 
 
+def test_momentum_capped_solves_all_three_components(ctx):
+    """With nsw = 2 (the reference's examples run nsw 20, sor 1e-2) no component converges: calcuvw
+    still solves u, v and w in turn (src/calcuvw.f90:441,498,556), so all three phi and 1/a_P are
+    written, each equal to the oracle's BiCGStab after the same 2 iterations, and the call reports
+    FC_NOT_CONVERGED."""
+    m = M.tet_box(5, grade_axis=1, grade_ratio=1.1, permute_seed=3)
+    geo = O.mesh_geometry(m)
+    csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    flux = O.face_flux_central(m, geo, u, v, w)
+    vis = np.full(m.n_cells, 1e-3)
+    n = m.n_cells
+    rng = np.random.default_rng(12)
+    sp = [rng.random(n) * geo.vol * 10 for _ in range(3)]
+    src = [W.blob_rhs(geo.xc, geo.yc, geo.zc, geo.vol, 4, seed=30 + c) for c in range(3)]
+    phi0 = [0.1 * rng.standard_normal(n) for _ in range(3)]
+    urf = [0.7, 0.6, 0.8]
+    s = fcap.System(m, ctx)
+    try:
+        s.assemble_convdiff(vis, flux, 0.7, None)
+        rc, phi, ap, infos = s.solve_momentum(sp, src, urf, phi0, fcap.METHOD_BICGSTAB, 1e-30, 2)
+        assert rc == fcap.FC_NOT_CONVERGED, infos
+        a_base = O.convdiff(m, geo, csr, vis, flux, 0.7)
+        for c in range(3):
+            a_c, su_c, ap_c = O.momentum_rows(csr, a_base, sp[c], src[c], phi0[c], urf[c])
+            np.testing.assert_array_equal(ap[c], ap_c, err_msg=f"ap component {c}")
+            ref = O.bicgstab(csr, a_c, su_c, phi0[c], sor=1e-30, nsw=2)
+            assert infos[c]["iters"] == ref.iters == 2, (c, infos[c]["iters"], ref.iters)
+            assert np.linalg.norm(phi[c] - ref.fi) <= 1e-12 * np.linalg.norm(ref.fi), c
+    finally:
+        s.close()
+
+
 # ------------------------------------------------- pressure correction on the resident system
 
 def _calcp_fields(m, geo, seed):
@@ -299,11 +329,14 @@ def test_calcp_resident_then_iccg_matches_oracle(case):
     name, m, geo, csr, s = case
     f = _calcp_fields(m, geo, seed=7)
     ref = O.calcp_assemble(m, geo, csr, f)
+    # a stale solution vector (e.g. the last momentum component's): the assembly resets pp to
+    # zero as calcp does before its solve (calcp:112), so no set_solution is needed
+    s.set_solution(np.random.default_rng(11).standard_normal(m.n_cells))
     fl = s.calcp_assemble(*[f[k] for k in _CP_ORDER])
     np.testing.assert_array_equal(fl, ref["flmass"])
     np.testing.assert_array_equal(s.get("a"), ref["a"])
     np.testing.assert_array_equal(s.get("su"), ref["su"])
-    s.set_solution(np.zeros(m.n_cells))
+    assert not s.solution().any()
     if not np.all(np.isfinite(O.dic_pivots(csr, ref["a"]))):
         with pytest.raises(fcap.FcError) as e:
             s.solve(fcap.METHOD_ICCG, fcap.STOP_L1, 1e-8, 2000)
@@ -328,6 +361,7 @@ def test_calcp_resident_then_iccg_matches_oracle(case):
     su_o, fl_o, _, _ = O.calcp_nonorth(m, geo, f["den"], f["apu"], dpdx, got[0])
     np.testing.assert_array_equal(fl2, fl_o)
     np.testing.assert_array_equal(s.get("su"), su_o)
+    assert not s.solution().any()          # the next corrector pass starts from pp = 0
 
 
 def test_calcp_resident_with_device_fields(ctx):

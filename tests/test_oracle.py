@@ -294,6 +294,31 @@ def test_face_flux_and_divergence():
     assert np.abs(su1[interior]).max() < 1e-15
 
 
+def test_face_flux_and_divergence_of_a_linear_field_on_a_graded_box():
+    """u = (x, y, z) on an x-graded orthogonal box (no perturbation): the facint-weighted face
+    value is exact for a linear field whose face centre lies on the P-N line, so the flux of every
+    inner face is x_f S_x + y_f S_y + z_f S_z, and the discrete divergence of a cell without
+    boundary faces is div u * Vol = 3 Vol, entered with calcp's sign (su_P -= F, su_N += F,
+    src/calcp-multiple_correction_SIMPLE.f90:74-75): su = -3 Vol.  Interchanged interpolation
+    weights (facint is 0.38-0.5 here, not 1/2) or a flipped flux sign fail both checks."""
+    m = M.hex_box(9, 5, 4, grade_axis=0, grade_ratio=1.35, grade_max_aspect=20.0)
+    geo = O.mesh_geometry(m)
+    nif = m.n_inner
+    assert geo.facint.min() < 0.45                     # grading makes the weights unequal
+    flux = O.face_flux_central(m, geo, geo.xc, geo.yc, geo.zc)
+    exact = geo.xf[:nif] * geo.arx[:nif] + geo.yf[:nif] * geo.ary[:nif] + geo.zf[:nif] * geo.arz[:nif]
+    np.testing.assert_allclose(flux, exact, rtol=1e-12, atol=1e-15)
+    # the interchanged weights miss by far more than rounding
+    o, nb = m.owner[:nif], m.neighbour
+    fx = geo.facint
+    swapped = (fx * geo.xc[o] + (1 - fx) * geo.xc[nb]) * geo.arx[:nif]
+    assert np.abs(swapped - exact).max() > 1e-4
+    su = O.rhs_divergence(m, flux, np.zeros(m.n_cells))
+    interior = np.setdiff1d(np.arange(m.n_cells), np.unique(m.owner[nif:]))
+    assert interior.size > 0
+    np.testing.assert_allclose(su[interior], -3.0 * geo.vol[interior], rtol=1e-11)
+
+
 # ------------------------------------------------------- pressure correction
 
 def _calcp_inputs(g):
