@@ -284,6 +284,12 @@ extern "C" fc_status fc_set_sweep_wait_limit(fc_context *ctx, int microseconds) 
     return FC_OK;
 }
 
+extern "C" fc_status fc_set_reduction_order(fc_context *ctx, int order) {
+    if (!ctx || (order != FC_REDUCE_TREE && order != FC_REDUCE_SEQUENTIAL)) return FC_ERR_ARGUMENT;
+    ctx->reduce_seq = order == FC_REDUCE_SEQUENTIAL;
+    return FC_OK;
+}
+
 extern "C" fc_status fc_set_graphs(fc_context *ctx, int on) {
     if (!ctx) return FC_ERR_ARGUMENT;
     ctx->use_graphs = on ? 1 : 0;

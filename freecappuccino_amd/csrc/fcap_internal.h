@@ -138,6 +138,9 @@ struct fc_context {
     int num_cus = 256;
     // replay solver iterations as HIP graphs (fc_set_graphs; off: launch kernels directly)
     int use_graphs = 1;
+    // sums of the FP64 solvers in index order, as the reference forms them
+    // (fc_set_reduction_order FC_REDUCE_SEQUENTIAL), instead of a fixed tree
+    int reduce_seq = 0;
     // triangular sweeps: FC_SWEEP_LEVELS or FC_SWEEP_DEPENDENT (fc_set_sweep_mode)
     int sweep_mode = 2;
     // wall-clock limit of one wait of the dependency-driven sweeps, microseconds

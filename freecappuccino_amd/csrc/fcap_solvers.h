@@ -47,6 +47,7 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
 fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info *info);
 // FP32 inner solves (DIC-PCG or DILU-BiCGStab) inside FP64 iterative refinement.
 fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicgstab, fc_solve_info *info);
+
 void cast_d2f(hipStream_t s, long long n, const double *x, float *y);
 // Truncated flexible CG (each direction A-orthogonalised against the previous
 // kFcgM) in FP32 with preconditioner `P` inside FP64 iterative refinement; on

@@ -401,7 +401,7 @@ __global__ void k_scale_cast(int n, const double *__restrict__ r, float *__restr
 struct Ctx {
     fc_context *ctx;
     hipStream_t s;
-    Workspace &ws;
+    Workspace *ws;   // the context's workspace (null for the allocation-free solve)
     int n;
     Scalars *sc;
     double *pa, *pb;
@@ -447,12 +447,57 @@ void fin(Ctx &c, int step, int nparts, bool two) {
     k_fin<<<1, kFinBlock, 0, c.s>>>(step, nparts, c.pa, two ? c.pb : nullptr, c.sc);
 }
 
+// Sums in index order, as the reference forms them (sum(res*zk), sum(abs(res)):
+// one running sum from the first entry to the last, src/iccg.f90:65,114,143,154).
+// One wave loads 64 terms at a time and adds them to the running sum one by one
+// through lane broadcasts, so the sum has the reference's rounding; it goes to
+// pa[0] (and pb[0]) for a finaliser that reads one partial.
+//   OP 0: a = sum x_i y_i;  OP 1: a = sum |x_i|, b = sum x_i x_i;  OP 2: a = sum x_i y_i, b = sum y_i y_i
+template <int OP>
+__global__ __launch_bounds__(64) void k_seq_sum(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                double *__restrict__ pa, double *__restrict__ pb,
+                                                const Scalars *__restrict__ sc) {
+    if (sc && sc->done) return;
+    const int lane = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        double ta = 0.0, tb = 0.0;
+        if (i < n) {
+            const double xv = x[i];
+            const double yv = OP == 1 ? 0.0 : y[i];
+            ta = OP == 1 ? fabs(xv) : xv * yv;
+            tb = OP == 1 ? xv * xv : yv * yv;
+        }
+        const int cnt = min(64, n - base);
+        for (int l = 0; l < cnt; ++l) {
+            a = a + __shfl(ta, l, 64);
+            if (OP != 0) b = b + __shfl(tb, l, 64);
+        }
+    }
+    if (lane == 0) {
+        pa[0] = a;
+        if (OP != 0) pb[0] = b;
+    }
+}
+
+// The partial count for the finaliser of a reduction whose per-block partials
+// the kernels just wrote: with sequential reductions (fc_set_reduction_order)
+// the same sums are formed again in index order into one partial.
+int seq_or(Ctx &c, int nparts, int op, const double *x, const double *y) {
+    if (!c.ctx->reduce_seq) return nparts;
+    if (op == 0) k_seq_sum<0><<<1, 64, 0, c.s>>>(c.n, x, y, c.pa, c.pb, c.sc);
+    else if (op == 1) k_seq_sum<1><<<1, 64, 0, c.s>>>(c.n, x, y, c.pa, c.pb, c.sc);
+    else k_seq_sum<2><<<1, 64, 0, c.s>>>(c.n, x, y, c.pa, c.pb, c.sc);
+    return 1;
+}
+
 // b-norm^2 into sc->bnorm2 (host-side read; used by L2 stopping)
 double norm2_host(Ctx &c, const double *v) {
     dot_launch64(c.s, c.n, v, v, c.pa);
-    sum_parts_launch(c.s, red_grid(c.n), c.pa, nullptr, c.ws.sums.as<double>());
+    sum_parts_launch(c.s, red_grid(c.n), c.pa, nullptr, c.ws->sums.as<double>());
     double h[2];
-    FC_HIP_CHECK(hipMemcpyAsync(h, c.ws.sums.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipMemcpyAsync(h, c.ws->sums.get(), sizeof(double) * 2, hipMemcpyDeviceToHost, c.s));
     FC_HIP_CHECK(hipStreamSynchronize(c.s));
     return h[0];
 }
@@ -518,15 +563,15 @@ void Workspace::ensure(int n_, int nnz_) {
 
 static Ctx make_ctx(fc_context *ctx, int n) {
     Workspace &ws = *ctx->ws;
-    Ctx c{ctx, ctx->stream, ws, n, ws.sc.as<Scalars>(), ws.pa.as<double>(), ws.pb.as<double>(), red_grid(n), {}};
+    Ctx c{ctx, ctx->stream, &ws, n, ws.sc.as<Scalars>(), ws.pa.as<double>(), ws.pb.as<double>(), red_grid(n), {}};
     return c;
 }
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
     if (c.ctx->sweep_mode == FC_SWEEP_DEPENDENT && syncfree_sweeps(st)) {
-        syncfree_reset(c.ctx, c.s, c.ws.sf.as<SyncFree>());
-        c.sw.sf = c.ws.sf.as<SyncFree>();
-        c.sw.tmp = c.ws.sf_tmp.get();
+        syncfree_reset(c.ctx, c.s, c.ws->sf.as<SyncFree>());
+        c.sw.sf = c.ws->sf.as<SyncFree>();
+        c.sw.tmp = c.ws->sf_tmp.get();
         c.sw.waves = c.ctx->num_cus;
     }
 }
@@ -545,7 +590,7 @@ template <class T>
 static PackedFactor<T> pack_for_solve(Ctx &c, const Structure &st, const T *a, const T *d) {
     PackedFactor<T> pf;
     if (!packed_sweeps(st)) return pf;
-    Workspace &ws = c.ws;
+    Workspace &ws = *c.ws;
     constexpr bool f32 = sizeof(T) == sizeof(float);
     DevBuf &af = f32 ? ws.pk_af32 : ws.pk_af, &df = f32 ? ws.pk_df32 : ws.pk_df;
     DevBuf &ab = f32 ? ws.pk_ab32 : ws.pk_ab, &db = f32 ? ws.pk_db32 : ws.pk_db;
@@ -567,7 +612,7 @@ static PackedFactor<T> pack_for_solve(Ctx &c, const Structure &st, const T *a, c
 
 static void check_pivots(Ctx &c) {
     int h = 0;
-    FC_HIP_CHECK(hipMemcpyAsync(&h, c.ws.err.get(), sizeof(int), hipMemcpyDeviceToHost, c.s));
+    FC_HIP_CHECK(hipMemcpyAsync(&h, c.ws->err.get(), sizeof(int), hipMemcpyDeviceToHost, c.s));
     FC_HIP_CHECK(hipStreamSynchronize(c.s));
     if (h) throw Error(FC_ERR_ZERO_PIVOT, "incomplete factorisation: zero or non-finite pivot");
 }
@@ -602,13 +647,13 @@ static void initial_residual(Ctx &c, const SolveArgs &args, double *r) {
     if (!args.oc.noc) {
         spmv_launch64(c.s, 1, c.n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
                       nullptr, c.pa, c.pb, nullptr);
-        fin(c, kFinRes0, spmv_grid(c.n), true);
+        fin(c, kFinRes0, seq_or(c, spmv_grid(c.n), 1, r, nullptr), true);
     } else {
         spmv_launch64(c.s, 1, c.n, st.ioffset.as<int>(), st.ja.as<int>(), args.a, args.fi, args.su, nullptr, r,
                       nullptr, c.pa, c.pb, nullptr);
         oc_launch64(c.s, c.n, args.oc, args.fi, r, -1, nullptr);
         norms_launch64(c.s, c.n, r, c.pa, c.pb);
-        fin(c, kFinRes0, red_grid(c.n), true);
+        fin(c, kFinRes0, seq_or(c, red_grid(c.n), 1, r, nullptr), true);
     }
 }
 
@@ -649,19 +694,19 @@ fc_status solve_pcg64(fc_context *ctx, const SolveArgs &args, bool dic, fc_solve
         } else {
             k_jacobi<double, double><<<c.gv, kRedBlock, 0, c.s>>>(n, r, args.a, dg, z, c.sc, c.pa);
         }
-        fin(c, kFinSk, c.gv, false);
+        fin(c, kFinSk, seq_or(c, c.gv, 0, r, z), false);
         k_p_update<double><<<c.gv, 256, 0, c.s>>>(n, p, z, c.sc);
         if (!args.oc.noc) {
             spmv_launch64(c.s, 2, n, ptr, ja, args.a, p, nullptr, p, q, nullptr, c.pa, nullptr, c.sc);
-            fin(c, kFinAlf, gs, false);
+            fin(c, kFinAlf, seq_or(c, gs, 0, p, q), false);
         } else {
             spmv_launch64(c.s, 0, n, ptr, ja, args.a, p, nullptr, nullptr, q, nullptr, c.pa, nullptr, c.sc);
             oc_launch64(c.s, n, args.oc, p, q, +1, c.sc);
             k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, p, q, c.sc, c.pa);
-            fin(c, kFinAlf, c.gv, false);
+            fin(c, kFinAlf, seq_or(c, c.gv, 0, p, q), false);
         }
         k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, r, p, q, c.sc, c.pa, c.pb);
-        fin(c, kFinPcgEnd, c.gv, true);
+        fin(c, kFinPcgEnd, seq_or(c, c.gv, 1, r, nullptr), true);
     };
     ctx->timer.start(c.s);
     run_batches(c, dic ? 1 : 8, iter);
@@ -705,17 +750,17 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
     const int gs = spmv_grid(n);
     auto iter = [&]() {
         k_dot<double><<<c.gv, kRedBlock, 0, c.s>>>(n, res, reso, c.sc, c.pa);
-        fin(c, kFinBet, c.gv, false);
+        fin(c, kFinBet, seq_or(c, c.gv, 0, res, reso), false);
         k_bicg_p<double><<<c.gv, 256, 0, c.s>>>(n, pk, res, uk, c.sc);
         lu_launch64(c.s, st, args.a, d, small, pk, zk, c.sc, &c.sw, ppf);
         spmv_launch64(c.s, 2, n, ptr, ja, args.a, zk, nullptr, reso, uk, nullptr, c.pa, nullptr, c.sc);
-        fin(c, kFinGam, gs, false);
+        fin(c, kFinGam, seq_or(c, gs, 0, uk, reso), false);
         k_bicg_xr1<double, double><<<c.gv, 256, 0, c.s>>>(n, args.fi, res, zk, uk, c.sc);
         lu_launch64(c.s, st, args.a, d, small, res, zk, c.sc, &c.sw, ppf);
         spmv_launch64(c.s, 3, n, ptr, ja, args.a, zk, nullptr, res, vk, nullptr, c.pa, c.pb, c.sc);
-        fin(c, kFinAlfB, gs, true);
+        fin(c, kFinAlfB, seq_or(c, gs, 2, res, vk), true);
         k_pcg_xr64<<<c.gv, kRedBlock, 0, c.s>>>(n, args.fi, res, zk, vk, c.sc, c.pa, c.pb);
-        fin(c, kFinBicgEnd, c.gv, true);
+        fin(c, kFinBicgEnd, seq_or(c, c.gv, 1, res, nullptr), true);
     };
     ctx->timer.start(c.s);
     run_batches(c, 1, iter);
@@ -729,7 +774,7 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 // FP32 inner solve of A e = r32 to ||r_inner||_2 <= eta ||r32||_2, e32 out.
 static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, double r32norm2, int cap,
                          const PackedFactor<float> *pf) {
-    Workspace &ws = c.ws;
+    Workspace &ws = *c.ws;
     const int n = c.n;
     float *a32 = ws.a32.as<float>(), *d32 = ws.d32.as<float>();
     float *r = ws.r32.as<float>(), *z = ws.z32.as<float>(), *p = ws.p32.as<float>(), *q = ws.q32.as<float>();
@@ -863,7 +908,7 @@ static int fcg_window(const fc_context *ctx) {
 // Returns the inner status (FC_OK, FC_NOT_CONVERGED, FC_FALLBACK_USED on
 // stagnation, FC_ERR_NONFINITE) and adds the iterations to `iters`.
 static int inner_fcg32(Ctx &c, const Structure &st, const PrecondHook &P, double eta, int cap, int &iters) {
-    Workspace &ws = c.ws;
+    Workspace &ws = *c.ws;
     const int n = c.n;
     float *a32 = ws.a32.as<float>();
     float *r = ws.r32.as<float>(), *z = ws.z32.as<float>(), *p = ws.p32.as<float>(), *q = ws.q32.as<float>();
@@ -1000,5 +1045,6 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
     }
     return status;
 }
+
 
 }  // namespace fcap

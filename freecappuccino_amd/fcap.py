@@ -38,6 +38,7 @@ METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
 METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
 SWEEP_LEVELS, SWEEP_DEPENDENT = 0, 2
+REDUCE_TREE, REDUCE_SEQUENTIAL = 0, 1
 SWEEP_DEFAULT = SWEEP_DEPENDENT
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20, gnn_perm=21)
@@ -80,6 +81,7 @@ def library() -> C.CDLL:
         "fc_set_sweep_mode": ([vp, i], i), "fc_set_fcg_smoothing": ([vp, i, d], i),
         "fc_set_fcg_stagnation": ([vp, i, d], i),
         "fc_set_sweep_wait_limit": ([vp, i], i), "fc_set_graphs": ([vp, i], i),
+        "fc_set_reduction_order": ([vp, i], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,
@@ -226,6 +228,11 @@ class Context:
     def set_sweep_wait_limit(self, microseconds: int = 0):
         """Wall-clock limit of one wait of the dependency-driven sweeps (0: the built-in 2 s)."""
         self.check(self.L.fc_set_sweep_wait_limit(self.h, int(microseconds)))
+
+    def set_reduction_order(self, order: int = 0):
+        """REDUCE_TREE (default) or REDUCE_SEQUENTIAL: the FP64 solvers' sums in index order, which
+        makes their iterates bit-identical to the reference's serial solvers (verification mode)."""
+        self.check(self.L.fc_set_reduction_order(self.h, int(order)))
 
     def set_graphs(self, on: bool = True):
         """Replay solver iterations as HIP graphs (default) or launch kernels one by one."""

@@ -266,6 +266,21 @@ fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *ioffset, const
 enum { FC_VARIANT_SRC = 0, FC_VARIANT_TESTS = 1 };
 fc_status fc_set_variant(fc_context *ctx, int variant);
 
+/* Order of the sums of the FP64 solvers (iccg, dpcg, bicgstab: fc_iccg,
+ * fc_dpcg, fc_bicgstab, fc_sys_solve with FC_METHOD_ICCG / _DPCG / _BICGSTAB,
+ * fc_sys_solve_momentum with FC_METHOD_BICGSTAB).
+ *   FC_REDUCE_TREE        per-block partials added in a fixed tree (default;
+ *                         deterministic, full speed).
+ *   FC_REDUCE_SEQUENTIAL  every dot product and norm formed in index order, as
+ *                         the reference's sum(...) forms it: the iterates,
+ *                         residuals, L1 history and iteration count are then
+ *                         bit-identical to the reference's serial solvers.  One
+ *                         wave adds the terms one by one (about 10 ns per entry
+ *                         and sum): a verification mode, not a fast path.
+ * No counterpart in the reference. */
+enum { FC_REDUCE_TREE = 0, FC_REDUCE_SEQUENTIAL = 1 };
+fc_status fc_set_reduction_order(fc_context *ctx, int order);
+
 /* Tuning of the FP32-in-FP64 solvers; no counterpart in the reference.
  * fc_set_inner_tolerance: relative residual reduction of each inner FP32 solve
  * (0 = the method's default: 1e-2 for FC_METHOD_FCG_LEARNED, 1e-3 otherwise).

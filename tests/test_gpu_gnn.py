@@ -329,9 +329,9 @@ def test_nonfinite_network_falls_back_to_ic0_pcg(ctx):
 
 
 def test_stagnation_window_falls_back_to_ic0_pcg(ctx, trained):
-    """Finite (trained) weights with the window rule of the inner solves made to fail (window 8,
+    """Finite (trained) weights with the window rule of the inner solves made to fail (window 4,
     factor 0: no window after the first can beat 0 x the best before it): the first inner solve
-    stops after 16 iterations and plain FP64 IC(0)-PCG finishes the solve from the refined
+    stops after 8 iterations and plain FP64 IC(0)-PCG finishes the solve from the refined
     iterate, with the caller's stopping rule (L2 relative to ||su||, or the reference's L1 rule
     relative to the first ||r0||_1) and what is left of nsw."""
     net, _ = trained
@@ -339,22 +339,22 @@ def test_stagnation_window_falls_back_to_ic0_pcg(ctx, trained):
     s = _system(m, ctx, seed=8)
     n_net = gnn.Network(net.params, ctx)
     n_net.attach(s)
-    ctx.set_fcg_stagnation(8, 0.0)
-    ctx.set_inner_tolerance(1e-6)          # the first inner solve runs past 16 iterations
+    ctx.set_fcg_stagnation(4, 0.0)
+    ctx.set_inner_tolerance(1e-6)          # the first inner solve runs past 8 iterations
     try:
         for rule in (fcap.STOP_L2, fcap.STOP_L1):
             s.set_solution(np.zeros(s.n))
             rc, info = s.solve(fcap.METHOD_FCG_LEARNED, rule, 1e-8, 5000)
             assert rc == fcap.FC_FALLBACK_USED, (rule, info)
-            assert info["iters"] == 16 and info["fallback_iters"] > 0, info
+            assert info["iters"] == 8 and info["fallback_iters"] > 0, info
             if rule == fcap.STOP_L2:
                 assert info["rel_l2"] < 1e-8, info
             else:
                 assert info["rsm"] < 1e-8 and info["rel_l2"] < 1e-6, info
-        # the fallback gets what is left of nsw: 16 FCG iterations + at most 4 IC(0)-PCG ones
+        # the fallback gets what is left of nsw: 8 FCG iterations + at most 4 IC(0)-PCG ones
         s.set_solution(np.zeros(s.n))
-        rc, info = s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, 1e-8, 20)
-        assert rc == fcap.FC_NOT_CONVERGED and info["iters"] == 16 and 1 <= info["fallback_iters"] <= 4, info
+        rc, info = s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, 1e-8, 12)
+        assert rc == fcap.FC_NOT_CONVERGED and info["iters"] == 8 and 1 <= info["fallback_iters"] <= 4, info
         # and with the test switched off the same weights converge without it
         ctx.set_fcg_stagnation(0, 0.81)
         s.set_solution(np.zeros(s.n))
