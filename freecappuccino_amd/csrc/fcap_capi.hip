@@ -193,6 +193,7 @@ extern "C" fc_status fc_init(fc_context **out, int device) {
         int khz = 0;
         FC_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
         if (khz > 0) ctx->wall_clock_khz = khz;
+        noalloc_init(device);   // this device's address of the allocation-free scratch
         return FC_OK;
     });
     if (s != FC_OK) {
@@ -1550,4 +1551,18 @@ extern "C" double fc_copy_bench(fc_context *ctx, size_t bytes, int reps) {
         return FC_OK;
     });
     return s == FC_OK ? out : -1.0;
+}
+
+extern "C" fc_status fc_dpcg_noalloc(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                                     const int *diag, const double *a, const double *su, double *fi, double *res,
+                                     double sor, int nsw, fc_solve_info *info) {
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(ioffset && ja && diag && a && su && fi, FC_ERR_ARGUMENT, "arrays");
+        const void *dev[7] = {ioffset, ja, diag, a, su, fi, res ? (const void *)res : (const void *)fi};
+        for (const void *p : dev)
+            FC_REQUIRE(device_resident(p), FC_ERR_ARGUMENT,
+                       "fc_dpcg_noalloc takes device pointers on the context's device only");
+        FC_REQUIRE(nsw > 0 && sor > 0.0, FC_ERR_ARGUMENT, "sor/nsw");
+        return solve_dpcg_noalloc(ctx, n, nnz, ioffset, ja, diag, a, su, fi, res, sor, nsw, info);
+    });
 }

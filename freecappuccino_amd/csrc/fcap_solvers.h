@@ -48,6 +48,15 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 // FP32 inner solves (DIC-PCG or DILU-BiCGStab) inside FP64 iterative refinement.
 fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicgstab, fc_solve_info *info);
 
+// Allocation-free Jacobi-PCG (fc_dpcg_noalloc): the device addresses of its
+// file-scope scratch are taken once per device (noalloc_init, with that device
+// current); the solve then uses caller-owned device arrays and that scratch only.
+void noalloc_init(int device);
+size_t noalloc_bytes();
+int noalloc_max_cells();
+fc_status solve_dpcg_noalloc(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                             const double *a, const double *su, double *fi, double *res, double sor, int nsw,
+                             fc_solve_info *info);
 void cast_d2f(hipStream_t s, long long n, const double *x, float *y);
 // Truncated flexible CG (each direction A-orthogonalised against the previous
 // kFcgM) in FP32 with preconditioner `P` inside FP64 iterative refinement; on

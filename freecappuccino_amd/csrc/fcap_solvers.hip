@@ -10,9 +10,11 @@
 // iterations in batches and reads one flag per batch.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <mutex>
 
 #include "fcap_solvers.h"
 #include "fcap_gnn.h"
@@ -1046,5 +1048,119 @@ fc_status solve_fcg(fc_context *ctx, const SolveArgs &args, const PrecondHook &P
     return status;
 }
 
+// ====================================== allocation-free Jacobi-PCG (fc_dpcg_noalloc)
+//
+// dpcg (src/dpcg.f90) on caller-owned device arrays for callers that cannot
+// allocate device memory.  Every buffer beyond the caller's arrays -- the search
+// direction, the preconditioned residual, A p, the residual when the caller
+// passes none, the reduction partials and the solver scalars -- is carved from
+// one file-scope device array.  The 1-based CSR arrays are read in place: the
+// kernels index a, ja and x with 1-based positions and columns through pointers
+// shifted back by one entry, while ioffset and diag are indexed by 0-based row
+// and hold 1-based positions.  The kernels and sums are those of solve_pcg64,
+// so the result is bitwise that of fc_dpcg on the same inputs.
+
+constexpr int kNoAllocCells = 1 << 20;
+constexpr size_t kNoAllocVec = sizeof(double) * (size_t)kNoAllocCells;
+constexpr int kNoAllocParts = kNoAllocCells / kSpmvRows > kMaxRedGrid ? kNoAllocCells / kSpmvRows : kMaxRedGrid;
+constexpr size_t kNoAllocPartBytes = sizeof(double) * (size_t)kNoAllocParts;
+constexpr size_t kNoAllocTail = 1024;   // Scalars, then three sums
+static_assert(sizeof(Scalars) + 3 * sizeof(double) <= kNoAllocTail, "scratch tail");
+constexpr size_t kNoAllocBytes = 4 * kNoAllocVec + 2 * kNoAllocPartBytes + kNoAllocTail;
+
+// Each GPU holds its own copy of this array until the process ends.  Host code
+// never uses its name as an address: it uses the per-device pointer that
+// hipGetSymbolAddress returns (g_noalloc).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+static void *g_noalloc[kNoAllocDevices];
+static std::mutex g_noalloc_mu;
+
+size_t noalloc_bytes() { return kNoAllocBytes; }
+int noalloc_max_cells() { return kNoAllocCells; }
+
+void noalloc_init(int device) {
+    FC_REQUIRE(device >= 0 && device < kNoAllocDevices, FC_ERR_ARGUMENT, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc[device]) return;
+    FC_HIP_CHECK(hipSetDevice(device));
+    void *ptr = nullptr;
+    FC_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    FC_REQUIRE(ptr != nullptr, FC_ERR_HIP, "no device address for the allocation-free scratch");
+    g_noalloc[device] = ptr;
+}
+
+fc_status solve_dpcg_noalloc(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja, const int *diag,
+                             const double *a, const double *su, double *fi, double *res, double sor, int nsw,
+                             fc_solve_info *info) {
+    FC_REQUIRE(n >= 1 && n <= kNoAllocCells, FC_ERR_ARGUMENT,
+               "fc_dpcg_noalloc: numCells must be 1..1048576 (the size of its scratch)");
+    FC_REQUIRE(nnz >= n, FC_ERR_ARGUMENT, "nnz < numCells");
+    void *base = ctx->device >= 0 && ctx->device < kNoAllocDevices ? g_noalloc[ctx->device] : nullptr;
+    FC_REQUIRE(base != nullptr, FC_ERR_HIP, "allocation-free scratch not set up on this device");
+    const hipStream_t s = ctx->stream;
+    // the CSR bounds, read with two 4-byte copies into host variables
+    int first = 0, last = 0;
+    FC_HIP_CHECK(hipMemcpyAsync(&first, ioffset, sizeof(int), hipMemcpyDeviceToHost, s));
+    FC_HIP_CHECK(hipMemcpyAsync(&last, ioffset + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+    FC_REQUIRE(first == 1 && last == nnz + 1, FC_ERR_STRUCTURE, "ioffset is not 1-based with ioffset(n+1) = nnz+1");
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned char *b = static_cast<unsigned char *>(base);
+    double *r = res ? res : reinterpret_cast<double *>(b);
+    double *z = reinterpret_cast<double *>(b + kNoAllocVec);
+    double *p = reinterpret_cast<double *>(b + 2 * kNoAllocVec);
+    double *q = reinterpret_cast<double *>(b + 3 * kNoAllocVec);
+    double *pa = reinterpret_cast<double *>(b + 4 * kNoAllocVec);
+    double *pb = reinterpret_cast<double *>(b + 4 * kNoAllocVec + kNoAllocPartBytes);
+    Scalars *sc = reinterpret_cast<Scalars *>(b + 4 * kNoAllocVec + 2 * kNoAllocPartBytes);
+    double *sums = reinterpret_cast<double *>(b + 4 * kNoAllocVec + 2 * kNoAllocPartBytes + sizeof(Scalars));
+    const double *a1 = a - 1;    // a1[k], k a 1-based position
+    const int *ja1 = ja - 1;     // ja1[k] a 1-based column
+    Ctx c{ctx, s, nullptr, n, sc, pa, pb, red_grid(n), {}};
+    SolveArgs args;
+    args.sor = sor;
+    args.nsw = nsw;
+    args.variant = ctx->variant;
+    args.stop_rule = FC_STOP_L1;
+    const double small = args.variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+    const Scalars h = init_scalars(args, small);
+    upload_scalars(c, h);
+    k_zero<double><<<c.gv, 256, 0, s>>>(n, p);
+    spmv_launch64(s, 1, n, ioffset, ja1, a1, fi - 1, su, nullptr, r, nullptr, pa, pb, nullptr);
+    fin(c, kFinRes0, seq_or(c, spmv_grid(n), 1, r, nullptr), true);
+    const int gs = spmv_grid(n);
+    if (!read_done(c)) {
+        for (;;) {
+            for (int k = 0; k < 8; ++k) {
+                k_jacobi<double, double><<<c.gv, kRedBlock, 0, s>>>(n, r, a1, diag, z, sc, pa);
+                fin(c, kFinSk, seq_or(c, c.gv, 0, r, z), false);
+                k_p_update<double><<<c.gv, 256, 0, s>>>(n, p, z, sc);
+                spmv_launch64(s, 2, n, ioffset, ja1, a1, p - 1, nullptr, p, q, nullptr, pa, nullptr, sc);
+                fin(c, kFinAlf, seq_or(c, gs, 0, p, q), false);
+                k_pcg_xr64<<<c.gv, kRedBlock, 0, s>>>(n, fi, r, p, q, sc, pa, pb);
+                fin(c, kFinPcgEnd, seq_or(c, c.gv, 1, r, nullptr), true);
+            }
+            FC_HIP_CHECK(hipGetLastError());
+            if (read_done(c)) break;
+        }
+    }
+    const Scalars hs = read_scalars(c);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    fill_info(info, hs, 0.0, ms);
+    if (info) {
+        // ||su - A fi||_2 / ||su||_2 from scratch: the true residual into z (q's partials)
+        spmv_launch64(s, 1, n, ioffset, ja1, a1, fi - 1, su, nullptr, z, nullptr, pa, pb, nullptr);
+        sum_parts_launch(s, gs, pb, nullptr, sums);
+        norms_launch64(s, n, su, pa, pb);
+        sum_parts_launch(s, red_grid(n), pb, nullptr, sums + 1);
+        double hsum[3] = {0.0, 0.0, 0.0};
+        FC_HIP_CHECK(hipMemcpyAsync(hsum, sums, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+        FC_HIP_CHECK(hipStreamSynchronize(s));
+        info->rel_l2 = hsum[1] > 0.0 ? std::sqrt(hsum[0] / hsum[1]) : std::sqrt(hsum[0]);
+    }
+    return (fc_status)hs.status;
+}
 
 }  // namespace fcap

@@ -82,6 +82,7 @@ def library() -> C.CDLL:
         "fc_set_fcg_stagnation": ([vp, i, d], i),
         "fc_set_sweep_wait_limit": ([vp, i], i), "fc_set_graphs": ([vp, i], i),
         "fc_set_reduction_order": ([vp, i], i),
+        "fc_dpcg_noalloc": ([vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, i, si], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
         "fc_laplacian": ([vp, i, i, ip, ip] + [dp] * 12 + [i, i, i, i, i, i, ip, ip, dp, ip, ip, ip, ip, ip, i,

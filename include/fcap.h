@@ -259,6 +259,21 @@ fc_status fc_bicgstab(fc_context *ctx, int n, int nnz, const int *ioffset, const
                       int noc, const int *ijl, const int *ijr, const double *al, const double *ar,
                       double sor, int nsw, fc_solve_info *info);
 
+/* dpcg with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  As fc_dpcg, but every
+ * array is a device pointer on the context's device (res may be NULL), and
+ * there are no O-C couplings.  numCells is at most 1,048,576 (larger:
+ * FC_ERR_ARGUMENT); nnz is not limited, the CSR arrays are read in place.
+ * Every other buffer lives in one static device array of about 32 MiB that
+ * each GPU the library runs on holds until the process ends; calls that
+ * overlap on one GPU share it and race.  Its kernels and sums are those of
+ * fc_dpcg, so the results are bitwise equal.  info->solve_ms is host wall
+ * time, setup_ms 0. */
+fc_status fc_dpcg_noalloc(fc_context *ctx, int n, int nnz, const int *ioffset, const int *ja,
+                          const int *diag, const double *a, const double *su, double *fi, double *res,
+                          double sor, int nsw, fc_solve_info *info);
+
 /* Solver variants.  FC_VARIANT_SRC: the module solvers of src/ (small = 1e-20,
  * early exit below 1e-13, O-C terms).  FC_VARIANT_TESTS: the explicit-argument
  * copies of tests/test_sparse_solvers.f90 (small = 1e-30, sor = 1e-13,
