@@ -270,6 +270,26 @@ __device__ inline void flux_mass(int p, int q, double xf, double yf, double zf, 
     fluxmass = dene * ((ue * arx + ve * ary) + we * arz);
 }
 
+// facefluxmass_piso (src/facefluxmass.f90:294-358): cap (= can) =
+// -rho_f (fxp Vol_P apu_P + fxn Vol_N apu_N) |S| / |d_PN| and the mass flux rho_f (u_f . S)
+// without the Rhie-Chow pressure terms, in the reference's evaluation order
+__device__ inline void flux_mass_piso(int p, int q, double xf, double yf, double zf, double arx, double ary,
+                                      double arz, double lambda, const RhieChowDev &F, double &cap,
+                                      double &fluxmass) {
+    const double fxn = lambda;
+    const double fxp = 1.0 - lambda;
+    const double xpn = F.xc[q] - F.xc[p], ypn = F.yc[q] - F.yc[p], zpn = F.zc[q] - F.zc[p];
+    const double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+    const double are = sqrt((arx * arx + ary * ary) + arz * arz);
+    const double dene = F.den[p] * fxp + F.den[q] * fxn;
+    cap = -(((dene * ((fxp * F.vol[p]) * F.apu[p] + (fxn * F.vol[q]) * F.apu[q])) * are) / dpn);
+    const double ui = face_value_central(p, q, xf, yf, zf, F.u, F.dudx, F);
+    const double vi = face_value_central(p, q, xf, yf, zf, F.v, F.dvdx, F);
+    const double wi = face_value_central(p, q, xf, yf, zf, F.w, F.dwdx, F);
+    fluxmass = dene * ((ui * arx + vi * ary) + wi * arz);
+}
+
+template <bool PISO>
 __global__ void k_calcp_faces(int nif, const int *__restrict__ own, const int *__restrict__ nei,
                               const int *__restrict__ ij, const int *__restrict__ ji, const double *__restrict__ xf,
                               const double *__restrict__ yf, const double *__restrict__ zf,
@@ -279,12 +299,16 @@ __global__ void k_calcp_faces(int nif, const int *__restrict__ own, const int *_
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nif) return;
     double cap, fl;
-    flux_mass(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], F, cap, fl);
+    if (PISO)
+        flux_mass_piso(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], F, cap, fl);
+    else
+        flux_mass(own[f], nei[f], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], facint[f], F, cap, fl);
     a[ij[f]] = cap;   // can
     a[ji[f]] = cap;   // cap
     flmass[f] = fl;
 }
 
+template <bool PISO>
 __global__ void k_calcp_oc(int noc, int oc_face0, const int *__restrict__ ijl, const int *__restrict__ ijr,
                            const double *__restrict__ foc, const double *__restrict__ xf,
                            const double *__restrict__ yf, const double *__restrict__ zf,
@@ -295,7 +319,10 @@ __global__ void k_calcp_oc(int noc, int oc_face0, const int *__restrict__ ijl, c
     if (i >= noc) return;
     const int f = oc_face0 + i;
     double cap, fl;
-    flux_mass(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], F, cap, fl);
+    if (PISO)
+        flux_mass_piso(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], F, cap, fl);
+    else
+        flux_mass(ijl[i], ijr[i], xf[f], yf[f], zf[f], arx[f], ary[f], arz[f], foc[i], F, cap, fl);
     al[i] = cap;
     ar[i] = cap;
     fmoc[i] = fl;
@@ -522,6 +549,74 @@ __global__ void k_outlet_scale(int nout, int out_face0, int out_val0, const int 
     u[b] = u[c] * k;
     v[b] = v[c] * k;
     w[b] = w[c] * k;
+}
+
+// correctBoundaryConditionsVelocity's symmetry loop
+// (src/correctBoundaryConditionsVelocity.f90:63-75): the boundary velocity is the owner's
+// minus (u . S) S, with S the face's area vector as the reference writes it
+__global__ void k_symmetry_bc(int nsym, int sym_face0, int sym_val0, const int *__restrict__ owner,
+                              const double *__restrict__ arx, const double *__restrict__ ary,
+                              const double *__restrict__ arz, double *__restrict__ u, double *__restrict__ v,
+                              double *__restrict__ w) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsym) return;
+    const int f = sym_face0 + i, c = owner[f], b = sym_val0 + i;
+    const double un = (u[c] * arx[f] + v[c] * ary[f]) + w[c] * arz[f];
+    u[b] = u[c] - un * arx[f];
+    v[b] = v[c] - un * ary[f];
+    w[b] = w[c] - un * arz[f];
+}
+
+// bpres (src/bpres.f90:3-152) on p (cells, then boundary values in group order).  Item t
+// of the groups' sequence (inlet, outlet, symmetry, wall, pressure outlet), then the O-C
+// cuts.  Stage 1: every boundary value takes its owner cell's value.  Stage >= 2: the wall
+// and pressure-outlet values, and the O-C values, are extrapolated from the owner cell
+// with dpdx (3, numCells): ((p_P + g_x dx) + g_y dy) + g_z dz, d = x_f - x_P; the O-C
+// branch takes the face centre of face number i, the cut's own index (bpres.f90:140-142),
+// as the reference does.
+__global__ void k_bpres(GradGroups G, int noc, int oc_face0, int oc_val0, const int *__restrict__ owner,
+                        const double *__restrict__ xc, const double *__restrict__ yc, const double *__restrict__ zc,
+                        const double *__restrict__ xf, const double *__restrict__ yf, const double *__restrict__ zf,
+                        const double *__restrict__ dpdx, int stage, double *__restrict__ p) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nq = G.qstart[5];
+    if (t >= nq + noc) return;
+    int face, b, cf;
+    bool extrapolate;
+    if (t < nq) {
+        int g = 0;
+        while (t >= G.qstart[g + 1]) ++g;
+        const int i = t - G.qstart[g];
+        face = G.fstart[g] + i;
+        b = G.vstart[g] + i;
+        cf = face;
+        extrapolate = g >= 3;        // wall, pressure outlet
+    } else {
+        const int i = t - nq;
+        face = oc_face0 + i;
+        b = oc_val0 + i;
+        cf = i;
+        extrapolate = true;
+    }
+    const int c = owner[face];
+    if (stage <= 1) {
+        p[b] = p[c];
+        return;
+    }
+    if (!extrapolate) return;
+    const double xpb = xf[cf] - xc[c], ypb = yf[cf] - yc[c], zpb = zf[cf] - zc[c];
+    const double *g3 = dpdx + 3 * (size_t)c;
+    p[b] = ((p[c] + g3[0] * xpb) + g3[1] * ypb) + g3[2] * zpb;
+}
+
+// The reference pressure of PISO (src/PISO_multiple_correction.f90:188-192): row pref of a
+// zeroed, its diagonal 1, su(pref) = p(pref)
+__global__ void k_pin_reference(const int *__restrict__ ptr, const int *__restrict__ diagpos, int pref,
+                                const double *__restrict__ p, double *__restrict__ a, double *__restrict__ su) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = ptr[pref]; k < ptr[pref + 1]; ++k) a[k] = 0.0;
+    a[diagpos[pref]] = 1.0;
+    su[pref] = p[pref];
 }
 
 // su(i) -= fmi over the cell's inlet faces in order, then -= fmo over its outlet faces
@@ -949,15 +1044,26 @@ void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
                const int *ij_slot, const int *ji_slot, const int *kface, const int *kface_order, const double *xf,
                const double *yf, const double *zf, const double *arx, const double *ary, const double *arz,
                const double *facint, const RhieChowDev &F, const OcDev &oc, double *a, double *su, double *flmass,
-               double *fmoc) {
+               double *fmoc, bool piso) {
     hipStream_t s = ctx->stream;
-    if (nif)
-        k_calcp_faces<<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, xf, yf, zf, arx, ary,
-                                                       arz, facint, F, a, flmass);
-    if (oc.noc)
-        k_calcp_oc<<<grid_for(oc.noc), kBlock, 0, s>>>(oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(),
-                                                       oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx, ary,
-                                                       arz, F, oc.al.as<double>(), oc.ar.as<double>(), fmoc);
+    if (nif) {
+        if (piso)
+            k_calcp_faces<true><<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, xf, yf, zf,
+                                                                 arx, ary, arz, facint, F, a, flmass);
+        else
+            k_calcp_faces<false><<<grid_for(nif), kBlock, 0, s>>>(nif, owner0, neigh0, ij_slot, ji_slot, xf, yf, zf,
+                                                                  arx, ary, arz, facint, F, a, flmass);
+    }
+    if (oc.noc) {
+        if (piso)
+            k_calcp_oc<true><<<grid_for(oc.noc), kBlock, 0, s>>>(
+                oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(), oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx,
+                ary, arz, F, oc.al.as<double>(), oc.ar.as<double>(), fmoc);
+        else
+            k_calcp_oc<false><<<grid_for(oc.noc), kBlock, 0, s>>>(
+                oc.noc, oc.ioc_faces_start, oc.ijl.as<int>(), oc.ijr.as<int>(), oc.foc.as<double>(), xf, yf, zf, arx,
+                ary, arz, F, oc.al.as<double>(), oc.ar.as<double>(), fmoc);
+    }
     k_calcp_rows<<<grid_for(st.n), kBlock, 0, s>>>(
         st.n, st.ioffset.as<int>(), st.diag.as<int>(), kface_order, kface, flmass,
         oc.noc ? oc.co_ptr.as<int>() : nullptr, oc.noc ? oc.co_idx.as<int>() : nullptr,
@@ -1092,6 +1198,38 @@ void adjust_mass_flow_dev(fc_context *ctx, int n, const int *owner0_all, const d
         k_adjust_rows<<<grid_for(n), kBlock, 0, s>>>(n, ninl, cb_ptr.as<int>(), cb_q.as<int>(), fmi, fmo, su);
     FC_HIP_CHECK(hipGetLastError());
     FC_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void velocity_bc_dev(fc_context *ctx, const int *owner0_all, const double *arx, const double *ary, const double *arz,
+                     int nout, int outlet_face0, int outlet_val0, int nsym, int sym_face0, int sym_val0, double flomas,
+                     const double *den, double *u, double *v, double *w, double *fmo, double *fac_scratch) {
+    hipStream_t s = ctx->stream;
+    if (nout > 0) {
+        k_outlet_flux<<<grid_for(nout), kBlock, 0, s>>>(nout, outlet_face0, owner0_all, arx, ary, arz, den, u, v, w,
+                                                        fmo);
+        k_outlet_factor<<<1, 64, 0, s>>>(nout, fmo, flomas, fac_scratch);
+        k_outlet_scale<<<grid_for(nout), kBlock, 0, s>>>(nout, outlet_face0, outlet_val0, owner0_all, fac_scratch,
+                                                         fmo, u, v, w);
+    }
+    if (nsym > 0)
+        k_symmetry_bc<<<grid_for(nsym), kBlock, 0, s>>>(nsym, sym_face0, sym_val0, owner0_all, arx, ary, arz, u, v,
+                                                        w);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void bpres_dev(fc_context *ctx, const GradGroups &G, int noc, int oc_face0, int oc_val0, const int *owner0_all,
+               const double *xc, const double *yc, const double *zc, const double *xf, const double *yf,
+               const double *zf, const double *dpdx, int stage, double *p) {
+    const int total = G.qstart[5] + noc;
+    if (total <= 0) return;
+    k_bpres<<<grid_for(total), kBlock, 0, ctx->stream>>>(G, noc, oc_face0, oc_val0, owner0_all, xc, yc, zc, xf, yf,
+                                                         zf, dpdx, stage, p);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void pin_reference_dev(fc_context *ctx, const Structure &st, int pref0, const double *p, double *a, double *su) {
+    k_pin_reference<<<1, 64, 0, ctx->stream>>>(st.ioffset.as<int>(), st.diag.as<int>(), pref0, p, a, su);
+    FC_HIP_CHECK(hipGetLastError());
 }
 
 void calcp_correct_dev(fc_context *ctx, int n, int nif, const int *owner0, const int *neigh0, const int *ij_slot,

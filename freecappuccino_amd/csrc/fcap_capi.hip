@@ -473,26 +473,29 @@ extern "C" fc_status fc_convdiff(fc_context *ctx, int numCells, int numInnerFace
     });
 }
 
-extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
-                                       const int *neighbour, const double *xc, const double *yc, const double *zc,
-                                       const double *vol, const double *xf, const double *yf, const double *zf,
-                                       const double *arx, const double *ary, const double *arz,
-                                       const double *facint, const double *den, const double *apu,
-                                       const double *apv, const double *apw, const double *u, const double *v,
-                                       const double *w, const double *p, const double *dUdxi, const double *dVdxi,
-                                       const double *dWdxi, const double *dPdxi, int noc, int iOCFacesStart,
-                                       const int *ijl, const int *ijr, const double *foc, const int *ioffset,
-                                       const int *ja, const int *diag, const int *ij_slot, const int *ji_slot,
-                                       int nnz, double *a, double *su, double *flmass, double *al, double *ar,
-                                       double *fmoc) {
+// The face loops of calcp (facefluxmass) or PISO (facefluxmass_piso, which reads
+// neither apv/apw nor p and dPdxi: those may be null) on host arrays.
+static fc_status calcp_assemble_host(bool piso, fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                     const int *neighbour, const double *xc, const double *yc, const double *zc,
+                                     const double *vol, const double *xf, const double *yf, const double *zf,
+                                     const double *arx, const double *ary, const double *arz, const double *facint,
+                                     const double *den, const double *apu, const double *apv, const double *apw,
+                                     const double *u, const double *v, const double *w, const double *p,
+                                     const double *dUdxi, const double *dVdxi, const double *dWdxi,
+                                     const double *dPdxi, int noc, int iOCFacesStart, const int *ijl,
+                                     const int *ijr, const double *foc, const int *ioffset, const int *ja,
+                                     const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
+                                     double *su, double *flmass, double *al, double *ar, double *fmoc) {
     return guarded(ctx, [&]() {
         const int n = numCells, nif = numInnerFaces;
         FC_REQUIRE(n > 0 && nif >= 0 && nnz == 2 * nif + n, FC_ERR_ARGUMENT, "nnz must be 2*numInnerFaces+numCells");
         FC_REQUIRE(a && su && (flmass || nif == 0), FC_ERR_ARGUMENT, "a/su/flmass");
         const double *cf[12] = {xc, yc, zc, vol, den, apu, apv, apw, u, v, w, p};
-        for (const double *c : cf) FC_REQUIRE(c != nullptr, FC_ERR_ARGUMENT, "cell fields");
+        for (int k = 0; k < 12; ++k)
+            FC_REQUIRE(cf[k] != nullptr || (piso && (k == 6 || k == 7 || k == 11)), FC_ERR_ARGUMENT, "cell fields");
         const double *gf[4] = {dUdxi, dVdxi, dWdxi, dPdxi};
-        for (const double *g : gf) FC_REQUIRE(g != nullptr, FC_ERR_ARGUMENT, "gradients");
+        for (int k = 0; k < 4; ++k)
+            FC_REQUIRE(gf[k] != nullptr || (piso && k == 3), FC_ERR_ARGUMENT, "gradients");
         const int nfaces_needed = std::max(nif, noc > 0 ? iOCFacesStart + noc : 0);
         Structure &st = cached_structure(ctx, n, nnz, ioffset, ja, diag);
         FaceSet fs;
@@ -500,8 +503,8 @@ extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInn
         DevBuf dx[6], dc[12], dg[4], dfi, dsu, da, dfl, dfm;
         const double *fa[6] = {arx, ary, arz, xf, yf, zf};
         for (int k = 0; k < 6; ++k) upload_dbl(ctx, dx[k], fa[k], nfaces_needed);
-        for (int k = 0; k < 12; ++k) upload_dbl(ctx, dc[k], cf[k], n);
-        for (int k = 0; k < 4; ++k) upload_dbl(ctx, dg[k], gf[k], 3 * (size_t)n);
+        for (int k = 0; k < 12; ++k) upload_dbl(ctx, dc[k], cf[k], cf[k] ? n : 0);
+        for (int k = 0; k < 4; ++k) upload_dbl(ctx, dg[k], gf[k], gf[k] ? 3 * (size_t)n : 0);
         upload_dbl(ctx, dfi, facint, nif);
         dsu.alloc(sizeof(double) * n);
         da.alloc(sizeof(double) * nnz);
@@ -541,7 +544,7 @@ extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInn
                   fs.kface.as<int>(), fs.needs_order ? fs.kord.as<int>() : nullptr, dx[3].as<double>(),
                   dx[4].as<double>(), dx[5].as<double>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(),
                   dfi.as<double>(), F, oc, da.as<double>(), dsu.as<double>(), dfl.as<double>(),
-                  noc > 0 ? dfm.as<double>() : nullptr);
+                  noc > 0 ? dfm.as<double>() : nullptr, piso);
         download_dbl(ctx, a, da.get(), nnz);
         download_dbl(ctx, su, dsu.get(), n);
         if (nif) download_dbl(ctx, flmass, dfl.get(), nif);
@@ -552,6 +555,40 @@ extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInn
         }
         return FC_OK;
     });
+}
+
+extern "C" fc_status fc_calcp_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                       const int *neighbour, const double *xc, const double *yc, const double *zc,
+                                       const double *vol, const double *xf, const double *yf, const double *zf,
+                                       const double *arx, const double *ary, const double *arz,
+                                       const double *facint, const double *den, const double *apu,
+                                       const double *apv, const double *apw, const double *u, const double *v,
+                                       const double *w, const double *p, const double *dUdxi, const double *dVdxi,
+                                       const double *dWdxi, const double *dPdxi, int noc, int iOCFacesStart,
+                                       const int *ijl, const int *ijr, const double *foc, const int *ioffset,
+                                       const int *ja, const int *diag, const int *ij_slot, const int *ji_slot,
+                                       int nnz, double *a, double *su, double *flmass, double *al, double *ar,
+                                       double *fmoc) {
+    return calcp_assemble_host(false, ctx, numCells, numInnerFaces, owner, neighbour, xc, yc, zc, vol, xf, yf, zf,
+                               arx, ary, arz, facint, den, apu, apv, apw, u, v, w, p, dUdxi, dVdxi, dWdxi, dPdxi,
+                               noc, iOCFacesStart, ijl, ijr, foc, ioffset, ja, diag, ij_slot, ji_slot, nnz, a, su,
+                               flmass, al, ar, fmoc);
+}
+
+extern "C" fc_status fc_piso_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                                      const int *neighbour, const double *xc, const double *yc, const double *zc,
+                                      const double *vol, const double *xf, const double *yf, const double *zf,
+                                      const double *arx, const double *ary, const double *arz, const double *facint,
+                                      const double *den, const double *apu, const double *u, const double *v,
+                                      const double *w, const double *dUdxi, const double *dVdxi,
+                                      const double *dWdxi, int noc, int iOCFacesStart, const int *ijl,
+                                      const int *ijr, const double *foc, const int *ioffset, const int *ja,
+                                      const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
+                                      double *su, double *flmass, double *al, double *ar, double *fmoc) {
+    return calcp_assemble_host(true, ctx, numCells, numInnerFaces, owner, neighbour, xc, yc, zc, vol, xf, yf, zf,
+                               arx, ary, arz, facint, den, apu, nullptr, nullptr, u, v, w, nullptr, dUdxi, dVdxi,
+                               dWdxi, nullptr, noc, iOCFacesStart, ijl, ijr, foc, ioffset, ja, diag, ij_slot,
+                               ji_slot, nnz, a, su, flmass, al, ar, fmoc);
 }
 
 extern "C" fc_status fc_calcp_nonorth(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
@@ -634,6 +671,85 @@ extern "C" fc_status fc_adjust_mass_flow(fc_context *ctx, int numCells, int numT
                              flomas, dden.as<double>(), duvw[0].as<double>(), duvw[1].as<double>(),
                              duvw[2].as<double>(), dsu.as<double>(), dfmo.as<double>());
         download_dbl(ctx, su, dsu.get(), n);
+        for (int k = 0; k < 3; ++k) download_dbl(ctx, vel[k], duvw[k].get(), numTotal);
+        download_dbl(ctx, fmo, dfmo.get(), nout);
+        return FC_OK;
+    });
+}
+
+// The boundary groups of bnd (count, faces before the group; inlet, outlet, symmetry,
+// wall, pressure outlet) with their values after the cells, in the reference's order.
+static GradGroups groups_of(const int *bnd, int n, int nif, int nf, int ntotal) {
+    GradGroups G;
+    int q = 0, v = n;
+    for (int g = 0; g < 5; ++g) {
+        const int cnt = bnd[2 * g], fs = bnd[2 * g + 1];
+        FC_REQUIRE(cnt >= 0 && (cnt == 0 || (fs >= nif && fs + cnt <= nf)), FC_ERR_ARGUMENT,
+                   "boundary group outside the boundary faces");
+        G.qstart[g] = q;
+        G.fstart[g] = fs;
+        G.vstart[g] = v;
+        q += cnt;
+        v += cnt;
+    }
+    G.qstart[5] = q;
+    FC_REQUIRE(v <= ntotal, FC_ERR_ARGUMENT, "boundary values beyond numTotal");
+    return G;
+}
+
+extern "C" fc_status fc_bpres(fc_context *ctx, int numCells, int numTotal, int numFaces, int numInnerFaces,
+                              const int *owner, const double *xc, const double *yc, const double *zc,
+                              const double *xf, const double *yf, const double *zf, const int *bnd, int noc,
+                              int iOCFacesStart, const double *dPdxi, int istage, double *p) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nf = numFaces;
+        FC_REQUIRE(n > 0 && numInnerFaces >= 0 && nf >= numInnerFaces && numTotal >= n && noc >= 0, FC_ERR_ARGUMENT,
+                   "sizes");
+        FC_REQUIRE(owner && xc && yc && zc && xf && yf && zf && bnd && p && istage >= 1 && (istage == 1 || dPdxi),
+                   FC_ERR_ARGUMENT, "arguments");
+        const GradGroups G = groups_of(bnd, n, numInnerFaces, nf, numTotal);
+        const int oc_val0 = n + G.qstart[5];
+        FC_REQUIRE(noc == 0 || (iOCFacesStart >= 0 && iOCFacesStart + noc <= nf && oc_val0 + noc <= numTotal &&
+                                noc <= nf),
+                   FC_ERR_ARGUMENT, "O-C faces or values out of range");
+        DevBuf own, dc[3], dx[3], dg, dp;
+        upload_idx(ctx, own, owner, nf);
+        const double *ca[3] = {xc, yc, zc};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dc[k], ca[k], n);
+        const double *fa[3] = {xf, yf, zf};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dx[k], fa[k], nf);
+        upload_dbl(ctx, dg, istage >= 2 ? dPdxi : nullptr, istage >= 2 ? 3 * (long long)n : 0);
+        upload_dbl(ctx, dp, p, numTotal);
+        bpres_dev(ctx, G, noc, iOCFacesStart, oc_val0, own.as<int>(), dc[0].as<double>(), dc[1].as<double>(),
+                  dc[2].as<double>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), dg.as<double>(),
+                  istage, dp.as<double>());
+        download_dbl(ctx, p, dp.get(), numTotal);
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_correct_velocity_bc(fc_context *ctx, int numCells, int numTotal, int numFaces,
+                                            int numInnerFaces, const int *owner, const double *arx,
+                                            const double *ary, const double *arz, const int *bnd, double flomas,
+                                            const double *den, double *u, double *v, double *w, double *fmo) {
+    return guarded(ctx, [&]() {
+        const int n = numCells, nf = numFaces;
+        FC_REQUIRE(n > 0 && numInnerFaces >= 0 && nf >= numInnerFaces && numTotal >= n, FC_ERR_ARGUMENT, "sizes");
+        FC_REQUIRE(owner && arx && ary && arz && bnd && den && u && v && w, FC_ERR_ARGUMENT, "arguments");
+        const GradGroups G = groups_of(bnd, n, numInnerFaces, nf, numTotal);
+        const int nout = bnd[2], nsym = bnd[4];
+        FC_REQUIRE(fmo || nout == 0, FC_ERR_ARGUMENT, "fmo");
+        DevBuf own, dx[3], dden, duvw[3], dfmo(sizeof(double) * (size_t)std::max(nout, 1)), fac(sizeof(double));
+        upload_idx(ctx, own, owner, nf);
+        const double *fa[3] = {arx, ary, arz};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, dx[k], fa[k], nf);
+        upload_dbl(ctx, dden, den, n);
+        double *vel[3] = {u, v, w};
+        for (int k = 0; k < 3; ++k) upload_dbl(ctx, duvw[k], vel[k], numTotal);
+        velocity_bc_dev(ctx, own.as<int>(), dx[0].as<double>(), dx[1].as<double>(), dx[2].as<double>(), nout,
+                        G.fstart[1], G.vstart[1], nsym, G.fstart[2], G.vstart[2], flomas, dden.as<double>(),
+                        duvw[0].as<double>(), duvw[1].as<double>(), duvw[2].as<double>(), dfmo.as<double>(),
+                        fac.as<double>());
         for (int k = 0; k < 3; ++k) download_dbl(ctx, vel[k], duvw[k].get(), numTotal);
         download_dbl(ctx, fmo, dfmo.get(), nout);
         return FC_OK;
@@ -1093,10 +1209,12 @@ static const double *field_on_device(fc_context *ctx, DevBuf &stage, const doubl
     return stage.as<double>();
 }
 
-extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double *apu, const double *apv,
-                                           const double *apw, const double *u, const double *v, const double *w,
-                                           const double *p, const double *dUdxi, const double *dVdxi,
-                                           const double *dWdxi, const double *dPdxi, double *flmass) {
+// calcp's (or, piso, PISO's) face loops on the resident system; PISO reads neither
+// apv/apw nor p and dPdxi, which may then be null
+static fc_status sys_calcp_assemble(bool piso, fc_system *sys, const double *den, const double *apu,
+                                    const double *apv, const double *apw, const double *u, const double *v,
+                                    const double *w, const double *p, const double *dUdxi, const double *dVdxi,
+                                    const double *dWdxi, const double *dPdxi, double *flmass) {
     if (!sys) return FC_ERR_ARGUMENT;
     fc_context *ctx = sys->ctx;
     return guarded(ctx, [&]() {
@@ -1104,12 +1222,15 @@ extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, co
         FC_REQUIRE(flmass || nif == 0, FC_ERR_ARGUMENT, "flmass");
         FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
         const double *cf[8] = {den, apu, apv, apw, u, v, w, p};
-        for (const double *c : cf) FC_REQUIRE(c != nullptr, FC_ERR_ARGUMENT, "cell fields");
+        for (int k = 0; k < 8; ++k)
+            FC_REQUIRE(cf[k] != nullptr || (piso && (k == 2 || k == 3 || k == 7)), FC_ERR_ARGUMENT, "cell fields");
         const double *gf[4] = {dUdxi, dVdxi, dWdxi, dPdxi};
-        for (const double *g : gf) FC_REQUIRE(g != nullptr, FC_ERR_ARGUMENT, "gradients");
+        for (int k = 0; k < 4; ++k)
+            FC_REQUIRE(gf[k] != nullptr || (piso && k == 3), FC_ERR_ARGUMENT, "gradients");
         const double *dcf[8], *dgf[4];
-        for (int k = 0; k < 8; ++k) dcf[k] = field_on_device(ctx, sys->cp_stage[k], cf[k], n);
-        for (int k = 0; k < 4; ++k) dgf[k] = field_on_device(ctx, sys->cp_stage[8 + k], gf[k], 3 * (size_t)n);
+        for (int k = 0; k < 8; ++k) dcf[k] = cf[k] ? field_on_device(ctx, sys->cp_stage[k], cf[k], n) : nullptr;
+        for (int k = 0; k < 4; ++k)
+            dgf[k] = gf[k] ? field_on_device(ctx, sys->cp_stage[8 + k], gf[k], 3 * (size_t)n) : nullptr;
         const bool fl_dev = nif > 0 && device_resident(flmass);
         double *dfl = flmass;
         if (!fl_dev) {
@@ -1140,13 +1261,44 @@ extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, co
                   sys->ji.as<int>(), sys->kface.as<int>(), sys->needs_order ? sys->kord.as<int>() : nullptr,
                   g.xf.as<double>(), g.yf.as<double>(), g.zf.as<double>(), g.arx.as<double>(), g.ary.as<double>(),
                   g.arz.as<double>(), g.facint.as<double>(), F, sys->oc, sys->a.as<double>(), sys->su.as<double>(),
-                  dfl, nullptr);
-        // pp = 0 before the pressure-correction solve (calcp:112): the solution vector is the
-        // initial guess of the next fc_sys_solve, and the L1 rule's res0 depends on it
-        FC_HIP_CHECK(hipMemsetAsync(sys->fi.get(), 0, sizeof(double) * n, ctx->stream));
+                  dfl, nullptr, piso);
+        // calcp: pp = 0 before the pressure-correction solve (calcp:112): the solution vector is
+        // the initial guess of the next fc_sys_solve, and the L1 rule's res0 depends on it.  PISO
+        // solves for p itself and starts from whatever it holds (its pp = 0 is commented out,
+        // src/PISO_multiple_correction.f90:198-199), so the solution vector is left alone.
+        if (!piso) FC_HIP_CHECK(hipMemsetAsync(sys->fi.get(), 0, sizeof(double) * n, ctx->stream));
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         sys->assembly_version += 1;
         if (nif && !fl_dev) download_dbl(ctx, flmass, dfl, nif);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_calcp_assemble(fc_system *sys, const double *den, const double *apu, const double *apv,
+                                           const double *apw, const double *u, const double *v, const double *w,
+                                           const double *p, const double *dUdxi, const double *dVdxi,
+                                           const double *dWdxi, const double *dPdxi, double *flmass) {
+    return sys_calcp_assemble(false, sys, den, apu, apv, apw, u, v, w, p, dUdxi, dVdxi, dWdxi, dPdxi, flmass);
+}
+
+extern "C" fc_status fc_sys_piso_assemble(fc_system *sys, const double *den, const double *apu, const double *u,
+                                          const double *v, const double *w, const double *dUdxi, const double *dVdxi,
+                                          const double *dWdxi, double *flmass) {
+    return sys_calcp_assemble(true, sys, den, apu, nullptr, nullptr, u, v, w, nullptr, dUdxi, dVdxi, dWdxi, nullptr,
+                              flmass);
+}
+
+extern "C" fc_status fc_sys_pin_reference(fc_system *sys, int pRefCell, const double *p) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        const int n = sys->st.n;
+        FC_REQUIRE(pRefCell >= 1 && pRefCell <= n, FC_ERR_ARGUMENT, "pRefCell outside 1..numCells");
+        FC_REQUIRE(p != nullptr, FC_ERR_ARGUMENT, "p");
+        const double *dp = field_on_device(ctx, sys->cp_stage[0], p, n);
+        pin_reference_dev(ctx, sys->st, pRefCell - 1, dp, sys->a.as<double>(), sys->su.as<double>());
+        sys->assembly_version += 1;
         FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return FC_OK;
     });
@@ -1228,56 +1380,134 @@ extern "C" fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, con
     });
 }
 
+
+// Gauss gradient of du (device, the system's numTotal values) into out (device, (3, numCells))
+static void sys_grad(fc_system *sys, const int *bnd, const double *du, int nigrad, double *out) {
+    fc_context *ctx = sys->ctx;
+    FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+    const int n = sys->st.n, nif = sys->mesh.nif;
+    const GradGroups G = groups_of(bnd, n, nif, sys->mesh.nfaces, sys->ntotal);
+    if (!std::equal(bnd, bnd + 10, sys->grad_bnd)) {
+        std::fill(sys->grad_bnd, sys->grad_bnd + 10, -1);
+        build_boundary_lists(ctx, G, sys->mesh.owner.as<int>(), n, sys->grad_cb_ptr, sys->grad_cb_q);
+        std::copy(bnd, bnd + 10, sys->grad_bnd);
+    }
+    sys->grad_xyz.ensure(sizeof(double) * 3 * (size_t)n);
+    double *g = sys->grad_xyz.as<double>();
+    GeometryDev &geo = sys->mesh.g;
+    OcDev none;
+    grad_gauss_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(), sys->kface.as<int>(),
+                   sys->needs_order ? sys->kord.as<int>() : nullptr, geo.xf.as<double>(), geo.yf.as<double>(),
+                   geo.zf.as<double>(), geo.arx.as<double>(), geo.ary.as<double>(), geo.arz.as<double>(),
+                   geo.facint.as<double>(), geo.xc.as<double>(), geo.yc.as<double>(), geo.zc.as<double>(),
+                   geo.vol.as<double>(), du, G, sys->grad_cb_ptr.get() ? sys->grad_cb_ptr.as<int>() : nullptr,
+                   sys->grad_cb_q.get() ? sys->grad_cb_q.as<int>() : nullptr, none, nullptr, nigrad, g, g + n,
+                   g + 2 * (size_t)n, &sys->grad_scratch);
+    interleave3_dev(ctx, n, g, g + n, g + 2 * (size_t)n, out);
+}
+
 extern "C" fc_status fc_sys_grad_gauss(fc_system *sys, const int *bnd, const double *u, int nigrad,
                                        double *dudxi) {
     if (!sys) return FC_ERR_ARGUMENT;
     fc_context *ctx = sys->ctx;
     return guarded(ctx, [&]() {
         FC_REQUIRE(bnd && u && dudxi && nigrad >= 1, FC_ERR_ARGUMENT, "arguments");
-        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
-        const int n = sys->st.n, nif = sys->mesh.nif, nf = sys->mesh.nfaces;
-        GradGroups G;
-        int q = 0, v = n;
-        for (int g = 0; g < 5; ++g) {
-            const int cnt = bnd[2 * g], fs = bnd[2 * g + 1];
-            FC_REQUIRE(cnt >= 0 && (cnt == 0 || (fs >= nif && fs + cnt <= nf)), FC_ERR_ARGUMENT,
-                       "boundary group outside the boundary faces");
-            G.qstart[g] = q;
-            G.fstart[g] = fs;
-            G.vstart[g] = v;
-            q += cnt;
-            v += cnt;
-        }
-        G.qstart[5] = q;
-        FC_REQUIRE(v <= sys->ntotal, FC_ERR_ARGUMENT, "boundary values beyond numTotal of the system");
-        if (!std::equal(bnd, bnd + 10, sys->grad_bnd)) {
-            std::fill(sys->grad_bnd, sys->grad_bnd + 10, -1);
-            build_boundary_lists(ctx, G, sys->mesh.owner.as<int>(), n, sys->grad_cb_ptr, sys->grad_cb_q);
-            std::copy(bnd, bnd + 10, sys->grad_bnd);
-        }
+        const int n = sys->st.n;
         const double *du = field_on_device(ctx, sys->grad_u, u, sys->ntotal);
-        sys->grad_xyz.ensure(sizeof(double) * 3 * (size_t)n);
-        double *g = sys->grad_xyz.as<double>();
         const bool out_dev = device_resident(dudxi);
         double *out = dudxi;
         if (!out_dev) {
             sys->grad_out.ensure(sizeof(double) * 3 * (size_t)n);
             out = sys->grad_out.as<double>();
         }
-        GeometryDev &geo = sys->mesh.g;
-        OcDev none;
         ctx->timer.start(ctx->stream);
-        grad_gauss_dev(ctx, sys->st, nif, sys->mesh.owner.as<int>(), sys->mesh.neighbour.as<int>(),
-                       sys->kface.as<int>(), sys->needs_order ? sys->kord.as<int>() : nullptr, geo.xf.as<double>(),
-                       geo.yf.as<double>(), geo.zf.as<double>(), geo.arx.as<double>(), geo.ary.as<double>(),
-                       geo.arz.as<double>(), geo.facint.as<double>(), geo.xc.as<double>(), geo.yc.as<double>(),
-                       geo.zc.as<double>(), geo.vol.as<double>(), du, G,
-                       sys->grad_cb_ptr.get() ? sys->grad_cb_ptr.as<int>() : nullptr,
-                       sys->grad_cb_q.get() ? sys->grad_cb_q.as<int>() : nullptr, none, nullptr, nigrad, g, g + n,
-                       g + 2 * (size_t)n, &sys->grad_scratch);
-        interleave3_dev(ctx, n, g, g + n, g + 2 * (size_t)n, out);
+        sys_grad(sys, bnd, du, nigrad, out);
         sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
         if (!out_dev) download_dbl(ctx, dudxi, out, 3 * (size_t)n);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_bpres(fc_system *sys, const int *bnd, int istage, const double *dPdxi, double *p) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(bnd && p && istage >= 1 && (istage == 1 || dPdxi), FC_ERR_ARGUMENT, "arguments");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const int n = sys->st.n;
+        const GradGroups G = groups_of(bnd, n, sys->mesh.nif, sys->mesh.nfaces, sys->ntotal);
+        bool back = false;
+        double *dp = inout_on_device(ctx, sys->grad_u, p, sys->ntotal, &back);
+        const double *dg = istage >= 2 ? field_on_device(ctx, sys->cp_stage[0], dPdxi, 3 * (size_t)n) : nullptr;
+        GeometryDev &geo = sys->mesh.g;
+        bpres_dev(ctx, G, 0, 0, 0, sys->mesh.owner.as<int>(), geo.xc.as<double>(), geo.yc.as<double>(),
+                  geo.zc.as<double>(), geo.xf.as<double>(), geo.yf.as<double>(), geo.zf.as<double>(), dg, istage, dp);
+        if (back) download_dbl(ctx, p, dp, sys->ntotal);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_grad_pp(fc_system *sys, const int *bnd, int nipgrad, int nigrad, double *dPdxi) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(bnd && dPdxi && nipgrad >= 1 && nigrad >= 1, FC_ERR_ARGUMENT, "arguments");
+        FC_REQUIRE(sys->oc.noc == 0, FC_ERR_ARGUMENT, "resident systems carry no O-C faces");
+        const int n = sys->st.n;
+        const GradGroups G = groups_of(bnd, n, sys->mesh.nif, sys->mesh.nfaces, sys->ntotal);
+        // pp over numTotal: the solution's cells, boundary values from bpres
+        sys->pp_total.ensure(sizeof(double) * (size_t)sys->ntotal);
+        double *pp = sys->pp_total.as<double>();
+        FC_HIP_CHECK(hipMemsetAsync(pp, 0, sizeof(double) * (size_t)sys->ntotal, ctx->stream));
+        FC_HIP_CHECK(hipMemcpyAsync(pp, sys->fi.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+        const bool out_dev = device_resident(dPdxi);
+        double *out = dPdxi;
+        if (!out_dev) {
+            sys->grad_out.ensure(sizeof(double) * 3 * (size_t)n);
+            out = sys->grad_out.as<double>();
+        }
+        GeometryDev &geo = sys->mesh.g;
+        ctx->timer.start(ctx->stream);
+        for (int istage = 1; istage <= nipgrad; ++istage) {   // calcp:131-136
+            bpres_dev(ctx, G, 0, 0, 0, sys->mesh.owner.as<int>(), geo.xc.as<double>(), geo.yc.as<double>(),
+                      geo.zc.as<double>(), geo.xf.as<double>(), geo.yf.as<double>(), geo.zf.as<double>(), out, istage,
+                      pp);
+            sys_grad(sys, bnd, pp, nigrad, out);
+        }
+        sys->last_ms[0] = ctx->timer.stop_ms(ctx->stream);
+        if (!out_dev) download_dbl(ctx, dPdxi, out, 3 * (size_t)n);
+        FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return FC_OK;
+    });
+}
+
+extern "C" fc_status fc_sys_correct_velocity_bc(fc_system *sys, const int *bnd, double flomas, const double *den,
+                                                double *u, double *v, double *w, double *fmo) {
+    if (!sys) return FC_ERR_ARGUMENT;
+    fc_context *ctx = sys->ctx;
+    return guarded(ctx, [&]() {
+        FC_REQUIRE(bnd && den && u && v && w, FC_ERR_ARGUMENT, "arguments");
+        const int n = sys->st.n;
+        const GradGroups G = groups_of(bnd, n, sys->mesh.nif, sys->mesh.nfaces, sys->ntotal);
+        const int nout = bnd[2], nsym = bnd[4];
+        FC_REQUIRE(fmo || nout == 0, FC_ERR_ARGUMENT, "fmo");
+        const double *dden = field_on_device(ctx, sys->cp_stage[0], den, n);
+        double *io[3] = {u, v, w};
+        double *dio[3];
+        bool back[3];
+        for (int k = 0; k < 3; ++k) dio[k] = inout_on_device(ctx, sys->cp_stage[1 + k], io[k], sys->ntotal, &back[k]);
+        bool fback = false;
+        double *dfmo = nout ? inout_on_device(ctx, sys->cp_stage[4], fmo, nout, &fback) : nullptr;
+        sys->vbc_fac.ensure(sizeof(double));
+        GeometryDev &geo = sys->mesh.g;
+        velocity_bc_dev(ctx, sys->mesh.owner.as<int>(), geo.arx.as<double>(), geo.ary.as<double>(), geo.arz.as<double>(),
+                        nout, G.fstart[1], G.vstart[1], nsym, G.fstart[2], G.vstart[2], flomas, dden, dio[0], dio[1],
+                        dio[2], dfmo, sys->vbc_fac.as<double>());
+        for (int k = 0; k < 3; ++k)
+            if (back[k]) download_dbl(ctx, io[k], dio[k], sys->ntotal);
+        if (fback) download_dbl(ctx, fmo, dfmo, nout);
         FC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return FC_OK;
     });

@@ -268,7 +268,18 @@ void calcp_dev(fc_context *ctx, const Structure &st, int nif, const int *owner0,
                const int *ij_slot, const int *ji_slot, const int *kface, const int *kface_order, const double *xf,
                const double *yf, const double *zf, const double *arx, const double *ary, const double *arz,
                const double *facint, const RhieChowDev &F, const OcDev &oc, double *a, double *su, double *flmass,
-               double *fmoc);
+               double *fmoc, bool piso = false);
+// correctBoundaryConditionsVelocity: outlet values and fluxes scaled to flomas (fac_scratch:
+// one device double), symmetry values projected (0-based faces and values)
+void velocity_bc_dev(fc_context *ctx, const int *owner0_all, const double *arx, const double *ary, const double *arz,
+                     int nout, int outlet_face0, int outlet_val0, int nsym, int sym_face0, int sym_val0, double flomas,
+                     const double *den, double *u, double *v, double *w, double *fmo, double *fac_scratch);
+// bpres at one stage on p (numTotal); dpdx (3, numCells) is read from stage 2 on
+void bpres_dev(fc_context *ctx, const GradGroups &G, int noc, int oc_face0, int oc_val0, const int *owner0_all,
+               const double *xc, const double *yc, const double *zc, const double *xf, const double *yf,
+               const double *zf, const double *dpdx, int stage, double *p);
+// PISO's reference cell: row pref0 of a zeroed, diagonal 1, su(pref0) = p(pref0)
+void pin_reference_dev(fc_context *ctx, const Structure &st, int pref0, const double *p, double *a, double *su);
 // range/consistency check of caller-supplied owner/neighbour and slot maps (device, 0-based)
 void validate_faces(fc_context *ctx, const Structure &st, int nif, const int *own, const int *nei, const int *ij,
                     const int *ji);

@@ -23,6 +23,8 @@ struct fc_system {
     // the component arrays and the (3, numCells) output
     int grad_bnd[10] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     fcap::DevBuf grad_cb_ptr, grad_cb_q, grad_u, grad_xyz, grad_out;
+    // pp over numTotal for bpres + grad(pp), and the outlet factor of the velocity BCs
+    fcap::DevBuf pp_total, vbc_fac;
     fcap::GradScratch grad_scratch;
     bool su0_zero = true;
     double last_ms[3] = {0, 0, 0};

@@ -95,6 +95,15 @@ def library() -> C.CDLL:
         "fc_calcp_nonorth": ([vp, i, i, ip, ip] + [dp] * 14 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
                              + [dp] * 3, i),
         "fc_adjust_mass_flow": ([vp, i, i, i, ip, dp, dp, dp, i, i, dp, i, i, i, d] + [dp] * 6, i),
+        "fc_piso_assemble": ([vp, i, i, ip, ip] + [dp] * 19 + [i, i, ip, ip, dp, ip, ip, ip, ip, ip, i]
+                             + [dp] * 6, i),
+        "fc_bpres": ([vp, i, i, i, i, ip, dp, dp, dp, dp, dp, dp, ip, i, i, dp, i, dp], i),
+        "fc_correct_velocity_bc": ([vp, i, i, i, i, ip, dp, dp, dp, ip, d, dp, dp, dp, dp, dp], i),
+        "fc_sys_piso_assemble": ([vp] + [vp] * 9, i),
+        "fc_sys_pin_reference": ([vp, i, vp], i),
+        "fc_sys_bpres": ([vp, ip, i, vp, vp], i),
+        "fc_sys_grad_pp": ([vp, ip, i, i, vp], i),
+        "fc_sys_correct_velocity_bc": ([vp, ip, d, vp, vp, vp, vp, vp], i),
         "fc_calcp_correct": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip, i] + [dp] * 7 + [d, i, i, ip, ip]
                              + [dp] * 7, i),
         "fc_rhs_divergence": ([vp, i, i, ip, ip, dp, dp], i),
@@ -450,6 +459,67 @@ def _oc_host(oc):
     return noc, (int(oc["faces"]) if noc else 0), ijl, ijr, foc
 
 
+PISO_FIELDS = ("den", "apu", "u", "v", "w")
+PISO_GRADS = ("dudx", "dvdx", "dwdx")
+
+
+def piso_assemble(mesh, geo: Geometry, csr: CSR, fields, oc=None, ctx: Optional[Context] = None):
+    """PISO's pressure assembly on the GPU: the face loop of src/PISO_multiple_correction.f90:
+    107-148 around facefluxmass_piso (src/facefluxmass.f90:294-358).  fields: den, apu, u, v, w
+    (n each) and dudx, dvdx, dwdx (n x 3).  Returns a dict with a, su, flmass (and al, ar, fmoc
+    with cuts)."""
+    ctx = ctx or default_context()
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    fl = [_f64(fields[k]) for k in PISO_FIELDS]
+    gr = [_f64(np.asarray(fields[k]).reshape(n, 3)) for k in PISO_GRADS]
+    noc, oc0, ijl, ijr, foc = _oc_host(oc)
+    out = {"a": np.zeros(mesh.nnz), "su": np.zeros(n), "flmass": np.zeros(max(nif, 1)),
+           "al": np.zeros(max(noc, 1)), "ar": np.zeros(max(noc, 1)), "fmoc": np.zeros(max(noc, 1))}
+    g = [_f64(getattr(geo, k)) for k in ("xc", "yc", "zc", "vol", "xf", "yf", "zf", "arx", "ary", "arz", "facint")]
+    ctx.check(ctx.L.fc_piso_assemble(ctx.h, n, nif, _ip(own1), _ip(nei1), *[_dp(x) for x in g],
+                                     *[_dp(x) for x in fl], *[_dp(x) for x in gr], noc, oc0, _ip(ijl), _ip(ijr),
+                                     _dp(foc), _ip(csr.ioffset), _ip(csr.ja), _ip(csr.diag),
+                                     _ip(_i32(csr.ij_slot)), _ip(_i32(csr.ji_slot)), mesh.nnz,
+                                     *[_dp(out[k]) for k in ("a", "su", "flmass", "al", "ar", "fmoc")]))
+    out["flmass"] = out["flmass"][:nif]
+    for k in ("al", "ar", "fmoc"):
+        out[k] = out[k][:noc]
+    return out
+
+
+def bpres(mesh, geo: Geometry, p, dpdx, istage, noc=0, oc_faces_start=0, ctx: Optional[Context] = None):
+    """bpres(p, istage) on the GPU (src/bpres.f90:3-152): returns p (numTotal) with its boundary
+    values set; dpdx (n x 3) is read from istage 2 on."""
+    ctx = ctx or default_context()
+    n = mesh.n_cells
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    out = _f64(p).copy()
+    dg = _f64(np.asarray(dpdx).reshape(n, 3)) if dpdx is not None else np.zeros(1)
+    g = [_f64(getattr(geo, k)) for k in ("xc", "yc", "zc", "xf", "yf", "zf")]
+    ctx.check(ctx.L.fc_bpres(ctx.h, n, out.shape[0], mesh.n_faces, mesh.n_inner, _ip(own1), *[_dp(x) for x in g],
+                             _ip(boundary_groups(mesh)), int(noc), int(oc_faces_start), _dp(dg), int(istage),
+                             _dp(out)))
+    return out
+
+
+def correct_velocity_bc(mesh, geo: Geometry, flomas, den, u, v, w, ctx: Optional[Context] = None):
+    """correctBoundaryConditionsVelocity on the GPU (src/correctBoundaryConditionsVelocity.f90:1-79):
+    returns (u, v, w, fmo); u, v, w hold the cells then the boundary values in group order."""
+    ctx = ctx or default_context()
+    n = mesh.n_cells
+    bnd = boundary_groups(mesh)
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    vel = [_f64(x).copy() for x in (u, v, w)]
+    nout = int(bnd[2])
+    fmo = np.zeros(max(nout, 1))
+    ctx.check(ctx.L.fc_correct_velocity_bc(ctx.h, n, vel[0].shape[0], mesh.n_faces, mesh.n_inner, _ip(own1),
+                                           _dp(_f64(geo.arx)), _dp(_f64(geo.ary)), _dp(_f64(geo.arz)), _ip(bnd),
+                                           float(flomas), _dp(_f64(den)), *[_dp(x) for x in vel], _dp(fmo)))
+    return (*vel, fmo[:nout])
+
+
 def calcp_nonorth(mesh, geo: Geometry, csr: CSR, den, apu, dpdx, flmass, oc=None, fmoc=None,
                   ctx: Optional[Context] = None):
     """Non-orthogonal corrector source of calcp on the GPU
@@ -697,6 +767,88 @@ class System:
         args = [arg(x) for x in (den, apu, apv, apw, u, v, w, p, dudx, dvdx, dwdx, dpdx, flmass)]
         self.ctx.check(self.ctx.L.fc_sys_calcp_assemble(self.h, *args))
         return None if out is None else out[:self.mesh.n_inner]
+
+    def piso_assemble(self, den, apu, u, v, w, dudx, dvdx, dwdx, flmass=None):
+        """PISO's pressure assembly (facefluxmass_piso) on the system's own mesh: a and su go
+        into the system, its solution vector is left as it is.  Arguments as for
+        calcp_assemble."""
+        keep = []
+
+        def arg(x):
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.ascontiguousarray(x, dtype=np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        out = None
+        if flmass is None:
+            out = np.zeros(max(self.mesh.n_inner, 1))
+            flmass = out
+        args = [arg(x) for x in (den, apu, u, v, w, dudx, dvdx, dwdx, flmass)]
+        self.ctx.check(self.ctx.L.fc_sys_piso_assemble(self.h, *args))
+        return None if out is None else out[:self.mesh.n_inner]
+
+    def pin_reference(self, pref_cell, p):
+        """PISO's reference pressure in the system's matrix (row pref_cell, 0-based, zeroed, its
+        diagonal 1, su = p[pref_cell]); p: numpy array or int device address."""
+        arr = None if isinstance(p, int) else np.ascontiguousarray(p, np.float64)
+        ptr = C.c_void_p(p) if isinstance(p, int) else arr.ctypes.data_as(C.c_void_p)
+        self.ctx.check(self.ctx.L.fc_sys_pin_reference(self.h, int(pref_cell) + 1, ptr))
+
+    def bpres(self, p, istage, dpdx=None):
+        """bpres(p, istage) on the system's mesh; p (numTotal) numpy (a new array is returned) or
+        int device address (updated in place)."""
+        bnd = _i32(boundary_groups(self.mesh))
+        keep = []
+
+        def arg(x, copy=False):
+            if x is None:
+                return C.c_void_p(None)
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.array(x, dtype=np.float64, copy=True) if copy else np.ascontiguousarray(x, np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        pa = arg(p, copy=True)
+        da = arg(dpdx)
+        self.ctx.check(self.ctx.L.fc_sys_bpres(self.h, _ip(bnd), int(istage), da, pa))
+        return None if isinstance(p, int) else keep[0]
+
+    def grad_pp(self, nipgrad=2, nigrad=1, out=None):
+        """The corrector's gradient of the system's solution pp (calcp:131-136): nipgrad stages
+        of bpres + Gauss gradient; returns (numCells, 3) or None when out is a device address."""
+        bnd = _i32(boundary_groups(self.mesh))
+        res = None
+        if out is None:
+            res = np.zeros((self.n, 3))
+            out = res
+        o_arg = C.c_void_p(out) if isinstance(out, int) else out.ctypes.data_as(C.c_void_p)
+        self.ctx.check(self.ctx.L.fc_sys_grad_pp(self.h, _ip(bnd), int(nipgrad), int(nigrad), o_arg))
+        return res
+
+    def correct_velocity_bc(self, flomas, den, u, v, w):
+        """correctBoundaryConditionsVelocity on the system's mesh; numpy u, v, w are copied and
+        (u, v, w, fmo) returned; int device addresses are updated in place (fmo then returned)."""
+        bnd = _i32(boundary_groups(self.mesh))
+        keep = []
+
+        def arg(x, copy=False):
+            if isinstance(x, int):
+                return C.c_void_p(x)
+            arr = np.array(x, dtype=np.float64, copy=True) if copy else np.ascontiguousarray(x, np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(C.c_void_p)
+
+        fmo = np.zeros(max(int(bnd[2]), 1))
+        dn = arg(den)
+        vel = [arg(x, copy=True) for x in (u, v, w)]
+        self.ctx.check(self.ctx.L.fc_sys_correct_velocity_bc(self.h, _ip(bnd), float(flomas), dn, *vel,
+                                                             fmo.ctypes.data_as(C.c_void_p)))
+        if isinstance(u, int):
+            return fmo[:int(bnd[2])]
+        return (*keep[1:4], fmo[:int(bnd[2])])
 
     def grad_gauss(self, u, nigrad=1, out=None):
         """Gauss gradient of u (cells, then boundary values in group order) on the system's own

@@ -166,6 +166,49 @@ fc_status fc_calcp_nonorth(fc_context *ctx, int numCells, int numInnerFaces, con
                            const int *ioffset, const int *ja, const int *diag, const int *ij_slot,
                            const int *ji_slot, int nnz, double *su, double *flmass, double *fmoc);
 
+/* PISO's pressure assembly: the face loop of src/PISO_multiple_correction.f90:
+ * 107-148 around facefluxmass_piso (src/facefluxmass.f90:294-358), with the
+ * same scatter and arguments as fc_calcp_assemble, but the coefficient cap =
+ * can = -rho_f (fxp Vol_P apu_P + fxn Vol_N apu_N) |S| / |d_PN| and the mass
+ * flux rho_f (u_f . S) without Rhie-Chow terms (so no apv/apw, p or dPdxi).
+ * The reference-pressure pin (PISO:188-192) is fc_sys_pin_reference, or the
+ * caller's own loop on the host arrays. */
+fc_status fc_piso_assemble(fc_context *ctx, int numCells, int numInnerFaces, const int *owner,
+                           const int *neighbour, const double *xc, const double *yc, const double *zc,
+                           const double *vol, const double *xf, const double *yf, const double *zf,
+                           const double *arx, const double *ary, const double *arz, const double *facint,
+                           const double *den, const double *apu, const double *u, const double *v,
+                           const double *w, const double *dUdxi, const double *dVdxi,
+                           const double *dWdxi, int noc, int iOCFacesStart, const int *ijl,
+                           const int *ijr, const double *foc, const int *ioffset, const int *ja,
+                           const int *diag, const int *ij_slot, const int *ji_slot, int nnz, double *a,
+                           double *su, double *flmass, double *al, double *ar, double *fmoc);
+
+/* bpres(p, istage) (src/bpres.f90:3-152): boundary values of p (numTotal:
+ * cells, then the boundary values in the groups' order of bnd, then the O-C
+ * values).  istage 1: every boundary value takes its owner cell's value;
+ * istage >= 2: the wall and pressure-outlet values, and the O-C values, are
+ * extrapolated linearly from the owner with dPdxi (3, numCells), the other
+ * values are left as they are.  The O-C branch reads the face centre of face
+ * number i for cut i, as the reference does (bpres.f90:140-142).  bnd as for
+ * fc_grad_gauss; owner and the face centres cover numFaces. */
+fc_status fc_bpres(fc_context *ctx, int numCells, int numTotal, int numFaces, int numInnerFaces,
+                   const int *owner, const double *xc, const double *yc, const double *zc,
+                   const double *xf, const double *yf, const double *zf, const int *bnd, int noc,
+                   int iOCFacesStart, const double *dPdxi, int istage, double *p);
+
+/* correctBoundaryConditionsVelocity (src/correctBoundaryConditionsVelocity.f90:
+ * 1-79), called by calcp after the corrections (line 183): over the outlet
+ * faces the boundary velocity takes the owner's, fmo = den (u . S), flowo =
+ * their sum in outlet order, fac = flomas / (flowo + small), then fmo and the
+ * boundary velocities are scaled by fac; over the symmetry faces the boundary
+ * velocity is the owner's minus (u . S) S (S the area vector, as written in
+ * the reference).  u, v, w have numTotal values; den numCells; fmo (nout) out. */
+fc_status fc_correct_velocity_bc(fc_context *ctx, int numCells, int numTotal, int numFaces,
+                                 int numInnerFaces, const int *owner, const double *arx,
+                                 const double *ary, const double *arz, const int *bnd, double flomas,
+                                 const double *den, double *u, double *v, double *w, double *fmo);
+
 /* adjustMassFlow (src/adjustMassFlow.f90:1-71), called by calcp between the
  * face loops and the solve (src/calcp-multiple_correction_SIMPLE.f90:107):
  * su(owner) -= fmi over the inlet faces iInletFacesStart+1..+ninl; over the
@@ -413,6 +456,32 @@ fc_status fc_sys_calcp_correct(fc_system *sys, const double *pp, const double *d
  * for the next corrector pass (calcp:112). */
 fc_status fc_sys_calcp_nonorth(fc_system *sys, const double *den, const double *apu,
                                const double *dPdxi, double *flmass);
+/* PISO's pressure assembly (fc_piso_assemble) on the resident system; a and su
+ * are written into the system and its solution vector is left as it is (PISO
+ * starts each solve from the last p, src/PISO_multiple_correction.f90:198-199).
+ * Host or device memory, as for fc_sys_calcp_assemble. */
+fc_status fc_sys_piso_assemble(fc_system *sys, const double *den, const double *apu, const double *u,
+                               const double *v, const double *w, const double *dUdxi, const double *dVdxi,
+                               const double *dWdxi, double *flmass);
+/* PISO's reference pressure (src/PISO_multiple_correction.f90:188-192) in the
+ * system's matrix: row pRefCell (1-based) zeroed, its diagonal 1, su(pRefCell)
+ * = p(pRefCell).  p (numCells) host or device memory. */
+fc_status fc_sys_pin_reference(fc_system *sys, int pRefCell, const double *p);
+/* bpres on the resident system (fc_bpres without O-C cuts): p has the system's
+ * numTotal values and is updated in place; dPdxi (3, numCells) is read from
+ * istage 2 on.  Host or device memory. */
+fc_status fc_sys_bpres(fc_system *sys, const int *bnd, int istage, const double *dPdxi, double *p);
+/* The corrector's pressure-correction gradient (src/calcp-multiple_correction_
+ * SIMPLE.f90:131-136) on the system's solution pp: for istage = 1..nipgrad,
+ * bpres(pp, istage) then the Gauss gradient with nigrad iterations into dPdxi
+ * (3, numCells, host or device), each stage extrapolating with the previous
+ * stage's gradient.  pp's boundary values are kept in the system. */
+fc_status fc_sys_grad_pp(fc_system *sys, const int *bnd, int nipgrad, int nigrad, double *dPdxi);
+/* correctBoundaryConditionsVelocity (fc_correct_velocity_bc) on the resident
+ * system's mesh: u, v, w (numTotal) updated in place, fmo (nout) written; host
+ * or device memory. */
+fc_status fc_sys_correct_velocity_bc(fc_system *sys, const int *bnd, double flomas, const double *den,
+                                     double *u, double *v, double *w, double *fmo);
 /* Upload a RHS (numCells) replacing su. */
 fc_status fc_sys_set_rhs(fc_system *sys, const double *su);
 /* Momentum solves of calcuvw (src/calcuvw.f90:414-556): the off-diagonal

@@ -808,6 +808,165 @@ int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
     return 0;
 }
 
+/* facefluxmass_piso (src/facefluxmass.f90:294-358) */
+static void flux_mass_piso(int ijp, int ijn, double xf, double yf, double zf, double arx, double ary, double arz,
+                           double lambda, const RcFields *F, double *cap, double *can, double *fluxmass)
+{
+    double fxn = lambda;
+    double fxp = 1.0 - lambda;
+    double xpn = I1(F->xc, ijn) - I1(F->xc, ijp);
+    double ypn = I1(F->yc, ijn) - I1(F->yc, ijp);
+    double zpn = I1(F->zc, ijn) - I1(F->zc, ijp);
+    double dpn = sqrt(xpn * xpn + ypn * ypn + zpn * zpn);
+    double are = sqrt(arx * arx + ary * ary + arz * arz);
+    double dene = I1(F->den, ijp) * fxp + I1(F->den, ijn) * fxn;
+    *cap = -dene * (fxp * I1(F->vol, ijp) * I1(F->apu, ijp) + fxn * I1(F->vol, ijn) * I1(F->apu, ijn)) * are / dpn;
+    *can = *cap;
+    double ui = face_value_central(ijp, ijn, xf, yf, zf, F->u, F->dudx, F);
+    double vi = face_value_central(ijp, ijn, xf, yf, zf, F->v, F->dvdx, F);
+    double wi = face_value_central(ijp, ijn, xf, yf, zf, F->w, F->dwdx, F);
+    *fluxmass = dene * (ui * arx + vi * ary + wi * arz);
+}
+
+/* PISO's pressure assembly (src/PISO_multiple_correction.f90:107-171): the face loop of
+ * orc_calcp_assemble with facefluxmass_piso, then (pref_cell > 0) the reference pressure
+ * (:188-192): row pref_cell of a zeroed, a(diag) = 1, su(pref_cell) = p(pref_cell). */
+int orc_piso_assemble(int n, int nif, const int *owner1, const int *neighbour1,
+                      const double *xc, const double *yc, const double *zc, const double *vol,
+                      const double *xf, const double *yf, const double *zf,
+                      const double *arx, const double *ary, const double *arz, const double *facint,
+                      const double *den, const double *apu, const double *u, const double *v, const double *w,
+                      const double *dudx, const double *dvdx, const double *dwdx,
+                      int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                      const int *ioffset, const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                      int pref_cell, const double *p,
+                      double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
+                      double *fmoc)
+{
+    RcFields F = {xc, yc, zc, vol, den, apu, NULL, NULL, u, v, w, NULL, dudx, dvdx, dwdx, NULL};
+    memset(a, 0, sizeof(double) * nnz);
+    memset(su, 0, sizeof(double) * n);
+    for (int i = 0; i < nif; ++i) {
+        int ijp = owner1[i], ijn = neighbour1[i];
+        double cap, can;
+        flux_mass_piso(ijp, ijn, xf[i], yf[i], zf[i], arx[i], ary[i], arz[i], facint[i], &F, &cap, &can, &flmass[i]);
+        if (cap_out) cap_out[i] = cap;
+        I1(a, ij_slot[i]) = can;
+        I1(a, ji_slot[i]) = cap;
+        int k = I1(diag, ijp);
+        I1(a, k) = I1(a, k) - can;
+        k = I1(diag, ijn);
+        I1(a, k) = I1(a, k) - cap;
+        I1(su, ijp) = I1(su, ijp) - flmass[i];
+        I1(su, ijn) = I1(su, ijn) + flmass[i];
+    }
+    for (int i = 0; i < noc; ++i) {
+        int iface = ioc_faces_start + i;
+        int ijp = ijl1[i], ijn = ijr1[i];
+        flux_mass_piso(ijp, ijn, xf[iface], yf[iface], zf[iface], arx[iface], ary[iface], arz[iface], foc[i], &F,
+                       &al[i], &ar[i], &fmoc[i]);
+        int k = I1(diag, ijp);
+        I1(a, k) = I1(a, k) - ar[i];
+        k = I1(diag, ijn);
+        I1(a, k) = I1(a, k) - al[i];
+        I1(su, ijp) = I1(su, ijp) - fmoc[i];
+        I1(su, ijn) = I1(su, ijn) + fmoc[i];
+    }
+    if (pref_cell > 0) {
+        for (int k = I1(ioffset, pref_cell); k <= I1(ioffset, pref_cell + 1) - 1; ++k) I1(a, k) = 0.0;
+        I1(a, I1(diag, pref_cell)) = 1.0;
+        I1(su, pref_cell) = I1(p, pref_cell);
+    }
+    return 0;
+}
+
+/* bpres (src/bpres.f90:3-152).  Boundary values follow the cells in the groups' order (inlet,
+ * outlet, symmetry, wall, pressure outlet; src/mesh_geometry_and_topology.f90:593-605), then
+ * the O-C values.  owner1 covers every face; face arrays are 0-based per face. */
+void orc_bpres(int n, const int *owner1, const double *xc, const double *yc, const double *zc,
+               const double *xf, const double *yf, const double *zf, const int *bnd, int noc,
+               int ioc_faces_start, const double *dpdx, int istage, double *p)
+{
+    int fstart[5], vstart[6];
+    int v = n;
+    for (int g = 0; g < 5; ++g) {
+        fstart[g] = bnd[2 * g + 1];
+        vstart[g] = v;
+        v += bnd[2 * g];
+    }
+    vstart[5] = v; /* iOCStart */
+    if (istage == 1) {
+        for (int g = 0; g < 5; ++g)
+            for (int i = 1; i <= bnd[2 * g]; ++i) {
+                int ijp = I1(owner1, fstart[g] + i);
+                I1(p, vstart[g] + i) = I1(p, ijp);
+            }
+        for (int i = 1; i <= noc; ++i) {
+            int ijp = I1(owner1, ioc_faces_start + i);
+            I1(p, vstart[5] + i) = I1(p, ijp);
+        }
+        return;
+    }
+    for (int g = 3; g < 5; ++g)   /* wall, then pressure outlet */
+        for (int i = 1; i <= bnd[2 * g]; ++i) {
+            int iface = fstart[g] + i;
+            int ijp = I1(owner1, iface);
+            double xpb = I1(xf, iface) - I1(xc, ijp);
+            double ypb = I1(yf, iface) - I1(yc, ijp);
+            double zpb = I1(zf, iface) - I1(zc, ijp);
+            I1(p, vstart[g] + i) = I1(p, ijp) + dpdx[3 * (ijp - 1)] * xpb + dpdx[3 * (ijp - 1) + 1] * ypb +
+                                   dpdx[3 * (ijp - 1) + 2] * zpb;
+        }
+    for (int i = 1; i <= noc; ++i) {
+        int iface = ioc_faces_start + i;
+        int ijp = I1(owner1, iface);
+        /* the face centre of face number i, as src/bpres.f90:140-142 index it */
+        double xpb = I1(xf, i) - I1(xc, ijp);
+        double ypb = I1(yf, i) - I1(yc, ijp);
+        double zpb = I1(zf, i) - I1(zc, ijp);
+        I1(p, vstart[5] + i) = I1(p, ijp) + dpdx[3 * (ijp - 1)] * xpb + dpdx[3 * (ijp - 1) + 1] * ypb +
+                               dpdx[3 * (ijp - 1) + 2] * zpb;
+    }
+}
+
+/* correctBoundaryConditionsVelocity (src/correctBoundaryConditionsVelocity.f90:1-79) */
+void orc_correct_velocity_bc(int n, const int *owner1, const double *arx, const double *ary, const double *arz,
+                             const int *bnd, double flomas, const double *den, double *u, double *v, double *w,
+                             double *fmo)
+{
+    int ninl = bnd[0], nout = bnd[2], nsym = bnd[4];
+    int out_faces = bnd[3], sym_faces = bnd[5];
+    int out_start = n + ninl, sym_start = n + ninl + nout;
+    double flowo = 0.0;
+    for (int i = 1; i <= nout; ++i) {
+        int iface = out_faces + i;
+        int ijp = I1(owner1, iface);
+        int ijb = out_start + i;
+        I1(u, ijb) = I1(u, ijp);
+        I1(v, ijb) = I1(v, ijp);
+        I1(w, ijb) = I1(w, ijp);
+        I1(fmo, i) = I1(den, ijp) * (I1(u, ijb) * I1(arx, iface) + I1(v, ijb) * I1(ary, iface) + I1(w, ijb) * I1(arz, iface));
+        flowo = flowo + I1(fmo, i);
+    }
+    double fac = flomas / (flowo + orc_small());
+    for (int i = 1; i <= nout; ++i) {
+        int ijb = out_start + i;
+        I1(fmo, i) = I1(fmo, i) * fac;
+        I1(u, ijb) = I1(u, ijb) * fac;
+        I1(v, ijb) = I1(v, ijb) * fac;
+        I1(w, ijb) = I1(w, ijb) * fac;
+    }
+    for (int i = 1; i <= nsym; ++i) {
+        int iface = sym_faces + i;
+        int ijp = I1(owner1, iface);
+        int ijb = sym_start + i;
+        double unmag = I1(u, ijp) * I1(arx, iface) + I1(v, ijp) * I1(ary, iface) + I1(w, ijp) * I1(arz, iface);
+        I1(u, ijb) = I1(u, ijp) - unmag * I1(arx, iface);
+        I1(v, ijb) = I1(v, ijp) - unmag * I1(ary, iface);
+        I1(w, ijb) = I1(w, ijp) - unmag * I1(arz, iface);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Momentum component rows of calcuvw (src/calcuvw.f90:417-437, and the v/w
  * twins :468-489, :522-543): the diagonal slot is reset to zero, then

@@ -89,6 +89,30 @@ int orc_calcp_assemble(int n, int nif, const int *owner1, const int *neighbour1,
                        double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
                        double *fmoc);
 
+/* PISO's pressure assembly (src/PISO_multiple_correction.f90:107-171, facefluxmass_piso
+ * src/facefluxmass.f90:294-358) and, for pref_cell > 0, its reference pressure (:188-192). */
+int orc_piso_assemble(int n, int nif, const int *owner1, const int *neighbour1,
+                      const double *xc, const double *yc, const double *zc, const double *vol,
+                      const double *xf, const double *yf, const double *zf,
+                      const double *arx, const double *ary, const double *arz, const double *facint,
+                      const double *den, const double *apu, const double *u, const double *v, const double *w,
+                      const double *dudx, const double *dvdx, const double *dwdx,
+                      int noc, int ioc_faces_start, const int *ijl1, const int *ijr1, const double *foc,
+                      const int *ioffset, const int *diag, const int *ij_slot, const int *ji_slot, int nnz,
+                      int pref_cell, const double *p,
+                      double *a, double *su, double *flmass, double *cap_out, double *al, double *ar,
+                      double *fmoc);
+
+/* bpres(p, istage) (src/bpres.f90:3-152) on p (numTotal); bnd as for orc_grad_gauss. */
+void orc_bpres(int n, const int *owner1, const double *xc, const double *yc, const double *zc,
+               const double *xf, const double *yf, const double *zf, const int *bnd, int noc,
+               int ioc_faces_start, const double *dpdx, int istage, double *p);
+
+/* correctBoundaryConditionsVelocity (src/correctBoundaryConditionsVelocity.f90:1-79). */
+void orc_correct_velocity_bc(int n, const int *owner1, const double *arx, const double *ary, const double *arz,
+                             const int *bnd, double flomas, const double *den, double *u, double *v, double *w,
+                             double *fmo);
+
 /* Gauss gradient of u (numTotal values: cells, then the boundary groups' values) as
  * src/grad_gauss.f90 computes it, nigrad iterations.  owner1 covers every face; the
  * face arrays are 0-based per face.  bnd: (count, face start) of the inlet, outlet,

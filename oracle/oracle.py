@@ -59,6 +59,10 @@ def lib():
                                            + [dp] * 5)
         _lib.orc_calcp_assemble.argtypes = ([i, i, ip, ip] + [dp] * 23 + [i, i, ip, ip, dp, ip, ip, ip, i]
                                             + [dp] * 7)
+        _lib.orc_piso_assemble.argtypes = ([i, i, ip, ip] + [dp] * 19 + [i, i, ip, ip, dp, ip, ip, ip, ip, i, i, dp]
+                                           + [dp] * 7)
+        _lib.orc_bpres.argtypes = [i, ip, dp, dp, dp, dp, dp, dp, ip, i, i, dp, i, dp]
+        _lib.orc_correct_velocity_bc.argtypes = [i, ip, dp, dp, dp, ip, d, dp, dp, dp, dp, dp]
         _lib.orc_spmv.argtypes = [i, ip, ip, dp, dp, dp]
         _lib.orc_residual.argtypes = [i, ip, ip, dp, dp, dp, i, ip, ip, dp, dp, dp]
         _lib.orc_dic_pivots.argtypes = [i, ip, ip, ip, dp, dp]
@@ -323,6 +327,62 @@ def momentum_rows(csr: CSR, a, sp, src, phi, urf):
 
 RC_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p")
 RC_GRADS = ("dudx", "dvdx", "dwdx", "dpdx")
+
+
+def piso_assemble(mesh, g: Geometry, csr: CSR, fields, oc=None, pref_cell=0, p=None):
+    """PISO's pressure matrix, source and mass fluxes (src/PISO_multiple_correction.f90:107-171
+    around facefluxmass_piso, src/facefluxmass.f90:294-358); fields: den, apu, u, v, w (n each)
+    and dudx, dvdx, dwdx (n x 3).  pref_cell (1-based, 0: none) and p: the reference pressure
+    of :188-192.  Returns a dict with a, su, flmass, cap (and al, ar, fmoc with cuts)."""
+    n, nif = mesh.n_cells, mesh.n_inner
+    own1 = _i32(mesh.owner[:nif].astype(np.int64) + 1)
+    nei1 = _i32(mesh.neighbour.astype(np.int64) + 1)
+    fl = [_f64(fields[k]) for k in ("den", "apu", "u", "v", "w")]
+    gr = [_f64(np.asarray(fields[k]).reshape(n, 3)) for k in ("dudx", "dvdx", "dwdx")]
+    noc = 0 if oc is None else len(oc["ijl"])
+    ijl = _i32(np.asarray(oc["ijl"]) + 1) if noc else np.zeros(1, np.int32)
+    ijr = _i32(np.asarray(oc["ijr"]) + 1) if noc else np.zeros(1, np.int32)
+    foc = _f64(oc["foc"]) if noc else np.zeros(1)
+    oc0 = int(oc["faces"]) if noc else 0
+    pp = _f64(p) if p is not None else np.zeros(n)
+    out = {"a": np.zeros(mesh.nnz), "su": np.zeros(n), "flmass": np.zeros(max(nif, 1)),
+           "cap": np.zeros(max(nif, 1)), "al": np.zeros(max(noc, 1)), "ar": np.zeros(max(noc, 1)),
+           "fmoc": np.zeros(max(noc, 1))}
+    lib().orc_piso_assemble(n, nif, _ip(own1), _ip(nei1), _dp(g.xc), _dp(g.yc), _dp(g.zc), _dp(g.vol),
+                            _dp(g.xf), _dp(g.yf), _dp(g.zf), _dp(g.arx), _dp(g.ary), _dp(g.arz),
+                            _dp(g.facint), *[_dp(x) for x in fl], *[_dp(x) for x in gr], noc, oc0,
+                            _ip(ijl), _ip(ijr), _dp(foc), _ip(csr.ioffset), _ip(csr.diag), _ip(csr.ij_slot),
+                            _ip(csr.ji_slot), mesh.nnz, int(pref_cell), _dp(pp),
+                            *[_dp(out[k]) for k in ("a", "su", "flmass", "cap", "al", "ar", "fmoc")])
+    out["flmass"], out["cap"] = out["flmass"][:nif], out["cap"][:nif]
+    for k in ("al", "ar", "fmoc"):
+        out[k] = out[k][:noc]
+    return out
+
+
+def bpres(mesh, g: Geometry, p, dpdx, istage, noc=0, oc_faces_start=0):
+    """bpres(p, istage) (src/bpres.f90:3-152): p (numTotal) with its boundary values set;
+    dpdx (n x 3) read from istage 2 on."""
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    out = _f64(p).copy()
+    dg = _f64(np.asarray(dpdx).reshape(mesh.n_cells, 3)) if dpdx is not None else np.zeros(3 * mesh.n_cells)
+    lib().orc_bpres(mesh.n_cells, _ip(own1), _dp(_f64(g.xc)), _dp(_f64(g.yc)), _dp(_f64(g.zc)), _dp(_f64(g.xf)),
+                    _dp(_f64(g.yf)), _dp(_f64(g.zf)), _ip(boundary_groups(mesh)), int(noc), int(oc_faces_start),
+                    _dp(dg), int(istage), _dp(out))
+    return out
+
+
+def correct_velocity_bc(mesh, g: Geometry, flomas, den, u, v, w):
+    """correctBoundaryConditionsVelocity (src/correctBoundaryConditionsVelocity.f90:1-79): returns
+    (u, v, w, fmo), u, v, w with numTotal values."""
+    bnd = boundary_groups(mesh)
+    own1 = _i32(mesh.owner.astype(np.int64) + 1)
+    outs = [_f64(x).copy() for x in (u, v, w)]
+    nout = int(bnd[2])
+    fmo = np.zeros(max(nout, 1))
+    lib().orc_correct_velocity_bc(mesh.n_cells, _ip(own1), _dp(_f64(g.arx)), _dp(_f64(g.ary)), _dp(_f64(g.arz)),
+                                  _ip(bnd), float(flomas), _dp(_f64(den)), *[_dp(x) for x in outs], _dp(fmo))
+    return outs[0], outs[1], outs[2], fmo[:nout]
 
 
 def calcp_assemble(mesh, g: Geometry, csr: CSR, fields, oc=None):

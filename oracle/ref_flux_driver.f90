@@ -16,6 +16,10 @@
 ! (a = 0 and su = 0, then per inner face in order: a(ij) = can, a(ji) = cap,
 ! a(diag(P)) -= can, a(diag(N)) -= cap, su(P) -= F, su(N) += F), and fmcor of
 ! the reference's fluxmc (non-orthogonal mass flux correction) for every face.
+! Writes piso_out.bin (stream): float64 cap(nif), flmass(nif), a(nnz), su(numCells)
+! of the same face loop around facefluxmass_piso (PISO's assembly,
+! src/PISO_multiple_correction.f90:107-148) followed by its reference pressure
+! (:188-192) at pRefCell = numCells/2 + 1.
 program ref_flux_driver
   use types
   use parameters
@@ -24,7 +28,7 @@ program ref_flux_driver
   use variables
   use faceflux_mass
   implicit none
-  integer :: un, i, k, ijp, ijn
+  integer :: un, i, k, ijp, ijn, pref
   real(dp) :: capf, canf
   real(dp), allocatable :: capv(:), canv(:), fmc(:)
 
@@ -82,5 +86,31 @@ program ref_flux_driver
   open(newunit=un, file='flux_out.bin', access='stream', form='unformatted', status='replace')
   write(un) numCells, numInnerFaces, nnz
   write(un) capv, canv, flmass, a, su, fmc
+  close(un)
+
+  a = 0.0_dp
+  su = 0.0_dp
+  do i = 1, numInnerFaces
+    ijp = owner(i)
+    ijn = neighbour(i)
+    call facefluxmass_piso(ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), facint(i), capf, canf, flmass(i))
+    capv(i) = capf
+    k = icell_jcell_csr_value_index(i)
+    a(k) = canf
+    k = jcell_icell_csr_value_index(i)
+    a(k) = capf
+    k = diag(ijp)
+    a(k) = a(k) - canf
+    k = diag(ijn)
+    a(k) = a(k) - capf
+    su(ijp) = su(ijp) - flmass(i)
+    su(ijn) = su(ijn) + flmass(i)
+  end do
+  pref = numCells / 2 + 1
+  a(ioffset(pref):ioffset(pref+1)-1) = 0.0_dp
+  a(diag(pref)) = 1.0_dp
+  su(pref) = p(pref)
+  open(newunit=un, file='piso_out.bin', access='stream', form='unformatted', status='replace')
+  write(un) capv, flmass, a, su
   close(un)
 end program ref_flux_driver

@@ -16,6 +16,8 @@ CONVDIFF_CASES = ["tet4_convdiff", "hex8_convdiff"]
 CALCP_CASES = ["hex6_calcp", "tet3_calcp"]
 GRAD_CASES = ["hex5_grad", "tet3_grad"]
 ADJUST_CASES = ["hex5_adjust", "tet3_adjust"]
+BC_CASES = ["hex5_bc", "tet3_bc"]
+PISO_FIELDS = ("den", "apu", "u", "v", "w", "dudx", "dvdx", "dwdx")
 CALCP_FIELDS = ("den", "apu", "apv", "apw", "u", "v", "w", "p", "dudx", "dvdx", "dwdx", "dpdx")
 
 

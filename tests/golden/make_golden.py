@@ -38,6 +38,7 @@ DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 GRAD_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_grad_driver")
 ADJUST_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_adjust_driver")
 FLUX_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_flux_driver")
+BC_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_bc_driver")
 REF_TESTS = "/root/reference/tests"
 
 SOLVERS = {"none": 0, "iccg": 1, "dpcg": 2, "bicgstab": 3}
@@ -179,13 +180,19 @@ def run_reference_flux(mesh, fields, fmt="native"):
         env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
         subprocess.run([FLUX_DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
         raw = open(os.path.join(d, "flux_out.bin"), "rb").read()
+        praw = open(os.path.join(d, "piso_out.bin"), "rb").read()
     hdr = np.frombuffer(raw, np.int32, 3)
     assert tuple(hdr) == (n, nif, mesh.nnz), (hdr, n, nif, mesh.nnz)
     vals = np.frombuffer(raw, np.float64, offset=12)
     assert vals.size == 4 * nif + mesh.nnz + n
     o = 3 * nif + mesh.nnz
-    return {"cap": vals[:nif].copy(), "can": vals[nif:2 * nif].copy(), "flmass": vals[2 * nif:3 * nif].copy(),
-            "a": vals[3 * nif:o].copy(), "su": vals[o:o + n].copy(), "fmcor": vals[o + n:].copy()}
+    out = {"cap": vals[:nif].copy(), "can": vals[nif:2 * nif].copy(), "flmass": vals[2 * nif:3 * nif].copy(),
+           "a": vals[3 * nif:o].copy(), "su": vals[o:o + n].copy(), "fmcor": vals[o + n:].copy()}
+    pv = np.frombuffer(praw, np.float64)
+    assert pv.size == 2 * nif + mesh.nnz + n
+    piso = {"cap": pv[:nif].copy(), "flmass": pv[nif:2 * nif].copy(), "a": pv[2 * nif:2 * nif + mesh.nnz].copy(),
+            "su": pv[2 * nif + mesh.nnz:].copy(), "pref_cell": np.array([n // 2 + 1], np.int32)}
+    return out, piso
 
 
 def calcp_fields(g, seed):
@@ -214,13 +221,13 @@ def calcp_case(name, mesh, seed):
     g = O.Geometry(*(base[k] for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
                                        "facint")))
     fields = calcp_fields(g, seed)
-    out = run_reference_flux(mesh, fields)
+    out, piso = run_reference_flux(mesh, fields)
     for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint", "ioffset", "ja",
               "diag", "ij_slot", "ji_slot"):
         out[k] = base[k]
     inputs = {k: np.asarray(v) for k, v in fields.items()}
     inputs["fmt"] = np.array("native")
-    save_case(name, mesh, inputs, {"calcp": out})
+    save_case(name, mesh, inputs, {"calcp": out, "piso": piso})
 
 
 def run_reference_grad(mesh, u, nigrad, fmt="native"):
@@ -333,6 +340,62 @@ def grad_meshes():
     return [("hex5_grad", m, 23), ("tet3_grad", t, 24)]
 
 
+def run_reference_bc(mesh, flomas, p, dpdx, den, u, v, w, fmt="native"):
+    """p after the reference's bpres(p, 1) and then bpres(p, 2), and u, v, w, fmo after its
+    correctBoundaryConditionsVelocity."""
+    n, ntot = mesh.n_cells, mesh.n_cells + mesh.n_boundary
+    with tempfile.TemporaryDirectory() as d:
+        M.write_polymesh(mesh, os.path.join(d, "polyMesh"), fmt=fmt)
+        with open(os.path.join(d, "bc_in.bin"), "wb") as f:
+            for arr in (np.array([flomas]), p, np.asarray(dpdx).reshape(n, 3), den, u, v, w):
+                f.write(np.ascontiguousarray(arr, np.float64).tobytes())
+        env = dict(os.environ, MALLOC_MMAP_THRESHOLD_="0")
+        subprocess.run([BC_DRIVER], cwd=d, capture_output=True, text=True, check=True, env=env)
+        raw = open(os.path.join(d, "bc_out.bin"), "rb").read()
+    hdr = np.frombuffer(raw, np.int32, 3)
+    assert hdr[0] == n and hdr[1] == ntot, hdr
+    nout = int(hdr[2])
+    vals = np.frombuffer(raw, np.float64, offset=12)
+    assert vals.size == 5 * ntot + nout
+    out, o = {}, 0
+    for k, sz in (("p1", ntot), ("p2", ntot), ("u", ntot), ("v", ntot), ("w", ntot), ("fmo", nout)):
+        out[k] = vals[o:o + sz].copy()
+        o += sz
+    return out
+
+
+def bc_case(name, mesh, seed):
+    """bpres (stages 1 and 2) and correctBoundaryConditionsVelocity on a mesh with every
+    boundary group: random cell and boundary values, gradients, density and velocities."""
+    n, ntot = mesh.n_cells, mesh.n_cells + mesh.n_boundary
+    base = run_reference(mesh, np.full(n, -1.0), np.zeros(ntot), np.zeros(n), np.zeros(n), solver="none")
+    rng = np.random.default_rng(seed)
+    inputs = {"flomas": np.array([0.5 + rng.random()]), "p": rng.standard_normal(ntot),
+              "dpdx": rng.standard_normal((n, 3)), "den": 1.0 + 0.2 * rng.random(ntot),
+              "u": 1.0 + 0.3 * rng.standard_normal(ntot), "v": 0.3 * rng.standard_normal(ntot),
+              "w": 0.3 * rng.standard_normal(ntot), "fmt": np.array("native")}
+    out = run_reference_bc(mesh, inputs["flomas"][0], inputs["p"], inputs["dpdx"], inputs["den"], inputs["u"],
+                           inputs["v"], inputs["w"])
+    for k in ("arx", "ary", "arz", "xf", "yf", "zf", "xc", "yc", "zc", "vol"):
+        out[k] = base[k]
+    save_case(name, mesh, inputs, {"bc": out})
+
+
+def bc_meshes():
+    m = M.hex_box(5, 4, 3, perturb=0.2, seed=45)
+    types = ["inlet", "outlet", "symmetry", "symmetry", "wall", "prOutlet"]
+    m.patches = [(t, nf, st) for t, (_, nf, st) in zip(types, m.patches)]
+    t = M.tet_box(3, grade_axis=0, grade_ratio=1.2, grade_max_aspect=10.0, permute_seed=9)
+    t.patches = [(ty, nf, st) for ty, (_, nf, st) in zip(["inlet", "outlet", "symmetry", "wall", "wall", "prOutlet"],
+                                                          t.patches)]
+    return [("hex5_bc", m, 27), ("tet3_bc", t, 28)]
+
+
+def bc_only():
+    for name, mesh, seed in bc_meshes():
+        bc_case(name, mesh, seed)
+
+
 def grad_only():
     for name, mesh, seed in grad_meshes():
         grad_case(name, mesh, seed)
@@ -377,6 +440,7 @@ def main():
     calcp_only()
     grad_only()
     adjust_only()
+    bc_only()
 
 
 def calcp_only():
@@ -394,5 +458,7 @@ if __name__ == "__main__":
         grad_only()
     elif len(sys.argv) > 1 and sys.argv[1] == "adjust":
         adjust_only()
+    elif len(sys.argv) > 1 and sys.argv[1] == "bc":
+        bc_only()
     else:
         main()

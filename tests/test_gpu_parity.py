@@ -12,8 +12,8 @@ FP64 residual.
 import numpy as np
 import pytest
 
-from conftest import (ADJUST_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, GRAD_CASES, MESH_CASES, calcp_uniform_case,
-                      load_transcript)
+from conftest import (ADJUST_CASES, BC_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, GRAD_CASES, MESH_CASES,
+                      PISO_FIELDS, calcp_uniform_case, load_transcript)
 from freecappuccino_amd import fcap
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
@@ -618,3 +618,69 @@ def test_grad_gauss_oc_cuts_match_oracle(ctx):
         out = fcap.grad_gauss(m, geo, u, nig, oc=cuts, ctx=ctx)
         for a, b, k in zip(out, ref, ("dudx", "dudy", "dudz")):
             np.testing.assert_array_equal(a, b, err_msg=f"{k} nigrad={nig}")
+
+
+# ------------------------------------------------------- PISO assembly, bpres, velocity BCs
+
+def _calcp_geo_csr(g):
+    geo = fcap.Geometry(*(g.out("calcp", k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                                      "facint")))
+    csr = fcap.CSR(*(g.out("calcp", k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")))
+    return geo, csr
+
+
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_piso_assemble_bitwise_vs_reference(ctx, golden_loader, case):
+    """PISO's face loop around facefluxmass_piso on the GPU: flmass, a and su are the reference's
+    bits (golden vectors of ref_flux_driver; the reference-pressure pin, which the host entry
+    point leaves to the caller, applied to the GPU result on the host first)."""
+    g = golden_loader(case)
+    geo, csr = _calcp_geo_csr(g)
+    out = fcap.piso_assemble(g.mesh, geo, csr, {k: g.inp(k) for k in PISO_FIELDS}, ctx=ctx)
+    np.testing.assert_array_equal(out["flmass"], g.out("piso", "flmass"))
+    a, su = out["a"].copy(), out["su"].copy()
+    r = int(g.out("piso", "pref_cell")[0]) - 1
+    a[csr.ioffset[r] - 1:csr.ioffset[r + 1] - 1] = 0.0
+    a[csr.diag[r] - 1] = 1.0
+    su[r] = g.inp("p")[r]
+    np.testing.assert_array_equal(a, g.out("piso", "a"))
+    np.testing.assert_array_equal(su, g.out("piso", "su"))
+
+
+@pytest.mark.parametrize("case", BC_CASES)
+def test_bpres_and_velocity_bc_bitwise_vs_reference(ctx, golden_loader, case):
+    """bpres (stage 1 copies owner values, stage 2 extrapolates wall and pressure-outlet values)
+    and correctBoundaryConditionsVelocity (outlet scaled to flomas in outlet order, symmetry
+    projected) on the GPU: the reference's bits on meshes with every boundary group."""
+    g = golden_loader(case)
+    m = g.mesh
+    geo = fcap.mesh_geometry(m, ctx)
+    p1 = fcap.bpres(m, geo, g.inp("p"), None, 1, ctx=ctx)
+    np.testing.assert_array_equal(p1, g.out("bc", "p1"))
+    p2 = fcap.bpres(m, geo, p1, g.inp("dpdx"), 2, ctx=ctx)
+    np.testing.assert_array_equal(p2, g.out("bc", "p2"))
+    u, v, w, fmo = fcap.correct_velocity_bc(m, geo, g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"),
+                                            g.inp("w"), ctx=ctx)
+    for k, x in (("u", u), ("v", v), ("w", w), ("fmo", fmo)):
+        np.testing.assert_array_equal(x, g.out("bc", k), err_msg=k)
+
+
+def test_bpres_oc_cuts_match_oracle(ctx, golden_loader):
+    """bpres with O-C cuts after the pressure-outlet values: stage 1 copies the owners' values,
+    stage 2 extrapolates with the face centre the reference indexes by the cut number
+    (bpres.f90:140-142); bitwise the oracle."""
+    g = golden_loader("hex5_bc")
+    m = g.mesh
+    geo = fcap.mesh_geometry(m, ctx)
+    noc, oc0 = 5, m.n_inner + 3          # five boundary faces treated as cuts
+    rng = np.random.default_rng(4)
+    p = rng.standard_normal(m.n_cells + m.n_boundary + noc)
+    dpdx = rng.standard_normal((m.n_cells, 3))
+    og = O.Geometry(*(getattr(geo, k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc",
+                                                "facint")))
+    for stage, dg in ((1, None), (2, dpdx)):
+        got = fcap.bpres(m, geo, p, dg, stage, noc=noc, oc_faces_start=oc0, ctx=ctx)
+        want = O.bpres(m, og, p, dg, stage, noc=noc, oc_faces_start=oc0)
+        np.testing.assert_array_equal(got, want, err_msg=f"stage {stage}")
+        assert not np.array_equal(got, p)
+        p = got

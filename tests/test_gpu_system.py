@@ -501,3 +501,138 @@ def test_resident_step_entry_points_reject_bad_arguments(ctx):
         np.testing.assert_array_equal(s.get("a"), ref["a"])
     finally:
         s.close()
+
+
+# ------------------------------------------- PISO, bpres, velocity BCs and the whole calcp step
+
+PISO_ORDER = ("den", "apu", "u", "v", "w", "dudx", "dvdx", "dwdx")
+
+
+@pytest.mark.parametrize("name", ["hex6_calcp", "tet3_calcp"])
+def test_piso_resident_bitwise_vs_reference(ctx, golden_loader, name):
+    """fc_sys_piso_assemble then fc_sys_pin_reference on a system built from the mesh: the
+    reference's PISO flmass, a and su bit for bit; the solution vector is left as it was."""
+    g = golden_loader(name)
+    s = fcap.System(g.mesh, ctx)
+    try:
+        x0 = np.random.default_rng(2).standard_normal(g.mesh.n_cells)
+        s.set_solution(x0)
+        fl = s.piso_assemble(*[g.inp(k) for k in PISO_ORDER])
+        np.testing.assert_array_equal(fl, g.out("piso", "flmass"))
+        s.pin_reference(int(g.out("piso", "pref_cell")[0]) - 1, g.inp("p"))
+        np.testing.assert_array_equal(s.get("a"), g.out("piso", "a"))
+        np.testing.assert_array_equal(s.get("su"), g.out("piso", "su"))
+        np.testing.assert_array_equal(s.solution(), x0)
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["hex5_bc", "tet3_bc"])
+def test_bpres_and_velocity_bc_resident_bitwise_vs_reference(ctx, golden_loader, name):
+    """fc_sys_bpres (stages 1 and 2) and fc_sys_correct_velocity_bc on a system built from the
+    mesh: the reference's boundary values, velocities and outlet fluxes bit for bit."""
+    g = golden_loader(name)
+    s = fcap.System(g.mesh, ctx)
+    try:
+        p1 = s.bpres(g.inp("p"), 1)
+        np.testing.assert_array_equal(p1, g.out("bc", "p1"))
+        p2 = s.bpres(p1, 2, g.inp("dpdx"))
+        np.testing.assert_array_equal(p2, g.out("bc", "p2"))
+        u, v, w, fmo = s.correct_velocity_bc(g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"), g.inp("w"))
+        for k, x in (("u", u), ("v", v), ("w", w), ("fmo", fmo)):
+            np.testing.assert_array_equal(x, g.out("bc", k), err_msg=k)
+    finally:
+        s.close()
+
+
+def _step_mesh():
+    m = M.hex_box(9, 8, 7, perturb=0.15, seed=61)
+    types = ["inlet", "outlet", "symmetry", "symmetry", "wall", "prOutlet"]
+    m.patches = [(t, nf, st) for t, (_, nf, st) in zip(types, m.patches)]
+    return m
+
+
+def test_calcp_step_resident_bitwise_vs_oracle(ctx):
+    """The pressure-correction step of calcp (src/calcp-multiple_correction_SIMPLE.f90:34-229)
+    end to end on a resident system, with the solver's sums in index order: gradients of U, V, W
+    and p, the Rhie-Chow assembly, adjustMassFlow, then two corrector passes of the IC(0)-PCG
+    solve, bpres + grad(pp) in two stages, the flux / velocity / pressure corrections,
+    correctBoundaryConditionsVelocity and (between the passes) the non-orthogonal corrector's
+    source.  Every field it updates -- u, v, w and p with their boundary values, the face mass
+    fluxes and the outlet fluxes -- equals the oracle's (each part pinned bitwise against the
+    reference) bit for bit.  The system's pp stays on the device throughout."""
+    m = _step_mesh()
+    n, nb = m.n_cells, m.n_boundary
+    geo = O.mesh_geometry(m)
+    csr = O.csr_build(n, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    bnd = O.boundary_groups(m)
+    ninl = int(bnd[0])
+    rng = np.random.default_rng(17)
+    u0, v0, w0 = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    fb = slice(m.n_inner, m.n_faces)
+    ub, vb, wb = W.taylor_green(geo.xf[fb], geo.yf[fb], geo.zf[fb])
+    u_t, v_t, w_t = (np.concatenate([c, b]) for c, b in ((u0 + 0.5, ub + 0.5), (v0, vb), (w0, wb)))
+    p_t = np.concatenate([np.sin(3 * geo.xc) * geo.yc + geo.zc, np.zeros(nb)])
+    den = 1.0 + 0.1 * rng.random(n + nb)
+    apu = (0.5 + rng.random(n)) / geo.vol
+    apv = (0.5 + rng.random(n)) / geo.vol
+    apw = (0.5 + rng.random(n)) / geo.vol
+    # inlet fluxes (negative: into the domain) that carry flomas, so the source sums to zero
+    flomas = 0.8
+    wts = 1.0 + 0.1 * rng.random(ninl)
+    fmi = -flomas * wts / wts.sum()
+    urf, pref, nipgrad, nigrad, npcor, sor, nsw = 0.3, 11, 2, 1, 2, 1e-8, 500
+
+    def grads(gfun):
+        return [np.stack(gfun(x), axis=1) for x in (u_t, v_t, w_t, p_t)]
+
+    # ---- the oracle
+    du, dv, dw, dp = grads(lambda x: O.grad_gauss(m, geo, x, nigrad))
+    f = {"den": den[:n], "apu": apu, "apv": apv, "apw": apw, "u": u_t[:n], "v": v_t[:n], "w": w_t[:n],
+         "p": p_t[:n], "dudx": du, "dvdx": dv, "dwdx": dw, "dpdx": dp}
+    ref = O.calcp_assemble(m, geo, csr, f)
+    su, uo, vo, wo, fmo_o = O.adjust_mass_flow(m, geo, flomas, den, u_t, v_t, w_t, ref["su"], fmi)
+    a, fl_o, po = ref["a"], ref["flmass"], p_t.copy()
+    for ipcorr in range(1, npcor + 1):
+        pp = O.iccg(csr, a, su, np.zeros(n), sor=sor, nsw=nsw).fi
+        ppt = np.concatenate([pp, np.zeros(nb)])
+        dpp = None
+        for stage in range(1, nipgrad + 1):
+            ppt = O.bpres(m, geo, ppt, dpp, stage)
+            dpp = np.stack(O.grad_gauss(m, geo, ppt, nigrad), axis=1)
+        fl_o, _, uc, vc, wc, pc = O.calcp_correct(m, csr, a, pp, dpp, geo.vol, apu, apv, apw, urf, pref, fl_o,
+                                                  uo[:n], vo[:n], wo[:n], po[:n])
+        uo[:n], vo[:n], wo[:n], po[:n] = uc, vc, wc, pc
+        uo, vo, wo, fmo_o = O.correct_velocity_bc(m, geo, flomas, den, uo, vo, wo)
+        if ipcorr != npcor:
+            su, fl_o, _, _ = O.calcp_nonorth(m, geo, den[:n], apu, dpp, fl_o)
+
+    # ---- the GPU, one resident system
+    s = fcap.System(m, ctx)
+    ctx.set_reduction_order(fcap.REDUCE_SEQUENTIAL)
+    try:
+        gu, gv, gw, gp = grads(lambda x: tuple(s.grad_gauss(x, nigrad).T))
+        for x, y in ((gu, du), (gv, dv), (gw, dw), (gp, dp)):
+            np.testing.assert_array_equal(x, y)
+        fl = s.calcp_assemble(den[:n], apu, apv, apw, u_t[:n], v_t[:n], w_t[:n], p_t[:n], gu, gv, gw, gp)
+        su_g, ug, vg, wg, fmo = fcap.adjust_mass_flow(m, fcap.Geometry(*(s.get(k) for k in (
+            "arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint"))), flomas, den, u_t, v_t, w_t,
+            s.get("su"), fmi, ctx=ctx)
+        s.set_rhs(su_g)
+        pg = p_t.copy()
+        for ipcorr in range(1, npcor + 1):
+            rc, info = s.solve(fcap.METHOD_ICCG, fcap.STOP_L1, sor, nsw)
+            assert rc == fcap.FC_OK, info
+            dpg = s.grad_pp(nipgrad, nigrad)
+            fl, uc, vc, wc, pc = s.calcp_correct(dpg, apu, apv, apw, urf, pref, fl, ug[:n], vg[:n], wg[:n], pg[:n])
+            ug[:n], vg[:n], wg[:n], pg[:n] = uc, vc, wc, pc
+            ug, vg, wg, fmo = s.correct_velocity_bc(flomas, den, ug, vg, wg)
+            if ipcorr != npcor:
+                fl = s.calcp_nonorth(den[:n], apu, dpg, fl)
+        for k, x, y in (("u", ug, uo), ("v", vg, vo), ("w", wg, wo), ("p", pg, po), ("flmass", fl, fl_o),
+                        ("fmo", fmo, fmo_o), ("dPdxi", dpg, dpp)):
+            np.testing.assert_array_equal(x, y, err_msg=k)
+        assert np.abs(fmo.sum() - flomas) < 1e-12
+    finally:
+        ctx.set_reduction_order(fcap.REDUCE_TREE)
+        s.close()

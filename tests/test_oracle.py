@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import ADJUST_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case, load_transcript
+from conftest import (ADJUST_CASES, BC_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIFF_CASES, MESH_CASES, calcp_uniform_case,
+                      load_transcript)
 from freecappuccino_amd import mesh as M
 from oracle import oracle as O
 
@@ -454,6 +455,97 @@ def test_adjust_mass_flow_bitwise_vs_reference(golden_loader, case):
     out = O.adjust_mass_flow(g.mesh, geo, *_adjust_args(g))
     for k, x in zip(ADJUST_KEYS, out):
         np.testing.assert_array_equal(x, g.out("adjust", k), err_msg=k)
+
+
+@pytest.mark.parametrize("case", CALCP_CASES)
+def test_piso_assembly_bitwise_vs_reference(golden_loader, case):
+    """facefluxmass_piso and PISO's face loop and reference-pressure pin of the reference
+    (oracle/_ref/ref_flux_driver, src/facefluxmass.f90:294-358, PISO_multiple_correction.f90:
+    107-148,188-192)."""
+    g = golden_loader(case)
+    geo, c, f = _calcp_inputs(g)
+    pref = int(g.out("piso", "pref_cell")[0])
+    out = O.piso_assemble(g.mesh, geo, c, f, pref_cell=pref, p=g.inp("p"))
+    np.testing.assert_array_equal(out["cap"], g.out("piso", "cap"))
+    np.testing.assert_array_equal(out["flmass"], g.out("piso", "flmass"))
+    np.testing.assert_array_equal(out["a"], g.out("piso", "a"))
+    np.testing.assert_array_equal(out["su"], g.out("piso", "su"))
+    # the pinned row: zero off the diagonal, 1 on it, su = p
+    r = pref - 1
+    row = slice(c.ioffset[r] - 1, c.ioffset[r + 1] - 1)
+    assert np.count_nonzero(out["a"][row]) == 1 and out["a"][c.diag[r] - 1] == 1.0
+    assert out["su"][r] == g.inp("p")[r]
+
+
+def test_piso_coefficient_closed_form():
+    """On a box of unit cubes with uniform density and 1/a_P the PISO coefficient is
+    -rho Vol apu |S| / |d| = -rho apu exactly, and a uniform velocity with zero gradients gives
+    the flux rho u . S."""
+    m, fields, vel, _ = calcp_uniform_case()
+    geo = O.mesh_geometry(m)
+    csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    n = m.n_cells
+    f = {"den": fields["den"], "apu": np.full(n, 0.8), "u": np.full(n, vel[0]), "v": np.full(n, vel[1]),
+         "w": np.full(n, vel[2])}
+    for k in ("dudx", "dvdx", "dwdx"):
+        f[k] = np.zeros((n, 3))
+    out = O.piso_assemble(m, geo, csr, f)
+    np.testing.assert_allclose(out["cap"], -fields["den"][0] * 0.8, rtol=1e-14)
+    nif = m.n_inner
+    flux = fields["den"][0] * (vel[0] * geo.arx[:nif] + vel[1] * geo.ary[:nif] + vel[2] * geo.arz[:nif])
+    np.testing.assert_allclose(out["flmass"], flux, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("case", BC_CASES)
+def test_bpres_and_velocity_bc_bitwise_vs_reference(golden_loader, case):
+    """bpres stages 1 and 2 and correctBoundaryConditionsVelocity of the reference compiled on
+    their own (oracle/_ref/ref_bc_driver), on meshes with every boundary group."""
+    g = golden_loader(case)
+    geo = _geom(g)
+    for k in ("xf", "xc", "arx"):
+        np.testing.assert_array_equal(getattr(geo, k), g.out("bc", k), err_msg=k)
+    p1 = O.bpres(g.mesh, geo, g.inp("p"), None, 1)
+    np.testing.assert_array_equal(p1, g.out("bc", "p1"))
+    p2 = O.bpres(g.mesh, geo, p1, g.inp("dpdx"), 2)
+    np.testing.assert_array_equal(p2, g.out("bc", "p2"))
+    u, v, w, fmo = O.correct_velocity_bc(g.mesh, geo, g.inp("flomas")[0], g.inp("den"), g.inp("u"), g.inp("v"),
+                                         g.inp("w"))
+    for k, x in (("u", u), ("v", v), ("w", w), ("fmo", fmo)):
+        np.testing.assert_array_equal(x, g.out("bc", k), err_msg=k)
+
+
+@pytest.mark.parametrize("case", BC_CASES)
+def test_bpres_and_velocity_bc_invariants(golden_loader, case):
+    """Stage 1 copies owner values into every boundary value; stage 2 is exact for a linear
+    field with its exact gradient on the wall and pressure-outlet faces; the outlet fluxes carry
+    flomas; a symmetry value has the owner's velocity minus (u . S) S."""
+    g = golden_loader(case)
+    m = g.mesh
+    geo = _geom(g)
+    n, nb = m.n_cells, m.n_boundary
+    bnd = O.boundary_groups(m)
+    grad = np.array([0.3, -1.2, 0.7])
+    lin = lambda x, y, z: 2.0 + grad[0] * x + grad[1] * y + grad[2] * z  # noqa: E731
+    p = np.concatenate([lin(geo.xc, geo.yc, geo.zc), np.zeros(nb)])
+    p1 = O.bpres(m, geo, p, None, 1)
+    own_of_value = np.concatenate([m.owner[bnd[2 * k + 1]:bnd[2 * k + 1] + bnd[2 * k]] for k in range(5)])
+    np.testing.assert_array_equal(p1[n:n + own_of_value.size], p[own_of_value])
+    p2 = O.bpres(m, geo, p1, np.tile(grad, (n, 1)), 2)
+    v0 = n + int(bnd[0] + bnd[2] + bnd[4])
+    faces = np.concatenate([np.arange(bnd[7], bnd[7] + bnd[6]), np.arange(bnd[9], bnd[9] + bnd[8])])
+    np.testing.assert_allclose(p2[v0:v0 + faces.size], lin(geo.xf[faces], geo.yf[faces], geo.zf[faces]),
+                               rtol=1e-13, atol=1e-13)
+    np.testing.assert_array_equal(p2[n:v0], p1[n:v0])        # inlet, outlet, symmetry untouched
+    flomas = g.inp("flomas")[0]
+    u, v, w, fmo = O.correct_velocity_bc(m, geo, flomas, g.inp("den"), g.inp("u"), g.inp("v"), g.inp("w"))
+    assert abs(fmo.sum() - flomas) < 1e-13 * flomas * max(fmo.size, 1)
+    s0, ns = int(bnd[5]), int(bnd[4])
+    own = m.owner[s0:s0 + ns]
+    S = np.stack([geo.arx[s0:s0 + ns], geo.ary[s0:s0 + ns], geo.arz[s0:s0 + ns]], axis=1)
+    U = np.stack([g.inp("u")[own], g.inp("v")[own], g.inp("w")[own]], axis=1)
+    want = U - (U * S).sum(axis=1)[:, None] * S
+    b0 = n + int(bnd[0] + bnd[2])
+    np.testing.assert_allclose(np.stack([u, v, w], axis=1)[b0:b0 + ns], want, rtol=1e-13, atol=1e-15)
 
 
 @pytest.mark.parametrize("case", ADJUST_CASES)
