@@ -41,7 +41,7 @@ SWEEP_LEVELS, SWEEP_DEPENDENT = 0, 2
 REDUCE_TREE, REDUCE_SEQUENTIAL = 0, 1
 SWEEP_DEFAULT = SWEEP_DEPENDENT
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
-           arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20, gnn_perm=21)
+           arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
 
 
 class FcError(RuntimeError):
@@ -949,8 +949,8 @@ class System:
         nf = self.mesh.n_faces
         count = {"a": nnz, "su": n, "ioffset": n + 1, "ja": nnz, "diag": n, "ij_slot": nif, "ji_slot": nif,
                  "vol": n, "xc": n, "yc": n, "zc": n, "facint": nif, "arx": nf, "ary": nf, "arz": nf,
-                 "xf": nf, "yf": nf, "zf": nf, "res": n, "pivots": n, "gnn_perm": n}[name]
-        dtype = np.int32 if name in ("ioffset", "ja", "diag", "ij_slot", "ji_slot", "gnn_perm") else np.float64
+                 "xf": nf, "yf": nf, "zf": nf, "res": n, "pivots": n}[name]
+        dtype = np.int32 if name in ("ioffset", "ja", "diag", "ij_slot", "ji_slot") else np.float64
         out = np.zeros(max(count, 1), dtype)
         self.ctx.check(self.ctx.L.fc_sys_get(self.h, ARR[name], out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out[:count]

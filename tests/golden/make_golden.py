@@ -391,6 +391,43 @@ def bc_meshes():
     return [("hex5_bc", m, 27), ("tet3_bc", t, 28)]
 
 
+def reader_case(name, mesh, seed, store_arrays):
+    """The reference's geometry, CSR structure, Laplacian (mu = -1, wall values zero) and ICCG
+    solve on a mesh read by this package's polyMesh reader, written back in the reference's
+    native layout.  store_arrays False: SHA-256 digests of the arrays' bytes instead of the
+    arrays (the pitzDaily mesh: 49,180 faces), plus the iteration count and res0."""
+    import hashlib
+
+    rng = np.random.default_rng(seed)
+    n = mesh.n_cells
+    su0 = rng.standard_normal(n)
+    inputs = {"mu": np.full(n, -1.0), "phi": np.zeros(n + mesh.n_boundary), "su0": su0, "fi0": np.zeros(n),
+              "sor": np.array([1e-8]), "nsw": np.array([2000], np.int32), "fmt": np.array("native")}
+    out = run_reference(mesh, inputs["mu"], inputs["phi"], su0, inputs["fi0"], solver="iccg", sor=1e-8, nsw=2000)
+    if store_arrays:
+        save_case(name, mesh, inputs, {"iccg": out})
+        return
+    dig = {}
+    for k, v in out.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind in "fi" and k not in ("iters", "resor", "resl_history_4d"):
+            dig[k] = np.array(hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest())
+    dig["iters"], dig["resor"] = out["iters"], out["resor"]
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **{f"in_{k}": v for k, v in inputs.items() if k != "su0"},
+                        in_seed=np.array([seed]), **{f"iccg_{k}": v for k, v in dig.items()})
+    print(f"wrote {path}  ({os.path.getsize(path)} bytes)")
+
+
+def reader_only():
+    from freecappuccino_amd.polymesh import read_polymesh
+    reader_case("cavity_reader", read_polymesh(os.path.join(HERE, "polymesh_cavity")), 51, True)
+    import tarfile
+    with tempfile.TemporaryDirectory() as d:
+        with tarfile.open(os.path.join(HERE, "polymesh_pitzDaily.tar.gz")) as t:
+            t.extractall(d, filter="data")
+        reader_case("pitzdaily_reader_digest", read_polymesh(os.path.join(d, "polyMesh")), 52, False)
+
+
 def bc_only():
     for name, mesh, seed in bc_meshes():
         bc_case(name, mesh, seed)
@@ -441,6 +478,7 @@ def main():
     grad_only()
     adjust_only()
     bc_only()
+    reader_only()
 
 
 def calcp_only():
@@ -460,5 +498,7 @@ if __name__ == "__main__":
         adjust_only()
     elif len(sys.argv) > 1 and sys.argv[1] == "bc":
         bc_only()
+    elif len(sys.argv) > 1 and sys.argv[1] == "reader":
+        reader_only()
     else:
         main()

@@ -129,3 +129,47 @@ def test_errors_are_reported(tmp_path, case):
     with pytest.raises(fcap.FcError) as ei:
         read_polymesh(d if case != "missing" else os.path.join(d, "nope"))
     assert str(ei.value)
+
+
+# ------------------------------------------- the reference on meshes this reader reads
+
+def _sha(x):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest()
+
+
+def _oracle_chain(m, z):
+    """Oracle geometry, CSR, Laplacian (mu, zero wall values, su0 seeded as the fixture's) and
+    ICCG on a reader mesh."""
+    geo = O.mesh_geometry(m)
+    csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    su0 = z["in_su0"] if "in_su0" in z.files else np.random.default_rng(int(z["in_seed"][0])).standard_normal(m.n_cells)
+    a, su = O.laplacian(m, geo, csr, z["in_mu"], z["in_phi"], su0)
+    r = O.iccg(csr, a, su, z["in_fi0"], sor=float(z["in_sor"][0]), nsw=int(z["in_nsw"][0]))
+    out = {k: getattr(geo, k) for k in ("arx", "ary", "arz", "xf", "yf", "zf", "vol", "xc", "yc", "zc", "facint")}
+    out.update({k: getattr(csr, k) for k in ("ioffset", "ja", "diag", "ij_slot", "ji_slot")})
+    out.update({"a": a, "su": su, "fi": r.fi, "res": r.res})
+    return out, r
+
+
+def test_oracle_matches_the_reference_on_the_cavity_mesh():
+    """The reference's mesh_geometry, create_CSR_matrix_from_mesh_data, laplacian and iccg
+    (oracle/_ref/ref_driver) on the cavity mesh as this reader reads it: the oracle's arrays
+    bit for bit, and its iteration count and initial residual."""
+    m = read_polymesh(CAVITY)
+    z = np.load(os.path.join(GOLDEN, "cavity_reader.npz"), allow_pickle=False)
+    out, r = _oracle_chain(m, z)
+    for k, v in out.items():
+        np.testing.assert_array_equal(v, z[f"iccg_{k}"], err_msg=k)
+    assert r.iters == int(z["iccg_iters"][0]) and r.res0 == float(z["iccg_resor"][0])
+
+
+def test_oracle_matches_the_reference_on_the_pitzdaily_mesh(pitzdaily):
+    """The same chain on pitzDaily (12,225 cells): every array's SHA-256 equals the digest of
+    the reference's array, and the iteration count and initial residual are the reference's."""
+    m = read_polymesh(pitzdaily)
+    z = np.load(os.path.join(GOLDEN, "pitzdaily_reader_digest.npz"), allow_pickle=False)
+    out, r = _oracle_chain(m, z)
+    for k, v in out.items():
+        assert _sha(v) == str(z[f"iccg_{k}"]), k
+    assert r.iters == int(z["iccg_iters"][0]) and r.res0 == float(z["iccg_resor"][0])
