@@ -274,8 +274,15 @@ extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double ome
 }
 
 extern "C" fc_status fc_set_sweep_mode(fc_context *ctx, int mode) {
-    if (!ctx || (mode != FC_SWEEP_LEVELS && mode != FC_SWEEP_DEPENDENT)) return FC_ERR_ARGUMENT;
+    if (!ctx || (mode != FC_SWEEP_LEVELS && mode != FC_SWEEP_DEPENDENT && mode != FC_SWEEP_PARTS))
+        return FC_ERR_ARGUMENT;
     ctx->sweep_mode = mode;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_sweep_waves(fc_context *ctx, int waves) {
+    if (!ctx || waves < 0) return FC_ERR_ARGUMENT;
+    ctx->part_waves = waves;
     return FC_OK;
 }
 
@@ -959,7 +966,7 @@ static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const
         FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx->stream));
         SweepCfg cfg;
         DevBuf sfs;
-        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && syncfree_sweeps(st)) {
+        if ((ctx->sweep_mode == FC_SWEEP_DEPENDENT || ctx->sweep_mode == FC_SWEEP_PARTS) && syncfree_sweeps(st)) {
             sfs.alloc(sizeof(SyncFree));
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
@@ -1003,13 +1010,20 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         // the solvers' sweeps: level-packed operands when the structure has them
         DevBuf af, df, ab, db;
         PackedFactor<double> pf;
+        const bool dep = (ctx->sweep_mode == FC_SWEEP_DEPENDENT || ctx->sweep_mode == FC_SWEEP_PARTS) &&
+                         syncfree_sweeps(st);
+        const bool parts = dep && ctx->sweep_mode == FC_SWEEP_PARTS && part_sweeps(st);
         if (packed_sweeps(st)) {
             af.alloc(sizeof(double) * (size_t)st.ell_f * n);
             ab.alloc(sizeof(double) * (size_t)st.ell_b * n);
             df.alloc(sizeof(double) * n);
             db.alloc(sizeof(double) * n);
-            pack_factor64(ctx->stream, st, da.as<double>(), dd.as<double>(), af.as<double>(), df.as<double>(),
-                          ab.as<double>(), db.as<double>());
+            if (parts)
+                pack_factor_parts64(ctx->stream, st, da.as<double>(), dd.as<double>(), af.as<double>(),
+                                    df.as<double>(), ab.as<double>(), db.as<double>());
+            else
+                pack_factor64(ctx->stream, st, da.as<double>(), dd.as<double>(), af.as<double>(), df.as<double>(),
+                              ab.as<double>(), db.as<double>());
             pf.af = af.as<double>();
             pf.df = df.as<double>();
             pf.ab = ab.as<double>();
@@ -1017,13 +1031,14 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         }
         SweepCfg cfg;
         DevBuf sfs, sft;
-        if (ctx->sweep_mode == FC_SWEEP_DEPENDENT && pf.af && syncfree_sweeps(st)) {
+        if (dep && pf.af) {
             sfs.alloc(sizeof(SyncFree));
             sft.alloc(sizeof(double) * n);
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
-            cfg.waves = ctx->num_cus;
+            cfg.waves = parts ? (ctx->part_waves > 0 ? ctx->part_waves : 4 * ctx->num_cus) : ctx->num_cus;
+            cfg.parts = parts;
         }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
                     nullptr, &cfg, pf.af ? &pf : nullptr);

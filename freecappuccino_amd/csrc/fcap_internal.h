@@ -103,6 +103,15 @@ struct Structure {
     DevBuf cnt_f, cnt_b;             // entries per packed row
     DevBuf kpos_f, kpos_b;           // CSR position of each slot
     DevBuf col_f, col_b;             // column of each slot
+    // Part-wise sweeps: the rows split into parts of kPartRows consecutive indices
+    // (in which the forward sweep reads only lower and the backward sweep only
+    // higher indices); each part's rows in the order of their levels, packed like
+    // the level order above (same widths).  int_frac: the fraction of the
+    // triangle's entries whose column lies in the row's own part.
+    bool parts = false;
+    DevBuf rows_pf, cnt_pf, kpos_pf, col_pf;
+    DevBuf rows_pb, cnt_pb, kpos_pb, col_pb;
+    double int_frac_f = 0.0, int_frac_b = 0.0;
 };
 
 struct Workspace;   // solver vectors, partial sums, device scalars
@@ -141,8 +150,9 @@ struct fc_context {
     // sums of the FP64 solvers in index order, as the reference forms them
     // (fc_set_reduction_order FC_REDUCE_SEQUENTIAL), instead of a fixed tree
     int reduce_seq = 0;
-    // triangular sweeps: FC_SWEEP_LEVELS or FC_SWEEP_DEPENDENT (fc_set_sweep_mode)
-    int sweep_mode = 2;
+    // triangular sweeps: FC_SWEEP_LEVELS, FC_SWEEP_DEPENDENT or FC_SWEEP_PARTS (fc_set_sweep_mode)
+    int sweep_mode = 3;
+    int part_waves = 0;   // persistent waves of the part-wise sweeps (0: four per CU)
     // wall-clock limit of one wait of the dependency-driven sweeps, microseconds
     // (fc_set_sweep_wait_limit; 0: the built-in 2 s)
     int sf_wait_us = 0;

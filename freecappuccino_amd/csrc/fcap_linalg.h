@@ -53,7 +53,11 @@ struct SweepCfg {
     SyncFree *sf = nullptr;
     int waves = 0;
     void *tmp = nullptr;
+    bool parts = false;   // part-wise schedule (Structure::rows_pf ...; factor packed in part order)
 };
+// part-wise sweeps (FC_SWEEP_PARTS): the structure has them, and at least half of each
+// triangle's entries lie in their row's own part (else the dependency-driven schedule)
+bool part_sweeps(const Structure &st);
 // dependency-driven sweeps when the structure is packed at a width they handle
 bool syncfree_sweeps(const Structure &st);
 
@@ -70,6 +74,11 @@ void pack_factor64(hipStream_t s, const Structure &st, const double *a, const do
                    double *ab, double *db);
 void pack_factor32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
                    float *ab, float *db);
+// the same values in the part-wise order (Structure::rows_pf / rows_pb)
+void pack_factor_parts64(hipStream_t s, const Structure &st, const double *a, const double *d, double *af,
+                         double *df, double *ab, double *db);
+void pack_factor_parts32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
+                         float *ab, float *db);
 
 // DIC (src/iccg.f90:77-83) / DILU (src/bicgstab.f90:68-79) pivots: one launch per
 // level, or with cfg->sf one dependency-driven launch (bit-identical).

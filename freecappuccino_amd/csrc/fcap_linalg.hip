@@ -602,6 +602,152 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
     }
 }
 
+// ---------------------------------------------------- part-wise sweeps
+// One launch per sweep.  The rows are split into parts of kPartRows consecutive
+// indices: the forward sweep reads only rows of lower index and the backward one
+// only rows of higher index, so a part depends only on the parts before it (after
+// it, backward) in index order, and tickets hand the parts out in that order: every
+// part a part reads has been handed to a wave that is running or has finished.
+// A persistent wave computes its part's rows in the order of their levels, 64 at
+// a time.  The part's values live in the wave's LDS: an input inside the part is
+// read there (rows of lower level come earlier in the part's order, so they are
+// computed by then or within this window), an input outside it is polled in global
+// memory like k_sweep_syncfree's (the signalling-NaN mark means "not yet").  Each
+// window first waits for its outside inputs (a global round trip), then resolves
+// its inside inputs through LDS (one LDS round trip per level).  A row's value is
+// computed exactly as in k_sweep_syncfree and stored both to LDS and, with one
+// coherent store, to global memory.  Waits are bounded in wall-clock time as there.
+template <class T, int W, bool BWD>
+__global__ __launch_bounds__(64) void k_sweep_parts(int n, const int *__restrict__ rows, const int *__restrict__ cntp,
+                                                     const int *__restrict__ col, const T *__restrict__ av,
+                                                     const T *__restrict__ dv, const T *__restrict__ in, T small,
+                                                     T *out, SyncFree *sf, const Scalars *__restrict__ sc) {
+    using U = typename SfBits<T>::U;
+    __shared__ U lv[kPartRows];
+    if (sc && sc->done) return;
+    const int lane = threadIdx.x;
+    const unsigned nparts = (unsigned)((n + kPartRows - 1) / kPartRows);
+    const unsigned long long limit = sf_wait_ticks(sf, sf->rate_khz);
+    bool quit = false;
+    while (!quit) {
+        unsigned t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = __shfl(t, 0);
+        if (t >= nparts) break;
+        const int part = BWD ? (int)(nparts - 1u - t) : (int)t;
+        const int r0 = part * kPartRows, r1 = min(n, r0 + kPartRows);
+        for (int q = lane; q < kPartRows; q += 64) lv[q] = SfBits<T>::kMark;
+        __syncthreads();
+        for (int q0 = r0; q0 < r1; q0 += 64) {
+            const int q = q0 + lane;
+            bool done = q >= r1;
+            int i = 0, c = 0;
+            int cj[W];
+            T ak[W], wk[W];
+            T acc = T(0), di = T(0);
+            unsigned ext = 0, ready = 0;
+            if (!done) {
+                i = rows[q];
+                c = cntp[q];
+                di = dv[q];
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    cj[k] = col[(size_t)k * n + q];
+                    ak[k] = av[(size_t)k * n + q];
+                    wk[k] = T(0);
+                    const bool inside = BWD ? cj[k] < r1 : cj[k] >= r0;
+                    if (k < c && !inside) ext |= 1u << k;
+                }
+                acc = BWD ? in[i] / (di + small) : in[i];
+            }
+            // inputs outside the part
+            int spins = 0;
+            const unsigned long long t0 = wall_clock64();
+            for (;;) {
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    if (((ext & ~ready) >> k) & 1u) {
+                        const U bits = sf_load_bits(out + cj[k]);
+                        if (bits != SfBits<T>::kMark) {
+                            T v;
+                            __builtin_memcpy(&v, &bits, sizeof(U));
+                            wk[k] = v;
+                            ready |= 1u << k;
+                        }
+                    }
+                }
+                if (__all((ready & ext) == ext)) break;
+                ++spins;
+                bool give_up = false;
+                if ((spins & 63) == 0)
+                    give_up = wall_clock64() - t0 > limit ||
+                              __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (__any(give_up)) {
+                    if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    quit = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (quit) break;
+            // inputs inside the part, through LDS
+            spins = 0;
+            for (;;) {
+                if (!done) {
+                    bool all = true;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        if (k < c && !((ready >> k) & 1u)) {
+                            const U bits = lv[cj[k] - r0];
+                            if (bits != SfBits<T>::kMark) {
+                                T v;
+                                __builtin_memcpy(&v, &bits, sizeof(U));
+                                wk[k] = v;
+                                ready |= 1u << k;
+                            } else {
+                                all = false;
+                            }
+                        }
+                    }
+                    if (all) {
+                        T r = acc;
+#pragma unroll
+                        for (int k = 0; k < W; ++k)
+                            if (k < c) r = r - ak[k] * wk[k];
+                        const T v = r * di;
+                        U vb;
+                        __builtin_memcpy(&vb, &v, sizeof(U));
+                        lv[i - r0] = vb;
+                        sf_store(out + i, v);
+                        done = true;
+                    }
+                }
+                __syncthreads();
+                if (__all(done)) break;
+                ++spins;
+                bool give_up = false;
+                if ((spins & 1023) == 0)
+                    give_up = wall_clock64() - t0 > limit ||
+                              __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                if (__any(give_up)) {
+                    if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    quit = true;
+                    break;
+                }
+            }
+            if (quit) break;
+        }
+        __syncthreads();   // the next part re-marks the LDS copy
+    }
+    if (lane == 0) {
+        const unsigned f = __hip_atomic_fetch_add(&sf->fin, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (f == gridDim.x - 1) {
+            __hip_atomic_store(&sf->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sf->fin, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // The pivot recurrences of k_dic_level / k_dilu_level with the same schedule:
 // d_i = 1 / (a_ii - sum over the strict lower row, in CSR order, of a_k^2 d_j
 // (DIC) or (a_k d_j) a_kT (DILU)); W bounds the strict lower row (the packed
@@ -726,6 +872,47 @@ bool syncfree_sweeps(const Structure &st) {
     return packed_sweeps(st) && ok(st.ell_f) && ok(st.ell_b);
 }
 
+bool part_sweeps(const Structure &st) {
+    return syncfree_sweeps(st) && st.parts && st.int_frac_f >= 0.5 && st.int_frac_b >= 0.5;
+}
+
+template <class T, int WF, int WB>
+void lu_parts_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                const Scalars *sc, const SweepCfg &cfg) {
+    const int n = st.n;
+    T *tmp = static_cast<T *>(cfg.tmp);
+    const int nparts = (n + kPartRows - 1) / kPartRows;
+    const int waves = std::max(1, std::min(cfg.waves, nparts));
+    k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, tmp, sc);
+    k_sweep_parts<T, WF, false><<<waves, 64, 0, s>>>(n, st.rows_pf.as<int>(), st.cnt_pf.as<int>(), st.col_pf.as<int>(),
+                                                       pf.af, pf.df, rhs, small, tmp, cfg.sf, sc);
+    k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, w, sc);
+    k_sweep_parts<T, WB, true><<<waves, 64, 0, s>>>(n, st.rows_pb.as<int>(), st.cnt_pb.as<int>(), st.col_pb.as<int>(),
+                                                      pf.ab, pf.db, tmp, small, w, cfg.sf, sc);
+}
+
+template <class T, int WF>
+void lu_parts_b(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+                const Scalars *sc, const SweepCfg &cfg) {
+    switch (st.ell_b) {
+    case 3: lu_parts_w<T, WF, 3>(s, st, pf, small, rhs, w, sc, cfg); break;
+    case 4: lu_parts_w<T, WF, 4>(s, st, pf, small, rhs, w, sc, cfg); break;
+    default: lu_parts_w<T, WF, 8>(s, st, pf, small, rhs, w, sc, cfg); break;
+    }
+}
+
+template <class T>
+void lu_parts(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
+              const Scalars *sc, const SweepCfg &cfg) {
+    FC_REQUIRE(part_sweeps(st) && cfg.tmp && cfg.sf, FC_ERR_ARGUMENT, "part-wise sweeps not set up");
+    switch (st.ell_f) {
+    case 3: lu_parts_b<T, 3>(s, st, pf, small, rhs, w, sc, cfg); break;
+    case 4: lu_parts_b<T, 4>(s, st, pf, small, rhs, w, sc, cfg); break;
+    default: lu_parts_b<T, 8>(s, st, pf, small, rhs, w, sc, cfg); break;
+    }
+    FC_HIP_CHECK(hipGetLastError());
+}
+
 template <class T, int WF, int WB>
 void lu_syncfree_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T small, const T *rhs, T *w,
                    const Scalars *sc, const SweepCfg &cfg) {
@@ -766,6 +953,10 @@ void lu_syncfree(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, 
 template <class T>
 void lu_launch_t(hipStream_t s, const Structure &st, const T *a, const T *d, T small, const T *rhs, T *w,
                  const Scalars *sc, const SweepCfg *cfg, const PackedFactor<T> *pf) {
+    if (cfg && cfg->sf && cfg->parts && pf && pf->af) {
+        lu_parts<T>(s, st, *pf, small, rhs, w, sc, *cfg);
+        return;
+    }
     if (cfg && cfg->sf && pf && pf->af) {
         lu_syncfree<T>(s, st, *pf, small, rhs, w, sc, *cfg);
         return;
@@ -807,6 +998,27 @@ void pack_factor64(hipStream_t s, const Structure &st, const double *a, const do
 void pack_factor32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
                    float *ab, float *db) {
     pack_factor_t<float>(s, st, a, d, af, df, ab, db);
+}
+
+template <class T>
+void pack_factor_parts_t(hipStream_t s, const Structure &st, const T *a, const T *d, T *af, T *df, T *ab, T *db) {
+    FC_REQUIRE(st.parts && packed_sweeps(st), FC_ERR_ARGUMENT, "structure has no part-wise sweeps");
+    const int n = st.n;
+    k_pack_factor<T><<<grid_for(n, 256), 256, 0, s>>>(n, st.ell_f, st.rows_pf.as<int>(), st.cnt_pf.as<int>(),
+                                                      st.kpos_pf.as<int>(), a, d, af, df);
+    k_pack_factor<T><<<grid_for(n, 256), 256, 0, s>>>(n, st.ell_b, st.rows_pb.as<int>(), st.cnt_pb.as<int>(),
+                                                      st.kpos_pb.as<int>(), a, d, ab, db);
+    FC_HIP_CHECK(hipGetLastError());
+}
+
+void pack_factor_parts64(hipStream_t s, const Structure &st, const double *a, const double *d, double *af,
+                         double *df, double *ab, double *db) {
+    pack_factor_parts_t<double>(s, st, a, d, af, df, ab, db);
+}
+
+void pack_factor_parts32(hipStream_t s, const Structure &st, const float *a, const float *d, float *af, float *df,
+                         float *ab, float *db) {
+    pack_factor_parts_t<float>(s, st, a, d, af, df, ab, db);
 }
 
 // ----------------------------------------------------------- reductions

@@ -17,6 +17,7 @@
 #include <cstring>
 
 #include "fcap_internal.h"
+#include "fcap_kernels.h"
 
 namespace fcap {
 
@@ -617,6 +618,48 @@ static void pack_triangle(fc_context *ctx, Structure &st, bool upper) {
     width = ell;
 }
 
+// The part-wise order of one triangle: rows in level order, stably grouped by part
+// (index / kPartRows), so that each part's rows keep their level order; packed
+// with the level-order ELL's width.  Also counts the entries whose column lies in
+// the row's own part.
+static void pack_parts(fc_context *ctx, Structure &st, bool upper) {
+    hipStream_t s = ctx->stream;
+    const int n = st.n, ell = upper ? st.ell_b : st.ell_f;
+    DevBuf &rows_p = upper ? st.rows_pb : st.rows_pf;
+    DevBuf &cnt = upper ? st.cnt_pb : st.cnt_pf;
+    DevBuf &kpos = upper ? st.kpos_pb : st.kpos_pf;
+    DevBuf &col = upper ? st.col_pb : st.col_pf;
+    std::vector<int> lev((size_t)n), order((size_t)n);
+    d2h(lev.data(), (upper ? st.rows_b : st.rows_f).get(), sizeof(int) * (size_t)n, s);
+    const int nparts = (n + kPartRows - 1) / kPartRows;
+    std::vector<int> start((size_t)nparts + 1, 0);
+    for (int p = 0; p < n; ++p) start[lev[p] / kPartRows + 1] += 1;
+    for (int t = 0; t < nparts; ++t) start[t + 1] += start[t];
+    for (int p = 0; p < n; ++p) order[start[lev[p] / kPartRows]++] = lev[p];
+    rows_p.alloc(sizeof(int) * (size_t)n);
+    h2d(rows_p.get(), order.data(), sizeof(int) * (size_t)n, s);
+    cnt.alloc(sizeof(int) * (size_t)n);
+    kpos.alloc(sizeof(int) * (size_t)ell * n);
+    col.alloc(sizeof(int) * (size_t)ell * n);
+    k_tri_ell<<<grid_for(n), kBlock, 0, s>>>(rows_p.as<int>(), st.ioffset.as<int>(), st.diag.as<int>(),
+                                             st.ja.as<int>(), n, ell, upper, cnt.as<int>(), kpos.as<int>(),
+                                             col.as<int>());
+    FC_HIP_CHECK(hipGetLastError());
+    // entries inside their row's part
+    std::vector<int> c((size_t)n), cl((size_t)ell * n);
+    d2h(c.data(), cnt.get(), sizeof(int) * (size_t)n, s);
+    d2h(cl.data(), col.get(), sizeof(int) * (size_t)ell * n, s);
+    long long inside = 0, total = 0;
+    for (int p = 0; p < n; ++p) {
+        const int part = order[p] / kPartRows;
+        for (int k = 0; k < c[p]; ++k) {
+            inside += cl[(size_t)k * n + p] / kPartRows == part;
+            total += 1;
+        }
+    }
+    (upper ? st.int_frac_b : st.int_frac_f) = total ? (double)inside / (double)total : 0.0;
+}
+
 void analyse_levels(fc_context *ctx, Structure &st) {
     levels_one(ctx, st, false, st.rows_f, st.lvl_f, st.nlev_f, st.max_level_f);
     levels_one(ctx, st, true, st.rows_b, st.lvl_b, st.nlev_b, st.max_level_b);
@@ -626,6 +669,12 @@ void analyse_levels(fc_context *ctx, Structure &st) {
     h2d(st.lvl_b_d.get(), st.lvl_b.data(), sizeof(int) * st.lvl_b.size(), ctx->stream);
     pack_triangle(ctx, st, false);
     pack_triangle(ctx, st, true);
+    st.parts = false;
+    if (st.ell_f > 0 && st.ell_b > 0) {
+        pack_parts(ctx, st, false);
+        pack_parts(ctx, st, true);
+        st.parts = true;
+    }
 }
 
 void build_tpos(fc_context *ctx, Structure &st) {

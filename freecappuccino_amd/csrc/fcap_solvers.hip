@@ -570,11 +570,16 @@ static Ctx make_ctx(fc_context *ctx, int n) {
 }
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
-    if (c.ctx->sweep_mode == FC_SWEEP_DEPENDENT && syncfree_sweeps(st)) {
+    const int mode = c.ctx->sweep_mode;
+    if ((mode == FC_SWEEP_DEPENDENT || mode == FC_SWEEP_PARTS) && syncfree_sweeps(st)) {
         syncfree_reset(c.ctx, c.s, c.ws->sf.as<SyncFree>());
         c.sw.sf = c.ws->sf.as<SyncFree>();
         c.sw.tmp = c.ws->sf_tmp.get();
         c.sw.waves = c.ctx->num_cus;
+        if (mode == FC_SWEEP_PARTS && part_sweeps(st)) {
+            c.sw.parts = true;
+            c.sw.waves = c.ctx->part_waves > 0 ? c.ctx->part_waves : 4 * c.ctx->num_cus;
+        }
     }
 }
 
@@ -601,10 +606,17 @@ static PackedFactor<T> pack_for_solve(Ctx &c, const Structure &st, const T *a, c
     ab.ensure(sizeof(T) * n * (size_t)st.ell_b);
     df.ensure(sizeof(T) * n);
     db.ensure(sizeof(T) * n);
-    if constexpr (f32)
-        pack_factor32(c.s, st, a, d, af.as<float>(), df.as<float>(), ab.as<float>(), db.as<float>());
-    else
-        pack_factor64(c.s, st, a, d, af.as<double>(), df.as<double>(), ab.as<double>(), db.as<double>());
+    if constexpr (f32) {
+        if (c.sw.parts)
+            pack_factor_parts32(c.s, st, a, d, af.as<float>(), df.as<float>(), ab.as<float>(), db.as<float>());
+        else
+            pack_factor32(c.s, st, a, d, af.as<float>(), df.as<float>(), ab.as<float>(), db.as<float>());
+    } else {
+        if (c.sw.parts)
+            pack_factor_parts64(c.s, st, a, d, af.as<double>(), df.as<double>(), ab.as<double>(), db.as<double>());
+        else
+            pack_factor64(c.s, st, a, d, af.as<double>(), df.as<double>(), ab.as<double>(), db.as<double>());
+    }
     pf.af = af.as<T>();
     pf.df = df.as<T>();
     pf.ab = ab.as<T>();

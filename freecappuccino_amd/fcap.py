@@ -37,9 +37,9 @@ VARIANT_SRC, VARIANT_TESTS = 0, 1
 METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
 METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
-SWEEP_LEVELS, SWEEP_DEPENDENT = 0, 2
+SWEEP_LEVELS, SWEEP_DEPENDENT, SWEEP_PARTS = 0, 2, 3
 REDUCE_TREE, REDUCE_SEQUENTIAL = 0, 1
-SWEEP_DEFAULT = SWEEP_DEPENDENT
+SWEEP_DEFAULT = SWEEP_PARTS
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
 
@@ -81,6 +81,7 @@ def library() -> C.CDLL:
         "fc_set_sweep_mode": ([vp, i], i), "fc_set_fcg_smoothing": ([vp, i, d], i),
         "fc_set_fcg_stagnation": ([vp, i, d], i),
         "fc_set_sweep_wait_limit": ([vp, i], i), "fc_set_graphs": ([vp, i], i),
+        "fc_set_sweep_waves": ([vp, i], i),
         "fc_set_reduction_order": ([vp, i], i),
         "fc_dpcg_noalloc": ([vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, i, si], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
@@ -231,9 +232,13 @@ class Context:
         self.check(self.L.fc_set_fcg_stagnation(self.h, int(window), float(factor)))
 
     def set_sweep_mode(self, mode: int):
-        """Schedule of the triangular sweeps: SWEEP_LEVELS or SWEEP_DEPENDENT
+        """Schedule of the triangular sweeps: SWEEP_LEVELS, SWEEP_DEPENDENT or SWEEP_PARTS
         (bit-identical results)."""
         self.check(self.L.fc_set_sweep_mode(self.h, int(mode)))
+
+    def set_sweep_waves(self, waves: int = 0):
+        """Persistent waves of the part-wise sweeps (0: four per compute unit)."""
+        self.check(self.L.fc_set_sweep_waves(self.h, int(waves)))
 
     def set_sweep_wait_limit(self, microseconds: int = 0):
         """Wall-clock limit of one wait of the dependency-driven sweeps (0: the built-in 2 s)."""

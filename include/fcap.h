@@ -365,12 +365,22 @@ fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
  * operations in the same order, so results are bit-identical.
  *   FC_SWEEP_LEVELS    one launch per level;
  *   FC_SWEEP_DEPENDENT one launch per sweep, each row waiting only for the rows
- *                      it reads (default; packed structures of width 3, 4 or 8,
- *                      other structures keep the level launches).  A wait that
- *                      runs past its time limit fails the call with FC_ERR_HIP.
+ *                      it reads (packed structures of width 3, 4 or 8; other
+ *                      structures keep the level launches);
+ *   FC_SWEEP_PARTS     (default) one launch per sweep over parts of 2,048
+ *                      consecutive rows, one persistent wave per part at a time:
+ *                      inputs inside the part are read from the wave's LDS, only
+ *                      inputs from other parts wait on global memory.  Used when
+ *                      at least half of each triangle's entries lie inside their
+ *                      row's part (meshes numbered with locality, as the
+ *                      reference's are), else FC_SWEEP_DEPENDENT.
+ * A wait that runs past its time limit fails the call with FC_ERR_HIP.
  * No counterpart in the reference. */
-enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_DEPENDENT = 2 };
+enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_DEPENDENT = 2, FC_SWEEP_PARTS = 3 };
 fc_status fc_set_sweep_mode(fc_context *ctx, int mode);
+/* Persistent waves of the part-wise sweeps (0: four per compute unit).  No
+ * counterpart in the reference. */
+fc_status fc_set_sweep_waves(fc_context *ctx, int waves);
 /* Wall-clock limit of one wait of the dependency-driven sweeps, in
  * microseconds (0: the built-in 2 s).  A wait that runs past it ends the
  * sweep and the call returns FC_ERR_HIP.  No counterpart in the reference. */

@@ -16,6 +16,7 @@ from conftest import (ADJUST_CASES, BC_CASES, CALCP_CASES, CALCP_FIELDS, CONVDIF
                       PISO_FIELDS, calcp_uniform_case, load_transcript)
 from freecappuccino_amd import fcap
 from freecappuccino_amd import mesh as M
+from freecappuccino_amd import workloads as W
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -109,8 +110,9 @@ def test_spmv_pivots_sweeps_bitwise(ctx, golden_loader, case):
 
 @pytest.mark.parametrize("case", MESH_CASES + CONVDIFF_CASES)
 def test_sweeps_dependent_schedule_bitwise(ctx, golden_loader, case):
-    """Both schedules of the pivots and sweeps, one launch per level and one launch per sweep with
-    each row waiting for the rows it reads, give the oracle's DIC and DILU values bit for bit."""
+    """Every schedule of the pivots and sweeps -- one launch per level, one launch per sweep with
+    each row waiting for the rows it reads, and the part-wise sweeps -- gives the oracle's DIC and
+    DILU values bit for bit."""
     g = golden_loader(case)
     csr = _csr_of(g)
     oc = O.CSR(csr.ioffset, csr.ja, csr.diag, csr.ij_slot, csr.ji_slot)
@@ -120,7 +122,7 @@ def test_sweeps_dependent_schedule_bitwise(ctx, golden_loader, case):
     r = np.random.default_rng(2).standard_normal(csr.diag.shape[0])
     ref = O.lu_apply(oc, a, d_ref, r)
     try:
-        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT):
+        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT, fcap.SWEEP_PARTS):
             ctx.set_sweep_mode(mode)
             d = fcap.dilu_pivots(csr, a, ctx) if dilu else fcap.dic_pivots(csr, a, ctx)
             np.testing.assert_array_equal(d, d_ref, err_msg=f"pivots, mode {mode}")
@@ -684,3 +686,33 @@ def test_bpres_oc_cuts_match_oracle(ctx, golden_loader):
         np.testing.assert_array_equal(got, want, err_msg=f"stage {stage}")
         assert not np.array_equal(got, p)
         p = got
+
+
+@pytest.mark.parametrize("mesh", ["hex24", "hex20_graded", "tet10"])
+def test_part_sweeps_bitwise_on_multi_part_meshes(ctx, mesh):
+    """Meshes of several parts of 2,048 rows (hex 24^3: 7 parts; graded 20^3: 4; Kuhn tets 10^3:
+    3): the part-wise sweeps (inputs inside a part through LDS, across parts through global
+    memory) give the DIC and DILU sweeps of the oracle bit for bit, as the other schedules do."""
+    m = {"hex24": lambda: M.hex_box(24, perturb=0.2, seed=3),
+         "hex20_graded": lambda: M.hex_box(20, perturb=0.2, seed=4, grade_axis=0, grade_ratio=1.2,
+                                           grade_max_aspect=50.0),
+         "tet10": lambda: M.tet_box(10, grade_axis=1, grade_ratio=1.1)}[mesh]()
+    geo = O.mesh_geometry(m)
+    oc = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+    csr = fcap.CSR(oc.ioffset, oc.ja, oc.diag, oc.ij_slot, oc.ji_slot)
+    a, _ = O.laplacian(m, geo, oc, np.full(m.n_cells, -1.0), np.zeros(m.n_cells + m.n_boundary),
+                       np.zeros(m.n_cells))
+    u, v, w = W.taylor_green(geo.xc, geo.yc, geo.zc)
+    ac = O.convdiff(m, geo, oc, np.full(m.n_cells, 1e-3), O.face_flux_central(m, geo, u, v, w), 0.7)
+    r = np.random.default_rng(5).standard_normal(m.n_cells)
+    try:
+        for mat, dilu in ((a, False), (ac, True)):
+            d_ref = O.dilu_pivots(oc, mat) if dilu else O.dic_pivots(oc, mat)
+            ref = O.lu_apply(oc, mat, d_ref, r)
+            for mode in (fcap.SWEEP_PARTS, fcap.SWEEP_DEPENDENT):
+                ctx.set_sweep_mode(mode)
+                d = fcap.dilu_pivots(csr, mat, ctx) if dilu else fcap.dic_pivots(csr, mat, ctx)
+                np.testing.assert_array_equal(d, d_ref, err_msg=f"pivots, mode {mode}")
+                np.testing.assert_array_equal(fcap.lu_apply(csr, mat, d, r, ctx), ref, err_msg=f"mode {mode}")
+    finally:
+        ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)

@@ -113,7 +113,7 @@ def test_dependent_sweeps_give_identical_solves(case, ctx, method):
     sys_.assemble_laplacian(np.full(m.n_cells, -1.0), None, _rhs(geo, 8, 13))
     out = []
     try:
-        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT):
+        for mode in (fcap.SWEEP_LEVELS, fcap.SWEEP_DEPENDENT, fcap.SWEEP_PARTS):
             ctx.set_sweep_mode(mode)
             sys_.set_solution(np.zeros(m.n_cells))
             rc, info = sys_.solve(method, fcap.STOP_L2, 1e-8, 5000)
@@ -121,8 +121,9 @@ def test_dependent_sweeps_give_identical_solves(case, ctx, method):
             out.append((info["iters"], info["outer"], sys_.solution()))
     finally:
         ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
-    assert out[0][:2] == out[1][:2]
-    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for k in (1, 2):
+        assert out[0][:2] == out[k][:2]
+        np.testing.assert_array_equal(out[0][2], out[k][2])
 
 
 def test_dependent_wait_limit_fails_the_solve_cleanly():
