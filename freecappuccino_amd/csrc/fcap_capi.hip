@@ -281,7 +281,7 @@ extern "C" fc_status fc_set_sweep_mode(fc_context *ctx, int mode) {
 }
 
 extern "C" fc_status fc_set_sweep_waves(fc_context *ctx, int waves) {
-    if (!ctx || waves < 0) return FC_ERR_ARGUMENT;
+    if (!ctx || waves < 0 || waves > 65536) return FC_ERR_ARGUMENT;
     ctx->part_waves = waves;
     return FC_OK;
 }
@@ -1037,7 +1037,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
-            cfg.waves = parts ? (ctx->part_waves > 0 ? ctx->part_waves : 4 * ctx->num_cus) : ctx->num_cus;
+            cfg.waves = parts ? part_wave_count(ctx) : ctx->num_cus;
             cfg.parts = parts;
         }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),

@@ -107,10 +107,12 @@ struct Structure {
     // (in which the forward sweep reads only lower and the backward sweep only
     // higher indices); each part's rows in the order of their levels, packed like
     // the level order above (same widths).  int_frac: the fraction of the
-    // triangle's entries whose column lies in the row's own part.
+    // triangle's entries whose column lies in the row's own part.  meta: per
+    // packed row, its entry count | its step within its 64-row window << 8 | the
+    // window's number of steps << 16 (the static schedule of the part sweeps).
     bool parts = false;
-    DevBuf rows_pf, cnt_pf, kpos_pf, col_pf;
-    DevBuf rows_pb, cnt_pb, kpos_pb, col_pb;
+    DevBuf rows_pf, cnt_pf, kpos_pf, col_pf, meta_pf;
+    DevBuf rows_pb, cnt_pb, kpos_pb, col_pb, meta_pb;
     double int_frac_f = 0.0, int_frac_b = 0.0;
 };
 
@@ -152,7 +154,7 @@ struct fc_context {
     int reduce_seq = 0;
     // triangular sweeps: FC_SWEEP_LEVELS, FC_SWEEP_DEPENDENT or FC_SWEEP_PARTS (fc_set_sweep_mode)
     int sweep_mode = 3;
-    int part_waves = 0;   // persistent waves of the part-wise sweeps (0: four per CU)
+    int part_waves = 0;   // persistent waves of the part-wise sweeps (0: two per CU)
     // wall-clock limit of one wait of the dependency-driven sweeps, microseconds
     // (fc_set_sweep_wait_limit; 0: the built-in 2 s)
     int sf_wait_us = 0;
@@ -174,6 +176,13 @@ struct fc_context {
 };
 
 namespace fcap {
+
+// persistent waves of the part-wise sweeps: the context's setting, else two per
+// CU (192^3 box, FP64 IC(0)-PCG: 320 waves 1.35 s, 512 1.16 s, 640 1.16 s,
+// 1024 1.25 s; more waves than that only add polling traffic)
+inline int part_wave_count(const fc_context *ctx) {
+    return ctx->part_waves > 0 ? ctx->part_waves : 2 * ctx->num_cus;
+}
 
 // ---------------------------------------------------------------- launchers
 // Structure analysis (fcap_mesh.hip)

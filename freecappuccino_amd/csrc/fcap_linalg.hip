@@ -608,17 +608,151 @@ __global__ __launch_bounds__(64) void k_sweep_syncfree(int n, const int *__restr
 // only rows of higher index, so a part depends only on the parts before it (after
 // it, backward) in index order, and tickets hand the parts out in that order: every
 // part a part reads has been handed to a wave that is running or has finished.
-// A persistent wave computes its part's rows in the order of their levels, 64 at
-// a time.  The part's values live in the wave's LDS: an input inside the part is
-// read there (rows of lower level come earlier in the part's order, so they are
-// computed by then or within this window), an input outside it is polled in global
-// memory like k_sweep_syncfree's (the signalling-NaN mark means "not yet").  Each
-// window first waits for its outside inputs (a global round trip), then resolves
-// its inside inputs through LDS (one LDS round trip per level).  A row's value is
-// computed exactly as in k_sweep_syncfree and stored both to LDS and, with one
-// coherent store, to global memory.  Waits are bounded in wall-clock time as there.
+// A persistent wave computes its part 64 packed rows (a window) at a time, the
+// rows in the order of their levels.  The part's values live in the wave's LDS.
+// A window first waits until every input it reads outside the part is in global
+// memory (polled like k_sweep_syncfree's: the signalling-NaN mark means "not yet"),
+// then computes its rows in the static schedule of Structure::meta_*: at step s the
+// rows of step s, whose inputs inside the part are in LDS by then (earlier steps or
+// earlier windows).  Only this wave touches its LDS and a wave's LDS operations
+// complete in order, so the steps need no barrier.  A row's value is computed
+// exactly as in k_sweep_syncfree and published with one coherent store at the end
+// of its window.
+//
+// The loads run two windows ahead: a window's operands are loaded two windows
+// before it, its right-hand side and the first poll of its outside inputs one
+// window before; three register sets rotate through the unrolled window loop, and
+// every lane issues the same loads and stores on every path (the padding lanes of
+// a part's last window repeat its last row: same loads, same value, same store),
+// so the waits on the vector-memory counter only cover loads issued a window ago.
+// Waits are bounded in wall-clock time as in k_sweep_syncfree.
+template <class T, int W>
+struct PartRow {
+    int i, meta;
+    int cj[W];
+    T ak[W];
+    T di;
+};
+template <class T, int W>
+struct PartWin {
+    PartRow<T, W> o;
+    T rhs;
+    typename SfBits<T>::U pb[W];
+};
+
+template <class T, int W>
+__device__ __forceinline__ void part_row_load(PartRow<T, W> &o, int q, int last, int n, const int *__restrict__ rows,
+                                              const int *__restrict__ meta, const int *__restrict__ col,
+                                              const T *__restrict__ av, const T *__restrict__ dv) {
+    q = min(q, last);
+    o.i = rows[q];
+    o.meta = meta[q];
+    o.di = dv[q];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        o.cj[k] = col[(size_t)k * n + q];
+        o.ak[k] = av[(size_t)k * n + q];
+    }
+}
+
+template <bool BWD>
+__device__ __forceinline__ bool part_inside(int j, int r0, int r1) {
+    return BWD ? j < r1 : j >= r0;
+}
+
+// the right-hand side and the first poll of the row's inputs outside the part:
+// every slot issues a load, of the row's own output where the slot is inside the
+// part or padding, so that every lane issues the same loads (the address is
+// blended arithmetically: a select between two operand fields would keep the
+// register sets in scratch memory)
 template <class T, int W, bool BWD>
-__global__ __launch_bounds__(64) void k_sweep_parts(int n, const int *__restrict__ rows, const int *__restrict__ cntp,
+__device__ __forceinline__ void part_inputs(PartWin<T, W> &S, int r0, int r1, const T *__restrict__ in,
+                                            const T *out) {
+    S.rhs = in[S.o.i];
+    const int c = S.o.meta & 255;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int j = S.o.cj[k];
+        const int own = -(int)(k >= c || part_inside<BWD>(j, r0, r1));   // all ones or zero
+        S.pb[k] = sf_load_bits(out + (j + ((S.o.i - j) & own)));
+    }
+}
+
+// one window: S0 its operands and inputs, S1 the next window (operands loaded, its
+// inputs issued here), N2 the window after (operands issued here); false: gave up
+template <class T, int W, bool BWD>
+__device__ __forceinline__ bool part_window(PartWin<T, W> &S0, PartWin<T, W> &S1, PartRow<T, W> &N2, int q0, int r0,
+                                            int r1, int n, typename SfBits<T>::U *lv, const int *__restrict__ rows,
+                                            const int *__restrict__ meta, const int *__restrict__ col,
+                                            const T *__restrict__ av, const T *__restrict__ dv,
+                                            const T *__restrict__ in, T small, T *out, SyncFree *sf,
+                                            unsigned long long limit) {
+    using U = typename SfBits<T>::U;
+    const int lane = threadIdx.x;
+    const int c = S0.o.meta & 255, step = (S0.o.meta >> 8) & 255;
+    const int nst = __shfl((S0.o.meta >> 16) & 255, 0);
+    T wk[W];
+    unsigned need = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        wk[k] = __builtin_bit_cast(T, S0.pb[k]);
+        if (k < c && !part_inside<BWD>(S0.o.cj[k], r0, r1) && S0.pb[k] == SfBits<T>::kMark) need |= 1u << k;
+    }
+    if (__any(need != 0)) {
+        const unsigned long long t0 = wall_clock64();
+        do {
+            __builtin_amdgcn_s_sleep(1);
+            U r[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) r[k] = ((need >> k) & 1u) ? sf_load_bits(out + S0.o.cj[k]) : SfBits<T>::kMark;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (((need >> k) & 1u) && r[k] != SfBits<T>::kMark) {
+                    wk[k] = __builtin_bit_cast(T, r[k]);
+                    need &= ~(1u << k);
+                }
+            }
+            const bool give_up = wall_clock64() - t0 > limit ||
+                                 __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+            if (__any(give_up)) {
+                if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        } while (__any(need != 0));
+    }
+    const int last = r1 - 1;
+    __builtin_amdgcn_sched_barrier(0);
+    part_inputs<T, W, BWD>(S1, r0, r1, in, out);
+    __builtin_amdgcn_sched_barrier(0);
+    part_row_load<T, W>(N2, q0 + 128 + lane, last, n, rows, meta, col, av, dv);
+    __builtin_amdgcn_sched_barrier(0);
+    const T acc = BWD ? S0.rhs / (S0.o.di + small) : S0.rhs;
+    T v = T(0);
+    for (int st = 0; st < nst; ++st) {
+        if (step == st) {
+            U b[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                const bool ins = k < c && part_inside<BWD>(S0.o.cj[k], r0, r1);
+                b[k] = lv[ins ? S0.o.cj[k] - r0 : 0];
+            }
+            T r = acc;
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                T x = wk[k];
+                if (k < c && part_inside<BWD>(S0.o.cj[k], r0, r1)) x = __builtin_bit_cast(T, b[k]);
+                if (k < c) r = r - S0.o.ak[k] * x;
+            }
+            v = r * S0.o.di;
+            lv[S0.o.i - r0] = __builtin_bit_cast(U, v);
+        }
+    }
+    sf_store(out + S0.o.i, v);
+    return true;
+}
+
+template <class T, int W, bool BWD>
+__global__ __launch_bounds__(64) void k_sweep_parts(int n, const int *__restrict__ rows, const int *__restrict__ meta,
                                                      const int *__restrict__ col, const T *__restrict__ av,
                                                      const T *__restrict__ dv, const T *__restrict__ in, T small,
                                                      T *out, SyncFree *sf, const Scalars *__restrict__ sc) {
@@ -628,116 +762,38 @@ __global__ __launch_bounds__(64) void k_sweep_parts(int n, const int *__restrict
     const int lane = threadIdx.x;
     const unsigned nparts = (unsigned)((n + kPartRows - 1) / kPartRows);
     const unsigned long long limit = sf_wait_ticks(sf, sf->rate_khz);
-    bool quit = false;
-    while (!quit) {
+    for (;;) {
         unsigned t = 0;
         if (lane == 0) t = __hip_atomic_fetch_add(&sf->ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         t = __shfl(t, 0);
         if (t >= nparts) break;
         const int part = BWD ? (int)(nparts - 1u - t) : (int)t;
         const int r0 = part * kPartRows, r1 = min(n, r0 + kPartRows);
-        for (int q = lane; q < kPartRows; q += 64) lv[q] = SfBits<T>::kMark;
-        __syncthreads();
-        for (int q0 = r0; q0 < r1; q0 += 64) {
-            const int q = q0 + lane;
-            bool done = q >= r1;
-            int i = 0, c = 0;
-            int cj[W];
-            T ak[W], wk[W];
-            T acc = T(0), di = T(0);
-            unsigned ext = 0, ready = 0;
-            if (!done) {
-                i = rows[q];
-                c = cntp[q];
-                di = dv[q];
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    cj[k] = col[(size_t)k * n + q];
-                    ak[k] = av[(size_t)k * n + q];
-                    wk[k] = T(0);
-                    const bool inside = BWD ? cj[k] < r1 : cj[k] >= r0;
-                    if (k < c && !inside) ext |= 1u << k;
-                }
-                acc = BWD ? in[i] / (di + small) : in[i];
-            }
-            // inputs outside the part
-            int spins = 0;
-            const unsigned long long t0 = wall_clock64();
-            for (;;) {
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    if (((ext & ~ready) >> k) & 1u) {
-                        const U bits = sf_load_bits(out + cj[k]);
-                        if (bits != SfBits<T>::kMark) {
-                            T v;
-                            __builtin_memcpy(&v, &bits, sizeof(U));
-                            wk[k] = v;
-                            ready |= 1u << k;
-                        }
-                    }
-                }
-                if (__all((ready & ext) == ext)) break;
-                ++spins;
-                bool give_up = false;
-                if ((spins & 63) == 0)
-                    give_up = wall_clock64() - t0 > limit ||
-                              __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-                if (__any(give_up)) {
-                    if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    quit = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            if (quit) break;
-            // inputs inside the part, through LDS
-            spins = 0;
-            for (;;) {
-                if (!done) {
-                    bool all = true;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        if (k < c && !((ready >> k) & 1u)) {
-                            const U bits = lv[cj[k] - r0];
-                            if (bits != SfBits<T>::kMark) {
-                                T v;
-                                __builtin_memcpy(&v, &bits, sizeof(U));
-                                wk[k] = v;
-                                ready |= 1u << k;
-                            } else {
-                                all = false;
-                            }
-                        }
-                    }
-                    if (all) {
-                        T r = acc;
-#pragma unroll
-                        for (int k = 0; k < W; ++k)
-                            if (k < c) r = r - ak[k] * wk[k];
-                        const T v = r * di;
-                        U vb;
-                        __builtin_memcpy(&vb, &v, sizeof(U));
-                        lv[i - r0] = vb;
-                        sf_store(out + i, v);
-                        done = true;
-                    }
-                }
-                __syncthreads();
-                if (__all(done)) break;
-                ++spins;
-                bool give_up = false;
-                if ((spins & 1023) == 0)
-                    give_up = wall_clock64() - t0 > limit ||
-                              __hip_atomic_load(&sf->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-                if (__any(give_up)) {
-                    if (lane == 0) __hip_atomic_store(&sf->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    quit = true;
-                    break;
-                }
-            }
-            if (quit) break;
+        // the same issue order as inside the loop (inputs of A before the operands of
+        // B), so that the loop head's waits on its two entries nearly agree
+        PartWin<T, W> A, B, C;
+        part_row_load<T, W>(A.o, r0 + lane, r1 - 1, n, rows, meta, col, av, dv);
+        __builtin_amdgcn_sched_barrier(0);
+        part_inputs<T, W, BWD>(A, r0, r1, in, out);
+        __builtin_amdgcn_sched_barrier(0);
+        part_row_load<T, W>(B.o, r0 + 64 + lane, r1 - 1, n, rows, meta, col, av, dv);
+        __builtin_amdgcn_sched_barrier(0);
+        bool ok = true;
+        for (int q0 = r0;;) {
+            if (!(ok = part_window<T, W, BWD>(A, B, C.o, q0, r0, r1, n, lv, rows, meta, col, av, dv, in, small, out,
+                                               sf, limit)))
+                break;
+            if ((q0 += 64) >= r1) break;
+            if (!(ok = part_window<T, W, BWD>(B, C, A.o, q0, r0, r1, n, lv, rows, meta, col, av, dv, in, small, out,
+                                               sf, limit)))
+                break;
+            if ((q0 += 64) >= r1) break;
+            if (!(ok = part_window<T, W, BWD>(C, A, B.o, q0, r0, r1, n, lv, rows, meta, col, av, dv, in, small, out,
+                                               sf, limit)))
+                break;
+            if ((q0 += 64) >= r1) break;
         }
-        __syncthreads();   // the next part re-marks the LDS copy
+        if (!ok) break;
     }
     if (lane == 0) {
         const unsigned f = __hip_atomic_fetch_add(&sf->fin, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -884,10 +940,10 @@ void lu_parts_w(hipStream_t s, const Structure &st, const PackedFactor<T> &pf, T
     const int nparts = (n + kPartRows - 1) / kPartRows;
     const int waves = std::max(1, std::min(cfg.waves, nparts));
     k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, tmp, sc);
-    k_sweep_parts<T, WF, false><<<waves, 64, 0, s>>>(n, st.rows_pf.as<int>(), st.cnt_pf.as<int>(), st.col_pf.as<int>(),
+    k_sweep_parts<T, WF, false><<<waves, 64, 0, s>>>(n, st.rows_pf.as<int>(), st.meta_pf.as<int>(), st.col_pf.as<int>(),
                                                        pf.af, pf.df, rhs, small, tmp, cfg.sf, sc);
     k_sf_mark<T><<<grid_for(n, 256), 256, 0, s>>>(n, w, sc);
-    k_sweep_parts<T, WB, true><<<waves, 64, 0, s>>>(n, st.rows_pb.as<int>(), st.cnt_pb.as<int>(), st.col_pb.as<int>(),
+    k_sweep_parts<T, WB, true><<<waves, 64, 0, s>>>(n, st.rows_pb.as<int>(), st.meta_pb.as<int>(), st.col_pb.as<int>(),
                                                       pf.ab, pf.db, tmp, small, w, cfg.sf, sc);
 }
 

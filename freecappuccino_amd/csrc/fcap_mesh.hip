@@ -621,7 +621,11 @@ static void pack_triangle(fc_context *ctx, Structure &st, bool upper) {
 // The part-wise order of one triangle: rows in level order, stably grouped by part
 // (index / kPartRows), so that each part's rows keep their level order; packed
 // with the level-order ELL's width.  Also counts the entries whose column lies in
-// the row's own part.
+// the row's own part, and gives each row its step in the static schedule of its
+// window (the 64 consecutive packed rows of its part it is computed with): one
+// more than the latest step of the rows it reads inside that window, 0 if it reads
+// none there.  A window's rows only read rows of lower level, which come earlier in
+// the part's order, so a step only reads values of earlier steps or earlier windows.
 static void pack_parts(fc_context *ctx, Structure &st, bool upper) {
     hipStream_t s = ctx->stream;
     const int n = st.n, ell = upper ? st.ell_b : st.ell_f;
@@ -629,6 +633,7 @@ static void pack_parts(fc_context *ctx, Structure &st, bool upper) {
     DevBuf &cnt = upper ? st.cnt_pb : st.cnt_pf;
     DevBuf &kpos = upper ? st.kpos_pb : st.kpos_pf;
     DevBuf &col = upper ? st.col_pb : st.col_pf;
+    DevBuf &meta = upper ? st.meta_pb : st.meta_pf;
     std::vector<int> lev((size_t)n), order((size_t)n);
     d2h(lev.data(), (upper ? st.rows_b : st.rows_f).get(), sizeof(int) * (size_t)n, s);
     const int nparts = (n + kPartRows - 1) / kPartRows;
@@ -658,6 +663,29 @@ static void pack_parts(fc_context *ctx, Structure &st, bool upper) {
         }
     }
     (upper ? st.int_frac_b : st.int_frac_f) = total ? (double)inside / (double)total : 0.0;
+    // static schedule: meta = count | step << 8 | steps of the window << 16
+    std::vector<int> pos((size_t)n), step((size_t)n, 0), m((size_t)n);
+    for (int p = 0; p < n; ++p) pos[order[p]] = p;
+    for (int w0 = 0; w0 < n; w0 += 64) {
+        const int part = w0 / kPartRows;
+        const int w1 = std::min({n, w0 + 64, (part + 1) * kPartRows});
+        int top = 0;
+        for (int p = w0; p < w1; ++p) {
+            int sp = 0;
+            for (int k = 0; k < c[p]; ++k) {
+                const int j = cl[(size_t)k * n + p];
+                if (j / kPartRows != part) continue;
+                const int pj = pos[j];
+                FC_REQUIRE(pj < p, FC_ERR_STRUCTURE, "part order: an input after its row");
+                if (pj >= w0) sp = std::max(sp, step[pj] + 1);
+            }
+            step[p] = sp;
+            top = std::max(top, sp);
+        }
+        for (int p = w0; p < w1; ++p) m[p] = c[p] | step[p] << 8 | (top + 1) << 16;
+    }
+    meta.alloc(sizeof(int) * (size_t)n);
+    h2d(meta.get(), m.data(), sizeof(int) * (size_t)n, s);
 }
 
 void analyse_levels(fc_context *ctx, Structure &st) {

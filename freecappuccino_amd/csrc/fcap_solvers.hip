@@ -578,7 +578,7 @@ static void setup_sweeps(Ctx &c, const Structure &st) {
         c.sw.waves = c.ctx->num_cus;
         if (mode == FC_SWEEP_PARTS && part_sweeps(st)) {
             c.sw.parts = true;
-            c.sw.waves = c.ctx->part_waves > 0 ? c.ctx->part_waves : 4 * c.ctx->num_cus;
+            c.sw.waves = part_wave_count(c.ctx);
         }
     }
 }

@@ -237,7 +237,7 @@ class Context:
         self.check(self.L.fc_set_sweep_mode(self.h, int(mode)))
 
     def set_sweep_waves(self, waves: int = 0):
-        """Persistent waves of the part-wise sweeps (0: four per compute unit)."""
+        """Persistent waves of the part-wise sweeps (0: two per compute unit)."""
         self.check(self.L.fc_set_sweep_waves(self.h, int(waves)))
 
     def set_sweep_wait_limit(self, microseconds: int = 0):

@@ -368,7 +368,8 @@ fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
  *                      it reads (packed structures of width 3, 4 or 8; other
  *                      structures keep the level launches);
  *   FC_SWEEP_PARTS     (default) one launch per sweep over parts of 2,048
- *                      consecutive rows, one persistent wave per part at a time:
+ *                      consecutive rows, one persistent wave per part at a time,
+ *                      64 rows per window in a schedule fixed at analysis:
  *                      inputs inside the part are read from the wave's LDS, only
  *                      inputs from other parts wait on global memory.  Used when
  *                      at least half of each triangle's entries lie inside their
@@ -378,8 +379,8 @@ fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
  * No counterpart in the reference. */
 enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_DEPENDENT = 2, FC_SWEEP_PARTS = 3 };
 fc_status fc_set_sweep_mode(fc_context *ctx, int mode);
-/* Persistent waves of the part-wise sweeps (0: four per compute unit).  No
- * counterpart in the reference. */
+/* Persistent waves of the part-wise sweeps (0: two per compute unit; at most
+ * 65,536).  No counterpart in the reference. */
 fc_status fc_set_sweep_waves(fc_context *ctx, int waves);
 /* Wall-clock limit of one wait of the dependency-driven sweeps, in
  * microseconds (0: the built-in 2 s).  A wait that runs past it ends the

@@ -586,14 +586,17 @@ def test_step_entry_points_reject_bad_arguments(ctx, golden_loader):
     for bad in ((-1, 0.8), (9, 0.8), (3, 0.0), (3, 2.0)):
         with pytest.raises(fcap.FcError):
             ctx.set_fcg_smoothing(*bad)
-    with pytest.raises(fcap.FcError):
-        ctx.set_sweep_mode(3)
+    for bad_mode in (1, 4, -1):   # 1 was a schedule this library no longer has
+        with pytest.raises(fcap.FcError):
+            ctx.set_sweep_mode(bad_mode)
+    for bad_waves in (-1, 1 << 20):
+        with pytest.raises(fcap.FcError):
+            ctx.set_sweep_waves(bad_waves)
     ctx.set_fcg_smoothing()
+    ctx.set_sweep_waves(0)
     ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
 
 
-# ---------------------------------------------------------------- Gauss gradient
-# ---------------------------------------------------------------- Gauss gradient
 # ---------------------------------------------------------------- Gauss gradient
 
 @pytest.mark.parametrize("case", GRAD_CASES)
