@@ -966,11 +966,11 @@ static fc_status pivots_common(fc_context *ctx, bool dilu, int n, int nnz, const
         FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), ctx->stream));
         SweepCfg cfg;
         DevBuf sfs;
-        if ((ctx->sweep_mode == FC_SWEEP_DEPENDENT || ctx->sweep_mode == FC_SWEEP_PARTS) && syncfree_sweeps(st)) {
+        if (choose_sweeps(ctx, st).dep) {
             sfs.alloc(sizeof(SyncFree));
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
-            cfg.waves = ctx->num_cus;
+            cfg.waves = syncfree_wave_count(ctx, st);
         }
         pivots_launch(ctx->stream, st, da.as<double>(), dd.as<double>(), dilu, err.as<int>(), &cfg);
         if (cfg.sf) {
@@ -1010,9 +1010,8 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
         // the solvers' sweeps: level-packed operands when the structure has them
         DevBuf af, df, ab, db;
         PackedFactor<double> pf;
-        const bool dep = (ctx->sweep_mode == FC_SWEEP_DEPENDENT || ctx->sweep_mode == FC_SWEEP_PARTS) &&
-                         syncfree_sweeps(st);
-        const bool parts = dep && ctx->sweep_mode == FC_SWEEP_PARTS && part_sweeps(st);
+        const SweepChoice ch = choose_sweeps(ctx, st);
+        const bool dep = ch.dep, parts = ch.parts;
         if (packed_sweeps(st)) {
             af.alloc(sizeof(double) * (size_t)st.ell_f * n);
             ab.alloc(sizeof(double) * (size_t)st.ell_b * n);
@@ -1037,7 +1036,7 @@ extern "C" fc_status fc_lu_apply(fc_context *ctx, int n, int nnz, const int *iof
             syncfree_reset(ctx, ctx->stream, sfs.as<SyncFree>());
             cfg.sf = sfs.as<SyncFree>();
             cfg.tmp = sft.get();
-            cfg.waves = parts ? part_wave_count(ctx) : ctx->num_cus;
+            cfg.waves = parts ? part_wave_count(ctx) : syncfree_wave_count(ctx, st);
             cfg.parts = parts;
         }
         lu_launch64(ctx->stream, st, da.as<double>(), dd.as<double>(), small, dr.as<double>(), dz.as<double>(),
@@ -1718,8 +1717,75 @@ extern "C" fc_status fc_sys_sizes(fc_system *sys, int *n, int *nif, int *nnz, in
 }
 
 extern "C" double fc_sys_last_ms(fc_system *sys, int stage) {
-    if (!sys || stage < 0 || stage > 2) return -1.0;
+    if (!sys || stage < 0 || stage > 3) return -1.0;
     return sys->last_ms[stage];
+}
+
+extern "C" double fc_sys_sweep_bench(fc_system *sys, int dilu, int reps) {
+    if (!sys || reps <= 0 || (dilu != 0 && dilu != 1)) return -1.0;
+    fc_context *ctx = sys->ctx;
+    double out = -1.0;
+    fc_status s = guarded(ctx, [&]() {
+        Structure &st = sys->st;
+        ensure_levels(ctx, st);
+        if (dilu) build_tpos(ctx, st);
+        const int n = st.n;
+        hipStream_t hs = ctx->stream;
+        const double *a = sys->a.as<double>();
+        const SweepChoice ch = choose_sweeps(ctx, st);
+        const bool dep = ch.dep, parts = ch.parts;
+        DevBuf d(sizeof(double) * n), z(sizeof(double) * n), err(sizeof(int)), sfs, sft;
+        FC_HIP_CHECK(hipMemsetAsync(err.get(), 0, sizeof(int), hs));
+        SweepCfg cfg;
+        if (dep) {
+            sfs.alloc(sizeof(SyncFree));
+            sft.alloc(sizeof(double) * n);
+            syncfree_reset(ctx, hs, sfs.as<SyncFree>());
+            cfg.sf = sfs.as<SyncFree>();
+            cfg.tmp = sft.get();
+            cfg.waves = syncfree_wave_count(ctx, st);   // the pivots' dependency-driven launch
+        }
+        pivots_launch(hs, st, a, d.as<double>(), dilu != 0, err.as<int>(), dep ? &cfg : nullptr);
+        int herr = 0;
+        d2h(&herr, err.get(), sizeof(int), hs);
+        FC_REQUIRE(!herr, FC_ERR_ZERO_PIVOT, "sweep bench: a pivot is zero or not finite");
+        cfg.waves = parts ? part_wave_count(ctx) : syncfree_wave_count(ctx, st);
+        cfg.parts = parts;
+        DevBuf af, df, ab, db;
+        PackedFactor<double> pf;
+        if (packed_sweeps(st)) {
+            af.alloc(sizeof(double) * (size_t)st.ell_f * n);
+            ab.alloc(sizeof(double) * (size_t)st.ell_b * n);
+            df.alloc(sizeof(double) * n);
+            db.alloc(sizeof(double) * n);
+            if (parts)
+                pack_factor_parts64(hs, st, a, d.as<double>(), af.as<double>(), df.as<double>(), ab.as<double>(),
+                                    db.as<double>());
+            else
+                pack_factor64(hs, st, a, d.as<double>(), af.as<double>(), df.as<double>(), ab.as<double>(),
+                              db.as<double>());
+            pf = {af.as<double>(), df.as<double>(), ab.as<double>(), db.as<double>()};
+        }
+        const SweepCfg *pc = dep && pf.af ? &cfg : nullptr;
+        const double small = ctx->variant == FC_VARIANT_TESTS ? kSmallTests : kSmallSrc;
+        lu_launch64(hs, st, a, d.as<double>(), small, sys->su.as<double>(), z.as<double>(), nullptr, pc,
+                    pf.af ? &pf : nullptr);   // warm-up
+        ctx->timer.start(hs);
+        for (int r = 0; r < reps; ++r)
+            lu_launch64(hs, st, a, d.as<double>(), small, sys->su.as<double>(), z.as<double>(), nullptr, pc,
+                        pf.af ? &pf : nullptr);
+        FC_HIP_CHECK(hipGetLastError());
+        const double ms = ctx->timer.stop_ms(hs);
+        if (pc) {
+            SyncFree h;
+            d2h(&h, sfs.get(), sizeof(SyncFree), hs);
+            FC_REQUIRE(!h.abort, FC_ERR_HIP, "dependency-driven sweep: a wait exceeded its time limit");
+        }
+        out = ms / reps;
+        sys->last_ms[3] = out;
+        return FC_OK;
+    });
+    return s == FC_OK ? out : -1.0;
 }
 
 extern "C" double fc_sys_spmv_bench(fc_system *sys, int precision, int reps) {

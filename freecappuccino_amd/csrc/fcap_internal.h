@@ -2,6 +2,8 @@
 // kernel launchers shared between translation units.
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -154,7 +156,7 @@ struct fc_context {
     int reduce_seq = 0;
     // triangular sweeps: FC_SWEEP_LEVELS, FC_SWEEP_DEPENDENT or FC_SWEEP_PARTS (fc_set_sweep_mode)
     int sweep_mode = 3;
-    int part_waves = 0;   // persistent waves of the part-wise sweeps (0: two per CU)
+    int part_waves = 0;   // persistent waves of the dependency-driven sweeps (0: automatic)
     // wall-clock limit of one wait of the dependency-driven sweeps, microseconds
     // (fc_set_sweep_wait_limit; 0: the built-in 2 s)
     int sf_wait_us = 0;
@@ -182,6 +184,19 @@ namespace fcap {
 // 1024 1.25 s; more waves than that only add polling traffic)
 inline int part_wave_count(const fc_context *ctx) {
     return ctx->part_waves > 0 ? ctx->part_waves : 2 * ctx->num_cus;
+}
+
+// persistent waves of the row-wise dependency-driven sweeps and pivots: one 64-row
+// block per wave for the rows of an average level, at least one wave per CU and at
+// most sixteen.  A deep chain (192^3 box in natural order: 574 levels of 12K rows)
+// then runs with one wave per CU -- more waves only add polling -- and a shallow one
+// (96^3 permuted Kuhn tets: 16 levels of 330K rows) with enough waves to keep its
+// gathers in flight.
+inline int syncfree_wave_count(const fc_context *ctx, const Structure &st) {
+    if (ctx->part_waves > 0) return ctx->part_waves;
+    const long long nlev = std::max(1, std::max(st.nlev_f, st.nlev_b));
+    const long long blocks = ((long long)st.n / nlev + 63) / 64;
+    return (int)std::min<long long>(16LL * ctx->num_cus, std::max<long long>(ctx->num_cus, blocks));
 }
 
 // ---------------------------------------------------------------- launchers

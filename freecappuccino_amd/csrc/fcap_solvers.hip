@@ -570,13 +570,13 @@ static Ctx make_ctx(fc_context *ctx, int n) {
 }
 
 static void setup_sweeps(Ctx &c, const Structure &st) {
-    const int mode = c.ctx->sweep_mode;
-    if ((mode == FC_SWEEP_DEPENDENT || mode == FC_SWEEP_PARTS) && syncfree_sweeps(st)) {
+    const SweepChoice ch = choose_sweeps(c.ctx, st);
+    if (ch.dep) {
         syncfree_reset(c.ctx, c.s, c.ws->sf.as<SyncFree>());
         c.sw.sf = c.ws->sf.as<SyncFree>();
         c.sw.tmp = c.ws->sf_tmp.get();
-        c.sw.waves = c.ctx->num_cus;
-        if (mode == FC_SWEEP_PARTS && part_sweeps(st)) {
+        c.sw.waves = syncfree_wave_count(c.ctx, st);
+        if (ch.parts) {
             c.sw.parts = true;
             c.sw.waves = part_wave_count(c.ctx);
         }

@@ -27,7 +27,7 @@ struct fc_system {
     fcap::DevBuf pp_total, vbc_fac;
     fcap::GradScratch grad_scratch;
     bool su0_zero = true;
-    double last_ms[3] = {0, 0, 0};
+    double last_ms[4] = {0, 0, 0, 0};
     // bumped by every assembly; the network's graph data is rebuilt when stale
     long assembly_version = 0;
     fcap::GnnGraph graph;

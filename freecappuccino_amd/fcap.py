@@ -131,6 +131,7 @@ def library() -> C.CDLL:
         "fc_sys_sizes": ([vp, ip, ip, ip, ip, ip], i),
         "fc_sys_last_ms": ([vp, i], d),
         "fc_sys_spmv_bench": ([vp, i, i], d),
+        "fc_sys_sweep_bench": ([vp, i, i], d),
         "fc_copy_bench": ([vp, C.c_size_t, i], d),
         "fc_gnn_num_params": ([], i),
         "fc_gnn_create": ([vp, C.POINTER(vp), fp], i),
@@ -237,7 +238,7 @@ class Context:
         self.check(self.L.fc_set_sweep_mode(self.h, int(mode)))
 
     def set_sweep_waves(self, waves: int = 0):
-        """Persistent waves of the part-wise sweeps (0: two per compute unit)."""
+        """Persistent waves of the dependency-driven sweeps and pivots (0: automatic)."""
         self.check(self.L.fc_set_sweep_waves(self.h, int(waves)))
 
     def set_sweep_wait_limit(self, microseconds: int = 0):
@@ -970,6 +971,13 @@ class System:
 
     def last_ms(self, stage):
         return self.ctx.L.fc_sys_last_ms(self.h, stage)
+
+    def sweep_bench(self, dilu=False, reps=10):
+        """Average device ms of one FP64 IC(0) (or DILU) forward + backward sweep pair."""
+        ms = self.ctx.L.fc_sys_sweep_bench(self.h, 1 if dilu else 0, int(reps))
+        if ms < 0:
+            raise FcError(-2, self.ctx.L.fc_last_error(self.ctx.h).decode())
+        return ms
 
     def spmv_bench(self, precision=64, reps=20):
         ms = self.ctx.L.fc_sys_spmv_bench(self.h, precision, reps)

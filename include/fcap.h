@@ -374,13 +374,17 @@ fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
  *                      inputs from other parts wait on global memory.  Used when
  *                      at least half of each triangle's entries lie inside their
  *                      row's part (meshes numbered with locality, as the
- *                      reference's are), else FC_SWEEP_DEPENDENT.
+ *                      reference's are); else FC_SWEEP_DEPENDENT where levels
+ *                      are narrow (fewer than 65,536 rows on average) and
+ *                      FC_SWEEP_LEVELS where they are wide.
  * A wait that runs past its time limit fails the call with FC_ERR_HIP.
  * No counterpart in the reference. */
 enum { FC_SWEEP_LEVELS = 0, FC_SWEEP_DEPENDENT = 2, FC_SWEEP_PARTS = 3 };
 fc_status fc_set_sweep_mode(fc_context *ctx, int mode);
-/* Persistent waves of the part-wise sweeps (0: two per compute unit; at most
- * 65,536).  No counterpart in the reference. */
+/* Persistent waves of the dependency-driven sweeps and pivots (0: automatic --
+ * two per compute unit for the part-wise sweeps, for the row-wise ones one
+ * 64-row block per wave for an average level's rows, within 1 to 16 per compute
+ * unit; at most 65,536).  No counterpart in the reference. */
 fc_status fc_set_sweep_waves(fc_context *ctx, int waves);
 /* Wall-clock limit of one wait of the dependency-driven sweeps, in
  * microseconds (0: the built-in 2 s).  A wait that runs past it ends the
@@ -536,11 +540,19 @@ void *fc_sys_device_ptr(fc_system *sys, int which);
 fc_status fc_sys_sizes(fc_system *sys, int *n, int *nif, int *nnz, int *nlev_fwd, int *nlev_bwd);
 
 /* Device time of the last call of each stage, milliseconds (HIP events on the
- * context's stream): 0 assembly, 1 solve, 2 SpMV (last fc_sys_spmv_bench). */
+ * context's stream): 0 assembly, 1 solve, 2 SpMV (last fc_sys_spmv_bench),
+ * 3 sweep pair (last fc_sys_sweep_bench). */
 double fc_sys_last_ms(fc_system *sys, int stage);
 /* Time `reps` FP64 (precision 64) or FP32 (32) SpMVs y = A x on the resident
  * matrix; returns the average device time per SpMV in milliseconds, or < 0. */
 double fc_sys_spmv_bench(fc_system *sys, int precision, int reps);
+/* Time `reps` applications of the FP64 IC(0) (dilu 0, src/iccg.f90:94-111) or
+ * ILU(0)/DILU (dilu 1, src/bicgstab.f90:107-124) preconditioner of the resident
+ * matrix -- a forward and a backward sweep in the context's sweep schedule, each
+ * after the fill of its output with the "not ready" mark -- after one untimed
+ * application; returns the average device time per pair in milliseconds, or < 0.
+ * No counterpart in the reference. */
+double fc_sys_sweep_bench(fc_system *sys, int dilu, int reps);
 
 /* --------------------------------------------- learned preconditioner */
 

@@ -15,7 +15,8 @@
 !   int32  ioffset(numCells+1), ja(nnz), diag(numCells), ijslot(nif), jislot(nif)
 !   float64 a(nnz), su(numCells)            (after laplacian)
 !   float64 fi(numCells), res(numCells), resor(ip)   (after the solve)
-! The iteration count is taken from the solver's report line on stdout.
+! The iteration count is taken from the solver's report line on stdout, and the
+! solver call's wall time from the 'solve_seconds' line.
 program ref_driver
   use types
   use parameters
@@ -23,6 +24,7 @@ program ref_driver
   use sparse_matrix
   implicit none
   integer :: solver, nswv, u, uinp, i, amode
+  integer(8) :: c0, c1, crate
   real(dp) :: sorv
   real(dp), allocatable :: mu(:), phi(:), fi(:), su0(:)
 
@@ -63,6 +65,7 @@ program ref_driver
   sor(ip) = sorv
   nsw(ip) = nswv
   resor(ip) = 0.0_dp
+  call system_clock(c0, crate)
   select case (solver)
   case (1)
     call iccg(fi, ip)
@@ -71,6 +74,9 @@ program ref_driver
   case (3)
     call bicgstab(fi, ip)
   end select
+  call system_clock(c1)
+  ! wall time of the solver call alone (factorisation included), for the CPU calibration
+  write(*,'(a,es16.8)') ' solve_seconds ', real(c1 - c0, dp) / real(crate, dp)
   write(u) fi(1:numCells), res, resor(ip)
   close(u)
 end program ref_driver
