@@ -69,6 +69,8 @@ METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "fcg_jacobi_poly", "ic_
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
 # 192^3 graded box; summary and commands in profiles/gnn_layer_hex192_graded.md
 LAYER_TRAFFIC = {(5, 192): 1_171_501_056 + 452_984_832}
+# the same for the hidden layer over 256-cell clusters (k_gnn_layer_cl); None until measured
+LAYER_TRAFFIC_CL = {}
 
 
 def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
@@ -81,6 +83,16 @@ def gnn_layer_bytes(n, width=6):
     (64), its ELL slots (neighbour index + packed features, 8 per slot) and count (4); the
     neighbours' rows are the other cells' own rows and are not counted again."""
     return n * (64 + 64 + 8 * width + 4)
+
+
+def gnn_layer_bytes_cl(n, clusters, halo_total, width=6, cells_per_cluster=256):
+    """Algorithmic bytes of one hidden layer over the cluster layout: the cell's own BF16 row in
+    (64) and out (64); per position of the padded cluster layout its slot pairs' LDS row
+    indices (4 per pair) and its slots' packed features (4 per slot); per cluster its halo
+    count (4) and halo positions (4 each).  The halo rows are other cells' own rows and are
+    not counted again."""
+    n_pad = clusters * cells_per_cluster
+    return n * (64 + 64) + n_pad * (4 * (width // 2) + 4 * width) + 4 * clusters + 4 * halo_total
 
 
 def gnn_bytes(n, width=6, layers=4):
@@ -610,6 +622,8 @@ def main():
     ap.add_argument("--weights", type=str, default=None, help="network weights (.npy); default: the packaged ones")
     ap.add_argument("--cpu-iters", type=int, default=80, help="iterations of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--network-layout", choices=("clusters", "cells"), default="clusters",
+                    help="layout of the network's inference layers (fc_set_network_layout)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch each solver iteration's kernels one by one instead of replaying a "
                          "captured HIP graph (for profilers that do not follow graph launches)")
@@ -659,6 +673,8 @@ def main():
     ctx = fcap.Context(0)
     if args.no_graphs:
         ctx.set_graphs(False)
+    if args.network_layout == "cells":
+        ctx.set_network_layout(fcap.NET_CELL_ORDER)
     if args.profile_calcp or args.profile_momentum:
         cpu_n = 0 if args.no_cpu else args.calcp_cpu_n
         if args.profile_calcp:
@@ -742,6 +758,7 @@ def main():
             np.array_equal(sols["ic_pcg_fp64"], sols["ic_pcg_fp64_levels"]))
     if net is not None:
         layer_ms = net.bench_layer(s, args.spmv_reps)
+        layout = s.network_layout()
         net.close()
     if zero_net is not None:
         zero_net.close()
@@ -787,13 +804,20 @@ def main():
                       "pair_ms": sweep_ms, "bytes": sweep_pair_bytes(n, nnz),
                       "GBps": sweep_pair_bytes(n, nnz) / (sweep_ms * 1e-3) / 1e9}
     if layer_ms is not None:
-        lb = gnn_layer_bytes(n)
+        if layout[0] > 0:
+            lb = gnn_layer_bytes_cl(n, layout[0], layout[2])
+            kname = "k_gnn_layer_cl<6,false,false> (hidden message-passing layer, 256-cell clusters)"
+            traffic = LAYER_TRAFFIC_CL.get((args.case, args.n))
+        else:
+            lb = gnn_layer_bytes(n)
+            kname = "k_gnn_layer_quad<false,false,6> (hidden message-passing layer, cell order)"
+            traffic = LAYER_TRAFFIC.get((args.case, args.n))
         lgbs = lb / (layer_ms * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": round(lgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(lgbs / HBM_PEAK_GBS, 4),
-                            "traffic": LAYER_TRAFFIC.get((args.case, args.n)),
-                            "kernel": "k_gnn_layer_quad<false,false,6> (hidden message-passing layer)",
+                            "frac": round(lgbs / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                             "launch_ms": layer_ms, "bytes_per_launch": lb}
+        line["network_layout"] = {"clusters": layout[0], "halo_max": layout[1], "halo_rows": layout[2],
+                                  "halo_rows_per_cell": round(layout[2] / n, 4)}
     else:
         line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv<double,double,0>",

@@ -273,6 +273,12 @@ extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double ome
     return FC_OK;
 }
 
+extern "C" fc_status fc_set_network_layout(fc_context *ctx, int layout) {
+    if (!ctx || (layout != FC_NET_CLUSTERS && layout != FC_NET_CELL_ORDER)) return FC_ERR_ARGUMENT;
+    ctx->gnn_layout = layout;
+    return FC_OK;
+}
+
 extern "C" fc_status fc_set_sweep_mode(fc_context *ctx, int mode) {
     if (!ctx || (mode != FC_SWEEP_LEVELS && mode != FC_SWEEP_DEPENDENT && mode != FC_SWEEP_PARTS))
         return FC_ERR_ARGUMENT;

@@ -35,8 +35,35 @@ struct GnnGraph {
     // separates two neighbours).
     int nblk = 0;
     DevBuf border;
+    // Cluster layout of the inference layers (k_gnn_layer_cl).  The network's
+    // cells are renumbered: positions sorted by the Morton code of the cells'
+    // per-axis centroid ranks, so each run of kGnnClCells consecutive positions
+    // is a compact cluster (an 8 x 8 x 4 brick of a structured hex box, graded
+    // or not).  A workgroup stages its cluster's rows and its halo -- the
+    // distinct outside neighbours, at most kGnnClHalo -- in LDS and gathers every
+    // message from there.  cperm: position -> cell (n_pad = ncl * kGnnClCells
+    // entries, -1 past n); cinv: cell -> position; per cluster the halo count and
+    // kGnnClHalo halo positions; per slot pair (2q, 2q + 1) of a position the two
+    // LDS row indices as 16-bit halves at [q * n_pad + pos] (0xffff: padding
+    // slot); the edge features of the slots at [k * n_pad + pos].  Built once per
+    // structure (cl_n); the features again at every assembly.
+    int cl_n = -1, ncl = 0;
+    bool cl_ok = false;              // every halo fits
+    int cl_halo_max = 0;             // largest halo found
+    DevBuf cperm, cinv, hcnt, halo, lidx, efc;
+    // gate pairs of every layer for one set of weights (k_cl_gates), tagged with
+    // the weights' version; -1 after the features change
+    DevBuf cgate;
+    long long gate_wver = -1;
     bool ready = false;
 };
+
+constexpr int kGnnClCells = 256;   // cells per cluster (threads of the layer's workgroup)
+constexpr int kGnnClHalo = 384;    // halo rows a cluster may stage
+
+// (key, value) pairs sorted by key on the stream (radix sort; stable)
+void sort_pairs_u64(hipStream_t s, const unsigned long long *key_in, unsigned long long *key_out, const int *val_in,
+                    int *val_out, int n, DevBuf &tmp);
 
 // Build the graph data of an assembled system (device arrays of the system).
 void gnn_prepare_graph(fc_context *ctx, GnnGraph &g, const Structure &st, const int *kface, const int *owner0,

@@ -173,6 +173,7 @@ struct fc_context {
     double inner_eta = 0.0;
     int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG (fc_set_fcg_smoothing)
     float gnn_omega = 0.8f;   // their damping
+    int gnn_layout = 0;       // FC_NET_CLUSTERS or FC_NET_CELL_ORDER (fc_set_network_layout)
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

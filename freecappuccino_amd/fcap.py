@@ -39,6 +39,7 @@ METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
 STOP_L1, STOP_L2 = 0, 1
 SWEEP_LEVELS, SWEEP_DEPENDENT, SWEEP_PARTS = 0, 2, 3
 REDUCE_TREE, REDUCE_SEQUENTIAL = 0, 1
+NET_CLUSTERS, NET_CELL_ORDER = 0, 1
 SWEEP_DEFAULT = SWEEP_PARTS
 ARR = dict(a=1, su=2, ioffset=3, ja=4, diag=5, ij_slot=6, ji_slot=7, vol=8, xc=9, yc=10, zc=11, facint=12,
            arx=13, ary=14, arz=15, xf=16, yf=17, zf=18, res=19, pivots=20)
@@ -82,7 +83,8 @@ def library() -> C.CDLL:
         "fc_set_fcg_stagnation": ([vp, i, d], i),
         "fc_set_sweep_wait_limit": ([vp, i], i), "fc_set_graphs": ([vp, i], i),
         "fc_set_sweep_waves": ([vp, i], i),
-        "fc_set_reduction_order": ([vp, i], i),
+        "fc_set_reduction_order": ([vp, i], i), "fc_set_network_layout": ([vp, i], i),
+        "fc_sys_network_layout": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)], i),
         "fc_dpcg_noalloc": ([vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, i, si], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
@@ -226,6 +228,11 @@ class Context:
     def set_fcg_smoothing(self, steps: int = 3, omega: float = 0.8):
         """Damped-Jacobi steps after the network in the learned FCG's preconditioner."""
         self.check(self.L.fc_set_fcg_smoothing(self.h, int(steps), float(omega)))
+
+    def set_network_layout(self, layout: int = NET_CLUSTERS):
+        """Layout of the network's inference layers: NET_CLUSTERS (compact 256-cell clusters
+        staged in LDS, where every halo fits) or NET_CELL_ORDER; bitwise the same z."""
+        self.check(self.L.fc_set_network_layout(self.h, int(layout)))
 
     def set_fcg_stagnation(self, window: int = 64, factor: float = 0.81):
         """Stagnation test of the learned FCG's inner solves (window 0: off; factor 0 forces the
@@ -942,6 +949,13 @@ class System:
         fi = np.zeros(self.n)
         self.ctx.check(self.ctx.L.fc_sys_get_solution(self.h, _dp(fi)))
         return fi
+
+    def network_layout(self):
+        """(clusters, largest halo, halo rows in all) of the network's cluster layout on this
+        system (0 clusters: none); builds the network's graph data if needed."""
+        c, h, t = C.c_int(), C.c_int(), C.c_longlong()
+        self.ctx.check(self.ctx.L.fc_sys_network_layout(self.h, C.byref(c), C.byref(h), C.byref(t)))
+        return c.value, h.value, t.value
 
     def solve(self, method=METHOD_ICCG, stop_rule=STOP_L1, sor=1e-8, nsw=1000):
         info = SolveInfo()

@@ -350,6 +350,18 @@ fc_status fc_set_fcg_window(fc_context *ctx, int m);
  * steps z <- z + omega (r - A z) / a_ii (0..8 steps, 0 < omega < 2; default 3
  * steps, omega 0.8).  No counterpart in the reference. */
 fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega);
+/* Layout of the network's inference layers (FC_METHOD_FCG_LEARNED, fc_gnn_apply).
+ *   FC_NET_CLUSTERS   (default) the cells renumbered into compact clusters of
+ *                     256 (Morton order of their per-axis centroid ranks); one
+ *                     workgroup per cluster stages the cluster's rows and its
+ *                     halo (at most 384 outside neighbours) in LDS and reads
+ *                     every message from there.  Used for ELL widths 4, 6 and 8
+ *                     when every halo fits; else cell order.
+ *   FC_NET_CELL_ORDER rows in cell order, each neighbour row gathered from
+ *                     global memory by four lanes.
+ * Both give bitwise the same z.  No counterpart in the reference. */
+enum { FC_NET_CLUSTERS = 0, FC_NET_CELL_ORDER = 1 };
+fc_status fc_set_network_layout(fc_context *ctx, int layout);
 /* Stagnation test of FC_METHOD_FCG_LEARNED's inner solves: the first `window`
  * iterations only set the reference; after every later window the window's
  * smallest ||r||_2^2 must be below factor x the smallest before it, else plain
@@ -579,6 +591,11 @@ double fc_gnn_bench(fc_gnn *net, fc_system *sys, int reps);
 double fc_gnn_bench_layer(fc_gnn *net, fc_system *sys, int reps);
 /* Edge features of the system's CSR entries (nnz each; 0 on the diagonal) and its FP32 diagonal. */
 fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2, float *diag);
+/* The cluster layout of the system's network graph (fc_set_network_layout),
+ * built on first use: the number of clusters (0: the graph has none -- ELL width
+ * other than 4, 6 or 8, or a halo above 384), the largest halo found and the
+ * halo rows of all clusters together. */
+fc_status fc_sys_network_layout(fc_system *sys, int *clusters, int *halo_max, long long *halo_total);
 /* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
 fc_status fc_gnn_loss_grad(fc_gnn *net, fc_system *sys, const float *r, const float *e, double *loss, float *grad);
 /* Add (residual, error) pairs of IC(0)-PCG iterates on the homogeneous system

@@ -121,6 +121,38 @@ def test_fused_first_layer_matches_stored_encoder_rows(ctx, mesh):
         s.close()
 
 
+@pytest.mark.parametrize("case", ["hex6", "tet5", "hex23", "hex40_graded"])
+def test_cluster_layout_matches_cell_order(ctx, case):
+    """The inference layers over 256-cell clusters (rows renumbered, halos staged in LDS) give z
+    bitwise equal to the cell-order layers: every message, MFMA product and epilogue value is
+    formed with the same operations in the same order, so one wrong halo entry, local index or
+    position anywhere in the four layers changes z.  Meshes: one partial cluster (6^3), permuted
+    graded tets (ELL width 4, 3 clusters), 23^3 (12,167 cells, the last cluster partial) and a
+    graded 40^3 box (250 clusters)."""
+    m = {"hex6": lambda: M.hex_box(6, perturb=0.2, seed=2),
+         "tet5": lambda: M.tet_box(5, grade_axis=1, grade_ratio=1.1, permute_seed=3),
+         "hex23": lambda: M.hex_box(23, perturb=0.2, seed=4),
+         "hex40_graded": lambda: M.hex_box(40, perturb=0.2, seed=5, grade_axis=0, grade_ratio=1.05,
+                                           grade_max_aspect=100.0)}[case]()
+    s = _system(m, ctx)
+    net = gnn.Network(_params(4), ctx)
+    try:
+        clusters, halo_max, halo_total = s.network_layout()
+        assert clusters == (m.n_cells + 255) // 256, (clusters, halo_max)
+        assert 0 <= halo_max <= 384 and halo_max <= halo_total <= clusters * halo_max
+        assert (halo_max > 0) == (clusters > 1)
+        r = np.random.default_rng(5).standard_normal(m.n_cells).astype(np.float32)
+        z_cl = net.apply(s, r)
+        ctx.set_network_layout(fcap.NET_CELL_ORDER)
+        z_cells = net.apply(s, r)
+        assert np.abs(z_cells).max() > 0
+        np.testing.assert_array_equal(z_cl, z_cells)
+    finally:
+        ctx.set_network_layout(fcap.NET_CLUSTERS)
+        net.close()
+        s.close()
+
+
 @pytest.mark.parametrize("pseed", [3, 5])
 def test_loss_and_gradient_match_oracle(ctx, small, pseed):
     m, A, graph, feats, s = small
