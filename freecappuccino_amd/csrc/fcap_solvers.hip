@@ -786,8 +786,12 @@ fc_status solve_bicgstab64(fc_context *ctx, const SolveArgs &args, fc_solve_info
 }
 
 // FP32 inner solve of A e = r32 to ||r_inner||_2 <= eta ||r32||_2, e32 out.
+// s0_carry (PCG only, may be null): on entry the r.z of the previous cycle's last
+// iteration (<= 0: none), on exit this cycle's.  With a previous value the search
+// direction p is kept and the first step continues it (beta = r.z / s0_carry), so
+// the refinement replaces the residual without discarding the Krylov direction.
 static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, double r32norm2, int cap,
-                         const PackedFactor<float> *pf) {
+                         const PackedFactor<float> *pf, double *s0_carry = nullptr) {
     Workspace &ws = *c.ws;
     const int n = c.n;
     float *a32 = ws.a32.as<float>(), *d32 = ws.d32.as<float>();
@@ -798,8 +802,9 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
     const int gs = spmv_grid(n);
     Scalars h;
     std::memset(&h, 0, sizeof(h));
+    const bool keep = !bicg && s0_carry && *s0_carry > 0.0;
     h.small = kSmallSrc;
-    h.s0 = kS0;
+    h.s0 = keep ? *s0_carry : kS0;
     h.alf = 1.0;
     h.beto = 1.0;
     h.gam = 1.0;
@@ -810,7 +815,7 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
     h.status = kStatusRunning;
     upload_scalars(c, h);
     k_zero<float><<<c.gv, 256, 0, c.s>>>(n, e);
-    k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
+    if (!keep) k_zero<float><<<c.gv, 256, 0, c.s>>>(n, p);
     if (!bicg) {
         auto iter = [&]() {
             lu_launch32(c.s, st, a32, d32, small, r, z, c.sc, &c.sw, pf);
@@ -845,6 +850,7 @@ static int inner_solve32(Ctx &c, const Structure &st, bool bicg, double eta, dou
         run_batches(c, 1, iter);
     }
     Scalars hs = read_scalars(c);
+    if (s0_carry) *s0_carry = bicg || hs.status != FC_OK ? 0.0 : hs.s0;
     return hs.iters;
 }
 
@@ -871,6 +877,7 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
     const bool l2 = args.stop_rule == FC_STOP_L2;
     int total_inner = 0, outer = 0;
     double res0 = 0.0, resl = 0.0, rel = 0.0;
+    double scale = 1.0, s0c = 0.0;   // FP32 residual scale; r.z carried between PCG cycles
     fc_status status = FC_NOT_CONVERGED;
     ctx->timer.start(c.s);
     for (;;) {
@@ -889,10 +896,15 @@ fc_status solve_mixed(fc_context *ctx, const SolveArgs &args, bool bicg, fc_solv
         if (conv) { status = FC_OK; break; }
         if (outer >= args.max_outer || total_inner >= args.nsw) { status = FC_NOT_CONVERGED; break; }
         int cap = std::max(1, args.nsw - total_inner);
+        // PCG: every cycle's FP32 residual is scaled by the first cycle's norm, so that the
+        // kept search direction and r.z stay on one scale; BiCGStab: each cycle starts
+        // afresh on a unit-norm right-hand side
         const double rn = std::sqrt(ho.rnorm2);
-        k_scale_cast<<<c.gv, 256, 0, c.s>>>(n, r, ws.r32.as<float>(), 1.0 / rn);
-        total_inner += inner_solve32(c, st, bicg, args.inner_eta, 1.0, cap, pf.af ? &pf : nullptr);
-        k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), rn);
+        if (outer == 0 || bicg) scale = rn;
+        k_scale_cast<<<c.gv, 256, 0, c.s>>>(n, r, ws.r32.as<float>(), 1.0 / scale);
+        total_inner += inner_solve32(c, st, bicg, args.inner_eta, ho.rnorm2 / (scale * scale), cap,
+                                     pf.af ? &pf : nullptr, &s0c);
+        k_refine<<<c.gv, 256, 0, c.s>>>(n, args.fi, ws.e32.as<float>(), scale);
         ++outer;
     }
     double solve_ms = ctx->timer.stop_ms(c.s);

@@ -341,7 +341,10 @@ fc_status fc_set_reduction_order(fc_context *ctx, int order);
 
 /* Tuning of the FP32-in-FP64 solvers; no counterpart in the reference.
  * fc_set_inner_tolerance: relative residual reduction of each inner FP32 solve
- * (0 = the method's default: 1e-2 for FC_METHOD_FCG_LEARNED, 1e-3 otherwise).
+ * (0 = the method's default: 1e-2 for FC_METHOD_FCG_LEARNED and FC_METHOD_ICCG_MIXED,
+ * 1e-3 for FC_METHOD_BICGSTAB_MIXED).  FC_METHOD_ICCG_MIXED keeps its search direction
+ * from one refinement cycle to the next: each cycle replaces the FP32 residual by the
+ * refined FP64 one and continues the same conjugate-gradient sequence.
  * fc_set_fcg_window: directions kept by flexible CG, 1..16, rounded up to a
  * multiple of 4 (default 12). */
 fc_status fc_set_inner_tolerance(fc_context *ctx, double eta);
