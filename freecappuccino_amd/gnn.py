@@ -163,6 +163,11 @@ class Network:
         if getattr(self, "h", None) and getattr(system, "h", None):
             self.ctx.check(self.ctx.L.fc_gnn_drop_system(self.h, system.h))
 
+    def set_training_smoothing(self, steps: int = 3, omega: float = 0.8):
+        """Take the solver's damped-Jacobi steps after the network into the training loss (and
+        its gradient); 0 steps: the network's own output."""
+        self.ctx.check(self.ctx.L.fc_gnn_set_training_smoothing(self.h, int(steps), float(omega)))
+
     def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         losses = np.zeros(steps, np.float32)
         self.ctx.check(self.ctx.L.fc_gnn_train(self.h, steps, batch, float(lr), seed, _fp(losses)))

@@ -110,11 +110,17 @@ def main(argv=None):
     ap.add_argument("--steps-per-system", type=int, default=1)
     ap.add_argument("--graded-every", type=int, default=0,
                     help="with --stream: every k-th streamed system is a graded slab (0: none)")
+    ap.add_argument("--smooth", type=int, default=0,
+                    help="damped-Jacobi steps after the network taken into the loss (the solver applies 3)")
+    ap.add_argument("--omega", type=float, default=0.8, help="their damping (the solver's: 0.8)")
     args = ap.parse_args(argv)
 
     ctx = fcap.Context(0)
     init = np.load(args.init, allow_pickle=False) if args.init else gnn.init_params(args.seed)
     net = gnn.Network(init, ctx)
+    if args.smooth:
+        net.set_training_smoothing(args.smooth, args.omega)
+    command = " ".join(["python -m freecappuccino_amd.train_gnn", *(argv if argv is not None else sys.argv[1:])])
     iters = [int(x) for x in args.iters.split(",")]
     if args.stream:
         res = stream_train(net, ctx, n=args.n, count=args.stream, window=args.window,
@@ -124,8 +130,7 @@ def main(argv=None):
         net.save(args.out)
         if args.log:
             with open(args.log, "w") as f:
-                cmd = argv if argv is not None else sys.argv[1:]
-                json.dump(dict(res, command=" ".join(["python -m freecappuccino_amd.train_gnn", *cmd])), f)
+                json.dump(dict(res, command=command, smooth=args.smooth, omega=args.omega), f)
         h = res["loss"]
         print(f"saved {args.out}: {args.stream} systems in {res['wall_s']:.1f} s, first-100 mean loss "
               f"{np.mean(h[:100]):.4f}, last-100 mean loss {np.mean(h[-100:]):.4f}", flush=True)
@@ -151,9 +156,11 @@ def main(argv=None):
     net.save(args.out)
     if args.log:
         with open(args.log, "w") as f:
-            json.dump({"n": args.n, "systems": args.systems, "graded": args.graded, "graded_n": args.graded_n,
-                       "iters": iters, "steps": args.steps, "batch": args.batch, "lr": args.lr,
-                       "lr_final": args.lr_final, "seconds": time.time() - t0, "loss": history}, f)
+            json.dump({"command": command, "n": args.n, "systems": args.systems, "graded": args.graded,
+                       "graded_n": args.graded_n, "iters": iters, "steps": args.steps, "batch": args.batch,
+                       "lr": args.lr, "lr_final": args.lr_final, "seed": args.seed, "init": args.init,
+                       "smooth": args.smooth, "omega": args.omega, "seconds": time.time() - t0,
+                       "loss": history}, f)
     print(f"saved {args.out}: first-100 mean loss {np.mean(history[:100]):.4f}, "
           f"last-100 mean loss {np.mean(history[-100:]):.4f}", flush=True)
     net.clear_samples()

@@ -614,6 +614,12 @@ fc_status fc_gnn_clear_samples(fc_gnn *net);
 fc_status fc_gnn_drop_system(fc_gnn *net, fc_system *sys);
 /* Adam over `steps` mini-batches of `batch` samples; loss_out (steps floats, may be NULL). */
 fc_status fc_gnn_train(fc_gnn *net, int steps, int batch, float lr, unsigned seed, float *loss_out);
+/* Training with the preconditioner's damped-Jacobi steps in the loss (fc_gnn_train,
+ * fc_gnn_loss_grad): the loss is taken on z after `steps` steps
+ * z <- z + omega (r - A z) / a_ii, and the gradient passes back through them
+ * (0..8 steps, 0 < omega < 2; default 0: the network's own output).  Set it to
+ * the solver's fc_set_fcg_smoothing to train what the solver applies. */
+fc_status fc_gnn_set_training_smoothing(fc_gnn *net, int steps, double omega);
 /* Attach a network as the preconditioner of FC_METHOD_FCG_LEARNED solves of sys. */
 fc_status fc_sys_set_network(fc_system *sys, fc_gnn *net);
 

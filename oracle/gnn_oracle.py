@@ -25,7 +25,10 @@ BF16 points (as on the GPU): weights, the edge gates c_k, stored h, and agg
 before the GEMM are rounded to bfloat16 (round to nearest even); the rest is
 float64 here (FP32 on the GPU).  Rounding is identity in the backward pass.
 
-Loss of a (residual, error) pair: L = (z - e)^T A (z - e) / (e^T A e).
+Loss of a (residual, error) pair: L = (z - e)^T A (z - e) / (e^T A e).  With
+`smooth` > 0 the z in the loss is the preconditioner's output after that many
+damped-Jacobi steps z <- z + omega (r - A z) / a_ii, and the gradient passes
+back through them: dz <- D M^k D^-1 dz with M = I - omega D^-1 A (A symmetric).
 
 Pinning: the backward pass is checked against central finite differences of
 the forward in float64 and against torch autograd (CPU) in tests/.
@@ -108,18 +111,33 @@ def forward(theta, graph: Graph, r, bf=True, keep=False):
     return (z, cache, p) if keep else z
 
 
-def loss_and_grad(theta, graph: Graph, A, pairs, bf=True):
-    """Mean loss over (r, e) pairs and its gradient (explicit backward pass)."""
+def smooth_steps(z, r, A, diag, steps, omega):
+    """`steps` damped-Jacobi steps z <- z + omega (r - A z) / diag."""
+    z = np.asarray(z, np.float64)
+    for _ in range(steps):
+        z = z + omega * (np.asarray(r, np.float64) - A @ z) / diag
+    return z
+
+
+def loss_and_grad(theta, graph: Graph, A, pairs, bf=True, smooth=0, omega=0.8):
+    """Mean loss over (r, e) pairs and its gradient (explicit backward pass); the loss is
+    taken after `smooth` damped-Jacobi steps of the network's output."""
     grad = np.zeros(N_PARAMS)
     total = 0.0
     for r, e in pairs:
         z, cache, p = forward(theta, graph, r, bf, keep=True)
         e = np.asarray(e, np.float64)
-        d = z - e
+        d = smooth_steps(z, r, A, graph.diag, smooth, omega) - e
         Ad = A @ d
         eAe = float(e @ (A @ e))
         total += float(d @ Ad) / eAe
         dz = 2.0 * Ad / eAe / len(pairs)
+        if smooth:
+            # back through z_{k+1} = M z_k + omega D^-1 r: dz <- M^T dz = D M D^-1 dz
+            t = dz / graph.diag
+            for _ in range(smooth):
+                t = t - omega * (A @ t) / graph.diag
+            dz = graph.diag * t
         s, u = cache["s"], cache["u"]
         dy = s * dz
         h = cache["h"][-1]

@@ -82,6 +82,42 @@ def test_backward_matches_finite_differences():
         assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
 
 
+def test_smoothed_backward_matches_finite_differences():
+    """With damped-Jacobi steps after the network in the loss, the backward pass (adjoint of the
+    steps, D M^k D^-1) against central differences; the smoothed loss differs from the plain one,
+    and zero steps give the plain loss exactly."""
+    m, graph, A, _ = _small_graph(n=3)
+    rng = np.random.default_rng(14)
+    th = gnn.init_params(6).astype(np.float64)
+    th[G.OFF_DEC_W:G.OFF_DEC_W + G.H] *= 20.0
+    pairs = [(rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)) for _ in range(2)]
+    loss, grad = G.loss_and_grad(th, graph, A, pairs, bf=False, smooth=3, omega=0.8)
+    plain, _ = G.loss_and_grad(th, graph, A, pairs, bf=False)
+    zero, _ = G.loss_and_grad(th, graph, A, pairs, bf=False, smooth=0, omega=0.8)
+    assert zero == plain and abs(loss - plain) > 1e-3 * plain
+    eps = 1e-6
+    for i in _probe_indices(rng):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        lp, _ = G.loss_and_grad(tp, graph, A, pairs, bf=False, smooth=3, omega=0.8)
+        lm, _ = G.loss_and_grad(tm, graph, A, pairs, bf=False, smooth=3, omega=0.8)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
+
+
+def test_smoothing_steps_are_jacobi_iterations():
+    """smooth_steps from z = 0 is the Jacobi iteration for A z = r: its error contracts by
+    (I - omega D^-1 A) each step, checked against the matrix power."""
+    m, graph, A, _ = _small_graph(n=3)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(m.n_cells)
+    x = np.linalg.solve(A.toarray() if hasattr(A, "toarray") else A, r)
+    Mt = np.eye(m.n_cells) - 0.7 * (A.toarray() if hasattr(A, "toarray") else A) / graph.diag[:, None]
+    z = G.smooth_steps(np.zeros(m.n_cells), r, A, graph.diag, 4, 0.7)
+    np.testing.assert_allclose(z - x, np.linalg.matrix_power(Mt, 4) @ (-x), rtol=1e-10, atol=1e-12)
+
+
 def test_backward_matches_torch_autograd():
     """The same gradient from torch autograd of an independent torch restatement (CPU)."""
     import torch

@@ -153,16 +153,23 @@ def test_cluster_layout_matches_cell_order(ctx, case):
         s.close()
 
 
-@pytest.mark.parametrize("pseed", [3, 5])
-def test_loss_and_gradient_match_oracle(ctx, small, pseed):
+@pytest.mark.parametrize("pseed,smooth", [(3, 0), (5, 0), (3, 3)])
+def test_loss_and_gradient_match_oracle(ctx, small, pseed, smooth):
+    """Loss and hand-written gradient against the oracle; with smooth = 3 the loss is taken after
+    the solver's three damped-Jacobi steps (omega 0.8) and the gradient passes back through them."""
     m, A, graph, feats, s = small
     th = _params(pseed)
     net = gnn.Network(th, ctx)
+    if smooth:
+        net.set_training_smoothing(smooth, 0.8)
     rng = np.random.default_rng(20 + pseed)
     r = rng.standard_normal(m.n_cells).astype(np.float32)
     e = rng.standard_normal(m.n_cells).astype(np.float32)
     loss_gpu, g_gpu = net.loss_grad(s, r, e)
-    loss_ref, g_ref = G.loss_and_grad(th, graph, A, [(r, e)], bf=True)
+    loss_ref, g_ref = G.loss_and_grad(th, graph, A, [(r, e)], bf=True, smooth=smooth, omega=0.8)
+    if smooth:
+        loss_plain, _ = G.loss_and_grad(th, graph, A, [(r, e)], bf=True)
+        assert abs(loss_ref - loss_plain) > 10 * LOSS_RTOL * loss_plain   # the steps change the loss
     assert abs(loss_gpu - loss_ref) <= LOSS_RTOL * abs(loss_ref), (loss_gpu, loss_ref)
     g_gpu = g_gpu.astype(np.float64)
     cos = g_gpu @ g_ref / (np.linalg.norm(g_gpu) * np.linalg.norm(g_ref))
