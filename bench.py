@@ -69,8 +69,9 @@ METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "fcg_jacobi_poly", "ic_
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
 # 192^3 graded box; summary and commands in profiles/gnn_layer_hex192_graded.md
 LAYER_TRAFFIC = {(5, 192): 1_171_501_056 + 452_984_832}
-# the same for the hidden layer over 256-cell clusters (k_gnn_layer_cl); None until measured
-LAYER_TRAFFIC_CL = {}
+# the same for the hidden layer over 256-cell clusters (k_gnn_layer_cl): FETCH_SIZE 431,862 KiB
+# (x 2.0) + WRITE_SIZE 442,368 KiB (x 1.0); summary in profiles/gnn_layer_hex192_graded.md
+LAYER_TRAFFIC_CL = {(5, 192): 431_862 * 1024 * 2 + 442_368 * 1024}
 
 
 def spmv_bytes(n, nnz, value_bytes=8, index_bytes=4):
@@ -87,12 +88,11 @@ def gnn_layer_bytes(n, width=6):
 
 def gnn_layer_bytes_cl(n, clusters, halo_total, width=6, cells_per_cluster=256):
     """Algorithmic bytes of one hidden layer over the cluster layout: the cell's own BF16 row in
-    (64) and out (64); per position of the padded cluster layout its slot pairs' LDS row
-    indices (4 per pair) and its slots' packed features (4 per slot); per cluster its halo
-    count (4) and halo positions (4 each).  The halo rows are other cells' own rows and are
-    not counted again."""
+    (64) and out (64); per position of the padded cluster layout one 8-byte word per slot pair
+    (the two slots' LDS rows and BF16 gates); per cluster its halo count (4) and halo positions
+    (4 each).  The halo rows are other cells' own rows and are not counted again."""
     n_pad = clusters * cells_per_cluster
-    return n * (64 + 64) + n_pad * (4 * (width // 2) + 4 * width) + 4 * clusters + 4 * halo_total
+    return n * (64 + 64) + n_pad * 8 * (width // 2) + 4 * clusters + 4 * halo_total
 
 
 def gnn_bytes(n, width=6, layers=4):
