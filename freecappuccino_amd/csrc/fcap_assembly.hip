@@ -509,31 +509,46 @@ __global__ void k_outlet_flux(int nout, int out_face0, const int *__restrict__ o
     fmo[i] = den[c] * ((u[c] * arx[f] + v[c] * ary[f]) + w[c] * arz[f]);
 }
 
-// lane j's value, read into scalar registers (j uniform)
-__device__ inline double lane_value(double x, int j) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)b, j);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
-    return __hiloint2double(hi, lo);
-}
-
-// flowo = sum of fmo in outlet order, fac = flomas / (flowo + small).  One wave: each lane
-// holds one of 64 consecutive values, the next 64 are loaded while these are added, and
-// the adds form one chain in outlet order (every lane computes the same sum).
+// flowo = sum of fmo in outlet order, fac = flomas / (flowo + small).  One wave.  The
+// values pass through LDS in chunks of kOutChunk: all lanes load the next chunk (coalesced,
+// every load unconditional at a clamped index, so the loads stay in flight) while lane 0 adds
+// the current chunk's values one by one, in outlet order, from LDS; the adds are the only
+// dependent chain.
+constexpr int kOutChunk = 2048;
 __global__ void __launch_bounds__(64) k_outlet_factor(int nout, const double *__restrict__ fmo, double flomas,
                                                       double *__restrict__ fac) {
+    constexpr int PER = kOutChunk / 64;
+    __shared__ double buf[2][kOutChunk];
     const int lane = threadIdx.x;
-    double flowo = 0.0;
-    double cur = lane < nout ? fmo[lane] : 0.0;
-    int base = 0;
-    for (; base + 64 <= nout; base += 64) {
-        const int nx = base + 64 + lane;
-        const double next = nx < nout ? fmo[nx] : 0.0;
+    const int last = max(nout - 1, 0);
+    double stage[PER];
 #pragma unroll
-        for (int j = 0; j < 64; ++j) flowo = flowo + lane_value(cur, j);
-        cur = next;
+    for (int k = 0; k < PER; ++k) stage[k] = fmo[min(k * 64 + lane, last)];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) buf[0][k * 64 + lane] = stage[k];
+    __syncthreads();
+    double flowo = 0.0;
+    for (int c0 = 0, cb = 0; c0 < nout; c0 += kOutChunk, cb ^= 1) {
+        const int c1 = c0 + kOutChunk;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) stage[k] = fmo[min(c1 + k * 64 + lane, last)];
+        if (lane == 0) {
+            const int cnt = min(kOutChunk, nout - c0);
+            const double *b = buf[cb];
+            int j = 0;
+            for (; j + 32 <= cnt; j += 32) {   // 32 reads in flight, then the 32 adds
+                double v[32];
+#pragma unroll
+                for (int t = 0; t < 32; ++t) v[t] = b[j + t];
+#pragma unroll
+                for (int t = 0; t < 32; ++t) flowo = flowo + v[t];
+            }
+            for (; j < cnt; ++j) flowo = flowo + b[j];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) buf[cb ^ 1][k * 64 + lane] = stage[k];
+        __syncthreads();
     }
-    for (int j = 0; j < nout - base; ++j) flowo = flowo + lane_value(cur, j);
     if (lane == 0) fac[0] = flomas / (flowo + kSmallSrc);
 }
 
