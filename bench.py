@@ -55,6 +55,7 @@ import json
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -506,6 +507,36 @@ def cpu_baseline(mesh, a, su, gpu_iters, budget_iters):
     return out
 
 
+class OracleSolve(threading.Thread):
+    """The FP64 oracle's DIC-PCG (src/iccg.f90 restated in C) run to convergence on the bench
+    system, on one pinned host core, beside the GPU work: the checker that states each GPU
+    solution's error against the FP64 oracle.  It stops on the reference's own L1 rule at
+    rsm < 1e-10, two decades tighter than the GPU solves' 1e-8, so its own error does not
+    limit the comparison."""
+
+    SOR = 1e-10
+
+    def __init__(self, mesh, a, su):
+        super().__init__(daemon=True)
+        self.mesh, self.a, self.su = mesh, a, su
+        self.x = None
+        self.info = {}
+
+    def run(self):
+        from oracle import oracle as O
+        allowed = os.sched_getaffinity(0)
+        core = min(allowed)
+        os.sched_setaffinity(0, {core})          # this thread only
+        m = self.mesh
+        csr = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
+        t0 = time.perf_counter()
+        r = O.iccg(csr, self.a, self.su, np.zeros(m.n_cells), sor=self.SOR, nsw=20000)
+        self.x = r.fi
+        self.info = {"method": f"oracle DIC-PCG (C restatement of src/iccg.f90), L1 rsm < {self.SOR:g}, "
+                               f"1 thread pinned to core {core}, beside the GPU runs",
+                     "iters": r.iters, "seconds": time.perf_counter() - t0}
+
+
 def host_rel_residual(s, x):
     """||su - A x||_2 / ||su||_2 in FP64 on the host, from the resident system's own CSR arrays."""
     import scipy.sparse as sp
@@ -691,6 +722,10 @@ def main():
     mu = np.full(n, -1.0)
     s.assemble_laplacian(mu, None, su0)
     sizes = s.sizes()
+    oracle_solve = None
+    if not (args.no_cpu or args.profile_spmv or args.profile_gnn):
+        oracle_solve = OracleSolve(mesh, s.get("a"), s.get("su"))
+        oracle_solve.start()
     if args.profile_spmv:
         ms = s.spmv_bench(64, args.spmv_reps)
         copy_bytes = 1 << 30
@@ -762,6 +797,12 @@ def main():
         net.close()
     if zero_net is not None:
         zero_net.close()
+    if oracle_solve is not None:
+        oracle_solve.join()
+        if oracle_solve.x is not None:
+            xo = oracle_solve.x
+            for name in results:
+                results[name]["err_vs_oracle_fp64"] = float(np.linalg.norm(sols[name] - xo) / np.linalg.norm(xo))
     if "ic_pcg_fp64" in sols:
         x64 = sols["ic_pcg_fp64"]
         for name in results:
@@ -795,6 +836,8 @@ def main():
                  "fp32_ms": spmv32_ms,
                  "fp32_GBps": b32 / (spmv32_ms * 1e-3) / 1e9, "bytes_fp64": b64, "bytes_fp32": b32},
     }
+    if oracle_solve is not None and oracle_solve.x is not None:
+        line["oracle_fp64"] = dict(oracle_solve.info, rel_l2_host=host_rel_residual(s, oracle_solve.x))
     if "fcg_learned" in results and "ic_pcg_fp64" in results:
         ratio = results["ic_pcg_fp64"]["ms"] / results["fcg_learned"]["ms"]
         line["target"] = {"what": "learned mixed-precision FCG at least 2x faster than GPU FP64 IC(0)-PCG to 1e-8",

@@ -19,12 +19,16 @@ pytestmark = pytest.mark.gpu
 
 # Tolerances.  The GPU evaluates in FP32 between the BF16 rounding points and
 # rounds d_out to BF16 before its MFMA products; the oracle is float64 between
-# the same points.  A value near a BF16 rounding boundary can round the other
-# way, so comparisons are in relative L2 norm, not elementwise.
-FWD_RTOL = 2e-2       # ||y_gpu - y_oracle|| / ||y_oracle||, y = z/s - u (the network's part of z)
-LOSS_RTOL = 1e-2
-GRAD_COS = 0.99       # cosine similarity of the full gradient vectors
-GRAD_RTOL = 6e-2      # ||g_gpu - g_oracle|| / ||g_oracle||
+# the same points.  A value near a BF16 rounding boundary could round the other
+# way, so comparisons are in relative L2 norm, not elementwise.  Measured on
+# these cases (the tests print them): forward 8e-8..2e-7, loss 2e-9..4e-8,
+# gradient 3e-4..4e-4 (worst parameter block 8e-4..9e-3); the bounds leave room
+# for a few BF16 values rounding the other way, not for a wrong term.
+FWD_RTOL = 1e-4       # ||y_gpu - y_oracle|| / ||y_oracle||, y = z/s - u (the network's part of z)
+LOSS_RTOL = 1e-6
+GRAD_COS = 0.9999     # cosine similarity of the full gradient vectors
+GRAD_RTOL = 5e-3      # ||g_gpu - g_oracle|| / ||g_oracle||
+BLOCK_RTOL = 5e-2     # the same, per parameter block
 
 
 @pytest.fixture(scope="module")
@@ -208,7 +212,7 @@ def test_loss_and_gradient_match_oracle(ctx, small, pseed, smooth):
     loss_ref, g_ref = G.loss_and_grad(th, graph, A, [(r, e)], bf=True, smooth=smooth, omega=0.8)
     if smooth:
         loss_plain, _ = G.loss_and_grad(th, graph, A, [(r, e)], bf=True)
-        assert abs(loss_ref - loss_plain) > 10 * LOSS_RTOL * loss_plain   # the steps change the loss
+        assert abs(loss_ref - loss_plain) > 1e-2 * loss_plain   # the steps change the loss
     assert abs(loss_gpu - loss_ref) <= LOSS_RTOL * abs(loss_ref), (loss_gpu, loss_ref)
     g_gpu = g_gpu.astype(np.float64)
     cos = g_gpu @ g_ref / (np.linalg.norm(g_gpu) * np.linalg.norm(g_ref))
@@ -226,7 +230,7 @@ def test_loss_and_gradient_match_oracle(ctx, small, pseed, smooth):
     for lo, hi in blocks:
         nb = np.linalg.norm(g_ref[lo:hi])
         blk.append(np.linalg.norm(g_gpu[lo:hi] - g_ref[lo:hi]) / (nb + 1e-30))
-        assert np.linalg.norm(g_gpu[lo:hi] - g_ref[lo:hi]) <= 0.25 * nb + 1e-6 * np.linalg.norm(g_ref), (lo, hi)
+        assert np.linalg.norm(g_gpu[lo:hi] - g_ref[lo:hi]) <= BLOCK_RTOL * nb + 1e-6 * np.linalg.norm(g_ref), (lo, hi)
     print(f"  worst parameter block {max(blk):.2e}")
     net.close()
 
