@@ -180,11 +180,13 @@ struct fc_context {
 
 namespace fcap {
 
-// persistent waves of the part-wise sweeps: the context's setting, else two per
-// CU (192^3 box, FP64 IC(0)-PCG: 320 waves 1.35 s, 512 1.16 s, 640 1.16 s,
-// 1024 1.25 s; more waves than that only add polling traffic)
+// persistent waves of the part-wise sweeps: the context's setting, else three per
+// two CUs (192^3 box, FP64 IC(0)-PCG with 4,096-row parts: 256 waves 1.03 s, 384
+// 0.93 s, 512 0.95 s, 768 1.03 s; with 2,048-row parts 512 waves were best at
+// 1.15 s, with 1,024-row parts 1.71 s, with 8,192-row parts 1.61 s: bigger parts
+// cross fewer part boundaries until too few of them run at once)
 inline int part_wave_count(const fc_context *ctx) {
-    return ctx->part_waves > 0 ? ctx->part_waves : 2 * ctx->num_cus;
+    return ctx->part_waves > 0 ? ctx->part_waves : 3 * ctx->num_cus / 2;
 }
 
 // persistent waves of the row-wise dependency-driven sweeps and pivots: one 64-row

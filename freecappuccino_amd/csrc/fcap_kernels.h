@@ -25,7 +25,7 @@ constexpr int kFcgM = 16;                        // most directions truncated fl
 constexpr double kFcgInnerEta = 1e-2;            // default inner tolerance of flexible CG
 constexpr int kXcds = 8;                         // XCDs of an MI355X (workgroups are dealt round-robin)
 constexpr int kFinBlock = 1024;                  // block size of the finalize kernels
-constexpr int kPartRows = 2048;                  // rows per part of the part-wise triangular sweeps
+constexpr int kPartRows = 4096;                  // rows per part of the part-wise triangular sweeps
 
 // Device-side solver state shared by the kernels of one solve.  Kernels of an
 // iteration return immediately once `done` is set, so iterations can be

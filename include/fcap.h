@@ -382,7 +382,7 @@ fc_status fc_set_fcg_stagnation(fc_context *ctx, int window, double factor);
  *   FC_SWEEP_DEPENDENT one launch per sweep, each row waiting only for the rows
  *                      it reads (packed structures of width 3, 4 or 8; other
  *                      structures keep the level launches);
- *   FC_SWEEP_PARTS     (default) one launch per sweep over parts of 2,048
+ *   FC_SWEEP_PARTS     (default) one launch per sweep over parts of 4,096
  *                      consecutive rows, one persistent wave per part at a time,
  *                      64 rows per window in a schedule fixed at analysis:
  *                      inputs inside the part are read from the wave's LDS, only

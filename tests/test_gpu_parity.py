@@ -691,15 +691,15 @@ def test_bpres_oc_cuts_match_oracle(ctx, golden_loader):
         p = got
 
 
-@pytest.mark.parametrize("mesh", ["hex24", "hex20_graded", "tet10"])
+@pytest.mark.parametrize("mesh", ["hex32", "hex24_graded", "tet12"])
 def test_part_sweeps_bitwise_on_multi_part_meshes(ctx, mesh):
-    """Meshes of several parts of 2,048 rows (hex 24^3: 7 parts; graded 20^3: 4; Kuhn tets 10^3:
+    """Meshes of several parts of 4,096 rows (hex 32^3: 8 parts; graded 24^3: 4; Kuhn tets 12^3:
     3): the part-wise sweeps (inputs inside a part through LDS, across parts through global
     memory) give the DIC and DILU sweeps of the oracle bit for bit, as the other schedules do."""
-    m = {"hex24": lambda: M.hex_box(24, perturb=0.2, seed=3),
-         "hex20_graded": lambda: M.hex_box(20, perturb=0.2, seed=4, grade_axis=0, grade_ratio=1.2,
+    m = {"hex32": lambda: M.hex_box(32, perturb=0.2, seed=3),
+         "hex24_graded": lambda: M.hex_box(24, perturb=0.2, seed=4, grade_axis=0, grade_ratio=1.2,
                                            grade_max_aspect=50.0),
-         "tet10": lambda: M.tet_box(10, grade_axis=1, grade_ratio=1.1)}[mesh]()
+         "tet12": lambda: M.tet_box(12, grade_axis=1, grade_ratio=1.1)}[mesh]()
     geo = O.mesh_geometry(m)
     oc = O.csr_build(m.n_cells, m.owner[:m.n_inner] + 1, m.neighbour + 1)
     csr = fcap.CSR(oc.ioffset, oc.ja, oc.diag, oc.ij_slot, oc.ji_slot)
