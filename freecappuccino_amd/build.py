@@ -34,6 +34,10 @@ FLAGS = [
 ]
 
 
+# rocSOLVER (dense Cholesky of the coarse operator) and the rocBLAS it runs on
+ROCM_LIB = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
+LINK_LIBS = ["-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocsolver", "-lrocblas"]
+
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-Wall"]
 
@@ -57,7 +61,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp, *LINK_LIBS]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr}")

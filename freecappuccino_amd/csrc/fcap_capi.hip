@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "fcap_coarse.h"
 #include "fcap_internal.h"
 #include "fcap_solvers.h"
 #include "fcap_system.h"
@@ -211,6 +212,7 @@ extern "C" fc_status fc_finalize(fc_context *ctx) {
     ctx->cache.reset();
     ctx->ws.reset();
     ctx->timer.destroy();
+    coarse_release(ctx);
     if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -270,6 +272,13 @@ extern "C" fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double ome
     if (!ctx || steps < 0 || steps > 8 || !(omega > 0.0 && omega < 2.0)) return FC_ERR_ARGUMENT;
     ctx->gnn_smooth = steps;
     ctx->gnn_omega = (float)omega;
+    return FC_OK;
+}
+
+extern "C" fc_status fc_set_fcg_coarse(fc_context *ctx, int cells_per_aggregate) {
+    if (!ctx || cells_per_aggregate < 0 || cells_per_aggregate > (1 << 24) || cells_per_aggregate % 256 != 0)
+        return FC_ERR_ARGUMENT;
+    ctx->fcg_coarse = cells_per_aggregate;
     return FC_OK;
 }
 

@@ -1,6 +1,7 @@
 // Learned preconditioner: message-passing network over the cell graph (fcap_gnn.hip).
 #pragma once
 
+#include "fcap_coarse.h"
 #include "fcap_internal.h"
 #include "fcap_kernels.h"
 
@@ -55,6 +56,9 @@ struct GnnGraph {
     // the weights' version; -1 after the features change
     DevBuf cgate;
     long long gate_wver = -1;
+    // coarse correction of flexible CG: aggregates of consecutive positions of the
+    // cluster layout's order (cell order without one), rebuilt per assembly
+    CoarseSpace coarse;
     bool ready = false;
 };
 

@@ -174,6 +174,9 @@ struct fc_context {
     int gnn_smooth = 3;       // damped-Jacobi steps after the network in flexible CG (fc_set_fcg_smoothing)
     float gnn_omega = 0.8f;   // their damping
     int gnn_layout = 0;       // FC_NET_CLUSTERS or FC_NET_CELL_ORDER (fc_set_network_layout)
+    // cells per aggregate of flexible CG's coarse correction (fc_set_fcg_coarse; 0: off)
+    int fcg_coarse = 4096;
+    void *dense_handle = nullptr;   // rocBLAS handle of the coarse factorisation (created on first use)
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;
 };

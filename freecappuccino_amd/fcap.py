@@ -85,6 +85,9 @@ def library() -> C.CDLL:
         "fc_set_sweep_waves": ([vp, i], i),
         "fc_set_reduction_order": ([vp, i], i), "fc_set_network_layout": ([vp, i], i),
         "fc_sys_network_layout": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)], i),
+        "fc_set_fcg_coarse": ([vp, i], i),
+        "fc_sys_coarse_space": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), vp, vp], i),
+        "fc_sys_precondition": ([vp, vp, vp], i),
         "fc_dpcg_noalloc": ([vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, i, si], i),
         "fc_mesh_geometry": ([vp, i, dp, i, i, ip, ip, ip, ip, i] + [dp] * 11, i),
         "fc_csr_build": ([vp, i, i, ip, ip, ip, ip, ip, ip, ip], i),
@@ -229,6 +232,11 @@ class Context:
     def set_fcg_smoothing(self, steps: int = 3, omega: float = 0.8):
         """Damped-Jacobi steps after the network in the learned FCG's preconditioner."""
         self.check(self.L.fc_set_fcg_smoothing(self.h, int(steps), float(omega)))
+
+    def set_fcg_coarse(self, cells_per_aggregate: int = 4096):
+        """Coarse correction of the learned FCG's preconditioner over aggregates of this many
+        cells of the network's Morton order (a multiple of 256; 0 switches it off)."""
+        self.check(self.L.fc_set_fcg_coarse(self.h, int(cells_per_aggregate)))
 
     def set_network_layout(self, layout: int = NET_CLUSTERS):
         """Layout of the network's inference layers: NET_CLUSTERS (compact 256-cell clusters
@@ -957,6 +965,29 @@ class System:
         c, h, t = C.c_int(), C.c_int(), C.c_longlong()
         self.ctx.check(self.ctx.L.fc_sys_network_layout(self.h, C.byref(c), C.byref(h), C.byref(t)))
         return c.value, h.value, t.value
+
+    def coarse_space(self, with_arrays: bool = False):
+        """(aggregates, cells per aggregate, factored) of the learned FCG's coarse space on this
+        system's current matrix; with_arrays adds each cell's aggregate and A_c = P^T A P."""
+        nc, agg, ok = C.c_int(), C.c_int(), C.c_int()
+        self.ctx.check(self.ctx.L.fc_sys_coarse_space(self.h, C.byref(nc), C.byref(agg), C.byref(ok), None, None))
+        if not with_arrays or nc.value == 0:
+            return nc.value, agg.value, bool(ok.value)
+        cell_agg = np.zeros(self.n, np.int32)
+        ac = np.zeros((nc.value, nc.value))
+        self.ctx.check(self.ctx.L.fc_sys_coarse_space(self.h, C.byref(nc), C.byref(agg), C.byref(ok),
+                                                      cell_agg.ctypes.data_as(C.c_void_p),
+                                                      ac.ctypes.data_as(C.c_void_p)))
+        return nc.value, agg.value, bool(ok.value), cell_agg, ac.T.copy()
+
+    def precondition(self, r):
+        """One application of the learned FCG's preconditioner (attached network, Jacobi steps,
+        coarse correction) to r, in FP32."""
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros_like(r)
+        self.ctx.check(self.ctx.L.fc_sys_precondition(self.h, r.ctypes.data_as(C.c_void_p),
+                                                      z.ctypes.data_as(C.c_void_p)))
+        return z
 
     def solve(self, method=METHOD_ICCG, stop_rule=STOP_L1, sor=1e-8, nsw=1000):
         info = SolveInfo()

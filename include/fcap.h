@@ -353,6 +353,16 @@ fc_status fc_set_fcg_window(fc_context *ctx, int m);
  * steps z <- z + omega (r - A z) / a_ii (0..8 steps, 0 < omega < 2; default 3
  * steps, omega 0.8).  No counterpart in the reference. */
 fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega);
+/* Coarse correction of FC_METHOD_FCG_LEARNED's preconditioner, applied after the
+ * Jacobi steps: z <- z + P A_c^{-1} P^T (r - A z), P the indicator of aggregates
+ * of `cells_per_aggregate` consecutive cells of the network's Morton order
+ * (compact bricks of the mesh; cell order where the system has no cluster
+ * layout), A_c = P^T A P factored by dense Cholesky once per assembled matrix.
+ * A mesh of more than 4,096 x cells_per_aggregate cells gets larger aggregates
+ * (at most 4,096 of them).  0 switches it off; otherwise a multiple of 256
+ * (default 4,096).  The correction is skipped when A_c is not positive
+ * definite.  No counterpart in the reference. */
+fc_status fc_set_fcg_coarse(fc_context *ctx, int cells_per_aggregate);
 /* Layout of the network's inference layers (FC_METHOD_FCG_LEARNED, fc_gnn_apply).
  *   FC_NET_CLUSTERS   (default) the cells renumbered into compact clusters of
  *                     256 (Morton order of their per-axis centroid ranks); one
@@ -599,6 +609,17 @@ fc_status fc_gnn_edge_features(fc_gnn *net, fc_system *sys, float *e1, float *e2
  * other than 4, 6 or 8, or a halo above 384), the largest halo found and the
  * halo rows of all clusters together. */
 fc_status fc_sys_network_layout(fc_system *sys, int *clusters, int *halo_max, long long *halo_total);
+/* The coarse space FC_METHOD_FCG_LEARNED uses on the system (fc_set_fcg_coarse),
+ * built for its current matrix on first use: the number of aggregates, cells per
+ * aggregate and whether A_c was factored (all 0 when the correction is off);
+ * optionally each cell's aggregate (numCells ints, 0-based) and A_c = P^T A P
+ * (aggregates^2 doubles, column-major).  No counterpart in the reference. */
+fc_status fc_sys_coarse_space(fc_system *sys, int *aggregates, int *cells_per_aggregate, int *factored,
+                              int *cell_aggregate, double *matrix);
+/* One application of FC_METHOD_FCG_LEARNED's preconditioner to r (numCells
+ * floats): the attached network, the Jacobi steps and the coarse correction,
+ * as the solve applies them.  No counterpart in the reference. */
+fc_status fc_sys_precondition(fc_system *sys, const float *r, float *z);
 /* Loss (z-e)^T A (z-e) / (e^T A e) of one pair and its gradient (fc_gnn_num_params floats). */
 fc_status fc_gnn_loss_grad(fc_gnn *net, fc_system *sys, const float *r, const float *e, double *loss, float *grad);
 /* Add (residual, error) pairs of IC(0)-PCG iterates on the homogeneous system

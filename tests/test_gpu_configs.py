@@ -122,7 +122,8 @@ def test_config4_training_at_64_cubed_batch_8(ctx):
 def test_config5_box192_learned_fcg_reaches_1e8(ctx):
     """The learned flexible CG with the packaged weights reaches 1e-8 in the FP64 residual
     recomputed on the host, with no IC(0)-PCG fallback, and its solution is FP64 IC(0)-PCG's
-    within 1e-6."""
+    within 1e-6; its coarse correction needs fewer than half the iterations of the same solve
+    without it."""
     m = W.case_mesh(5, 192)
     s = fcap.System(m, ctx)
     net = gnn.Network.load(gnn.DEFAULT_WEIGHTS, ctx)
@@ -142,6 +143,15 @@ def test_config5_box192_learned_fcg_reaches_1e8(ctx):
               f"IC(0)-PCG {err:.2e}; IC(0)-PCG {info64['iters']} it, {info64['solve_ms']:.0f} ms")
         assert rc == fcap.FC_OK and info["fallback_iters"] == 0 and rel < 1e-8, info
         assert err < 1e-6
+        # the coarse correction over 4,096-cell aggregates (default) against none
+        ctx.set_fcg_coarse(0)
+        s.set_solution(np.zeros(m.n_cells))
+        rc0, info0 = s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, 1e-8, 20000)
+        print(f"config 5: learned FCG without the coarse correction {info0['iters']} it, "
+              f"{info0['solve_ms']:.0f} ms")
+        assert rc0 == fcap.FC_OK and info0["rel_l2"] < 1e-8, info0
+        assert info["iters"] < 0.5 * info0["iters"], (info["iters"], info0["iters"])
     finally:
+        ctx.set_fcg_coarse()
         net.close()
         s.close()

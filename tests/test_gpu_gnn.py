@@ -294,6 +294,7 @@ def test_fcg_learned_converges_and_beats_no_network(ctx, trained):
     m = M.hex_box(16, perturb=0.2, seed=99)      # not a training mesh
     s = _system(m, ctx, seed=123)
     ctx.set_fcg_smoothing(0, 0.8)               # the network alone against Jacobi
+    ctx.set_fcg_coarse(0)
     try:
         rc0, info0 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
         x0 = s.solution()
@@ -301,6 +302,7 @@ def test_fcg_learned_converges_and_beats_no_network(ctx, trained):
         x1 = s.solution()
     finally:
         ctx.set_fcg_smoothing()
+        ctx.set_fcg_coarse()
     assert rc0 == fcap.FC_OK and rc1 == fcap.FC_OK, (rc0, rc1, info0, info1)
     assert info0["rel_l2"] < 1e-8 and info1["rel_l2"] < 1e-8
     assert np.linalg.norm(x1 - x0) / np.linalg.norm(x0) < 1e-6
@@ -318,11 +320,13 @@ def test_fcg_with_zero_network_matches_jacobi_pcg(ctx):
     rc_j, info_j = s.solve(fcap.METHOD_DPCG, fcap.STOP_L2, 1e-3, 5000)
     ctx.set_inner_tolerance(1e-3)       # the inner solve runs to the same 1e-3
     ctx.set_fcg_smoothing(0, 0.8)       # the network alone
+    ctx.set_fcg_coarse(0)
     try:
         rc_f, info_f = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
     finally:
         ctx.set_inner_tolerance(0.0)
         ctx.set_fcg_smoothing()
+        ctx.set_fcg_coarse()
     assert rc_j == fcap.FC_OK and rc_f == fcap.FC_OK, (info_j, info_f)
     assert abs(info_f["iters"] - info_j["iters"]) <= 0.1 * info_j["iters"] + 5, (info_f["iters"], info_j["iters"])
     s.close()
@@ -373,6 +377,7 @@ def test_fcg_smoothing_steps_match_polynomial_jacobi_pcg(ctx):
     # whether the pair is positive or negative definite
     ref = _poly_jacobi_pcg(A, s.get("su"), 3, 0.8, 1e-3)
     ctx.set_inner_tolerance(1e-3)
+    ctx.set_fcg_coarse(0)
     try:
         ctx.set_fcg_smoothing(3, 0.8)
         rc3, info3 = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32), sor=1e-3)
@@ -381,6 +386,7 @@ def test_fcg_smoothing_steps_match_polynomial_jacobi_pcg(ctx):
     finally:
         ctx.set_inner_tolerance(0.0)
         ctx.set_fcg_smoothing()
+        ctx.set_fcg_coarse()
     assert rc3 == fcap.FC_OK and rc0 == fcap.FC_OK, (info3, info0)
     assert abs(info3["iters"] - ref) <= 0.1 * ref + 5, (info3["iters"], ref)
     assert info3["iters"] < 0.8 * info0["iters"], (info3["iters"], info0["iters"])
