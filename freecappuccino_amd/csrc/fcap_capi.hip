@@ -256,7 +256,7 @@ extern "C" fc_status fc_set_inner_tolerance(fc_context *ctx, double eta) {
 }
 
 extern "C" fc_status fc_set_fcg_window(fc_context *ctx, int m) {
-    if (!ctx || m < 1 || m > kFcgM) return FC_ERR_ARGUMENT;
+    if (!ctx || m < 0 || m > kFcgM) return FC_ERR_ARGUMENT;
     ctx->fcg_m = m;
     return FC_OK;
 }

@@ -59,6 +59,9 @@ struct GnnGraph {
     // coarse correction of flexible CG: aggregates of consecutive positions of the
     // cluster layout's order (cell order without one), rebuilt per assembly
     CoarseSpace coarse;
+    // structure the per-structure parts were built for (Structure::key, numCells)
+    uint64_t skey = 0;
+    int n_struct = -1;
     bool ready = false;
 };
 

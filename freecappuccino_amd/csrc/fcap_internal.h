@@ -166,8 +166,9 @@ struct fc_context {
     // fcg_stag_factor x the best before it (window 0: no test)
     int fcg_stag_window = 64;
     double fcg_stag_factor = 0.81;
-    // flexible CG directions kept, rounded up to 4, 8, 12 or 16 (fc_set_fcg_window)
-    int fcg_m = 12;
+    // flexible CG directions kept, rounded up to 4, 8, 12 or 16 (fc_set_fcg_window;
+    // 0: 4 with the coarse correction, 12 without)
+    int fcg_m = 0;
     // inner tolerance of the FP32 solves in FP64 refinement (fc_set_inner_tolerance;
     // 0: the method's default)
     double inner_eta = 0.0;

@@ -225,8 +225,9 @@ class Context:
         """Inner FP32 tolerance of the refinement solvers (0: the method's default)."""
         self.check(self.L.fc_set_inner_tolerance(self.h, float(eta)))
 
-    def set_fcg_window(self, m: int):
-        """Directions kept by flexible CG (1..16, rounded up to a multiple of 4)."""
+    def set_fcg_window(self, m: int = 0):
+        """Directions kept by flexible CG (1..16, rounded up to a multiple of 4; 0: 4 with the
+        coarse correction, 12 without)."""
         self.check(self.L.fc_set_fcg_window(self.h, int(m)))
 
     def set_fcg_smoothing(self, steps: int = 3, omega: float = 0.8):

@@ -346,7 +346,8 @@ fc_status fc_set_reduction_order(fc_context *ctx, int order);
  * from one refinement cycle to the next: each cycle replaces the FP32 residual by the
  * refined FP64 one and continues the same conjugate-gradient sequence.
  * fc_set_fcg_window: directions kept by flexible CG, 1..16, rounded up to a
- * multiple of 4 (default 12). */
+ * multiple of 4; 0 (default): 4 with the coarse correction (fc_set_fcg_coarse),
+ * 12 without it. */
 fc_status fc_set_inner_tolerance(fc_context *ctx, double eta);
 fc_status fc_set_fcg_window(fc_context *ctx, int m);
 /* Preconditioner of FC_METHOD_FCG_LEARNED: the network, then `steps` damped-Jacobi
