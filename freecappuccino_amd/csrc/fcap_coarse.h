@@ -20,6 +20,9 @@ namespace fcap {
 constexpr int kCoarseCells = 4096;
 constexpr int kCoarseMaxAggregates = 4096;
 
+// exclusive prefix sum of n ints on the stream (fcap_sort.hip)
+void exclusive_scan_i32(hipStream_t s, const int *in, int *out, int n, DevBuf &tmp);
+
 struct CoarseSpace {
     int n = 0, agg = 0, nc = 0;
     long long version = -1;   // assembly the operator was built for
@@ -31,7 +34,13 @@ struct CoarseSpace {
     DevBuf ainv;    // the inverse, FP32 row-major (both triangles)
     DevBuf amax;    // largest |a_ij| (FP64 bits as uint64)
     DevBuf info;    // factorisation status (int)
-    DevBuf rc, yc;  // P^T t and A_c^{-1} P^T t, FP32
+    DevBuf rc, yc;  // P^T (r - A z) and A_c^{-1} P^T (r - A z), FP32
+    // P^T A as the restriction reads it, by position p of aggregate I:
+    // colsum[p] = sum over rows i of I of a_{i,cell(p)}, and the entries of the
+    // aggregate's rows whose column lies in another aggregate, (column cell, a)
+    // in xcol/xval[xoff[p] .. xoff[p + 1]) (FP32, from the FP32 matrix)
+    DevBuf colsum, xoff, xcol, xval, xcnt, scan_tmp;
+    long long nx = 0;
 };
 
 // Cells per aggregate for n cells and a requested size (a multiple of 256).
@@ -40,18 +49,20 @@ int coarse_aggregate_cells(int n, int requested);
 // Build and factor A_c for the FP64 CSR matrix (0-based ptr/ja, n rows) on the
 // context's stream.  cs.ok is false afterwards when A_c is not positive definite
 // (the correction is then skipped).  perm/inv may be null (cell order).
-// max_row: the most entries of a row (bounds the fixed-point sums).
+// max_row: the most entries of a row (bounds the fixed-point sums); a32: the
+// FP32 copy of the matrix the preconditioner applies; trans: the CSR position
+// of each entry's transpose.
 void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const int *ja, const double *a,
-                  const int *perm, const int *inv, int agg, int max_row, long long version);
+                  const float *a32, const int *trans, const int *perm, const int *inv, int agg, int max_row,
+                  long long version);
 
 // P^T A P as built (FP64, column-major nc x nc) into host memory.
 void coarse_matrix_host(fc_context *ctx, const CoarseSpace &cs, double *out);
 
-// z = zf + P A_c^{-1} P^T (r - A zf) with the FP32 matrix; t: n floats of
-// scratch that may alias z but not zf; z may alias zf.  Kernels return at once
-// when sc->done is set.
-void coarse_apply(hipStream_t s, CoarseSpace &cs, const int *ptr, const int *ja, const float *a32, const float *r,
-                  const float *zf, float *t, float *z, const Scalars *sc);
+// z = zf + P A_c^{-1} P^T (r - A zf) with the FP32 matrix, P^T (r - A zf)
+// formed as P^T r - (P^T A) zf without a pass over the matrix; z may alias zf.
+// Kernels return at once when sc->done is set.
+void coarse_apply(hipStream_t s, CoarseSpace &cs, const float *r, const float *zf, float *z, const Scalars *sc);
 
 // Release the context's dense-solver handle (created on first use).
 void coarse_release(fc_context *ctx);

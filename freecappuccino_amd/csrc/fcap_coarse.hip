@@ -13,7 +13,6 @@
 
 #include "fcap_coarse.h"
 #include "fcap_kernels.h"
-#include "fcap_linalg.h"
 
 namespace fcap {
 namespace {
@@ -114,16 +113,96 @@ __global__ void k_co_inv32(int nc, const double *__restrict__ ac, float *__restr
     }
 }
 
-// rc[I] = sum of t over the cells at positions I agg .. (I + 1) agg - 1 (one block per aggregate)
+// Per position p (cell j of aggregate I): the number of entries of row j whose
+// column lies in another aggregate.
+__global__ void k_co_cross_count(int n, int agg, const int *__restrict__ ptr, const int *__restrict__ ja,
+                                 const int *__restrict__ perm, const int *__restrict__ inv, int *__restrict__ cnt) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n; p += gridDim.x * blockDim.x) {
+        int c = 0;
+        if (p < n) {
+            const int j = perm ? perm[p] : p, I = p / agg;
+            for (int k = ptr[j]; k < ptr[j + 1]; ++k) c += ((inv ? inv[ja[k]] : ja[k]) / agg != I);
+        }
+        cnt[p] = c;
+    }
+}
+
+// colsum[p] = sum over the rows i of I of a_ij (j = cell at p: the transposes of
+// row j's entries inside I), and row j's entries outside I in CSR order.
+__global__ void k_co_cross_fill(int n, int agg, const int *__restrict__ ptr, const int *__restrict__ ja,
+                                const float *__restrict__ a32, const int *__restrict__ trans,
+                                const int *__restrict__ perm, const int *__restrict__ inv,
+                                const int *__restrict__ xoff, float *__restrict__ colsum, int *__restrict__ xcol,
+                                float *__restrict__ xval) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int j = perm ? perm[p] : p, I = p / agg;
+        int o = xoff[p];
+        float s = 0.f;
+        for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
+            const int c = ja[k];
+            if ((inv ? inv[c] : c) / agg == I) {
+                s += a32[trans ? trans[k] : k];
+            } else {
+                xcol[o] = c;
+                xval[o] = a32[k];
+                ++o;
+            }
+        }
+        colsum[p] = s;
+    }
+}
+
+// rc[I] = sum over I of (r - A z) = sum_{j in I} (r_j - colsum_j z_j) - sum of the
+// aggregate's outside entries a z_c (one block per aggregate, a fixed order of sums)
 __global__ __launch_bounds__(kCoBlock) void k_co_restrict(int n, int agg, const int *__restrict__ perm,
-                                                         const float *__restrict__ t, float *__restrict__ rc,
+                                                         const float *__restrict__ r, const float *__restrict__ z,
+                                                         const float *__restrict__ colsum,
+                                                         const int *__restrict__ xoff, const int *__restrict__ xcol,
+                                                         const float *__restrict__ xval, float *__restrict__ rc,
                                                          const Scalars *__restrict__ sc) {
     if (sc && sc->done) return;
     __shared__ float part[kCoBlock / kWave];
     const int I = blockIdx.x;
     const int beg = I * agg, end = min(n, beg + agg);
+    // four positions (and four outside entries) per trip, their loads issued together
     float v = 0.f;
-    for (int p = beg + (int)threadIdx.x; p < end; p += kCoBlock) v += t[perm ? perm[p] : p];
+    int p = beg + (int)threadIdx.x;
+    for (; p + 3 * kCoBlock < end; p += 4 * kCoBlock) {
+        int j[4];
+        float c[4], rv[4], zv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            j[u] = perm ? perm[p + u * kCoBlock] : p + u * kCoBlock;
+            c[u] = colsum[p + u * kCoBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            rv[u] = r[j[u]];
+            zv[u] = z[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v += rv[u] - c[u] * zv[u];
+    }
+    for (; p < end; p += kCoBlock) {
+        const int j = perm ? perm[p] : p;
+        v += r[j] - colsum[p] * z[j];
+    }
+    const int e1 = xoff[end];
+    int e = xoff[beg] + (int)threadIdx.x;
+    for (; e + 3 * kCoBlock < e1; e += 4 * kCoBlock) {
+        int col[4];
+        float a[4], zv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            col[u] = xcol[e + u * kCoBlock];
+            a[u] = xval[e + u * kCoBlock];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) zv[u] = z[col[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v -= a[u] * zv[u];
+    }
+    for (; e < e1; e += kCoBlock) v -= xval[e] * z[xcol[e]];
     v = wave_sum(v);
     if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = v;
     __syncthreads();
@@ -176,7 +255,8 @@ int coarse_aggregate_cells(int n, int requested) {
 }
 
 void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const int *ja, const double *a,
-                  const int *perm, const int *inv, int agg, int max_row, long long version) {
+                  const float *a32, const int *trans, const int *perm, const int *inv, int agg, int max_row,
+                  long long version) {
     FC_REQUIRE(agg >= kCoBlock && agg % kCoBlock == 0, FC_ERR_ARGUMENT, "aggregate size must be a multiple of 256");
     const hipStream_t s = ctx->stream;
     cs.ok = false;
@@ -225,6 +305,21 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const
     FC_HIP_CHECK(hipStreamSynchronize(s));
     if (hinfo[0] != 0 || hinfo[1] != 0) return;   // not positive definite: no correction
     k_co_inv32<<<std::min(grid_for(cnt, 256), 8192), 256, 0, s>>>(nc, cs.ac.as<double>(), cs.ainv.as<float>());
+    // the restriction's operator P^T A
+    cs.xoff.ensure(sizeof(int) * ((size_t)n + 1));
+    cs.colsum.ensure(sizeof(float) * (size_t)n);
+    cs.xcnt.ensure(sizeof(int) * ((size_t)n + 1));
+    const int gb = std::min(grid_for((long long)n + 1, 256), 8192);
+    k_co_cross_count<<<gb, 256, 0, s>>>(n, agg, ptr, ja, perm, inv, cs.xcnt.as<int>());
+    exclusive_scan_i32(s, cs.xcnt.as<int>(), cs.xoff.as<int>(), n + 1, cs.scan_tmp);
+    int total = 0;
+    FC_HIP_CHECK(hipMemcpyAsync(&total, cs.xoff.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    FC_HIP_CHECK(hipStreamSynchronize(s));
+    cs.nx = total;
+    cs.xcol.ensure(sizeof(int) * (size_t)std::max(total, 1));
+    cs.xval.ensure(sizeof(float) * (size_t)std::max(total, 1));
+    k_co_cross_fill<<<gb, 256, 0, s>>>(n, agg, ptr, ja, a32, trans, perm, inv, cs.xoff.as<int>(),
+                                       cs.colsum.as<float>(), cs.xcol.as<int>(), cs.xval.as<float>());
     FC_HIP_CHECK(hipGetLastError());
     cs.ok = true;
 }
@@ -243,11 +338,10 @@ void coarse_matrix_host(fc_context *ctx, const CoarseSpace &cs, double *out) {
     for (size_t k = 0; k < cnt; ++k) out[k] = (double)fix[k] * inv_scale;
 }
 
-void coarse_apply(hipStream_t s, CoarseSpace &cs, const int *ptr, const int *ja, const float *a32, const float *r,
-                  const float *zf, float *t, float *z, const Scalars *sc) {
+void coarse_apply(hipStream_t s, CoarseSpace &cs, const float *r, const float *zf, float *z, const Scalars *sc) {
     const int n = cs.n;
-    residual32(s, n, ptr, ja, a32, zf, r, t, sc);
-    k_co_restrict<<<cs.nc, kCoBlock, 0, s>>>(n, cs.agg, cs.perm, t, cs.rc.as<float>(), sc);
+    k_co_restrict<<<cs.nc, kCoBlock, 0, s>>>(n, cs.agg, cs.perm, r, zf, cs.colsum.as<float>(), cs.xoff.as<int>(),
+                                             cs.xcol.as<int>(), cs.xval.as<float>(), cs.rc.as<float>(), sc);
     k_co_gemv<<<grid_for(cs.nc, kCoBlock / kWave), kCoBlock, 0, s>>>(cs.nc, cs.ainv.as<float>(), cs.rc.as<float>(),
                                                                      cs.yc.as<float>(), sc);
     k_co_prolong<<<std::min(grid_for(n, 256), 8192), 256, 0, s>>>(n, cs.agg, cs.inv, zf, cs.yc.as<float>(), z, sc);
