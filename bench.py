@@ -16,11 +16,15 @@ seeded Gaussian blobs with zero mean.  Four solves run on the same system:
                     steps (omega 0.8) and a coarse correction over 4,096-cell
                     aggregates of the mesh, inside FP64 iterative refinement,
                     IC(0)-PCG fallback on stagnation;
-  * fcg_learned_no_coarse -- fcg_learned without the coarse correction;
-  * fcg_jacobi_poly -- fcg_learned with the all-zero network: the preconditioner
-                    is then the Jacobi guess, the same three damped-Jacobi
-                    steps and the same coarse correction, which shows what the
-                    trained network itself buys;
+  * fcg_learned_every_iter -- fcg_learned with the network on every iteration
+                    (by default it runs on every second one, the Jacobi guess on
+                    the others);
+  * fcg_learned_no_coarse -- fcg_learned without the coarse correction (and
+                    with the network on every iteration);
+  * fcg_jacobi_poly -- the same flexible CG without the network
+                    (FC_METHOD_FCG_JACOBI): the Jacobi guess, the same three
+                    damped-Jacobi steps and the same coarse correction on every
+                    iteration, which shows what the trained network itself buys;
   * ic_pcg_fp64_levels -- ic_pcg_fp64 with one sweep launch per level (the same
                     bits, the schedule before the dependency-driven one).
 `value` is the time to 1e-8 of fcg_learned (ms per step, lower is better);
@@ -66,8 +70,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "fcg_learned_no_coarse", "fcg_jacobi_poly",
-           "ic_pcg_fp64_levels")
+METHODS = ("ic_pcg_fp64", "ic_pcg_mixed", "fcg_learned", "fcg_learned_every_iter", "fcg_learned_no_coarse",
+           "fcg_jacobi_poly", "ic_pcg_fp64_levels")
 
 # HBM bytes per launch of the hidden network layer, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (x2.0 and x1.0, the scales a 1 GiB device copy gives) on the
@@ -753,13 +757,11 @@ def main():
 
     net = zero_net = None
     layer_ms = None
-    if "fcg_learned" in methods or "fcg_learned_no_coarse" in methods:
+    if any(m in methods for m in ("fcg_learned", "fcg_learned_no_coarse", "fcg_learned_every_iter")):
         net = gnn.Network.load(args.weights or gnn.DEFAULT_WEIGHTS, ctx)
-    if "fcg_jacobi_poly" in methods:
-        zero_net = gnn.Network(np.zeros(gnn.N_PARAMS, np.float32), ctx)
     code = {"ic_pcg_fp64": fcap.METHOD_ICCG, "ic_pcg_mixed": fcap.METHOD_ICCG_MIXED,
-            "fcg_learned": fcap.METHOD_FCG_LEARNED, "fcg_jacobi_poly": fcap.METHOD_FCG_LEARNED,
-            "fcg_learned_no_coarse": fcap.METHOD_FCG_LEARNED,
+            "fcg_learned": fcap.METHOD_FCG_LEARNED, "fcg_jacobi_poly": fcap.METHOD_FCG_JACOBI,
+            "fcg_learned_no_coarse": fcap.METHOD_FCG_LEARNED, "fcg_learned_every_iter": fcap.METHOD_FCG_LEARNED,
             "ic_pcg_fp64_levels": fcap.METHOD_ICCG}
 
     def step(method):
@@ -773,10 +775,9 @@ def main():
     for name in methods:
         ctx.set_sweep_mode(fcap.SWEEP_LEVELS if name == "ic_pcg_fp64_levels" else fcap.SWEEP_DEFAULT)
         ctx.set_fcg_coarse(0 if name == "fcg_learned_no_coarse" else 4096)
-        if name in ("fcg_learned", "fcg_learned_no_coarse"):
+        ctx.set_fcg_network_period(1 if name == "fcg_learned_every_iter" else 0)
+        if name in ("fcg_learned", "fcg_learned_no_coarse", "fcg_learned_every_iter"):
             net.attach(s)
-        elif name == "fcg_jacobi_poly":
-            zero_net.attach(s)
         for _ in range(args.warmup):
             step(code[name])
         times, infos = [], []
@@ -794,6 +795,7 @@ def main():
         print(f"{name}: {results[name]}", file=sys.stderr, flush=True)
     ctx.set_sweep_mode(fcap.SWEEP_DEFAULT)
     ctx.set_fcg_coarse(4096)
+    ctx.set_fcg_network_period(0)
     coarse = s.coarse_space() if ("fcg_learned" in methods or "fcg_jacobi_poly" in methods) else None
     if "ic_pcg_fp64" in sols and "ic_pcg_fp64_levels" in sols:
         results["ic_pcg_fp64_levels"]["bitwise_equal_to_ic_pcg_fp64"] = bool(
@@ -832,13 +834,15 @@ def main():
         "value": round(value, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": {"ic_pcg_fp64": "f64", "ic_pcg_mixed": "f32+f64", "fcg_learned": "bf16+f32+f64",
-                  "fcg_learned_no_coarse": "bf16+f32+f64", "fcg_jacobi_poly": "f32+f64",
+                  "fcg_learned_no_coarse": "bf16+f32+f64", "fcg_learned_every_iter": "bf16+f32+f64",
+                  "fcg_jacobi_poly": "f32+f64",
                   "ic_pcg_fp64_levels": "f64"}[head],
         "data": "synthetic",
         "config": {"workload": workload, "cells": n, "nnz": nnz, "levels_fwd": sizes["nlev_fwd"],
                    "solver": head,
                    "preconditioner": ("BF16 message-passing network (4 layers, width 32) + 3 damped-Jacobi "
-                                      "steps, omega 0.8 + coarse correction over aggregates of the mesh"
+                                      "steps, omega 0.8 + coarse correction over aggregates of the mesh; "
+                                      "the network on every second iteration"
                                       if head == "fcg_learned" else "IC(0)")},
         "solvers": results,
         "spmv": {"fp64_ms": spmv_ms, "fp64_GBps": gbs, "fp64_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),

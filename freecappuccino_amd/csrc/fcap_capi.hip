@@ -282,6 +282,12 @@ extern "C" fc_status fc_set_fcg_coarse(fc_context *ctx, int cells_per_aggregate)
     return FC_OK;
 }
 
+extern "C" fc_status fc_set_fcg_network_period(fc_context *ctx, int period) {
+    if (!ctx || period < 0 || period > 4) return FC_ERR_ARGUMENT;
+    ctx->fcg_net_period = period;
+    return FC_OK;
+}
+
 extern "C" fc_status fc_set_network_layout(fc_context *ctx, int layout) {
     if (!ctx || (layout != FC_NET_CLUSTERS && layout != FC_NET_CELL_ORDER)) return FC_ERR_ARGUMENT;
     ctx->gnn_layout = layout;
@@ -1635,7 +1641,8 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         // keep the search direction: 604 iterations at 192^3 against 909 at 1e-3), and
         // SolveArgs' 1e-3 for the mixed BiCGStab, whose cycles restart
         if (ctx->inner_eta > 0.0 && ctx->inner_eta < 1.0) args.inner_eta = ctx->inner_eta;
-        else if (method == FC_METHOD_FCG_LEARNED || method == FC_METHOD_ICCG_MIXED) args.inner_eta = kFcgInnerEta;
+        else if (method == FC_METHOD_FCG_LEARNED || method == FC_METHOD_FCG_JACOBI || method == FC_METHOD_ICCG_MIXED)
+            args.inner_eta = kFcgInnerEta;
         fc_solve_info local;
         std::memset(&local, 0, sizeof local);
         fc_status s;
@@ -1648,6 +1655,11 @@ extern "C" fc_status fc_sys_solve(fc_system *sys, int method, int stop_rule, dou
         case FC_METHOD_FCG_LEARNED: {
             FC_REQUIRE(sys->net != nullptr, FC_ERR_ARGUMENT, "no network attached (fc_sys_set_network)");
             PrecondHook P = gnn_hook(sys);
+            s = solve_fcg(ctx, args, P, &local);
+            break;
+        }
+        case FC_METHOD_FCG_JACOBI: {
+            PrecondHook P = jacobi_hook(sys);
             s = solve_fcg(ctx, args, P, &local);
             break;
         }

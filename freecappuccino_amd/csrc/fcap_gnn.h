@@ -59,6 +59,7 @@ struct GnnGraph {
     // coarse correction of flexible CG: aggregates of consecutive positions of the
     // cluster layout's order (cell order without one), rebuilt per assembly
     CoarseSpace coarse;
+    DevBuf zscratch;   // the Jacobi steps' second vector
     // structure the per-structure parts were built for (Structure::key, numCells)
     uint64_t skey = 0;
     int n_struct = -1;
@@ -78,10 +79,11 @@ void gnn_prepare_graph(fc_context *ctx, GnnGraph &g, const Structure &st, const 
                        const double *arz, const double *xc, const double *yc, const double *zc, const double *a);
 
 // Preconditioner hook used by flexible CG: z = P(r) on the context's stream;
-// kernels return at once when sc->done is set.
+// kernels return at once when sc->done is set.  light: the application leaves
+// the network out (the Jacobi guess r / a_ii takes its place).
 struct PrecondHook {
     void *obj = nullptr;
-    void (*apply)(void *obj, hipStream_t s, const float *r, float *z, const Scalars *sc) = nullptr;
+    void (*apply)(void *obj, hipStream_t s, const float *r, float *z, const Scalars *sc, bool light) = nullptr;
 };
 
 }  // namespace fcap

@@ -177,6 +177,9 @@ struct fc_context {
     int gnn_layout = 0;       // FC_NET_CLUSTERS or FC_NET_CELL_ORDER (fc_set_network_layout)
     // cells per aggregate of flexible CG's coarse correction (fc_set_fcg_coarse; 0: off)
     int fcg_coarse = 4096;
+    // flexible CG iterations per network application (fc_set_fcg_network_period;
+    // 0: 2 with the coarse correction, 1 without)
+    int fcg_net_period = 0;
     void *dense_handle = nullptr;   // rocBLAS handle of the coarse factorisation (created on first use)
     // pinned host flag block read once per iteration batch
     int *host_flags = nullptr;

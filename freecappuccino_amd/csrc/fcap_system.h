@@ -40,4 +40,6 @@ namespace fcap {
 GnnGraph &system_graph(fc_system *sys);
 // The attached network as the preconditioner hook of flexible CG (apply == nullptr if none).
 PrecondHook gnn_hook(fc_system *sys);
+// the same preconditioner without the network (FC_METHOD_FCG_JACOBI)
+PrecondHook jacobi_hook(fc_system *sys);
 }  // namespace fcap

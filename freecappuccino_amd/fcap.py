@@ -35,7 +35,7 @@ STATUS_NAMES = {0: "FC_OK", 1: "FC_NOT_CONVERGED", 2: "FC_EARLY_EXIT", 3: "FC_FA
                 -5: "FC_ERR_STRUCTURE", -6: "FC_ERR_ALLOC", -7: "FC_ERR_NO_DEVICE"}
 VARIANT_SRC, VARIANT_TESTS = 0, 1
 METHOD_ICCG, METHOD_DPCG, METHOD_BICGSTAB = 1, 2, 3
-METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED = 11, 13, 21
+METHOD_ICCG_MIXED, METHOD_BICGSTAB_MIXED, METHOD_FCG_LEARNED, METHOD_FCG_JACOBI = 11, 13, 21, 22
 STOP_L1, STOP_L2 = 0, 1
 SWEEP_LEVELS, SWEEP_DEPENDENT, SWEEP_PARTS = 0, 2, 3
 REDUCE_TREE, REDUCE_SEQUENTIAL = 0, 1
@@ -85,7 +85,7 @@ def library() -> C.CDLL:
         "fc_set_sweep_waves": ([vp, i], i),
         "fc_set_reduction_order": ([vp, i], i), "fc_set_network_layout": ([vp, i], i),
         "fc_sys_network_layout": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(C.c_longlong)], i),
-        "fc_set_fcg_coarse": ([vp, i], i),
+        "fc_set_fcg_coarse": ([vp, i], i), "fc_set_fcg_network_period": ([vp, i], i),
         "fc_sys_coarse_space": ([vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), vp, vp], i),
         "fc_sys_precondition": ([vp, vp, vp], i),
         "fc_dpcg_noalloc": ([vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, i, si], i),
@@ -238,6 +238,11 @@ class Context:
         """Coarse correction of the learned FCG's preconditioner over aggregates of this many
         cells of the network's Morton order (a multiple of 256; 0 switches it off)."""
         self.check(self.L.fc_set_fcg_coarse(self.h, int(cells_per_aggregate)))
+
+    def set_fcg_network_period(self, period: int = 0):
+        """The learned FCG runs its network on the first of every `period` iterations (1..4; the
+        others start from the Jacobi guess); 0: 2 with the coarse correction, 1 without."""
+        self.check(self.L.fc_set_fcg_network_period(self.h, int(period)))
 
     def set_network_layout(self, layout: int = NET_CLUSTERS):
         """Layout of the network's inference layers: NET_CLUSTERS (compact 256-cell clusters

@@ -364,6 +364,13 @@ fc_status fc_set_fcg_smoothing(fc_context *ctx, int steps, double omega);
  * (default 4,096).  The correction is skipped when A_c is not positive
  * definite.  No counterpart in the reference. */
 fc_status fc_set_fcg_coarse(fc_context *ctx, int cells_per_aggregate);
+/* How often FC_METHOD_FCG_LEARNED's preconditioner runs the network: on the
+ * first of every `period` iterations (1..4); the others start from the Jacobi
+ * guess r / a_ii instead, with the same Jacobi steps and coarse correction
+ * (flexible CG allows a preconditioner that changes between iterations).
+ * 0 (default): 2 with the coarse correction, 1 without it.  No counterpart in
+ * the reference. */
+fc_status fc_set_fcg_network_period(fc_context *ctx, int period);
 /* Layout of the network's inference layers (FC_METHOD_FCG_LEARNED, fc_gnn_apply).
  *   FC_NET_CLUSTERS   (default) the cells renumbered into compact clusters of
  *                     256 (Morton order of their per-axis centroid ranks); one
@@ -429,8 +436,12 @@ enum {
     FC_METHOD_BICGSTAB = 3,        /* DILU-BiCGStab, FP64                                 */
     FC_METHOD_ICCG_MIXED = 11,     /* FP32 DIC-PCG inside FP64 iterative refinement       */
     FC_METHOD_BICGSTAB_MIXED = 13, /* FP32 DILU-BiCGStab inside FP64 iterative refinement */
-    FC_METHOD_FCG_LEARNED = 21     /* flexible CG, BF16 message-passing preconditioner,
+    FC_METHOD_FCG_LEARNED = 21,    /* flexible CG, BF16 message-passing preconditioner,
                                       FP32 inner / FP64 refinement, IC(0)-PCG fallback    */
+    FC_METHOD_FCG_JACOBI = 22      /* FC_METHOD_FCG_LEARNED without the network: the Jacobi
+                                      guess, the same damped-Jacobi steps and coarse
+                                      correction (no network needed); no counterpart in
+                                      the reference                                      */
 };
 
 /* Stopping rule of fc_sys_solve: FC_STOP_L1 is the reference's

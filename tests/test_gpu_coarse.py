@@ -131,8 +131,8 @@ def test_preconditioner_matches_oracle_and_annihilates_aggregate_residuals(ctx):
 
 def test_coarse_space_follows_the_matrix_and_the_setting(ctx):
     """A_c is rebuilt when the matrix is assembled again (mu = -2 doubles it) and when the
-    aggregate size changes; 0 switches the correction off; sizes that are not multiples of 256
-    are refused."""
+    aggregate size changes; a matrix whose A_c is not positive definite leaves the correction
+    out; 0 switches it off; sizes that are not multiples of 256 are refused."""
     m = M.hex_box(12, perturb=0.2, seed=3)
     s = _system(m, ctx)
     ctx.set_fcg_coarse(256)
@@ -143,6 +143,22 @@ def test_coarse_space_follows_the_matrix_and_the_setting(ctx):
         _, _, _, cell_agg2, ac2 = s.coarse_space(with_arrays=True)
         np.testing.assert_array_equal(cell_agg2, cell_agg)
         np.testing.assert_allclose(ac2, 2.0 * ac1, rtol=1e-9, atol=1e-9 * np.abs(ac1).max())
+        # a negative definite matrix (mu = +1) has a negative definite A_c: not factored, the
+        # correction is skipped, and flexible CG still converges with the zero network and the
+        # Jacobi steps (a preconditioner of the matrix's sign; CG takes either sign)
+        s.assemble_laplacian(np.full(m.n_cells, 1.0), None, np.zeros(m.n_cells))
+        assert s.coarse_space() == (nc, agg, False)
+        s.set_rhs(s.get("su") + np.random.default_rng(2).standard_normal(m.n_cells))
+        net = gnn.Network(np.zeros(gnn.N_PARAMS, np.float32), ctx)
+        net.attach(s)
+        try:
+            s.set_solution(np.zeros(s.n))
+            rc, info = s.solve(fcap.METHOD_FCG_LEARNED, fcap.STOP_L2, 1e-8, 5000)
+            assert rc == fcap.FC_OK and info["rel_l2"] < 1e-8, info
+        finally:
+            net.detach(s)
+            net.close()
+        s.assemble_laplacian(np.full(m.n_cells, -1.0), None, np.zeros(m.n_cells))
         ctx.set_fcg_coarse(512)
         nc3, agg3, ok3 = s.coarse_space()
         assert (nc3, agg3, ok3) == ((m.n_cells + 511) // 512, 512, True)
@@ -183,6 +199,7 @@ def test_coarse_correction_cuts_the_iterations(ctx):
         rc, info = s.solve(fcap.METHOD_ICCG, fcap.STOP_L2, 1e-12, 5000)
         x64 = s.solution()
         ctx.set_fcg_coarse(512)
+        ctx.set_fcg_network_period(1)          # the network on every iteration in all three
         rc1, info1, x1 = _fcg(ctx, s, th)
         rcz, infoz, _ = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
         ctx.set_fcg_coarse(0)
@@ -197,4 +214,23 @@ def test_coarse_correction_cuts_the_iterations(ctx):
         assert info1["iters"] < infoz["iters"], (info1["iters"], infoz["iters"])
     finally:
         ctx.set_fcg_coarse()
+        ctx.set_fcg_network_period()
+        s.close()
+
+
+def test_network_free_fcg_is_the_zero_network(ctx):
+    """FC_METHOD_FCG_JACOBI (no network attached) applies the preconditioner of the learned FCG
+    with the Jacobi guess r / a_ii in the network's place; the zero network's output is that
+    guess up to FP32 rounding, so both reach 1e-8 in the same number of iterations (within 2)."""
+    m = M.hex_box(24, perturb=0.2, seed=9)
+    s = _system(m, ctx, seed=4)
+    try:
+        s.set_solution(np.zeros(s.n))
+        rcj, infoj = s.solve(fcap.METHOD_FCG_JACOBI, fcap.STOP_L2, 1e-8, 5000)
+        rcz, infoz, _ = _fcg(ctx, s, np.zeros(gnn.N_PARAMS, np.float32))
+        print(f"network-free {infoj['iters']}, zero network {infoz['iters']}")
+        assert rcj == fcap.FC_OK and infoj["rel_l2"] < 1e-8 and infoj["fallback_iters"] == 0, infoj
+        assert rcz == fcap.FC_OK, infoz
+        assert abs(infoj["iters"] - infoz["iters"]) <= 2, (infoj["iters"], infoz["iters"])
+    finally:
         s.close()
