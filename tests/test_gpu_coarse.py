@@ -234,3 +234,28 @@ def test_network_free_fcg_is_the_zero_network(ctx):
         assert abs(infoj["iters"] - infoz["iters"]) <= 2, (infoj["iters"], infoz["iters"])
     finally:
         s.close()
+
+
+def test_network_period_setting(ctx):
+    """The network on every iteration, every second (the default with the coarse correction) and
+    every fourth: each reaches 1e-8 without fallback; fewer network applications take more
+    iterations here (the packaged weights on a perturbed 24^3 box), never fewer than the network
+    on every iteration; periods outside 0..4 are refused."""
+    m = M.hex_box(24, perturb=0.2, seed=9)
+    s = _system(m, ctx, seed=4)
+    th = np.load(gnn.DEFAULT_WEIGHTS).astype(np.float32)
+    try:
+        its = {}
+        for period in (1, 2, 4):
+            ctx.set_fcg_network_period(period)
+            rc, info, _ = _fcg(ctx, s, th)
+            assert rc == fcap.FC_OK and info["rel_l2"] < 1e-8 and info["fallback_iters"] == 0, (period, info)
+            its[period] = info["iters"]
+        print(f"iterations by network period: {its}")
+        assert its[1] <= its[2] and its[1] <= its[4], its
+        for bad in (-1, 5):
+            with pytest.raises(fcap.FcError):
+                ctx.set_fcg_network_period(bad)
+    finally:
+        ctx.set_fcg_network_period()
+        s.close()
