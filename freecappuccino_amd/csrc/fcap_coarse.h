@@ -40,7 +40,8 @@ struct CoarseSpace {
     // aggregate's rows whose column lies in another aggregate, (column cell, a)
     // in xcol/xval[xoff[p] .. xoff[p + 1]) (FP32, from the FP32 matrix)
     DevBuf colsum, xoff, xcol, xval, xcnt, scan_tmp;
-    long long nx = 0;
+    long long nx = -1;               // outside entries (-1: not counted yet)
+    const int *cross_ptr = nullptr;  // the CSR row pointer they were counted for
 };
 
 // Cells per aggregate for n cells and a requested size (a multiple of 256).
@@ -52,9 +53,9 @@ int coarse_aggregate_cells(int n, int requested);
 // max_row: the most entries of a row (bounds the fixed-point sums); a32: the
 // FP32 copy of the matrix the preconditioner applies; trans: the CSR position
 // of each entry's transpose.
-void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const int *ja, const double *a,
-                  const float *a32, const int *trans, const int *perm, const int *inv, int agg, int max_row,
-                  long long version);
+void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, long long nnz, const int *ptr, const int *ja,
+                  const double *a, const float *a32, const int *trans, const int *perm, const int *inv, int agg,
+                  int max_row, long long version);
 
 // P^T A P as built (FP64, column-major nc x nc) into host memory.
 void coarse_matrix_host(fc_context *ctx, const CoarseSpace &cs, double *out);

@@ -254,11 +254,14 @@ int coarse_aggregate_cells(int n, int requested) {
     return (int)std::max<long long>(requested, grown);
 }
 
-void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const int *ja, const double *a,
-                  const float *a32, const int *trans, const int *perm, const int *inv, int agg, int max_row,
-                  long long version) {
+void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, long long nnz, const int *ptr, const int *ja,
+                  const double *a, const float *a32, const int *trans, const int *perm, const int *inv, int agg,
+                  int max_row, long long version) {
     FC_REQUIRE(agg >= kCoBlock && agg % kCoBlock == 0, FC_ERR_ARGUMENT, "aggregate size must be a multiple of 256");
     const hipStream_t s = ctx->stream;
+    // the outside entries' positions depend on the structure and the aggregates only
+    const bool same_cross = cs.nx >= 0 && cs.n == n && cs.agg == agg && cs.perm == perm && cs.inv == inv &&
+                            cs.cross_ptr == ptr;
     cs.ok = false;
     cs.n = n;
     cs.agg = agg;
@@ -278,13 +281,6 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const
     cs.rc.ensure(sizeof(float) * (size_t)nc);
     cs.yc.ensure(sizeof(float) * (size_t)nc);
     const int nnz_rows = n;
-    long long nnz = 0;
-    {
-        int last = 0;
-        FC_HIP_CHECK(hipMemcpyAsync(&last, ptr + n, sizeof(int), hipMemcpyDeviceToHost, s));
-        FC_HIP_CHECK(hipStreamSynchronize(s));
-        nnz = last;
-    }
     k_co_zero<<<1, 1, 0, s>>>(1, reinterpret_cast<long long *>(cs.amax.get()));
     k_co_absmax<<<std::min(grid_for(nnz, kCoBlock), 4096), kCoBlock, 0, s>>>(nnz, a,
                                                                             cs.amax.as<unsigned long long>());
@@ -298,6 +294,8 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const
     FC_HIP_CHECK(hipGetLastError());
     rocblas_handle h = dense_handle(ctx);
     int *info = cs.info.as<int>();
+    // both enqueued before one status read; dpotri of a failed factor only reports a
+    // status of its own, and then neither result is used
     FC_BLAS_CHECK(rocsolver_dpotrf(h, rocblas_fill_lower, nc, cs.ac.as<double>(), nc, info));
     FC_BLAS_CHECK(rocsolver_dpotri(h, rocblas_fill_lower, nc, cs.ac.as<double>(), nc, info + 1));
     int hinfo[2] = {0, 0};
@@ -308,14 +306,18 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, const int *ptr, const
     // the restriction's operator P^T A
     cs.xoff.ensure(sizeof(int) * ((size_t)n + 1));
     cs.colsum.ensure(sizeof(float) * (size_t)n);
-    cs.xcnt.ensure(sizeof(int) * ((size_t)n + 1));
     const int gb = std::min(grid_for((long long)n + 1, 256), 8192);
-    k_co_cross_count<<<gb, 256, 0, s>>>(n, agg, ptr, ja, perm, inv, cs.xcnt.as<int>());
-    exclusive_scan_i32(s, cs.xcnt.as<int>(), cs.xoff.as<int>(), n + 1, cs.scan_tmp);
-    int total = 0;
-    FC_HIP_CHECK(hipMemcpyAsync(&total, cs.xoff.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    FC_HIP_CHECK(hipStreamSynchronize(s));
-    cs.nx = total;
+    if (!same_cross) {
+        cs.xcnt.ensure(sizeof(int) * ((size_t)n + 1));
+        k_co_cross_count<<<gb, 256, 0, s>>>(n, agg, ptr, ja, perm, inv, cs.xcnt.as<int>());
+        exclusive_scan_i32(s, cs.xcnt.as<int>(), cs.xoff.as<int>(), n + 1, cs.scan_tmp);
+        int last = 0;
+        FC_HIP_CHECK(hipMemcpyAsync(&last, cs.xoff.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        FC_HIP_CHECK(hipStreamSynchronize(s));
+        cs.nx = last;
+        cs.cross_ptr = ptr;
+    }
+    const int total = (int)cs.nx;
     cs.xcol.ensure(sizeof(int) * (size_t)std::max(total, 1));
     cs.xval.ensure(sizeof(float) * (size_t)std::max(total, 1));
     k_co_cross_fill<<<gb, 256, 0, s>>>(n, agg, ptr, ja, a32, trans, perm, inv, cs.xoff.as<int>(),
