@@ -154,7 +154,7 @@ def library() -> C.CDLL:
         "fc_gnn_clear_samples": ([vp], i),
         "fc_gnn_drop_system": ([vp, vp], i),
         "fc_gnn_train": ([vp, i, i, C.c_float, C.c_uint, fp], i),
-        "fc_gnn_set_training_smoothing": ([vp, i, d], i),
+        "fc_gnn_set_training_smoothing": ([vp, i, d], i), "fc_gnn_set_training_coarse": ([vp, i], i),
         "fc_sys_set_network": ([vp, vp], i),
     }
     for name in ("fc_iccg", "fc_dpcg", "fc_bicgstab"):

@@ -168,6 +168,11 @@ class Network:
         its gradient); 0 steps: the network's own output."""
         self.ctx.check(self.ctx.L.fc_gnn_set_training_smoothing(self.h, int(steps), float(omega)))
 
+    def set_training_coarse(self, on: bool = True):
+        """Take the solver's coarse correction (after the Jacobi steps) into the training loss
+        and its gradient."""
+        self.ctx.check(self.ctx.L.fc_gnn_set_training_coarse(self.h, int(bool(on))))
+
     def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
         losses = np.zeros(steps, np.float32)
         self.ctx.check(self.ctx.L.fc_gnn_train(self.h, steps, batch, float(lr), seed, _fp(losses)))

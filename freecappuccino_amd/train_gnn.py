@@ -113,6 +113,8 @@ def main(argv=None):
     ap.add_argument("--smooth", type=int, default=0,
                     help="damped-Jacobi steps after the network taken into the loss (the solver applies 3)")
     ap.add_argument("--omega", type=float, default=0.8, help="their damping (the solver's: 0.8)")
+    ap.add_argument("--coarse", action="store_true",
+                    help="take the solver's coarse correction (after the Jacobi steps) into the loss")
     args = ap.parse_args(argv)
 
     ctx = fcap.Context(0)
@@ -120,6 +122,8 @@ def main(argv=None):
     net = gnn.Network(init, ctx)
     if args.smooth:
         net.set_training_smoothing(args.smooth, args.omega)
+    if args.coarse:
+        net.set_training_coarse(True)
     command = " ".join(["python -m freecappuccino_amd.train_gnn", *(argv if argv is not None else sys.argv[1:])])
     iters = [int(x) for x in args.iters.split(",")]
     if args.stream:
@@ -130,7 +134,7 @@ def main(argv=None):
         net.save(args.out)
         if args.log:
             with open(args.log, "w") as f:
-                json.dump(dict(res, command=command, smooth=args.smooth, omega=args.omega), f)
+                json.dump(dict(res, command=command, smooth=args.smooth, omega=args.omega, coarse=args.coarse), f)
         h = res["loss"]
         print(f"saved {args.out}: {args.stream} systems in {res['wall_s']:.1f} s, first-100 mean loss "
               f"{np.mean(h[:100]):.4f}, last-100 mean loss {np.mean(h[-100:]):.4f}", flush=True)
@@ -159,7 +163,7 @@ def main(argv=None):
             json.dump({"command": command, "n": args.n, "systems": args.systems, "graded": args.graded,
                        "graded_n": args.graded_n, "iters": iters, "steps": args.steps, "batch": args.batch,
                        "lr": args.lr, "lr_final": args.lr_final, "seed": args.seed, "init": args.init,
-                       "smooth": args.smooth, "omega": args.omega, "seconds": time.time() - t0,
+                       "smooth": args.smooth, "omega": args.omega, "coarse": args.coarse, "seconds": time.time() - t0,
                        "loss": history}, f)
     print(f"saved {args.out}: first-100 mean loss {np.mean(history[:100]):.4f}, "
           f"last-100 mean loss {np.mean(history[-100:]):.4f}", flush=True)

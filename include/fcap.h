@@ -653,6 +653,11 @@ fc_status fc_gnn_train(fc_gnn *net, int steps, int batch, float lr, unsigned see
  * (0..8 steps, 0 < omega < 2; default 0: the network's own output).  Set it to
  * the solver's fc_set_fcg_smoothing to train what the solver applies. */
 fc_status fc_gnn_set_training_smoothing(fc_gnn *net, int steps, double omega);
+/* Training with the coarse correction after the Jacobi steps in the loss (on = 1;
+ * fc_gnn_train, fc_gnn_loss_grad): z <- z + P A_c^-1 P^T (r - A z) with each sample
+ * system's coarse space (fc_set_fcg_coarse of its context), the gradient through
+ * its adjoint dz - A P A_c^-1 P^T dz.  Default 0. */
+fc_status fc_gnn_set_training_coarse(fc_gnn *net, int on);
 /* Attach a network as the preconditioner of FC_METHOD_FCG_LEARNED solves of sys. */
 fc_status fc_sys_set_network(fc_system *sys, fc_gnn *net);
 

@@ -119,19 +119,37 @@ def smooth_steps(z, r, A, diag, steps, omega):
     return z
 
 
-def loss_and_grad(theta, graph: Graph, A, pairs, bf=True, smooth=0, omega=0.8):
+def coarse_correct(z, r, A, cell_agg):
+    """z + P A_c^-1 P^T (r - A z), P the indicator of the aggregates cell_agg, A_c = P^T A P
+    (dense FP64 solve)."""
+    import scipy.sparse as sp
+    n = z.shape[0]
+    nc = int(cell_agg.max()) + 1
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), cell_agg)), shape=(n, nc))
+    Ac = (P.T @ A @ P).toarray()
+    return z + P @ np.linalg.solve(Ac, P.T @ (np.asarray(r, np.float64) - A @ z)), P, Ac
+
+
+def loss_and_grad(theta, graph: Graph, A, pairs, bf=True, smooth=0, omega=0.8, cell_agg=None):
     """Mean loss over (r, e) pairs and its gradient (explicit backward pass); the loss is
-    taken after `smooth` damped-Jacobi steps of the network's output."""
+    taken after `smooth` damped-Jacobi steps of the network's output and, with cell_agg (each
+    cell's aggregate), the coarse correction after them."""
     grad = np.zeros(N_PARAMS)
     total = 0.0
     for r, e in pairs:
         z, cache, p = forward(theta, graph, r, bf, keep=True)
         e = np.asarray(e, np.float64)
-        d = smooth_steps(z, r, A, graph.diag, smooth, omega) - e
+        zs = smooth_steps(z, r, A, graph.diag, smooth, omega)
+        if cell_agg is not None:
+            zs, P, Ac = coarse_correct(zs, r, A, cell_agg)
+        d = zs - e
         Ad = A @ d
         eAe = float(e @ (A @ e))
         total += float(d @ Ad) / eAe
         dz = 2.0 * Ad / eAe / len(pairs)
+        if cell_agg is not None:
+            # back through z_c = (I - P A_c^-1 P^T A) z + ...: dz <- dz - A P A_c^-1 P^T dz
+            dz = dz - A @ (P @ np.linalg.solve(Ac, P.T @ dz))
         if smooth:
             # back through z_{k+1} = M z_k + omega D^-1 r: dz <- M^T dz = D M D^-1 dz
             t = dz / graph.diag

@@ -106,6 +106,33 @@ def test_smoothed_backward_matches_finite_differences():
         assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
 
 
+def test_coarse_corrected_backward_matches_finite_differences():
+    """With the Jacobi steps and the coarse correction in the loss, the backward pass (adjoint of
+    the correction, dz - A P A_c^-1 P^T dz, then of the steps) against central differences; after
+    the correction the residual of every aggregate is zero, whatever the network gives."""
+    m, graph, A, _ = _small_graph(n=4)
+    rng = np.random.default_rng(15)
+    th = gnn.init_params(7).astype(np.float64)
+    th[G.OFF_DEC_W:G.OFF_DEC_W + G.H] *= 20.0
+    cell_agg = (np.arange(m.n_cells) * 4) // m.n_cells          # four aggregates of consecutive cells
+    pairs = [(rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)) for _ in range(2)]
+    loss, grad = G.loss_and_grad(th, graph, A, pairs, bf=False, smooth=2, omega=0.8, cell_agg=cell_agg)
+    smoothed, _ = G.loss_and_grad(th, graph, A, pairs, bf=False, smooth=2, omega=0.8)
+    assert abs(loss - smoothed) > 1e-3 * smoothed
+    r = pairs[0][0]
+    zc, P, _ = G.coarse_correct(rng.standard_normal(m.n_cells), r, A, cell_agg)
+    assert np.abs(P.T @ (r - A @ zc)).max() < 1e-10 * np.abs(P.T @ r).max()
+    eps = 1e-6
+    for i in _probe_indices(rng):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += eps
+        tm[i] -= eps
+        lp, _ = G.loss_and_grad(tp, graph, A, pairs, bf=False, smooth=2, omega=0.8, cell_agg=cell_agg)
+        lm, _ = G.loss_and_grad(tm, graph, A, pairs, bf=False, smooth=2, omega=0.8, cell_agg=cell_agg)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - grad[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, grad[i])
+
+
 def test_smoothing_steps_are_jacobi_iterations():
     """smooth_steps from z = 0 is the Jacobi iteration for A z = r: its error contracts by
     (I - omega D^-1 A) each step, checked against the matrix power."""

@@ -259,3 +259,38 @@ def test_network_period_setting(ctx):
     finally:
         ctx.set_fcg_network_period()
         s.close()
+
+
+def test_training_loss_and_gradient_through_the_correction_match_oracle(ctx):
+    """Loss and gradient with the three Jacobi steps and the coarse correction in the loss
+    (fc_gnn_set_training_coarse) on the graded 20^3 box with 256-cell aggregates, against the
+    oracle at the same BF16 rounding points with the aggregate map the library reports."""
+    m = MESHES["hex20_graded"]()
+    geo, csr, a, A, graph, feats = gnn_case(m)
+    s = _system(m, ctx)
+    th = gnn.init_params(7).astype(np.float64)
+    th[G.OFF_DEC_W:G.OFF_DEC_W + G.H] *= 20.0
+    th = th.astype(np.float32)
+    net = gnn.Network(th, ctx)
+    ctx.set_fcg_coarse(256)
+    try:
+        _, _, ok, cell_agg, _ = s.coarse_space(with_arrays=True)
+        assert ok
+        net.set_training_smoothing(3, 0.8)
+        net.set_training_coarse(True)
+        rng = np.random.default_rng(41)
+        r = rng.standard_normal(m.n_cells).astype(np.float32)
+        e = rng.standard_normal(m.n_cells).astype(np.float32)
+        loss_gpu, g_gpu = net.loss_grad(s, r, e)
+        loss_ref, g_ref = G.loss_and_grad(th.astype(np.float64), graph, A, [(r, e)], bf=True, smooth=3,
+                                          omega=0.8, cell_agg=cell_agg)
+        g_gpu = g_gpu.astype(np.float64)
+        cos = g_gpu @ g_ref / (np.linalg.norm(g_gpu) * np.linalg.norm(g_ref))
+        rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
+        print(f"coarse-corrected loss {abs(loss_gpu - loss_ref) / loss_ref:.2e}, gradient cos {cos:.6f} rel {rel:.2e}")
+        assert abs(loss_gpu - loss_ref) <= 1e-5 * abs(loss_ref)
+        assert cos > 0.9999 and rel < 5e-3, (cos, rel)
+    finally:
+        ctx.set_fcg_coarse()
+        net.close()
+        s.close()
