@@ -20,8 +20,11 @@ namespace fcap {
 constexpr int kCoarseCells = 4096;
 constexpr int kCoarseMaxAggregates = 4096;
 
-// exclusive prefix sum of n ints on the stream (fcap_sort.hip)
+// exclusive prefix sum of n ints, and a radix sort of (64-bit key, int) pairs, on the
+// stream (fcap_sort.hip)
 void exclusive_scan_i32(hipStream_t s, const int *in, int *out, int n, DevBuf &tmp);
+void sort_pairs_u64(hipStream_t s, const unsigned long long *key_in, unsigned long long *key_out, const int *val_in,
+                    int *val_out, int n, DevBuf &tmp);
 
 struct CoarseSpace {
     int n = 0, agg = 0, nc = 0;
@@ -35,10 +38,14 @@ struct CoarseSpace {
     DevBuf amax;    // largest |a_ij| (FP64 bits as uint64)
     DevBuf info;    // factorisation status (int)
     DevBuf rc, yc;  // P^T (r - A z) and A_c^{-1} P^T (r - A z), FP32
-    // P^T A as the restriction reads it, by position p of aggregate I:
-    // colsum[p] = sum over rows i of I of a_{i,cell(p)}, and the entries of the
-    // aggregate's rows whose column lies in another aggregate, (column cell, a)
-    // in xcol/xval[xoff[p] .. xoff[p + 1]) (FP32, from the FP32 matrix)
+    // Each aggregate's cells in ascending cell order (slots I agg .. (I + 1) agg - 1;
+    // null with cell order): the restriction reads r and z in runs of consecutive cells.
+    DevBuf cells;
+    const int *cell_of = nullptr;
+    // P^T A as the restriction reads it, by slot q of aggregate I (cell j = cells[q]):
+    // colsum[q] = sum over rows i of I of a_ij, and row j's entries whose column lies in
+    // another aggregate, (column cell, a) in xcol/xval[xoff[q] .. xoff[q + 1]) (FP32,
+    // from the FP32 matrix)
     DevBuf colsum, xoff, xcol, xval, xcnt, scan_tmp;
     long long nx = -1;               // outside entries (-1: not counted yet)
     const int *cross_ptr = nullptr;  // the CSR row pointer they were counted for

@@ -113,14 +113,24 @@ __global__ void k_co_inv32(int nc, const double *__restrict__ ac, float *__restr
     }
 }
 
-// Per position p (cell j of aggregate I): the number of entries of row j whose
-// column lies in another aggregate.
+// (aggregate, cell) keys of the positions, to list each aggregate's cells in cell order
+__global__ void k_co_cell_keys(int n, int agg, const int *__restrict__ perm, unsigned long long *__restrict__ key,
+                               int *__restrict__ val) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int j = perm[p];
+        key[p] = ((unsigned long long)(p / agg) << 32) | (unsigned)j;
+        val[p] = j;
+    }
+}
+
+// Per slot q (cell j = cells[q] of aggregate I = q / agg): the number of entries of
+// row j whose column lies in another aggregate.
 __global__ void k_co_cross_count(int n, int agg, const int *__restrict__ ptr, const int *__restrict__ ja,
-                                 const int *__restrict__ perm, const int *__restrict__ inv, int *__restrict__ cnt) {
+                                 const int *__restrict__ cells, const int *__restrict__ inv, int *__restrict__ cnt) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n; p += gridDim.x * blockDim.x) {
         int c = 0;
         if (p < n) {
-            const int j = perm ? perm[p] : p, I = p / agg;
+            const int j = cells ? cells[p] : p, I = p / agg;
             for (int k = ptr[j]; k < ptr[j + 1]; ++k) c += ((inv ? inv[ja[k]] : ja[k]) / agg != I);
         }
         cnt[p] = c;
@@ -131,11 +141,11 @@ __global__ void k_co_cross_count(int n, int agg, const int *__restrict__ ptr, co
 // row j's entries inside I), and row j's entries outside I in CSR order.
 __global__ void k_co_cross_fill(int n, int agg, const int *__restrict__ ptr, const int *__restrict__ ja,
                                 const float *__restrict__ a32, const int *__restrict__ trans,
-                                const int *__restrict__ perm, const int *__restrict__ inv,
+                                const int *__restrict__ cells, const int *__restrict__ inv,
                                 const int *__restrict__ xoff, float *__restrict__ colsum, int *__restrict__ xcol,
                                 float *__restrict__ xval) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const int j = perm ? perm[p] : p, I = p / agg;
+        const int j = cells ? cells[p] : p, I = p / agg;
         int o = xoff[p];
         float s = 0.f;
         for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
@@ -154,7 +164,7 @@ __global__ void k_co_cross_fill(int n, int agg, const int *__restrict__ ptr, con
 
 // rc[I] = sum over I of (r - A z) = sum_{j in I} (r_j - colsum_j z_j) - sum of the
 // aggregate's outside entries a z_c (one block per aggregate, a fixed order of sums)
-__global__ __launch_bounds__(kCoBlock) void k_co_restrict(int n, int agg, const int *__restrict__ perm,
+__global__ __launch_bounds__(kCoBlock) void k_co_restrict(int n, int agg, const int *__restrict__ cells,
                                                          const float *__restrict__ r, const float *__restrict__ z,
                                                          const float *__restrict__ colsum,
                                                          const int *__restrict__ xoff, const int *__restrict__ xcol,
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(kCoBlock) void k_co_restrict(int n, int agg, const 
         float c[4], rv[4], zv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            j[u] = perm ? perm[p + u * kCoBlock] : p + u * kCoBlock;
+            j[u] = cells ? cells[p + u * kCoBlock] : p + u * kCoBlock;
             c[u] = colsum[p + u * kCoBlock];
         }
 #pragma unroll
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(kCoBlock) void k_co_restrict(int n, int agg, const 
         for (int u = 0; u < 4; ++u) v += rv[u] - c[u] * zv[u];
     }
     for (; p < end; p += kCoBlock) {
-        const int j = perm ? perm[p] : p;
+        const int j = cells ? cells[p] : p;
         v += r[j] - colsum[p] * z[j];
     }
     const int e1 = xoff[end];
@@ -308,8 +318,18 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, long long nnz, const 
     cs.colsum.ensure(sizeof(float) * (size_t)n);
     const int gb = std::min(grid_for((long long)n + 1, 256), 8192);
     if (!same_cross) {
+        cs.cell_of = nullptr;
+        if (perm) {
+            DevBuf ka(sizeof(unsigned long long) * (size_t)n), kb(sizeof(unsigned long long) * (size_t)n),
+                va(sizeof(int) * (size_t)n);
+            cs.cells.ensure(sizeof(int) * (size_t)n);
+            k_co_cell_keys<<<gb, 256, 0, s>>>(n, agg, perm, ka.as<unsigned long long>(), va.as<int>());
+            sort_pairs_u64(s, ka.as<unsigned long long>(), kb.as<unsigned long long>(), va.as<int>(),
+                           cs.cells.as<int>(), n, cs.scan_tmp);
+            cs.cell_of = cs.cells.as<int>();
+        }
         cs.xcnt.ensure(sizeof(int) * ((size_t)n + 1));
-        k_co_cross_count<<<gb, 256, 0, s>>>(n, agg, ptr, ja, perm, inv, cs.xcnt.as<int>());
+        k_co_cross_count<<<gb, 256, 0, s>>>(n, agg, ptr, ja, cs.cell_of, inv, cs.xcnt.as<int>());
         exclusive_scan_i32(s, cs.xcnt.as<int>(), cs.xoff.as<int>(), n + 1, cs.scan_tmp);
         int last = 0;
         FC_HIP_CHECK(hipMemcpyAsync(&last, cs.xoff.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -320,7 +340,7 @@ void coarse_setup(fc_context *ctx, CoarseSpace &cs, int n, long long nnz, const 
     const int total = (int)cs.nx;
     cs.xcol.ensure(sizeof(int) * (size_t)std::max(total, 1));
     cs.xval.ensure(sizeof(float) * (size_t)std::max(total, 1));
-    k_co_cross_fill<<<gb, 256, 0, s>>>(n, agg, ptr, ja, a32, trans, perm, inv, cs.xoff.as<int>(),
+    k_co_cross_fill<<<gb, 256, 0, s>>>(n, agg, ptr, ja, a32, trans, cs.cell_of, inv, cs.xoff.as<int>(),
                                        cs.colsum.as<float>(), cs.xcol.as<int>(), cs.xval.as<float>());
     FC_HIP_CHECK(hipGetLastError());
     cs.ok = true;
@@ -342,7 +362,7 @@ void coarse_matrix_host(fc_context *ctx, const CoarseSpace &cs, double *out) {
 
 void coarse_apply(hipStream_t s, CoarseSpace &cs, const float *r, const float *zf, float *z, const Scalars *sc) {
     const int n = cs.n;
-    k_co_restrict<<<cs.nc, kCoBlock, 0, s>>>(n, cs.agg, cs.perm, r, zf, cs.colsum.as<float>(), cs.xoff.as<int>(),
+    k_co_restrict<<<cs.nc, kCoBlock, 0, s>>>(n, cs.agg, cs.cell_of, r, zf, cs.colsum.as<float>(), cs.xoff.as<int>(),
                                              cs.xcol.as<int>(), cs.xval.as<float>(), cs.rc.as<float>(), sc);
     k_co_gemv<<<grid_for(cs.nc, kCoBlock / kWave), kCoBlock, 0, s>>>(cs.nc, cs.ainv.as<float>(), cs.rc.as<float>(),
                                                                      cs.yc.as<float>(), sc);
